@@ -1,0 +1,151 @@
+/* peridyn.h — C ABI of libperidyn_hip.so, the MI355X (gfx950) implementation of
+ * PeriDyn's explicit-dynamics hot path.
+ *
+ * The reference is Julia; its GPU seam is multiple dispatch on a device tag
+ * (DEVICE[], src/misc/const_macros.jl:10) reaching per-device methods such as
+ * force_density_T!(f, y, limits, mat, Val{device}) (src/materials/general_material.jl:274-276).
+ * A Julia shim binds these entry points with ccall.  Every
+ * signature uses plain pointers and sizes; arrays use the reference's layouts:
+ *   positions / velocities / forces  3 x n column-major Float64 (x[3*i + d])
+ *   family / intact                  M' x n column-major, 1-based ids, 0 padded
+ *   parameter matrices               ntype x ntype column-major Float64
+ *
+ * Ownership: host buffers are copied; nothing is retained after return.  Device
+ * state belongs to the context and is freed by pd_ctx_destroy / pd_env_destroy.
+ * Errors: every call returns 0 or a negative PD_E_* code; pd_last_error() gives
+ * the message.  A context is not re-entrant: use it from one host thread.
+ */
+#ifndef PERIDYN_H
+#define PERIDYN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PD_OK = 0,
+  PD_E_HIP = -1,       /* HIP runtime error (message has the HIP error string) */
+  PD_E_ARG = -2,       /* invalid argument */
+  PD_E_OVERFLOW = -3,  /* family degree >= max_neigh: the reference raises BoundsError */
+  PD_E_STATE = -4,     /* call not valid in the current state (e.g. env not committed) */
+  PD_E_NOMEM = -5,     /* device memory exhausted */
+  PD_E_NONFINITE = -6  /* NaN/Inf found in state (pd_env_check_finite) */
+};
+
+/* material kinds: BondBasedMaterial, OrdinaryStateBasedMaterial, ElastoPlasticSolidMaterial */
+enum { PD_BB = 0, PD_OSB = 1, PD_EP = 2 };
+
+/* arithmetic forms of the force passes */
+enum {
+  PD_PREC_EXACT = 0, /* FP64 in the single-threaded reference's summation order: bit-identical */
+  PD_PREC_FP64 = 1,  /* FP64, one pass per bond in neighbour order (plain FP64 form) */
+  PD_PREC_FP32C = 2  /* FP32 displacement-difference stretch + Kahan-compensated FP32 sums;
+                        FP64 Verlet update and weighted volume */
+};
+
+enum { PD_BC_FIX = 0, PD_BC_TOFRO = 1 };
+
+/* One material block = GeneralMaterial + *Specific after init()
+ * (general_material.jl:43-56,81-98; bond_based.jl:21-27,118-122;
+ *  ordinary_state_based.jl:18-24; elasto_plastic.jl:29-40,59-71). */
+typedef struct {
+  int32_t kind;                 /* PD_BB / PD_OSB / PD_EP */
+  int32_t ntype;                /* number of types in the block (type range length) */
+  int64_t n;                    /* particles */
+  int64_t type_start;           /* mat.type.start */
+  const double* x;              /* 3 x n reference positions */
+  const double* y;              /* 3 x n initial deformed positions */
+  const double* v;              /* 3 x n initial velocities */
+  const double* volume;         /* n */
+  const int64_t* type;          /* n global type ids */
+  double horizon;
+  double particle_size;         /* volume[1]^(1/3) as computed by the caller */
+  int64_t max_neigh;            /* GeneralMaterial max_neigh */
+  const double* density;        /* ntype */
+  const double* critical_stretch; /* ntype x ntype (symmetric) */
+  const double* bond_stiffness; /* PD_BB: 18K/pi/delta^4 matrix (only [1,1] is used, bond_based.jl:229) */
+  const double* bulk_modulus;   /* PD_OSB / PD_EP */
+  const double* shear_modulus;  /* PD_OSB / PD_EP */
+  const double* psi;            /* PD_EP: 75/(8pi) sigma_e^2 / delta^5 */
+  int32_t deformed;             /* GeneralMaterial.deformed (y0 != x) */
+  int32_t reserved;
+} pd_block_desc;
+
+/* NonLinearSpringContactModel12 / LinearSpringContactModel (exponent 1) /
+ * ShortRangeContactModel (contacts.jl:76-91, nonlinearspring.jl:63-76,124-145). */
+typedef struct {
+  const int64_t* types1; int64_t ntypes1; /* pair[1] type ids */
+  const int64_t* types2; int64_t ntypes2; /* pair[2] type ids */
+  double exponent;
+  int32_t exponent_is_int;      /* Julia Int exponent (power by squaring) vs Float64 */
+  int32_t reserved;
+  double stiffness;
+  double equi_dist;             /* (ps1 + ps2)/2 * distanceD */
+  double distance;              /* distanceX * equi_dist */
+  int64_t max_neighs;           /* min(max_neighs, n2) */
+} pd_contact_desc;
+
+/* FixBC / ToFroBC / MoveBC (= ToFroBC with freq = Inf), all dims
+ * (FixBC.jl:46-52, ToFroBC.jl:51-62,79-81,107-111). */
+typedef struct {
+  int32_t kind;                 /* PD_BC_FIX / PD_BC_TOFRO */
+  int32_t reserved;
+  const uint8_t* mask;          /* N (env particles) */
+  double rate[3];               /* ToFroBC rate (a velocity) */
+  double freq;                  /* ToFroBC freq (integer-valued or +Inf) */
+  int64_t applyafter;
+} pd_bc_desc;
+
+/* ---- context --------------------------------------------------------- */
+int pd_ctx_create(int device, void** ctx);
+int pd_ctx_destroy(void* ctx);
+const char* pd_last_error(void* ctx);
+int pd_sync(void* ctx);
+
+/* ---- environment (Env, src/environment.jl:166-235) ------------------- */
+int pd_env_create(void* ctx, double dt, int32_t precision, int64_t* env);
+int pd_env_add_block(void* ctx, int64_t env, const pd_block_desc* b, int64_t* block);
+int pd_env_add_contact(void* ctx, int64_t env, const pd_contact_desc* c);
+int pd_env_add_bc(void* ctx, int64_t env, const pd_bc_desc* bc);
+/* Uploads the env, builds every block's family on the GPU (cell list -> CSR),
+ * the weighted volume m and the per-particle BC map.  PD_E_OVERFLOW where
+ * cal_family/GeneralMaterial would raise BoundsError. */
+int pd_env_commit(void* ctx, int64_t env);
+int pd_env_destroy(void* ctx, int64_t env);
+int pd_env_info(void* ctx, int64_t env, int64_t* nparticles, int64_t* nbonds, int64_t* device_bytes);
+
+/* family in the reference ELL layout (general_material.jl:88-90): rows = max degree + 1 */
+int pd_block_family_rows(void* ctx, int64_t env, int64_t block, int64_t* rows);
+int pd_block_family_export(void* ctx, int64_t env, int64_t block, int64_t* family, uint8_t* intact, int64_t rows);
+int pd_block_weighted_volume(void* ctx, int64_t env, int64_t block, double* m);
+int pd_block_theta(void* ctx, int64_t env, int64_t block, double* theta);
+int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t rows);
+
+/* ---- stepping ----------------------------------------------------------- */
+/* run! physics cadence (solvers.jl:97-187) for the listed envs, stepped
+ * together: at an env's first call update_neighs! + apply_bc_at0; then nsteps
+ * velocity_verlet_step! (velocity_verlet.jl:57-89) with contact lists rebuilt
+ * after every step i with i % nuf == 0.  Asynchronous on the context stream. */
+int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf);
+/* update_acc! (solvers.jl:320-336) at the current positions; y may be NULL
+ * (resident state) or 3 x N positions to upload first.  f receives 3 x N. */
+int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f);
+/* force_density_T!(f, y, limits, mat, Val{:hip}) for one block: adds the block's
+ * material force density into f[:, lo:hi] (1-based inclusive limits, solvers.jl:236-237). */
+int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, double* f, int64_t lo, int64_t hi);
+/* update_contact_neighs! for every contact model of the env at the resident positions */
+int pd_env_update_neighs(void* ctx, int64_t env);
+
+/* ---- state transfer ----------------------------------------------------- */
+int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f);
+int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, const double* f);
+int pd_env_damage(void* ctx, int64_t env, double* damage);        /* cal_damage, peri.jl:587-594 */
+int pd_env_time_step(void* ctx, int64_t env, int64_t* time_step);
+int pd_env_check_finite(void* ctx, int64_t env);                  /* @check_nan, const_macros.jl:60-74 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PERIDYN_H */

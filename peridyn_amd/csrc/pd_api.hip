@@ -1,0 +1,1086 @@
+// C ABI of libperidyn_hip.so (declared in include/peridyn.h): contexts, env
+// upload + GPU family build, the resident run loop and state transfer.
+//
+// Each env owns its device arrays (layout in pd_internal.h).  All envs of a
+// context share one device table EnvDev[slot]; kernels index it through the
+// chunk descriptors, so any set of envs is stepped together by one launch per
+// kernel class (the batched / served form) and a single env is a set of one.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pd_internal.h"
+#include "pd_kernels.h"
+
+namespace {
+
+struct HostBlock {
+  pd_block_desc d{};
+  std::vector<double> x, y, v, vol;
+  std::vector<int64_t> type;
+  std::vector<double> density, s0, c, K, G, psi;
+  int32_t first = 0, chunk0 = 0, nchunks = 0, maxdeg = 0;
+};
+
+struct HostContact {
+  pd_contact_desc d{};
+  std::vector<int64_t> t1, t2;
+};
+
+struct HostBc {
+  pd_bc_desc d{};
+  std::vector<uint8_t> mask;
+};
+
+struct Env {
+  int64_t id = 0;
+  int32_t slot = -1;
+  double dt = 0.0;
+  int32_t precision = PD_PREC_FP64;
+  std::vector<HostBlock> blocks;
+  std::vector<HostContact> contacts;
+  std::vector<HostBc> bcs;
+  bool committed = false;
+  bool maybe_undeformed = true;
+  bool has_ep = false;
+  int64_t N = 0, nslots = 0, nbonds = 0;
+  int64_t time_step = 0;
+  int32_t cur = 0, edp_cur = 0;
+  std::vector<Chunk> chunks;
+  Chunk* d_chunks = nullptr;
+  int32_t* d_self = nullptr;  // this env's slot, for single-env bookkeeping launches
+  EnvDev dev{};
+  std::vector<ContactDev> cdev;
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<int64_t, Env*> envs;
+  int64_t next_id = 1;
+  EnvDev* d_envs = nullptr;
+  int32_t slot_cap = 0;
+  std::vector<int32_t> free_slots;
+  // cached run set
+  std::vector<int64_t> set_ids;
+  bool set_valid = false;
+  Chunk* d_set = nullptr;
+  size_t set_cap = 0;
+  int n_all = 0, n_theta = 0, n_td = 0, n_force[3] = {0, 0, 0};
+  int off_theta = 0, off_td = 0, off_force[3] = {0, 0, 0};
+  int32_t* d_slots = nullptr;
+  int n_slots = 0;
+  size_t slots_cap = 0;
+  bool set_has_ep = false, set_has_theta = false;
+  int2* d_latch = nullptr;
+  size_t latch_cap = 0;
+};
+
+template <typename T>
+T* dalloc(Env& e, size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  hipError_t r = hipMalloc(&p, n * sizeof(T));
+  if (r != hipSuccess) {
+    (void)hipGetLastError();
+    throw PdError(r == hipErrorOutOfMemory ? PD_E_NOMEM : PD_E_HIP,
+                  std::string("hipMalloc: ") + hipGetErrorString(r));
+  }
+  e.allocs.push_back(p);
+  e.bytes += n * sizeof(T);
+  return static_cast<T*>(p);
+}
+
+void free_env_device(Env& e) {
+  for (void* p : e.allocs) (void)hipFree(p);
+  e.allocs.clear();
+  e.bytes = 0;
+}
+
+Ctx* C(void* c) { return static_cast<Ctx*>(c); }
+
+Env& get_env(Ctx* c, int64_t id) {
+  auto it = c->envs.find(id);
+  if (it == c->envs.end()) throw PdError(PD_E_ARG, "unknown env id " + std::to_string(id));
+  return *it->second;
+}
+
+void check_sym(const std::vector<double>& m, int nt, const char* name) {
+  for (int a = 0; a < nt; ++a)
+    for (int b = 0; b < nt; ++b)
+      if (!(m[a + nt * b] == m[b + nt * a]) && !(std::isnan(m[a + nt * b]) && std::isnan(m[b + nt * a])))
+        throw PdError(PD_E_ARG, std::string(name) + " matrix must be symmetric (make_matrix)");
+}
+
+std::vector<double> copy_mat(const double* p, int nt, const char* name, bool required) {
+  std::vector<double> out((size_t)nt * nt, 0.0);
+  if (!p) {
+    if (required) throw PdError(PD_E_ARG, std::string(name) + " is required");
+    return out;
+  }
+  std::memcpy(out.data(), p, sizeof(double) * nt * nt);
+  check_sym(out, nt, name);
+  return out;
+}
+
+// ---- family build ---------------------------------------------------------
+struct FamTmp {
+  FamilyGrid g{};
+  int32_t *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cursor = nullptr, *items = nullptr;
+  std::vector<void*> mem;
+};
+
+void fam_free(FamTmp& t) {
+  for (void* p : t.mem) (void)hipFree(p);
+  t.mem.clear();
+}
+
+template <typename T>
+T* talloc(FamTmp& t, size_t n) {
+  void* p = nullptr;
+  hipError_t r = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+  if (r != hipSuccess) {
+    (void)hipGetLastError();
+    throw PdError(PD_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(r));
+  }
+  t.mem.push_back(p);
+  return static_cast<T*>(p);
+}
+
+void build_cells(Ctx* c, Env& e, HostBlock& b, FamTmp& t) {
+  const int n = (int)b.d.n;
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) { lo[d] = b.x[d]; hi[d] = b.x[d]; }
+  for (int i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], b.x[3 * i + d]);
+      hi[d] = std::max(hi[d], b.x[3 * i + d]);
+    }
+  double w = b.d.horizon * (1.0 + 9.313225746154785e-10);
+  if (!(w > 0.0) || !std::isfinite(w)) throw PdError(PD_E_ARG, "horizon must be positive and finite");
+  int64_t tot = 0;
+  int nc[3];
+  for (int it = 0; it < 200; ++it) {
+    tot = 1;
+    for (int d = 0; d < 3; ++d) {
+      nc[d] = (int)std::min(std::floor((hi[d] - lo[d]) / w) + 1.0, 1.0e6);
+      tot *= nc[d];
+    }
+    if (tot <= 8LL * n + 64) break;
+    w *= 1.25;
+  }
+  for (int d = 0; d < 3; ++d) { t.g.lo[d] = lo[d]; t.g.nc[d] = nc[d]; }
+  t.g.w = w;
+  t.cell_of = talloc<int32_t>(t, n);
+  t.cell_count = talloc<int32_t>(t, tot);
+  t.cell_start = talloc<int32_t>(t, tot);
+  t.cursor = talloc<int32_t>(t, tot);
+  t.items = talloc<int32_t>(t, n);
+  PD_HIP(hipMemsetAsync(t.cell_count, 0, sizeof(int32_t) * tot, c->stream));
+  PD_HIP(hipMemsetAsync(t.cursor, 0, sizeof(int32_t) * tot, c->stream));
+  pd_launch_cell_of(c->stream, e.dev.x, b.first, n, t.g, t.cell_of, t.cell_count);
+  std::vector<int32_t> cnt(tot), start(tot);
+  PD_HIP(hipMemcpyAsync(cnt.data(), t.cell_count, sizeof(int32_t) * tot, hipMemcpyDeviceToHost, c->stream));
+  PD_HIP(hipStreamSynchronize(c->stream));
+  int64_t run = 0;
+  for (int64_t i = 0; i < tot; ++i) { start[i] = (int32_t)run; run += cnt[i]; }
+  PD_HIP(hipMemcpyAsync(t.cell_start, start.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, c->stream));
+  pd_launch_cell_fill(c->stream, n, t.cell_of, t.cell_start, t.cursor, t.items, b.first);
+  PD_HIP(hipStreamSynchronize(c->stream));  // `start` goes out of scope
+}
+
+void upload_env_table(Ctx* c, Env& e) {
+  PD_HIP(hipMemcpyAsync(c->d_envs + e.slot, &e.dev, sizeof(EnvDev), hipMemcpyHostToDevice, c->stream));
+}
+
+void ensure_slot(Ctx* c, Env& e) {
+  if (e.slot >= 0) return;
+  if (c->free_slots.empty()) {
+    const int32_t ncap = std::max(64, c->slot_cap * 2);
+    EnvDev* nt = nullptr;
+    PD_HIP(hipMalloc(&nt, sizeof(EnvDev) * ncap));
+    if (c->d_envs) {
+      PD_HIP(hipMemcpyAsync(nt, c->d_envs, sizeof(EnvDev) * c->slot_cap, hipMemcpyDeviceToDevice, c->stream));
+      PD_HIP(hipStreamSynchronize(c->stream));
+      PD_HIP(hipFree(c->d_envs));
+    }
+    for (int32_t s = ncap - 1; s >= c->slot_cap; --s) c->free_slots.push_back(s);
+    c->d_envs = nt;
+    c->slot_cap = ncap;
+  }
+  e.slot = c->free_slots.back();
+  c->free_slots.pop_back();
+}
+
+void commit_env(Ctx* c, Env& e) {
+  if (e.committed) throw PdError(PD_E_STATE, "env already committed");
+  if (e.blocks.empty()) throw PdError(PD_E_ARG, "env has no material blocks");
+  int64_t N = 0;
+  for (auto& b : e.blocks) { b.first = (int32_t)N; N += b.d.n; }
+  if (N >= (1LL << 31) - 64) throw PdError(PD_E_ARG, "env too large (particles)");
+  e.N = N;
+  // ---- particle arrays (host staging)
+  std::vector<double4> x4(N), y4(N), v4(N);
+  std::vector<int32_t> pblock(N), bcid(N, -1);
+  std::vector<uint8_t> ptype(N);
+  std::vector<double> inv_den(N);
+  for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
+    HostBlock& b = e.blocks[bi];
+    for (int64_t i = 0; i < b.d.n; ++i) {
+      const int64_t p = b.first + i;
+      x4[p] = make_double4(b.x[3 * i], b.x[3 * i + 1], b.x[3 * i + 2], b.vol[i]);
+      y4[p] = make_double4(b.y[3 * i], b.y[3 * i + 1], b.y[3 * i + 2], 0.0);
+      v4[p] = make_double4(b.v[3 * i], b.v[3 * i + 1], b.v[3 * i + 2], 0.0);
+      pblock[p] = (int32_t)bi;
+      const int64_t t = b.type[i] - b.d.type_start;
+      ptype[p] = (uint8_t)t;
+      // Env: mass = volume * density[t]; density = mass ./ volume (environment.jl:173-214)
+      const double mass = b.vol[i] * b.density[t];
+      const double den = mass / b.vol[i];
+      inv_den[p] = 1.0 / den;
+    }
+  }
+  for (size_t k = 0; k < e.bcs.size(); ++k) {
+    if ((int64_t)e.bcs[k].mask.size() != N) throw PdError(PD_E_ARG, "BC mask length must equal env particles");
+    for (int64_t p = 0; p < N; ++p)
+      if (e.bcs[k].mask[p]) bcid[p] = (int32_t)k;
+  }
+  EnvDev& D = e.dev;
+  std::memset(&D, 0, sizeof(D));
+  D.N = (int32_t)N;
+  D.nblocks = (int32_t)e.blocks.size();
+  D.ncontacts = (int32_t)e.contacts.size();
+  D.nbc = (int32_t)e.bcs.size();
+  D.precision = e.precision;
+  D.dt = e.dt;
+  D.x = dalloc<double4>(e, N);
+  D.y[0] = dalloc<double4>(e, N);
+  D.y[1] = dalloc<double4>(e, N);
+  D.v = dalloc<double4>(e, N);
+  D.f = dalloc<double4>(e, N);
+  D.pblock = dalloc<int32_t>(e, N);
+  D.ptype = dalloc<uint8_t>(e, N);
+  D.inv_den = dalloc<double>(e, N);
+  D.bcid = dalloc<int32_t>(e, N);
+  D.deg = dalloc<int32_t>(e, N);
+  D.pslot0 = dalloc<uint32_t>(e, N);
+  D.m = dalloc<double>(e, N);
+  D.theta = dalloc<double>(e, N);
+  D.td = dalloc<double>(e, N);
+  hipStream_t s = c->stream;
+  PD_HIP(hipMemcpyAsync(D.x, x4.data(), sizeof(double4) * N, hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemcpyAsync(D.y[0], y4.data(), sizeof(double4) * N, hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemcpyAsync(D.y[1], y4.data(), sizeof(double4) * N, hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemcpyAsync(D.v, v4.data(), sizeof(double4) * N, hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemsetAsync(D.f, 0, sizeof(double4) * N, s));
+  PD_HIP(hipMemcpyAsync(D.pblock, pblock.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemcpyAsync(D.ptype, ptype.data(), N, hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemcpyAsync(D.inv_den, inv_den.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemcpyAsync(D.bcid, bcid.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemsetAsync(D.theta, 0, sizeof(double) * N, s));
+  PD_HIP(hipMemsetAsync(D.td, 0, sizeof(double) * N, s));
+  PD_HIP(hipMemsetAsync(D.f, 0, sizeof(double4) * N, s));
+
+  // ---- family: degrees
+  std::vector<FamTmp> tmp(e.blocks.size());
+  std::vector<int32_t> deg(N);
+  try {
+    for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
+      HostBlock& b = e.blocks[bi];
+      build_cells(c, e, b, tmp[bi]);
+      FamTmp& t = tmp[bi];
+      pd_launch_degree(s, D.x, b.first, (int)b.d.n, t.g, b.d.horizon, t.cell_of, t.cell_start, t.cell_count,
+                       t.items, D.deg);
+    }
+    PD_HIP(hipMemcpyAsync(deg.data(), D.deg, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    // ---- SELL-64 layout
+    std::vector<uint32_t> pslot0(N);
+    uint64_t slots = 0;
+    e.chunks.clear();
+    e.nbonds = 0;
+    for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
+      HostBlock& b = e.blocks[bi];
+      int32_t maxdeg = 0;
+      for (int64_t i = 0; i < b.d.n; ++i) maxdeg = std::max(maxdeg, deg[b.first + i]);
+      b.maxdeg = maxdeg;
+      // cal_family (peri.jl:571-576) keeps M = min(max_neigh, n) rows; filling
+      // beyond M or maxdeg == M (general_material.jl:88-89) is a BoundsError
+      const int64_t M = std::min<int64_t>(b.d.max_neigh, b.d.n);
+      if (maxdeg >= M && maxdeg > 0)
+        throw PdError(PD_E_OVERFLOW, "BoundsError: block " + std::to_string(bi) + " has a particle with " +
+                                         std::to_string(maxdeg) + " neighbours; max_neigh=" +
+                                         std::to_string(b.d.max_neigh) + " must exceed the maximum degree");
+      b.chunk0 = (int32_t)e.chunks.size();
+      for (int64_t c0 = 0; c0 < b.d.n; c0 += PD_WAVE) {
+        Chunk ch{};
+        ch.env = 0;
+        ch.block = (int32_t)bi;
+        ch.first = (int32_t)(b.first + c0);
+        ch.count = (int32_t)std::min<int64_t>(PD_WAVE, b.d.n - c0);
+        int32_t wdt = 0;
+        for (int l = 0; l < ch.count; ++l) wdt = std::max(wdt, deg[ch.first + l]);
+        ch.width = wdt;
+        ch.slot0 = (uint32_t)slots;
+        for (int l = 0; l < ch.count; ++l) {
+          pslot0[ch.first + l] = (uint32_t)(slots + l);
+          e.nbonds += deg[ch.first + l];
+        }
+        slots += (uint64_t)PD_WAVE * wdt;
+        if (slots >= 0xfffffff0ULL) throw PdError(PD_E_ARG, "env too large (bond slots exceed 2^32)");
+        e.chunks.push_back(ch);
+      }
+      b.nchunks = (int32_t)e.chunks.size() - b.chunk0;
+      if (b.d.kind == PD_EP) e.has_ep = true;
+    }
+    e.nslots = (int64_t)slots;
+    PD_HIP(hipMemcpyAsync(D.pslot0, pslot0.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s));
+    D.col = dalloc<int32_t>(e, slots);
+    D.state = dalloc<uint8_t>(e, slots);
+    PD_HIP(hipMemsetAsync(D.col, 0xff, sizeof(int32_t) * std::max<uint64_t>(slots, 1), s));
+    PD_HIP(hipMemsetAsync(D.state, 0, std::max<uint64_t>(slots, 1), s));
+    for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
+      HostBlock& b = e.blocks[bi];
+      FamTmp& t = tmp[bi];
+      pd_launch_fill(s, D.x, b.first, (int)b.d.n, t.g, b.d.horizon, t.cell_of, t.cell_start, t.cell_count,
+                     t.items, D.pslot0, D.deg, D.col, D.state);
+      pd_launch_wvol(s, D.x, b.first, (int)b.d.n, D.pslot0, D.deg, D.col, D.m);
+    }
+    if (e.has_ep) {
+      D.rev = dalloc<uint32_t>(e, slots);
+      D.edp[0] = dalloc<double>(e, slots);
+      D.edp[1] = dalloc<double>(e, slots);
+      PD_HIP(hipMemsetAsync(D.edp[0], 0, sizeof(double) * std::max<uint64_t>(slots, 1), s));
+      PD_HIP(hipMemsetAsync(D.edp[1], 0, sizeof(double) * std::max<uint64_t>(slots, 1), s));
+      for (auto& b : e.blocks)
+        pd_launch_rev(s, b.first, (int)b.d.n, D.pslot0, D.deg, D.col, D.rev);
+    }
+    PD_HIP(hipStreamSynchronize(s));
+    for (auto& t : tmp) fam_free(t);
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    for (auto& t : tmp) fam_free(t);
+    throw;
+  }
+  D.has_ep = e.has_ep ? 1 : 0;
+
+  // ---- blocks
+  std::vector<BlockDev> bd(e.blocks.size());
+  bool undeformed = false;
+  for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
+    HostBlock& b = e.blocks[bi];
+    BlockDev& B = bd[bi];
+    std::memset(&B, 0, sizeof(B));
+    const int nt = b.d.ntype;
+    B.kind = b.d.kind;
+    B.ntype = nt;
+    B.first = b.first;
+    B.n = (int32_t)b.d.n;
+    B.chunk0 = b.chunk0;
+    B.nchunks = b.nchunks;
+    B.deformed = b.d.deformed ? 1 : 0;
+    if (!B.deformed) undeformed = true;
+    B.c11 = (b.d.kind == PD_BB) ? b.c[0] : 0.0;
+    B.horizon = b.d.horizon;
+    for (int t = 0; t < nt; ++t) B.density[t] = b.density[t];
+    for (int q = 0; q < nt * nt; ++q) {
+      B.s0[q] = b.s0[q];
+      B.K[q] = b.K[q];
+      B.G[q] = b.G[q];
+      B.psi[q] = b.psi[q];
+    }
+  }
+  e.maybe_undeformed = undeformed;
+  D.blocks = dalloc<BlockDev>(e, bd.size());
+  PD_HIP(hipMemcpyAsync(D.blocks, bd.data(), sizeof(BlockDev) * bd.size(), hipMemcpyHostToDevice, s));
+
+  // ---- BCs
+  std::vector<BcDev> bcd(e.bcs.size());
+  for (size_t k = 0; k < e.bcs.size(); ++k) {
+    std::memset(&bcd[k], 0, sizeof(BcDev));
+    bcd[k].kind = e.bcs[k].d.kind;
+    for (int d = 0; d < 3; ++d) bcd[k].rate[d] = e.bcs[k].d.rate[d];
+    bcd[k].freq = e.bcs[k].d.freq;
+    bcd[k].applyafter = e.bcs[k].d.applyafter;
+  }
+  D.bcs = dalloc<BcDev>(e, std::max<size_t>(bcd.size(), 1));
+  if (!bcd.empty())
+    PD_HIP(hipMemcpyAsync(D.bcs, bcd.data(), sizeof(BcDev) * bcd.size(), hipMemcpyHostToDevice, s));
+
+  // ---- contacts
+  e.cdev.assign(e.contacts.size(), ContactDev{});
+  std::vector<std::vector<int32_t>> keep;  // staging kept alive until sync
+  for (size_t k = 0; k < e.contacts.size(); ++k) {
+    HostContact& hc = e.contacts[k];
+    ContactDev& cd = e.cdev[k];
+    std::memset(&cd, 0, sizeof(cd));
+    std::vector<int32_t> idx1, idx2, inv1(N, -1), inv2(N, -1);
+    std::vector<int64_t> types(N);
+    for (auto& b : e.blocks)
+      for (int64_t i = 0; i < b.d.n; ++i) types[b.first + i] = b.type[i];
+    for (int64_t p = 0; p < N; ++p) {
+      const bool in1 = std::find(hc.t1.begin(), hc.t1.end(), types[p]) != hc.t1.end();
+      const bool in2 = std::find(hc.t2.begin(), hc.t2.end(), types[p]) != hc.t2.end();
+      if (in1 && in2) throw PdError(PD_E_ARG, "contact pair type sets must be disjoint");
+      if (in1) { inv1[p] = (int32_t)idx1.size(); idx1.push_back((int32_t)p); }
+      if (in2) { inv2[p] = (int32_t)idx2.size(); idx2.push_back((int32_t)p); }
+    }
+    cd.n1 = (int32_t)idx1.size();
+    cd.n2 = (int32_t)idx2.size();
+    cd.max_neighs = (int32_t)std::max<int64_t>(1, std::min<int64_t>(hc.d.max_neighs, cd.n2));
+    cd.expo_is_int = hc.d.exponent_is_int;
+    cd.expo = hc.d.exponent;
+    cd.K = hc.d.stiffness;
+    cd.equi = hc.d.equi_dist;
+    cd.distance = hc.d.distance;
+    cd.ncell_cap = 2 * cd.n2 + 64;
+    cd.rev_cap = cd.n1 * cd.max_neighs;
+    int32_t* didx1 = dalloc<int32_t>(e, cd.n1);
+    int32_t* didx2 = dalloc<int32_t>(e, cd.n2);
+    int32_t* dinv1 = dalloc<int32_t>(e, N);
+    int32_t* dinv2 = dalloc<int32_t>(e, N);
+    cd.idx1 = didx1; cd.idx2 = didx2; cd.inv1 = dinv1; cd.inv2 = dinv2;
+    cd.own_cnt = dalloc<int32_t>(e, cd.n1);
+    cd.own = dalloc<int32_t>(e, (size_t)cd.n1 * cd.max_neighs);
+    cd.rev_cnt = dalloc<int32_t>(e, cd.n2);
+    cd.rev_off = dalloc<int32_t>(e, cd.n2);
+    cd.rev = dalloc<int32_t>(e, (size_t)cd.rev_cap);
+    cd.rev_cursor = dalloc<int32_t>(e, cd.n2);
+    cd.cell_start = dalloc<int32_t>(e, 2 * (size_t)cd.ncell_cap);
+    cd.cell_count = dalloc<int32_t>(e, cd.ncell_cap);
+    cd.cell_items = dalloc<int32_t>(e, cd.n2);
+    cd.cell_of = dalloc<int32_t>(e, cd.n2);
+    cd.box = dalloc<double>(e, 16 + 128 * 12);
+    PD_HIP(hipMemcpyAsync(didx1, idx1.data(), sizeof(int32_t) * idx1.size(), hipMemcpyHostToDevice, s));
+    PD_HIP(hipMemcpyAsync(didx2, idx2.data(), sizeof(int32_t) * idx2.size(), hipMemcpyHostToDevice, s));
+    PD_HIP(hipMemcpyAsync(dinv1, inv1.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+    PD_HIP(hipMemcpyAsync(dinv2, inv2.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+    PD_HIP(hipMemsetAsync(cd.own_cnt, 0, sizeof(int32_t) * std::max(cd.n1, 1), s));
+    PD_HIP(hipMemsetAsync(cd.rev_cnt, 0, sizeof(int32_t) * std::max(cd.n2, 1), s));
+    PD_HIP(hipMemsetAsync(cd.rev_off, 0, sizeof(int32_t) * std::max(cd.n2, 1), s));
+    PD_HIP(hipStreamSynchronize(s));
+  }
+  if (!e.cdev.empty()) {
+    D.contacts = dalloc<ContactDev>(e, e.cdev.size());
+    PD_HIP(hipMemcpyAsync(D.contacts, e.cdev.data(), sizeof(ContactDev) * e.cdev.size(), hipMemcpyHostToDevice, s));
+  }
+  ensure_slot(c, e);
+  for (auto& ch : e.chunks) ch.env = e.slot;
+  e.d_self = dalloc<int32_t>(e, 1);
+  PD_HIP(hipMemcpyAsync(e.d_self, &e.slot, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  e.d_chunks = dalloc<Chunk>(e, e.chunks.size());
+  PD_HIP(hipMemcpyAsync(e.d_chunks, e.chunks.data(), sizeof(Chunk) * e.chunks.size(), hipMemcpyHostToDevice, s));
+  upload_env_table(c, e);
+  PD_HIP(hipStreamSynchronize(s));
+  e.committed = true;
+  c->set_valid = false;
+}
+
+// ---- run sets ----------------------------------------------------------------
+template <typename T>
+void ensure_cap(T*& p, size_t& cap, size_t n) {
+  if (n <= cap && p) return;
+  if (p) PD_HIP(hipFree(p));
+  p = nullptr;
+  cap = std::max<size_t>(n, 16) * 2;
+  PD_HIP(hipMalloc(&p, sizeof(T) * cap));
+}
+
+void build_set(Ctx* c, const std::vector<int64_t>& ids) {
+  if (c->set_valid && ids == c->set_ids) return;
+  std::vector<Chunk> all, th, td, fk[3];
+  std::vector<int32_t> slots;
+  bool has_ep = false, has_th = false;
+  for (int64_t id : ids) {
+    Env& e = get_env(c, id);
+    if (!e.committed) throw PdError(PD_E_STATE, "env " + std::to_string(id) + " is not committed");
+    slots.push_back(e.slot);
+    for (const Chunk& ch : e.chunks) {
+      all.push_back(ch);
+      const int kind = e.blocks[ch.block].d.kind;
+      fk[kind].push_back(ch);
+      if (kind != PD_BB) { th.push_back(ch); has_th = true; }
+      if (kind == PD_EP) { td.push_back(ch); has_ep = true; }
+    }
+  }
+  std::vector<Chunk> cat;
+  cat.insert(cat.end(), all.begin(), all.end());
+  c->off_theta = (int)cat.size(); cat.insert(cat.end(), th.begin(), th.end());
+  c->off_td = (int)cat.size(); cat.insert(cat.end(), td.begin(), td.end());
+  for (int k = 0; k < 3; ++k) { c->off_force[k] = (int)cat.size(); cat.insert(cat.end(), fk[k].begin(), fk[k].end()); }
+  c->n_all = (int)all.size();
+  c->n_theta = (int)th.size();
+  c->n_td = (int)td.size();
+  for (int k = 0; k < 3; ++k) c->n_force[k] = (int)fk[k].size();
+  ensure_cap(c->d_set, c->set_cap, cat.size());
+  PD_HIP(hipMemcpyAsync(c->d_set, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
+  ensure_cap(c->d_slots, c->slots_cap, slots.size());
+  PD_HIP(hipMemcpyAsync(c->d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, c->stream));
+  PD_HIP(hipStreamSynchronize(c->stream));
+  c->n_slots = (int)slots.size();
+  c->set_has_ep = has_ep;
+  c->set_has_theta = has_th;
+  c->set_ids = ids;
+  c->set_valid = true;
+}
+
+int precision_of(Ctx* c, const std::vector<int64_t>& ids) {
+  int prec = -1;
+  for (int64_t id : ids) {
+    Env& e = get_env(c, id);
+    if (prec < 0) prec = e.precision;
+    else if (prec != e.precision) throw PdError(PD_E_ARG, "envs stepped together must share one precision");
+  }
+  return prec < 0 ? PD_PREC_FP64 : prec;
+}
+
+// deformed-latch work list for envs that may still hold undeformed blocks
+int prepare_latch(Ctx* c, const std::vector<int64_t>& ids) {
+  std::vector<int2> items;
+  for (int64_t id : ids) {
+    Env& e = get_env(c, id);
+    if (!e.maybe_undeformed) continue;
+    for (size_t b = 0; b < e.blocks.size(); ++b) items.push_back(make_int2(e.slot, (int)b));
+  }
+  if (items.empty()) return 0;
+  ensure_cap(c->d_latch, c->latch_cap, items.size());
+  PD_HIP(hipMemcpyAsync(c->d_latch, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, c->stream));
+  PD_HIP(hipStreamSynchronize(c->stream));
+  return (int)items.size();
+}
+
+void refresh_latch(Ctx* c, const std::vector<int64_t>& ids) {
+  for (int64_t id : ids) {
+    Env& e = get_env(c, id);
+    if (!e.maybe_undeformed) continue;
+    std::vector<BlockDev> bd(e.blocks.size());
+    PD_HIP(hipMemcpyAsync(bd.data(), e.dev.blocks, sizeof(BlockDev) * bd.size(), hipMemcpyDeviceToHost, c->stream));
+    PD_HIP(hipStreamSynchronize(c->stream));
+    bool any = false;
+    for (auto& B : bd) any = any || !B.deformed;
+    e.maybe_undeformed = any;
+  }
+}
+
+void contact_update(Ctx* c, Env& e) {
+  for (size_t k = 0; k < e.cdev.size(); ++k)
+    pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k);
+}
+
+// one force evaluation (update_acc!) for a chunk subset of one env
+void force_pass(Ctx* c, Env& e, const std::vector<Chunk>& sub, int flags_extra) {
+  std::vector<Chunk> th, td, fk[3];
+  for (auto& ch : sub) {
+    const int kind = e.blocks[ch.block].d.kind;
+    fk[kind].push_back(ch);
+    if (kind != PD_BB) th.push_back(ch);
+    if (kind == PD_EP) td.push_back(ch);
+  }
+  std::vector<Chunk> cat;
+  cat.insert(cat.end(), th.begin(), th.end());
+  const int o_td = (int)cat.size();
+  cat.insert(cat.end(), td.begin(), td.end());
+  int o_f[3];
+  for (int k = 0; k < 3; ++k) { o_f[k] = (int)cat.size(); cat.insert(cat.end(), fk[k].begin(), fk[k].end()); }
+  Chunk* d = nullptr;
+  PD_HIP(hipMalloc(&d, sizeof(Chunk) * std::max<size_t>(cat.size(), 1)));
+  try {
+    PD_HIP(hipMemcpyAsync(d, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
+    std::vector<int64_t> one{e.id};
+    const int nl = prepare_latch(c, one);
+    if (nl) pd_launch_latch(c->stream, c->d_envs, c->d_latch, nl);
+    pd_launch_theta(c->stream, e.precision, d, (int)th.size(), c->d_envs);
+    pd_launch_td(c->stream, e.precision, d + o_td, (int)td.size(), c->d_envs);
+    for (int k = 0; k < 3; ++k)
+      pd_launch_force(c->stream, k, e.precision, d + o_f[k], (int)fk[k].size(), c->d_envs, flags_extra);
+    if (e.has_ep && !td.empty()) {
+      // the force pass wrote the updated plastic strain into the other buffer
+      pd_launch_book(c->stream, c->d_envs, e.d_self, 1, PD_B_FLIP_EDP);
+      e.edp_cur ^= 1;
+    }
+    PD_HIP(hipGetLastError());
+    PD_HIP(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    throw;
+  }
+  PD_HIP(hipFree(d));
+  refresh_latch(c, {e.id});
+}
+
+void download_xyz(Ctx* c, const double4* src, int64_t N, double* out) {
+  std::vector<double4> h(N);
+  PD_HIP(hipMemcpyAsync(h.data(), src, sizeof(double4) * N, hipMemcpyDeviceToHost, c->stream));
+  PD_HIP(hipStreamSynchronize(c->stream));
+  for (int64_t p = 0; p < N; ++p) { out[3 * p] = h[p].x; out[3 * p + 1] = h[p].y; out[3 * p + 2] = h[p].z; }
+}
+
+void upload_xyz(Ctx* c, double4* dst, int64_t N, const double* in, bool keep_w) {
+  std::vector<double4> h(N);
+  if (keep_w) {
+    PD_HIP(hipMemcpyAsync(h.data(), dst, sizeof(double4) * N, hipMemcpyDeviceToHost, c->stream));
+    PD_HIP(hipStreamSynchronize(c->stream));
+  }
+  for (int64_t p = 0; p < N; ++p) {
+    h[p].x = in[3 * p]; h[p].y = in[3 * p + 1]; h[p].z = in[3 * p + 2];
+    if (!keep_w) h[p].w = 0.0;
+  }
+  PD_HIP(hipMemcpyAsync(dst, h.data(), sizeof(double4) * N, hipMemcpyHostToDevice, c->stream));
+  PD_HIP(hipStreamSynchronize(c->stream));
+}
+
+struct BondsHost {
+  std::vector<int32_t> col, deg;
+  std::vector<uint8_t> state;
+  std::vector<uint32_t> pslot0;
+};
+
+void download_bonds(Ctx* c, Env& e, BondsHost& h) {
+  h.col.resize(std::max<int64_t>(e.nslots, 1));
+  h.state.resize(std::max<int64_t>(e.nslots, 1));
+  h.deg.resize(e.N);
+  h.pslot0.resize(e.N);
+  PD_HIP(hipMemcpyAsync(h.col.data(), e.dev.col, sizeof(int32_t) * e.nslots, hipMemcpyDeviceToHost, c->stream));
+  PD_HIP(hipMemcpyAsync(h.state.data(), e.dev.state, e.nslots, hipMemcpyDeviceToHost, c->stream));
+  PD_HIP(hipMemcpyAsync(h.deg.data(), e.dev.deg, sizeof(int32_t) * e.N, hipMemcpyDeviceToHost, c->stream));
+  PD_HIP(hipMemcpyAsync(h.pslot0.data(), e.dev.pslot0, sizeof(uint32_t) * e.N, hipMemcpyDeviceToHost, c->stream));
+  PD_HIP(hipStreamSynchronize(c->stream));
+}
+
+HostBlock& get_block(Env& e, int64_t b) {
+  if (b < 0 || b >= (int64_t)e.blocks.size()) throw PdError(PD_E_ARG, "block index out of range");
+  return e.blocks[b];
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+// =============================================================================
+extern "C" {
+
+static int run_guard(void* ctx, const std::function<void()>& f) {
+  try {
+    f();
+    return PD_OK;
+  } catch (const PdError& e) {
+    if (ctx) C(ctx)->err = e.msg; else g_err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    if (ctx) C(ctx)->err = e.what(); else g_err = e.what();
+    return PD_E_ARG;
+  }
+}
+
+int pd_ctx_create(int device, void** out) {
+  return run_guard(nullptr, [&] {
+    if (!out) throw PdError(PD_E_ARG, "null output pointer");
+    int n = 0;
+    PD_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw PdError(PD_E_ARG, "no HIP device " + std::to_string(device));
+    PD_HIP(hipSetDevice(device));
+    Ctx* c = new Ctx();
+    c->device = device;
+    hipError_t r = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (r != hipSuccess) { delete c; PD_HIP(r); }
+    *out = c;
+  });
+}
+
+int pd_ctx_destroy(void* ctx) {
+  if (!ctx) return PD_E_ARG;
+  Ctx* c = C(ctx);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->envs) { free_env_device(*kv.second); delete kv.second; }
+  if (c->d_envs) (void)hipFree(c->d_envs);
+  if (c->d_set) (void)hipFree(c->d_set);
+  if (c->d_slots) (void)hipFree(c->d_slots);
+  if (c->d_latch) (void)hipFree(c->d_latch);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PD_OK;
+}
+
+const char* pd_last_error(void* ctx) { return ctx ? C(ctx)->err.c_str() : g_err.c_str(); }
+
+int pd_sync(void* ctx) {
+  return run_guard(ctx, [&] { PD_HIP(hipStreamSynchronize(C(ctx)->stream)); });
+}
+
+int pd_env_create(void* ctx, double dt, int32_t precision, int64_t* env) {
+  return run_guard(ctx, [&] {
+    if (!env) throw PdError(PD_E_ARG, "null output pointer");
+    if (!(std::isfinite(dt))) throw PdError(PD_E_ARG, "dt must be finite");
+    if (precision != PD_PREC_EXACT && precision != PD_PREC_FP64)
+      throw PdError(PD_E_ARG, "unsupported precision form " + std::to_string(precision));
+    Ctx* c = C(ctx);
+    Env* e = new Env();
+    e->id = c->next_id++;
+    e->dt = dt;
+    e->precision = precision;
+    c->envs[e->id] = e;
+    *env = e->id;
+  });
+}
+
+int pd_env_add_block(void* ctx, int64_t env, const pd_block_desc* d, int64_t* block) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (e.committed) throw PdError(PD_E_STATE, "env already committed");
+    if (!d || d->n <= 0 || !d->x || !d->volume || !d->type || !d->density || !d->critical_stretch)
+      throw PdError(PD_E_ARG, "block descriptor is missing required arrays");
+    if (d->kind < PD_BB || d->kind > PD_EP) throw PdError(PD_E_ARG, "unknown material kind");
+    if (d->ntype < 1 || d->ntype > PD_MAX_TYPES) throw PdError(PD_E_ARG, "ntype must be in 1..8");
+    HostBlock b;
+    b.d = *d;
+    const int64_t n = d->n;
+    const int nt = d->ntype;
+    b.x.assign(d->x, d->x + 3 * n);
+    b.y.assign(d->y ? d->y : d->x, (d->y ? d->y : d->x) + 3 * n);
+    if (d->v) b.v.assign(d->v, d->v + 3 * n); else b.v.assign(3 * n, 0.0);
+    b.vol.assign(d->volume, d->volume + n);
+    b.type.assign(d->type, d->type + n);
+    for (int64_t i = 0; i < n; ++i)
+      if (b.type[i] < d->type_start || b.type[i] >= d->type_start + nt)
+        throw PdError(PD_E_ARG, "particle type outside the block's type range");
+    b.density.assign(d->density, d->density + nt);
+    b.s0 = copy_mat(d->critical_stretch, nt, "critical_stretch", true);
+    if (d->kind == PD_BB) {
+      b.c = copy_mat(d->bond_stiffness, nt, "bond_stiffness", true);
+    } else {
+      b.K = copy_mat(d->bulk_modulus, nt, "bulk_modulus", true);
+      b.G = copy_mat(d->shear_modulus, nt, "shear_modulus", true);
+      if (d->kind == PD_EP) b.psi = copy_mat(d->psi, nt, "psi", true);
+    }
+    if (b.K.empty()) b.K.assign(nt * nt, 0.0);
+    if (b.G.empty()) b.G.assign(nt * nt, 0.0);
+    if (b.psi.empty()) b.psi.assign(nt * nt, 0.0);
+    if (b.c.empty()) b.c.assign(nt * nt, 0.0);
+    e.blocks.push_back(std::move(b));
+    if (block) *block = (int64_t)e.blocks.size() - 1;
+  });
+}
+
+int pd_env_add_contact(void* ctx, int64_t env, const pd_contact_desc* d) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (e.committed) throw PdError(PD_E_STATE, "env already committed");
+    if (!d || !d->types1 || !d->types2 || d->ntypes1 <= 0 || d->ntypes2 <= 0)
+      throw PdError(PD_E_ARG, "contact descriptor needs both type lists");
+    if ((int)e.contacts.size() >= PD_MAX_CONTACTS) throw PdError(PD_E_ARG, "too many contact models");
+    if (d->exponent_is_int && d->exponent != std::floor(d->exponent))
+      throw PdError(PD_E_ARG, "integer exponent must be integer-valued");
+    HostContact hc;
+    hc.d = *d;
+    hc.t1.assign(d->types1, d->types1 + d->ntypes1);
+    hc.t2.assign(d->types2, d->types2 + d->ntypes2);
+    e.contacts.push_back(std::move(hc));
+  });
+}
+
+int pd_env_add_bc(void* ctx, int64_t env, const pd_bc_desc* d) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (e.committed) throw PdError(PD_E_STATE, "env already committed");
+    if (!d || !d->mask) throw PdError(PD_E_ARG, "BC descriptor needs a mask");
+    if (d->kind != PD_BC_FIX && d->kind != PD_BC_TOFRO) throw PdError(PD_E_ARG, "unknown BC kind");
+    if (d->kind == PD_BC_TOFRO && !std::isinf(d->freq) && (d->freq != std::floor(d->freq) || d->freq <= 0))
+      throw PdError(PD_E_ARG, "ToFroBC freq must be a positive integer or Inf");
+    int64_t N = 0;
+    for (auto& b : e.blocks) N += b.d.n;
+    HostBc hb;
+    hb.d = *d;
+    hb.mask.assign(d->mask, d->mask + N);
+    e.bcs.push_back(std::move(hb));
+  });
+}
+
+int pd_env_commit(void* ctx, int64_t env) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    try {
+      commit_env(c, e);
+    } catch (...) {
+      free_env_device(e);
+      throw;
+    }
+  });
+}
+
+int pd_env_destroy(void* ctx, int64_t env) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    PD_HIP(hipStreamSynchronize(c->stream));
+    free_env_device(e);
+    if (e.slot >= 0) c->free_slots.push_back(e.slot);
+    c->envs.erase(env);
+    delete &e;
+    c->set_valid = false;
+  });
+}
+
+int pd_env_info(void* ctx, int64_t env, int64_t* np, int64_t* nb, int64_t* bytes) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (np) *np = e.N;
+    if (nb) *nb = e.nbonds;
+    if (bytes) *bytes = (int64_t)e.bytes;
+  });
+}
+
+int pd_block_family_rows(void* ctx, int64_t env, int64_t block, int64_t* rows) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    *rows = get_block(e, block).maxdeg + 1;
+  });
+}
+
+int pd_block_family_export(void* ctx, int64_t env, int64_t block, int64_t* family, uint8_t* intact, int64_t rows) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    HostBlock& b = get_block(e, block);
+    if (rows < b.maxdeg + 1) throw PdError(PD_E_ARG, "rows must be >= max degree + 1");
+    BondsHost h;
+    download_bonds(c, e, h);
+    for (int64_t i = 0; i < b.d.n; ++i) {
+      const int64_t p = b.first + i;
+      const int d = h.deg[p];
+      for (int64_t r = 0; r < rows; ++r) {
+        int64_t v = 0;
+        uint8_t s = 0;
+        if (r < d) {
+          const uint32_t slot = h.pslot0[p] + 64u * (uint32_t)(d - 1 - r);
+          v = (int64_t)h.col[slot] - b.first + 1;
+          s = h.state[slot];
+        }
+        if (family) family[i * rows + r] = v;
+        if (intact) intact[i * rows + r] = s;
+      }
+    }
+  });
+}
+
+static void download_block_vec(Ctx* c, Env& e, HostBlock& b, const double* src, double* out) {
+  PD_HIP(hipMemcpyAsync(out, src + b.first, sizeof(double) * b.d.n, hipMemcpyDeviceToHost, c->stream));
+  PD_HIP(hipStreamSynchronize(c->stream));
+}
+
+int pd_block_weighted_volume(void* ctx, int64_t env, int64_t block, double* m) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    download_block_vec(c, e, get_block(e, block), e.dev.m, m);
+  });
+}
+
+int pd_block_theta(void* ctx, int64_t env, int64_t block, double* theta) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    download_block_vec(c, e, get_block(e, block), e.dev.theta, theta);
+  });
+}
+
+int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t rows) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    HostBlock& b = get_block(e, block);
+    if (!e.has_ep) throw PdError(PD_E_STATE, "env has no elastoplastic block");
+    if (rows < b.maxdeg + 1) throw PdError(PD_E_ARG, "rows must be >= max degree + 1");
+    BondsHost h;
+    download_bonds(c, e, h);
+    std::vector<double> ed(std::max<int64_t>(e.nslots, 1));
+    PD_HIP(hipMemcpyAsync(ed.data(), e.dev.edp[e.edp_cur], sizeof(double) * e.nslots, hipMemcpyDeviceToHost, c->stream));
+    PD_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < b.d.n; ++i) {
+      const int64_t p = b.first + i;
+      const int d = h.deg[p];
+      for (int64_t r = 0; r < rows; ++r)
+        edp[i * rows + r] = (r < d) ? ed[h.pslot0[p] + 64u * (uint32_t)(d - 1 - r)] : 0.0;
+    }
+  });
+}
+
+int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    if (nenv <= 0 || !envs) throw PdError(PD_E_ARG, "empty env list");
+    if (nsteps < 0) throw PdError(PD_E_ARG, "nsteps must be >= 0");
+    if (nuf < 1) throw PdError(PD_E_ARG, "neigh_update_freq must be >= 1");
+    PD_HIP(hipSetDevice(c->device));
+    std::vector<int64_t> ids(envs, envs + nenv);
+    const int prec = precision_of(c, ids);
+    build_set(c, ids);
+    hipStream_t s = c->stream;
+    // run! start: update_neighs! (solvers.jl:114); apply_bc_at0 records
+    // BC.last = y, which the resident state already holds
+    for (int64_t id : ids) {
+      Env& e = get_env(c, id);
+      if (!e.contacts.empty()) contact_update(c, e);
+    }
+    if (nsteps == 0) return;
+    const int nl = prepare_latch(c, ids);
+    EnvDev* T = c->d_envs;
+    Chunk* d = c->d_set;
+    pd_launch_prologue(s, d, c->n_all, T);
+    pd_launch_book(s, T, c->d_slots, c->n_slots, PD_B_FLIP_Y);
+    for (int64_t st = 1; st <= nsteps; ++st) {
+      const bool last = (st == nsteps);
+      if (nl) pd_launch_latch(s, T, c->d_latch, nl);
+      if (c->set_has_theta) pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T);
+      if (c->set_has_ep) pd_launch_td(s, prec, d + c->off_td, c->n_td, T);
+      const int fl = PD_F_KICK | (last ? 0 : PD_F_DRIFT);
+      for (int k = 0; k < 3; ++k)
+        if (c->n_force[k]) pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl);
+      for (int64_t id : ids) {
+        Env& e = get_env(c, id);
+        if (!e.contacts.empty() && ((e.time_step + st) % nuf) == 0) contact_update(c, e);
+      }
+      pd_launch_book(s, T, c->d_slots, c->n_slots,
+                     PD_B_STEP | (last ? 0 : PD_B_FLIP_Y) | (c->set_has_ep ? PD_B_FLIP_EDP : 0));
+    }
+    PD_HIP(hipGetLastError());
+    for (int64_t id : ids) {
+      Env& e = get_env(c, id);
+      e.time_step += nsteps;
+      e.cur ^= (int32_t)(nsteps & 1);
+      if (e.has_ep) e.edp_cur ^= (int32_t)(nsteps & 1);
+    }
+    if (nl) refresh_latch(c, ids);
+  });
+}
+
+int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    if (y) upload_xyz(c, e.dev.y[e.cur], e.N, y, false);
+    force_pass(c, e, e.chunks, 0);
+    if (f) download_xyz(c, e.dev.f, e.N, f);
+  });
+}
+
+int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, double* f, int64_t lo, int64_t hi) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    HostBlock& b = get_block(e, block);
+    if (lo != b.first + 1 || hi != b.first + b.d.n) throw PdError(PD_E_ARG, "limits do not match the block");
+    if (y) upload_xyz(c, e.dev.y[e.cur], e.N, y, false);
+    std::vector<Chunk> sub(e.chunks.begin() + b.chunk0, e.chunks.begin() + b.chunk0 + b.nchunks);
+    // contacts are not part of force_density_T!: zero them out of this pass
+    EnvDev saved = e.dev;
+    e.dev.ncontacts = 0;
+    upload_env_table(c, e);
+    try {
+      force_pass(c, e, sub, 0);
+    } catch (...) {
+      e.dev = saved;
+      upload_env_table(c, e);
+      throw;
+    }
+    e.dev = saved;
+    upload_env_table(c, e);
+    PD_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> fb(3 * e.N);
+    download_xyz(c, e.dev.f, e.N, fb.data());
+    for (int64_t i = 0; i < b.d.n; ++i)
+      for (int d = 0; d < 3; ++d) f[3 * (b.first + i) + d] += fb[3 * (b.first + i) + d];
+  });
+}
+
+int pd_env_update_neighs(void* ctx, int64_t env) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    contact_update(c, e);
+    PD_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    PD_HIP(hipStreamSynchronize(c->stream));
+    if (y) download_xyz(c, e.dev.y[e.cur], e.N, y);
+    if (v) download_xyz(c, e.dev.v, e.N, v);
+    if (f) download_xyz(c, e.dev.f, e.N, f);
+  });
+}
+
+int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, const double* f) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    PD_HIP(hipStreamSynchronize(c->stream));
+    if (y) upload_xyz(c, e.dev.y[e.cur], e.N, y, false);
+    if (v) upload_xyz(c, e.dev.v, e.N, v, false);
+    if (f) upload_xyz(c, e.dev.f, e.N, f, false);
+  });
+}
+
+int pd_env_damage(void* ctx, int64_t env, double* damage) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    BondsHost h;
+    download_bonds(c, e, h);
+    for (int64_t p = 0; p < e.N; ++p) {
+      int64_t cnt = 0;
+      for (int k = 0; k < h.deg[p]; ++k) cnt += h.state[h.pslot0[p] + 64u * (uint32_t)k];
+      // cal_damage: 1 .- intact ./ intact0 (intact0 = initial family size)
+      damage[p] = 1.0 - (double)cnt / (double)h.deg[p];
+    }
+  });
+}
+
+int pd_env_time_step(void* ctx, int64_t env, int64_t* ts) {
+  return run_guard(ctx, [&] { *ts = get_env(C(ctx), env).time_step; });
+}
+
+int pd_env_check_finite(void* ctx, int64_t env) {
+  Ctx* c = C(ctx);
+  int rc = run_guard(ctx, [&] {
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    std::vector<double> a(3 * e.N);
+    const double4* srcs[3] = {e.dev.y[e.cur], e.dev.v, e.dev.f};
+    const char* names[3] = {"y", "v", "f"};
+    for (int q = 0; q < 3; ++q) {
+      download_xyz(c, srcs[q], e.N, a.data());
+      for (double x : a)
+        if (!std::isfinite(x)) throw PdError(PD_E_NONFINITE, std::string("nan/inf in env.") + names[q]);
+    }
+  });
+  return rc;
+}
+
+}  // extern "C"

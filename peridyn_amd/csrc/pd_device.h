@@ -1,0 +1,82 @@
+// Device helpers shared by the kernels.  The library is compiled with
+// -ffp-contract=off: every a*b+c below rounds twice, as in the reference's Julia
+// code (no @fastmath / @muladd there), and IEEE FP64 division and square root
+// are used throughout (no -ffast-math, no rcp/rsq shortcuts in the FP64 forms).
+#pragma once
+#include "pd_internal.h"
+
+// get_magnitude (src/misc/const_macros.jl:116): sqrt((a1*a1 + a2*a2) + a3*a3)
+__device__ __forceinline__ double pd_mag(double a1, double a2, double a3) {
+  return __builtin_sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+}
+
+// Julia `^(x::Float64, n::Integer)` power by squaring in double-double
+// (base/math.jl pow_body; muladd taken as fma).  n == 1 returns x, n == 2 x*x.
+__device__ __forceinline__ double pd_pow_int(double x, int64_t n) {
+  if (n == 0) return 1.0;
+  if (n == 3) return x * x * x;
+  double y = 1.0, xnlo = 0.0, ynlo = 0.0;
+  if (n < 0) {
+    double rx = 1.0 / x;
+    if (n == -2) return rx * rx;
+    if (__builtin_isfinite(x)) xnlo = -__builtin_fma(x, rx, -1.0) * rx;
+    x = rx;
+    n = -n;
+  }
+  while (n > 1) {
+    if (n & 1) {
+      double err = __builtin_fma(y, xnlo, x * ynlo);
+      double p = x * y;
+      double plo = __builtin_fma(x, y, -p);
+      y = p;
+      ynlo = plo + err;
+    }
+    double err = x * 2.0 * xnlo;
+    double p = x * x;
+    double plo = __builtin_fma(x, x, -p);
+    x = p;
+    xnlo = plo + err;
+    n >>= 1;
+  }
+  double err = __builtin_fma(y, xnlo, x * ynlo);
+  return (__builtin_isfinite(x) && __builtin_isfinite(err)) ? __builtin_fma(x, y, err) : x * y;
+}
+
+// Direction of a ToFroBC during step s = time_step + 1 (1-based), i.e. after the
+// check! calls of the completed steps, at time_step values t = 0 .. time_step-1
+// (ToFroBC.jl:107-111: flip when t > applyafter and t % freq == 0).
+__device__ __forceinline__ double pd_bc_dir(const BcDev& bc, int64_t time_step) {
+  const int64_t hi = time_step - 1;                  // last t checked
+  int64_t lo = bc.applyafter + 1;
+  if (lo < 0) lo = 0;
+  if (hi < lo) return 1.0;
+  int64_t flips;
+  if (__builtin_isinf(bc.freq)) {
+    flips = (lo == 0) ? 1 : 0;                       // t % Inf == t
+  } else {
+    const int64_t fr = (int64_t)bc.freq;
+    // multiples of fr in [lo, hi]
+    const int64_t upto_hi = hi / fr + 1;             // multiples in [0, hi]
+    const int64_t upto_lo = (lo > 0) ? ((lo - 1) / fr + 1) : 0;
+    flips = upto_hi - upto_lo;
+  }
+  return (flips & 1) ? -1.0 : 1.0;
+}
+
+__device__ __forceinline__ int pd_wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+__device__ __forceinline__ int pd_lane() { return threadIdx.x & 63; }
+
+// XCD-aware block id: consecutive logical blocks (spatial neighbours sharing
+// neighbour data) land on the same XCD / L2.  Hardware dispatches workgroup b
+// to XCD b % 8.
+// XCD x receives q + (x < r) workgroups (q = nblocks / 8, r = nblocks % 8), so
+// the map b -> start(x) + b / 8 is a bijection onto [0, nblocks).
+__device__ __forceinline__ int pd_xcd_block(int nblocks) {
+  const int b = blockIdx.x;
+  const int q = nblocks / 8, r = nblocks % 8;
+  const int xcd = b % 8, idx = b / 8;
+  const int start = xcd * q + (xcd < r ? xcd : r);
+  return start + idx;
+}

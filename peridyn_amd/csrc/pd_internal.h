@@ -1,0 +1,121 @@
+// Internal device/host structures of libperidyn_hip (gfx950, wave64).
+//
+// HBM layout (per env, all arrays device-resident, env-local particle ids):
+//   x      double4[N]  reference position xyz + particle volume in .w
+//   y[2]   double4[N]  deformed positions, double-buffered across the fused
+//                      force->Verlet kernel (read y[cur], write y[cur^1])
+//   v, f   double4[N]  velocity, force density (acceleration) — .w unused
+//   bonds  SELL-64: particles are grouped in chunks of 64 consecutive ids of one
+//          block (one wavefront, one lane per particle); chunk c stores slot k of
+//          lane l at slot0[c] + 64*k + l, k < width[c] = max degree in the chunk.
+//          col int32 (neighbour id, rows sorted ascending), state uint8 (1 intact).
+//          Loads of col/state are fully coalesced (64 lanes x 4 B / 1 B).
+//   EP     rev uint32 slot of the reverse bond, edp[2] double per slot.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/peridyn.h"
+
+#define PD_WAVE 64
+#define PD_WAVES_PER_BLOCK 4
+#define PD_MAX_TYPES 8
+#define PD_MAX_CONTACTS 8
+
+struct BlockDev {
+  int32_t kind, ntype;
+  int32_t first, n;          // env-local particle range
+  int32_t chunk0, nchunks;   // chunk range (env-local chunk ids)
+  int32_t deformed;          // latch (general_material.jl:243-255)
+  int32_t pad;
+  double c11;                // BB bond stiffness c[1,1]
+  double horizon;
+  double density[PD_MAX_TYPES];
+  double s0[PD_MAX_TYPES * PD_MAX_TYPES];   // [t1 + nt*t2] (symmetric)
+  double K[PD_MAX_TYPES * PD_MAX_TYPES];
+  double G[PD_MAX_TYPES * PD_MAX_TYPES];
+  double psi[PD_MAX_TYPES * PD_MAX_TYPES];
+};
+
+struct BcDev {
+  int32_t kind;
+  int32_t pad;
+  double rate[3];
+  double freq;               // integer-valued or +Inf
+  int64_t applyafter;
+};
+
+struct ContactDev {
+  int32_t n1, n2;            // members of pair[1] / pair[2]
+  int32_t max_neighs;
+  int32_t expo_is_int;
+  double expo, K, equi, distance;
+  const int32_t* idx1;       // member -> env particle id
+  const int32_t* idx2;
+  const int32_t* inv1;       // env particle id -> member or -1
+  const int32_t* inv2;
+  int32_t* own_cnt;          // n1
+  int32_t* own;              // n1 * max_neighs, block-2 member ids, descending
+  int32_t* rev_cnt;          // n2
+  int32_t* rev_off;          // n2 (exclusive offsets)
+  int32_t* rev;              // block-1 member ids, ascending per block-2 member
+  int32_t* rev_cursor;       // n2 scratch
+  int32_t* cell_start;       // cell list over in-box block-2 members
+  int32_t* cell_count;
+  int32_t* cell_items;       // n2
+  int32_t* cell_of;          // n2 (cell id or -1 outside the box)
+  double* box;               // [0..2] min, [3..5] max, [6] overlap flag (1/0), [7..] grid
+  int32_t ncell_cap;
+  int32_t rev_cap;
+};
+
+struct EnvDev {
+  int32_t N, nblocks, ncontacts, nbc;
+  int32_t precision, cur, edp_cur, has_ep;
+  double dt;
+  int64_t time_step;         // completed steps (env.time_step)
+  double4* x;
+  double4* y[2];
+  double4* v;
+  double4* f;
+  float4* u32;               // FP32C: float displacement y - x (+ vol in .w)
+  int32_t* pblock;           // block of each particle
+  uint8_t* ptype;            // 0-based type-local index
+  double* inv_den;           // 1 / env density (mass/volume), contacts.jl:231,235
+  int32_t* bcid;             // effective BC per particle (last in list order) or -1
+  int32_t* deg;              // degree per particle
+  uint32_t* pslot0;          // slot of (p, k = 0); slot(p, k) = pslot0[p] + 64 k
+  double* m;                 // weighted volume
+  double* theta;
+  double* td;                // EP td_norm2
+  int32_t* col;
+  uint8_t* state;
+  uint32_t* rev;             // EP only
+  double* edp[2];            // EP only
+  BlockDev* blocks;
+  BcDev* bcs;
+  ContactDev* contacts;
+};
+
+struct Chunk {
+  int32_t env, block;
+  int32_t first, count;      // env-local particle range of the chunk
+  uint32_t slot0;
+  int32_t width;
+};
+
+// ---- host-side helpers ----------------------------------------------------
+#define PD_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      throw PdError(PD_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                       \
+  } while (0)
+
+#include <string>
+struct PdError {
+  int code;
+  std::string msg;
+  PdError(int c, std::string m) : code(c), msg(std::move(m)) {}
+};

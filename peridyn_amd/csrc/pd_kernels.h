@@ -1,0 +1,44 @@
+// Host-side launch wrappers of the device kernels (all asynchronous on `s`).
+#pragma once
+#include "pd_internal.h"
+
+// k_force flags
+#define PD_F_KICK 1   // second half kick + velocity BCs of the current step
+#define PD_F_DRIFT 2  // first half kick + drift + position BCs of the next step
+// k_book flags
+#define PD_B_FLIP_Y 1
+#define PD_B_FLIP_EDP 2
+#define PD_B_STEP 4
+
+void pd_launch_theta(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* envs);
+void pd_launch_td(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* envs);
+void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int n, EnvDev* envs, int flags);
+void pd_launch_prologue(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
+void pd_launch_book(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int flags);
+void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n);
+
+// family build (pd_family.hip); all arrays device, env-local particle ids
+struct FamilyGrid {
+  double lo[3];
+  double w;          // cell width (>= horizon)
+  int32_t nc[3];     // cells per dimension
+};
+void pd_launch_cell_of(hipStream_t s, const double4* x, int first, int n, FamilyGrid g,
+                       int32_t* cell_of, int32_t* cell_count);
+void pd_launch_cell_fill(hipStream_t s, int n, const int32_t* cell_of, const int32_t* cell_start,
+                         int32_t* cell_cursor, int32_t* cell_items, int first);
+void pd_launch_degree(hipStream_t s, const double4* x, int first, int n, FamilyGrid g, double horizon,
+                      const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
+                      const int32_t* cell_items, int32_t* deg);
+void pd_launch_fill(hipStream_t s, const double4* x, int first, int n, FamilyGrid g, double horizon,
+                    const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
+                    const int32_t* cell_items, const uint32_t* pslot0, const int32_t* deg,
+                    int32_t* col, uint8_t* state);
+void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0,
+                    const int32_t* deg, const int32_t* col, double* m);
+void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, const int32_t* deg,
+                   const int32_t* col, uint32_t* rev);
+
+// contacts (pd_contact.hip)
+void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDev& host_env,
+                              const ContactDev& c, int cidx);

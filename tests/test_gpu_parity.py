@@ -1,0 +1,190 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars (written into each test):
+  * neighbour families: identical reference ELL arrays (sorted sets, exact);
+  * PD_PREC_EXACT: bit-identical y, v, f, bond state, theta, edp (== on FP64);
+  * PD_PREC_FP64:  relative L2 <= 1e-10 for forces and displacements after N
+    steps on stable runs; identical bond-state masks except bonds within 1e-9
+    of the critical stretch.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from peridyn_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pd():
+    from peridyn_amd import core
+    return core
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def run_both(pd, orc, spec, steps, nuf=1, prec="exact"):
+    o = orc.OracleEnv(spec)
+    o.run(steps, nuf)
+    g = pd.Env.from_spec(spec, precision=prec)
+    g.run(steps, nuf)
+    return o, g
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", ["lattice9", "cloud", "disk"])
+def test_family_and_weighted_volume_match_oracle(pd, orc, case):
+    say("family case", case)
+    if case == "lattice9":
+        x, h = W.lattice_box(9, 9, 9), 3.015
+    elif case == "cloud":
+        x, h = np.random.default_rng(7).uniform(0.0, 9.0, size=(700, 3)), 1.7
+    else:
+        x, h = W.lattice_disk(12.0, 4), 3.015
+    spec = {"dt": 1.0, "blocks": [W.block("bb", x, h, bulk_modulus=1.0, critical_stretch=1.0, density=1.0)],
+            "contacts": [], "bcs": []}
+    env = pd.Env.from_spec(spec)
+    fam, intact = env.family(0)
+    ofam, ointact = orc.cal_family(x, h, 200)
+    assert fam.shape == ofam.shape
+    assert np.array_equal(fam, ofam)
+    assert np.array_equal(intact, ointact)
+    m = env.weighted_volume(0)
+    assert np.array_equal(m, orc.weighted_volume(x, np.ones(x.shape[0]), ofam))
+    env.close()
+
+
+def test_family_overflow_is_reported(pd):
+    spec = {"dt": 1.0, "blocks": [W.block("bb", W.lattice_box(9, 9, 9), 3.015, max_neigh=122,
+                                          bulk_modulus=1.0, critical_stretch=1.0, density=1.0)],
+            "contacts": [], "bcs": []}
+    from peridyn_amd._lib import PeriDynError, PD_E_OVERFLOW
+    with pytest.raises(PeriDynError) as ei:
+        pd.Env.from_spec(spec)
+    assert ei.value.code == PD_E_OVERFLOW
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,dt,steps", [(8, 1e-4, 60), (10, 1e-3, 40)])
+def test_bb_exact_bit_identical(pd, orc, n, dt, steps):
+    """Tensile block with MoveBC grips; (10, 1e-3) is the literal tensile-block
+    time step (unstable), where only bit-identical arithmetic can agree."""
+    say("bb exact", n, dt, steps)
+    spec = W.tensile_bb(n=n, dt=dt)
+    o, g = run_both(pd, orc, spec, steps)
+    y, v, f = g.download()
+    assert np.array_equal(y, o.y)
+    assert np.array_equal(v, o.v)
+    assert np.array_equal(f, o.f)
+    fam, intact = g.family(0)
+    assert np.array_equal(intact, o.blocks[0].intact)
+    assert np.array_equal(g.damage(), o.damage())
+    assert g.time_step == steps
+
+
+def test_bb_exact_split_runs_equal_single_run(pd, orc):
+    spec = W.tensile_bb(n=7, dt=1e-4)
+    o = orc.OracleEnv(spec)
+    o.run(35)
+    g = pd.Env.from_spec(spec, precision="exact")
+    g.run(10); g.run(1); g.run(24)
+    y, v, f = g.download()
+    assert np.array_equal(y, o.y) and np.array_equal(v, o.v) and np.array_equal(f, o.f)
+
+
+def test_bb_fp64_within_1e10(pd, orc):
+    say("bb fp64")
+    spec = W.tensile_bb(n=10, dt=1e-4, pull_per_step=0.002)
+    o, g = run_both(pd, orc, spec, 100, prec="fp64")
+    y, v, f = g.download()
+    x = spec["blocks"][0]["x"]
+    assert rel_l2(f, o.f) <= 1e-10            # tolerance: 1e-10 relative L2 (FP64 form)
+    assert rel_l2(y - x, o.y - x) <= 1e-10
+    _, intact = g.family(0)
+    assert np.array_equal(intact, o.blocks[0].intact)
+
+
+@pytest.mark.parametrize("kind", ["osb", "ep"])
+def test_state_based_exact_bit_identical(pd, orc, kind):
+    say("state-based exact", kind)
+    spec = W.tensile_bb(n=8, dt=1e-4, kind=kind, pull_per_step=0.004)
+    if kind == "ep":
+        spec["blocks"][0]["yield_stress"] = 20.0   # yields within the run
+    o, g = run_both(pd, orc, spec, 40)
+    y, v, f = g.download()
+    assert np.array_equal(y, o.y)
+    assert np.array_equal(v, o.v)
+    assert np.array_equal(f, o.f)
+    assert np.array_equal(g.theta(0), o.blocks[0].theta)
+    _, intact = g.family(0)
+    assert np.array_equal(intact, o.blocks[0].intact)
+    if kind == "ep":
+        e = g.edp(0)
+        assert np.any(e != 0.0)
+        assert np.array_equal(e, o.blocks[0].edp)
+
+
+@pytest.mark.parametrize("kind", ["osb", "ep"])
+def test_state_based_fp64_within_1e10(pd, orc, kind):
+    spec = W.tensile_bb(n=8, dt=1e-4, kind=kind, pull_per_step=0.002)
+    if kind == "ep":
+        spec["blocks"][0]["yield_stress"] = 20.0
+    o, g = run_both(pd, orc, spec, 40, prec="fp64")
+    y, v, f = g.download()
+    x = spec["blocks"][0]["x"]
+    assert rel_l2(f, o.f) <= 1e-10
+    assert rel_l2(y - x, o.y - x) <= 1e-10
+
+
+@pytest.mark.parametrize("expo", [1, 2])
+def test_contact_plates_exact_bit_identical(pd, orc, expo):
+    """Two small EP plates colliding with a (non)linear spring contact, contact
+    lists rebuilt every 5 steps."""
+    say("contact plates exact", expo)
+    spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=expo)
+    o, g = run_both(pd, orc, spec, 30, nuf=5)
+    y, v, f = g.download()
+    assert np.array_equal(y, o.y)
+    assert np.array_equal(v, o.v)
+    assert np.array_equal(f, o.f)
+    assert np.abs(o.f).max() > 0
+
+
+def test_update_acc_and_force_density_seams(pd, orc):
+    spec = W.tensile_bb(n=7, kind="osb")
+    rng = np.random.default_rng(2)
+    spec["blocks"][0]["y"] = spec["blocks"][0]["x"] * 1.002 + rng.normal(scale=1e-3, size=(343, 3))
+    o = orc.OracleEnv(spec)
+    o.update_acc()
+    g = pd.Env.from_spec(spec, precision="exact")
+    f = g.update_acc()
+    assert np.array_equal(f, o.f)
+    g2 = pd.Env.from_spec(spec, precision="exact")
+    f2 = np.zeros((343, 3))
+    g2.force_density(0, np.ascontiguousarray(spec["blocks"][0]["y"]), f2)
+    assert np.array_equal(f2, o.f)
+
+
+def test_batch_equals_alone(pd, orc):
+    specs = [W.tensile_bb(n=6, dt=1e-4), W.tensile_bb(n=7, dt=1e-4, kind="osb", pull_per_step=0.003)]
+    alone = []
+    for s in specs:
+        e = pd.Env.from_spec(s, precision="exact")
+        e.run(25)
+        alone.append(e.download())
+    envs = [pd.Env.from_spec(s, precision="exact") for s in specs]
+    pd.run_batch(envs, 25)
+    for e, a in zip(envs, alone):
+        b = e.download()
+        for u, w in zip(a, b):
+            assert np.array_equal(u, w)
