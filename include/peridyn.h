@@ -145,6 +145,19 @@ int pd_env_damage(void* ctx, int64_t env, double* damage);        /* cal_damage,
 int pd_env_time_step(void* ctx, int64_t env, int64_t* time_step);
 int pd_env_check_finite(void* ctx, int64_t env);                  /* @check_nan, const_macros.jl:60-74 */
 
+/* ---- measurement ------------------------------------------------------------
+ * No reference counterpart (the reference's only timing hook is TimerOutputs,
+ * src/misc/const_macros.jl:7).  When enabled, every launch of a kernel class in
+ * pd_run is bracketed by HIP events on the context stream; reads return the
+ * summed device time and the launch count since the last enable. */
+enum {
+  PD_PROF_PROLOGUE = 0, PD_PROF_LATCH = 1, PD_PROF_THETA = 2, PD_PROF_TD = 3,
+  PD_PROF_FORCE_BB = 4, PD_PROF_FORCE_OSB = 5, PD_PROF_FORCE_EP = 6,
+  PD_PROF_CONTACT = 7, PD_PROF_BOOK = 8, PD_PROF_NCLASS = 9
+};
+int pd_ctx_profile(void* ctx, int32_t enable);
+int pd_ctx_profile_read(void* ctx, int32_t cls, double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,16 @@ class Context:
     def sync(self):
         self.check(self.lib.pd_sync(self.h))
 
+    def profile(self, enable: bool = True):
+        """Start (and reset) or stop per-kernel-class HIP-event timing."""
+        self.check(self.lib.pd_ctx_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self, cls: str):
+        """(total device ms, launches) of a kernel class since profile(True)."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self.check(self.lib.pd_ctx_profile_read(self.h, _lib.PROF_CLASSES[cls], ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def close(self):
         if self.h:
             self.lib.pd_ctx_destroy(self.h)

@@ -81,7 +81,52 @@ struct Ctx {
   bool set_has_ep = false, set_has_theta = false;
   int2* d_latch = nullptr;
   size_t latch_cap = 0;
+  // per-kernel-class HIP-event timing on the context stream (pd_ctx_profile)
+  bool prof = false;
+  struct Rec { int cls; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  double prof_ms[PD_PROF_NCLASS] = {};
+  int64_t prof_n[PD_PROF_NCLASS] = {};
 };
+
+hipEvent_t get_event(Ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  PD_HIP(hipEventCreate(&e));
+  return e;
+}
+
+// Runs `f` (kernel launches on c->stream) bracketed by events when profiling.
+template <typename F>
+void timed(Ctx* c, int cls, F&& f) {
+  if (!c->prof) {
+    f();
+    return;
+  }
+  hipEvent_t a = get_event(c), b = get_event(c);
+  PD_HIP(hipEventRecord(a, c->stream));
+  f();
+  PD_HIP(hipEventRecord(b, c->stream));
+  c->pending.push_back({cls, a, b});
+}
+
+void drain_profile(Ctx* c) {
+  PD_HIP(hipStreamSynchronize(c->stream));
+  for (auto& r : c->pending) {
+    float ms = 0.f;
+    PD_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    c->prof_ms[r.cls] += ms;
+    c->prof_n[r.cls] += 1;
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->pending.clear();
+}
 
 template <typename T>
 T* dalloc(Env& e, size_t n) {
@@ -705,6 +750,8 @@ int pd_ctx_destroy(void* ctx) {
   if (c->d_set) (void)hipFree(c->d_set);
   if (c->d_slots) (void)hipFree(c->d_slots);
   if (c->d_latch) (void)hipFree(c->d_latch);
+  for (auto& r : c->pending) { c->pool.push_back(r.a); c->pool.push_back(r.b); }
+  for (hipEvent_t ev : c->pool) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return PD_OK;
@@ -941,22 +988,28 @@ int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t
     const int nl = prepare_latch(c, ids);
     EnvDev* T = c->d_envs;
     Chunk* d = c->d_set;
-    pd_launch_prologue(s, d, c->n_all, T);
+    timed(c, PD_PROF_PROLOGUE, [&] { pd_launch_prologue(s, d, c->n_all, T); });
     pd_launch_book(s, T, c->d_slots, c->n_slots, PD_B_FLIP_Y);
     for (int64_t st = 1; st <= nsteps; ++st) {
       const bool last = (st == nsteps);
-      if (nl) pd_launch_latch(s, T, c->d_latch, nl);
-      if (c->set_has_theta) pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T);
-      if (c->set_has_ep) pd_launch_td(s, prec, d + c->off_td, c->n_td, T);
+      if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl); });
+      if (c->set_has_theta)
+        timed(c, PD_PROF_THETA, [&] { pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T); });
+      if (c->set_has_ep) timed(c, PD_PROF_TD, [&] { pd_launch_td(s, prec, d + c->off_td, c->n_td, T); });
       const int fl = PD_F_KICK | (last ? 0 : PD_F_DRIFT);
       for (int k = 0; k < 3; ++k)
-        if (c->n_force[k]) pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl);
+        if (c->n_force[k])
+          timed(c, PD_PROF_FORCE_BB + k,
+                [&] { pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl); });
       for (int64_t id : ids) {
         Env& e = get_env(c, id);
-        if (!e.contacts.empty() && ((e.time_step + st) % nuf) == 0) contact_update(c, e);
+        if (!e.contacts.empty() && ((e.time_step + st) % nuf) == 0)
+          timed(c, PD_PROF_CONTACT, [&] { contact_update(c, e); });
       }
-      pd_launch_book(s, T, c->d_slots, c->n_slots,
-                     PD_B_STEP | (last ? 0 : PD_B_FLIP_Y) | (c->set_has_ep ? PD_B_FLIP_EDP : 0));
+      timed(c, PD_PROF_BOOK, [&] {
+        pd_launch_book(s, T, c->d_slots, c->n_slots,
+                       PD_B_STEP | (last ? 0 : PD_B_FLIP_Y) | (c->set_has_ep ? PD_B_FLIP_EDP : 0));
+      });
     }
     PD_HIP(hipGetLastError());
     for (int64_t id : ids) {
@@ -1059,6 +1112,25 @@ int pd_env_damage(void* ctx, int64_t env, double* damage) {
       // cal_damage: 1 .- intact ./ intact0 (intact0 = initial family size)
       damage[p] = 1.0 - (double)cnt / (double)h.deg[p];
     }
+  });
+}
+
+int pd_ctx_profile(void* ctx, int32_t enable) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    drain_profile(c);
+    for (int k = 0; k < PD_PROF_NCLASS; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    c->prof = enable != 0;
+  });
+}
+
+int pd_ctx_profile_read(void* ctx, int32_t cls, double* total_ms, int64_t* launches) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    if (cls < 0 || cls >= PD_PROF_NCLASS) throw PdError(PD_E_ARG, "unknown kernel class");
+    drain_profile(c);
+    if (total_ms) *total_ms = c->prof_ms[cls];
+    if (launches) *launches = c->prof_n[cls];
   });
 }
 
