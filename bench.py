@@ -94,9 +94,14 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="plates_ep")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "exact"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "exact", "fp32c"])
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: chosen at upload)")
+    ap.add_argument("--autotune", action="store_true", help="time every schedule first and use the fastest")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy (x3) whose HBM counter readings "
+                         "scale FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
@@ -107,14 +112,28 @@ def main():
     from peridyn_amd import core
 
     desc, spec, nuf, dom = workload(args.workload)
+    if args.copy_calibration:
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ctx = core.Context(0)
+    lanes = args.lanes
+    tuned = None
+    if args.autotune and args.precision != "exact":
+        from peridyn_amd import autotune
+        tuned = autotune.tune(spec, args.precision, steps=10, nuf=nuf, use_cache=False, ctx=ctx)
+        lanes = tuned["lanes_per_particle"]
     # warm-up on a throwaway copy (code objects, caches), then time the canonical run
     if args.warmup > 0:
-        w = core.Env.from_spec(spec, precision=args.precision, ctx=ctx)
+        w = core.Env.from_spec(spec, precision=args.precision, ctx=ctx, lanes_per_particle=lanes)
         w.run(args.warmup, nuf)
         ctx.sync()
         w.close()
-    env = core.Env.from_spec(spec, precision=args.precision, ctx=ctx)
+    env = core.Env.from_spec(spec, precision=args.precision, ctx=ctx, lanes_per_particle=lanes)
+    lanes = env.lanes_per_particle
     info = env.info()
     N, nnz = info["particles"], info["bonds"]
     ctx.sync()
@@ -155,7 +174,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "particles": N, "bonds": nnz, "precision": args.precision,
-                   "neigh_update_freq": nuf},
+                   "neigh_update_freq": nuf, "lanes_per_particle": lanes,
+                   "autotune_ms_per_step": tuned["ms_per_step"] if tuned else None},
         "roofline": {"bound": "hbm", "kernel": f"k_force<{dom[6:].upper()},{args.precision}>",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_for(dom, args.workload),

@@ -109,6 +109,13 @@ int pd_env_create(void* ctx, double dt, int32_t precision, int64_t* env);
 int pd_env_add_block(void* ctx, int64_t env, const pd_block_desc* b, int64_t* block);
 int pd_env_add_contact(void* ctx, int64_t env, const pd_contact_desc* c);
 int pd_env_add_bc(void* ctx, int64_t env, const pd_bc_desc* bc);
+/* Bond-loop schedule of the env: lanes per particle (power of two 1..64; 1 =
+ * lane per particle, 64 = wavefront per particle; 0 = choose from the particle
+ * count at commit).  Set before commit; the reference-order form uses 1.  No
+ * reference counterpart: it selects among layouts of the same computation
+ * (peridyn_amd/autotune.py picks it by timing). */
+int pd_env_set_schedule(void* ctx, int64_t env, int32_t lanes_per_particle);
+int pd_env_schedule(void* ctx, int64_t env, int32_t* lanes_per_particle);
 /* Uploads the env, builds every block's family on the GPU (cell list -> CSR),
  * the weighted volume m and the per-particle BC map.  PD_E_OVERFLOW where
  * cal_family/GeneralMaterial would raise BoundsError. */

@@ -23,6 +23,7 @@ EXPORTS = [
     "pd_block_weighted_volume", "pd_block_theta", "pd_block_edp", "pd_run", "pd_env_update_acc",
     "pd_force_density", "pd_env_update_neighs", "pd_env_download", "pd_env_upload", "pd_env_damage",
     "pd_env_time_step", "pd_env_check_finite", "pd_ctx_profile", "pd_ctx_profile_read",
+    "pd_env_set_schedule", "pd_env_schedule",
 ]
 PROF_CLASSES = {"prologue": 0, "latch": 1, "theta": 2, "td": 3, "force_bb": 4, "force_osb": 5,
                 "force_ep": 6, "contact": 7, "book": 8}
@@ -99,6 +100,8 @@ def load() -> ctypes.CDLL:
     L.pd_env_damage.argtypes = [P, I64, P]
     L.pd_env_time_step.argtypes = [P, I64, ctypes.POINTER(I64)]
     L.pd_env_check_finite.argtypes = [P, I64]
+    L.pd_env_set_schedule.argtypes = [P, I64, I32]
+    L.pd_env_schedule.argtypes = [P, I64, ctypes.POINTER(I32)]
     L.pd_ctx_profile.argtypes = [P, I32]
     L.pd_ctx_profile_read.argtypes = [P, I32, ctypes.POINTER(D), ctypes.POINTER(I64)]
     _lib = L

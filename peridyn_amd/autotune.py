@@ -1,0 +1,82 @@
+"""Schedule autotuner for the bond loops.
+
+The bond layout (pd_internal.h) serves each particle with G lanes (G = 1 is one
+lane per particle, G = 64 one wavefront per particle).  Which G is fastest
+depends on the specimen size (enough wavefronts to hide gather latency), the
+degree distribution and the material passes.  `tune(spec, precision)` builds a
+replica of the env for each candidate G, times a few steps of the real run
+loop with HIP events, and returns the fastest G together with the timings;
+results are cached per (particle-count bucket, material kinds, precision) in
+a JSON file so later envs of similar shape reuse them.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+from typing import Dict, Iterable, Optional
+
+from . import core
+
+CANDIDATES = (1, 2, 4, 8, 16, 32, 64)
+_CACHE_PATH = os.environ.get("PERIDYN_TUNE_CACHE",
+                             os.path.join(os.path.expanduser("~"), ".cache", "peridyn_amd_tune.json"))
+
+
+def _key(spec: dict, precision: str) -> str:
+    n = sum(int(b["x"].shape[0]) for b in spec["blocks"])
+    kinds = "+".join(sorted({b["kind"] for b in spec["blocks"]}))
+    bucket = int(round(math.log2(max(n, 1)) * 2))  # half-octave buckets
+    return f"{precision}:{kinds}:{bucket}"
+
+
+def _load() -> Dict[str, dict]:
+    try:
+        with open(_CACHE_PATH) as f:
+            return json.load(f)
+    except Exception:
+        return {}
+
+
+def _store(db: Dict[str, dict]):
+    try:
+        os.makedirs(os.path.dirname(_CACHE_PATH), exist_ok=True)
+        with open(_CACHE_PATH, "w") as f:
+            json.dump(db, f, indent=1)
+    except OSError:
+        pass
+
+
+def time_schedule(spec: dict, precision: str, lanes: int, steps: int = 20, nuf: int = 10,
+                  ctx: Optional[core.Context] = None) -> float:
+    """Mean device time per step (ms) of `steps` steps with the given schedule."""
+    ctx = ctx or core.Context.default()
+    env = core.Env.from_spec(spec, precision=precision, ctx=ctx, lanes_per_particle=lanes)
+    try:
+        env.run(2, nuf)  # warm code objects and caches
+        ctx.sync()
+        t0 = time.perf_counter()
+        env.run(steps, nuf)
+        ctx.sync()
+        return 1e3 * (time.perf_counter() - t0) / steps
+    finally:
+        env.close()
+
+
+def tune(spec: dict, precision: str = "fp64", candidates: Iterable[int] = CANDIDATES, steps: int = 20,
+         nuf: int = 10, use_cache: bool = True, ctx: Optional[core.Context] = None) -> dict:
+    """Returns {"lanes_per_particle": best G, "ms_per_step": {G: ms}}."""
+    if precision == "exact":
+        return {"lanes_per_particle": 1, "ms_per_step": {}}
+    key = _key(spec, precision)
+    db = _load() if use_cache else {}
+    if use_cache and key in db:
+        return db[key]
+    times = {int(g): time_schedule(spec, precision, int(g), steps, nuf, ctx) for g in candidates}
+    best = min(times, key=times.get)
+    res = {"lanes_per_particle": best, "ms_per_step": times}
+    if use_cache:
+        db[key] = res
+        _store(db)
+    return res

@@ -79,12 +79,15 @@ class Env:
 
     # -- construction ---------------------------------------------------------
     @classmethod
-    def from_spec(cls, spec: dict, precision: str = "fp64", ctx: Optional[Context] = None) -> "Env":
+    def from_spec(cls, spec: dict, precision: str = "fp64", ctx: Optional[Context] = None,
+                  lanes_per_particle: int = 0) -> "Env":
         ctx = ctx or Context.default()
         L = ctx.lib
         eid = ctypes.c_int64()
         ctx.check(L.pd_env_create(ctx.h, float(spec["dt"]), _PREC[precision], ctypes.byref(eid)))
         env_id = eid.value
+        if lanes_per_particle:
+            ctx.check(L.pd_env_set_schedule(ctx.h, env_id, int(lanes_per_particle)))
         keep = []
         firsts, ns = [], []
         N = 0
@@ -264,6 +267,12 @@ class Env:
         self.ctx.check(self.ctx.lib.pd_env_info(self.ctx.h, self.id, ctypes.byref(n), ctypes.byref(b),
                                                 ctypes.byref(by)))
         return {"particles": n.value, "bonds": b.value, "device_bytes": by.value}
+
+    @property
+    def lanes_per_particle(self) -> int:
+        g = ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.pd_env_schedule(self.ctx.h, self.id, ctypes.byref(g)))
+        return g.value
 
     @property
     def time_step(self) -> int:

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "pd_device.h"
 #include "pd_internal.h"
 #include "pd_kernels.h"
 
@@ -41,6 +42,7 @@ struct Env {
   int32_t slot = -1;
   double dt = 0.0;
   int32_t precision = PD_PREC_FP64;
+  int32_t lgG = -1;  // log2 lanes per particle; -1 = choose at commit
   std::vector<HostBlock> blocks;
   std::vector<HostContact> contacts;
   std::vector<HostBc> bcs;
@@ -271,6 +273,15 @@ void commit_env(Ctx* c, Env& e) {
   for (auto& b : e.blocks) { b.first = (int32_t)N; N += b.d.n; }
   if (N >= (1LL << 31) - 64) throw PdError(PD_E_ARG, "env too large (particles)");
   e.N = N;
+  // bond layout: the reference-order form walks each row on one lane; the
+  // other forms spread a particle over G lanes so that small specimens still
+  // fill the chip (>= ~512k lanes), unless a schedule was set explicitly
+  if (e.precision == PD_PREC_EXACT) {
+    e.lgG = 0;
+  } else if (e.lgG < 0) {
+    e.lgG = 0;
+    while (e.lgG < 4 && (N << e.lgG) < (1LL << 19)) ++e.lgG;
+  }
   // ---- particle arrays (host staging)
   std::vector<double4> x4(N), y4(N), v4(N);
   std::vector<int32_t> pblock(N), bcid(N, -1);
@@ -364,14 +375,16 @@ void commit_env(Ctx* c, Env& e) {
                                          std::to_string(maxdeg) + " neighbours; max_neigh=" +
                                          std::to_string(b.d.max_neigh) + " must exceed the maximum degree");
       b.chunk0 = (int32_t)e.chunks.size();
-      for (int64_t c0 = 0; c0 < b.d.n; c0 += PD_WAVE) {
+      const int G = 1 << e.lgG, Cp = PD_WAVE >> e.lgG;
+      for (int64_t c0 = 0; c0 < b.d.n; c0 += Cp) {
         Chunk ch{};
         ch.env = 0;
         ch.block = (int32_t)bi;
         ch.first = (int32_t)(b.first + c0);
-        ch.count = (int32_t)std::min<int64_t>(PD_WAVE, b.d.n - c0);
-        int32_t wdt = 0;
-        for (int l = 0; l < ch.count; ++l) wdt = std::max(wdt, deg[ch.first + l]);
+        ch.count = (int32_t)std::min<int64_t>(Cp, b.d.n - c0);
+        int32_t md = 0;
+        for (int l = 0; l < ch.count; ++l) md = std::max(md, deg[ch.first + l]);
+        const int32_t wdt = (md + G - 1) / G;  // slots per lane
         ch.width = wdt;
         ch.slot0 = (uint32_t)slots;
         for (int l = 0; l < ch.count; ++l) {
@@ -395,9 +408,11 @@ void commit_env(Ctx* c, Env& e) {
       HostBlock& b = e.blocks[bi];
       FamTmp& t = tmp[bi];
       pd_launch_fill(s, D.x, b.first, (int)b.d.n, t.g, b.d.horizon, t.cell_of, t.cell_start, t.cell_count,
-                     t.items, D.pslot0, D.deg, D.col, D.state);
-      pd_launch_wvol(s, D.x, b.first, (int)b.d.n, D.pslot0, D.deg, D.col, D.m);
+                     t.items, D.pslot0, e.lgG, D.col, D.state);
+      pd_launch_wvol(s, D.x, b.first, (int)b.d.n, D.pslot0, e.lgG, D.deg, D.col, D.m);
     }
+    D.aux = dalloc<double4>(e, N);
+    pd_launch_aux_init(s, D.m, D.aux, (int)N);
     if (e.has_ep) {
       D.rev = dalloc<uint32_t>(e, slots);
       D.edp[0] = dalloc<double>(e, slots);
@@ -405,7 +420,30 @@ void commit_env(Ctx* c, Env& e) {
       PD_HIP(hipMemsetAsync(D.edp[0], 0, sizeof(double) * std::max<uint64_t>(slots, 1), s));
       PD_HIP(hipMemsetAsync(D.edp[1], 0, sizeof(double) * std::max<uint64_t>(slots, 1), s));
       for (auto& b : e.blocks)
-        pd_launch_rev(s, b.first, (int)b.d.n, D.pslot0, D.deg, D.col, D.rev);
+        pd_launch_rev(s, b.first, (int)b.d.n, D.pslot0, e.lgG, D.deg, D.col, D.rev);
+    }
+    if (e.precision == PD_PREC_FP32C) {
+      // x32 relative to each block's minimum corner (bonds never leave a block)
+      std::vector<float4> x32(N), a32(N);
+      std::vector<double> mh(N);
+      PD_HIP(hipMemcpyAsync(mh.data(), D.m, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+      PD_HIP(hipStreamSynchronize(s));
+      for (auto& b : e.blocks) {
+        double lo[3] = {INFINITY, INFINITY, INFINITY};
+        for (int64_t i = 0; i < b.d.n; ++i)
+          for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], b.x[3 * i + d]);
+        for (int64_t i = 0; i < b.d.n; ++i) {
+          const int64_t p = b.first + i;
+          x32[p] = make_float4((float)(b.x[3 * i] - lo[0]), (float)(b.x[3 * i + 1] - lo[1]),
+                               (float)(b.x[3 * i + 2] - lo[2]), (float)b.vol[i]);
+        }
+      }
+      for (int64_t p = 0; p < N; ++p) a32[p] = make_float4(0.0f, (float)(1.0 / mh[p]), 0.0f, 0.0f);
+      D.x32 = dalloc<float4>(e, N);
+      D.aux32 = dalloc<float4>(e, N);
+      PD_HIP(hipMemcpyAsync(D.x32, x32.data(), sizeof(float4) * N, hipMemcpyHostToDevice, s));
+      PD_HIP(hipMemcpyAsync(D.aux32, a32.data(), sizeof(float4) * N, hipMemcpyHostToDevice, s));
+      PD_HIP(hipStreamSynchronize(s));
     }
     PD_HIP(hipStreamSynchronize(s));
     for (auto& t : tmp) fam_free(t);
@@ -415,6 +453,7 @@ void commit_env(Ctx* c, Env& e) {
     throw;
   }
   D.has_ep = e.has_ep ? 1 : 0;
+  D.lgG = e.lgG;
 
   // ---- blocks
   std::vector<BlockDev> bd(e.blocks.size());
@@ -502,7 +541,8 @@ void commit_env(Ctx* c, Env& e) {
     cd.cell_count = dalloc<int32_t>(e, cd.ncell_cap);
     cd.cell_items = dalloc<int32_t>(e, cd.n2);
     cd.cell_of = dalloc<int32_t>(e, cd.n2);
-    cd.box = dalloc<double>(e, 16 + 128 * 12);
+    // reduction partials + scan scratch (pd_contact.hip)
+    cd.box = dalloc<double>(e, 16 + 128 * 12 + pd_scan_scratch_ints(std::max(cd.ncell_cap, cd.n2)) / 2 + 8);
     PD_HIP(hipMemcpyAsync(didx1, idx1.data(), sizeof(int32_t) * idx1.size(), hipMemcpyHostToDevice, s));
     PD_HIP(hipMemcpyAsync(didx2, idx2.data(), sizeof(int32_t) * idx2.size(), hipMemcpyHostToDevice, s));
     PD_HIP(hipMemcpyAsync(dinv1, inv1.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
@@ -616,7 +656,8 @@ void refresh_latch(Ctx* c, const std::vector<int64_t>& ids) {
 
 void contact_update(Ctx* c, Env& e) {
   for (size_t k = 0; k < e.cdev.size(); ++k)
-    pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k);
+    pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k,
+                             e.precision == PD_PREC_EXACT);
 }
 
 // one force evaluation (update_acc!) for a chunk subset of one env
@@ -767,7 +808,7 @@ int pd_env_create(void* ctx, double dt, int32_t precision, int64_t* env) {
   return run_guard(ctx, [&] {
     if (!env) throw PdError(PD_E_ARG, "null output pointer");
     if (!(std::isfinite(dt))) throw PdError(PD_E_ARG, "dt must be finite");
-    if (precision != PD_PREC_EXACT && precision != PD_PREC_FP64)
+    if (precision != PD_PREC_EXACT && precision != PD_PREC_FP64 && precision != PD_PREC_FP32C)
       throw PdError(PD_E_ARG, "unsupported precision form " + std::to_string(precision));
     Ctx* c = C(ctx);
     Env* e = new Env();
@@ -851,6 +892,29 @@ int pd_env_add_bc(void* ctx, int64_t env, const pd_bc_desc* d) {
   });
 }
 
+int pd_env_set_schedule(void* ctx, int64_t env, int32_t lanes_per_particle) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (e.committed) throw PdError(PD_E_STATE, "env already committed");
+    if (lanes_per_particle == 0) { e.lgG = -1; return; }
+    int lg = 0;
+    while ((1 << lg) < lanes_per_particle) ++lg;
+    if ((1 << lg) != lanes_per_particle || lg > 6)
+      throw PdError(PD_E_ARG, "lanes_per_particle must be a power of two in 1..64");
+    if (e.precision == PD_PREC_EXACT && lg != 0)
+      throw PdError(PD_E_ARG, "the reference-order form needs one lane per particle");
+    e.lgG = lg;
+  });
+}
+
+int pd_env_schedule(void* ctx, int64_t env, int32_t* lanes_per_particle) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    *lanes_per_particle = 1 << e.lgG;
+  });
+}
+
 int pd_env_commit(void* ctx, int64_t env) {
   return run_guard(ctx, [&] {
     Ctx* c = C(ctx);
@@ -911,7 +975,7 @@ int pd_block_family_export(void* ctx, int64_t env, int64_t block, int64_t* famil
         int64_t v = 0;
         uint8_t s = 0;
         if (r < d) {
-          const uint32_t slot = h.pslot0[p] + 64u * (uint32_t)(d - 1 - r);
+          const uint32_t slot = pd_slot(h.pslot0[p], d - 1 - r, e.lgG);
           v = (int64_t)h.col[slot] - b.first + 1;
           s = h.state[slot];
         }
@@ -956,13 +1020,14 @@ int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t row
     BondsHost h;
     download_bonds(c, e, h);
     std::vector<double> ed(std::max<int64_t>(e.nslots, 1));
-    PD_HIP(hipMemcpyAsync(ed.data(), e.dev.edp[e.edp_cur], sizeof(double) * e.nslots, hipMemcpyDeviceToHost, c->stream));
+    PD_HIP(hipMemcpyAsync(ed.data(), e.dev.edp[e.edp_cur], sizeof(double) * e.nslots, hipMemcpyDeviceToHost,
+                          c->stream));
     PD_HIP(hipStreamSynchronize(c->stream));
     for (int64_t i = 0; i < b.d.n; ++i) {
       const int64_t p = b.first + i;
       const int d = h.deg[p];
       for (int64_t r = 0; r < rows; ++r)
-        edp[i * rows + r] = (r < d) ? ed[h.pslot0[p] + 64u * (uint32_t)(d - 1 - r)] : 0.0;
+        edp[i * rows + r] = (r < d) ? ed[pd_slot(h.pslot0[p], d - 1 - r, e.lgG)] : 0.0;
     }
   });
 }
@@ -1108,7 +1173,7 @@ int pd_env_damage(void* ctx, int64_t env, double* damage) {
     download_bonds(c, e, h);
     for (int64_t p = 0; p < e.N; ++p) {
       int64_t cnt = 0;
-      for (int k = 0; k < h.deg[p]; ++k) cnt += h.state[h.pslot0[p] + 64u * (uint32_t)k];
+      for (int k = 0; k < h.deg[p]; ++k) cnt += h.state[pd_slot(h.pslot0[p], k, e.lgG)];
       // cal_damage: 1 .- intact ./ intact0 (intact0 = initial family size)
       damage[p] = 1.0 - (double)cnt / (double)h.deg[p];
     }

@@ -5,12 +5,13 @@
 // gates both sides with strict inequalities; a pair (i, j) is a contact when
 //   (a1-a2)*(a1-a2) + (b1-b2)*(b1-b2) + (c1-c2)*(c1-c2) < distance^2
 // (block-1 particle first); each block-1 particle keeps the `max_neighs`
-// LARGEST block-2 member indices in descending order (contacts.jl:468-469).
-// The brute-force O(n1 n2) scan is replaced by a cell list (cells >= distance
-// wide) over the in-box block-2 members.  For the gather-form force kernel the
-// lists are also transposed: each block-2 member gets its block-1 partners in
-// ascending order (the order in which the reference's loop over i scatters
-// into f2[:, j]).
+// LARGEST block-2 member indices (contacts.jl:468-469).  The brute-force
+// O(n1 n2) scan is replaced by a cell list (cells >= distance wide) over the
+// in-box block-2 members.  For the gather-form force kernel the lists are also
+// transposed: each block-2 member gets its block-1 partners.  In the
+// reference-order form (`ordered`) the own lists are sorted descending and the
+// transposed lists ascending, the order in which the reference's loop over i
+// scatters into f1[:, i] and f2[:, j]; the other forms skip both sorts.
 #include "pd_device.h"
 #include "pd_kernels.h"
 
@@ -19,6 +20,10 @@ namespace {
 constexpr int kRedBlocks = 128;
 
 __device__ __forceinline__ const double4* cur_y(const EnvDev& E) { return E.y[E.cur]; }
+
+__device__ __forceinline__ double red_op(int d, double a, double b) {
+  return ((d / 3) % 2 == 0) ? fmin(a, b) : fmax(a, b);
+}
 
 // stage 1: per-block partial min/max over both member sets
 __global__ __launch_bounds__(256) void k_cbox1(const EnvDev* __restrict__ envs, int slot, int cidx,
@@ -42,12 +47,8 @@ __global__ __launch_bounds__(256) void k_cbox1(const EnvDev* __restrict__ envs, 
     v[9] = fmax(v[9], p.x); v[10] = fmax(v[10], p.y); v[11] = fmax(v[11], p.z);
   }
   __shared__ double sh[12][4];
-  for (int o = 32; o > 0; o >>= 1) {
-    for (int d = 0; d < 12; ++d) {
-      const double t = __shfl_down(v[d], o);
-      v[d] = ((d / 3) % 2 == 0) ? fmin(v[d], t) : fmax(v[d], t);
-    }
-  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 12; ++d) v[d] = red_op(d, v[d], __shfl_down(v[d], o));
   const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
     for (int d = 0; d < 12; ++d) sh[d][wv] = v[d];
@@ -55,23 +56,24 @@ __global__ __launch_bounds__(256) void k_cbox1(const EnvDev* __restrict__ envs, 
   if (threadIdx.x == 0) {
     for (int d = 0; d < 12; ++d) {
       double r = sh[d][0];
-      for (int q = 1; q < 4; ++q) r = ((d / 3) % 2 == 0) ? fmin(r, sh[d][q]) : fmax(r, sh[d][q]);
+      for (int q = 1; q < 4; ++q) r = red_op(d, r, sh[d][q]);
       part[blockIdx.x * 12 + d] = r;
     }
   }
 }
 
-// stage 2: box, overlap flag and cell grid (<= ncell_cap cells) into C.box
-__global__ void k_cbox2(const EnvDev* __restrict__ envs, int slot, int cidx, const double* __restrict__ part,
-                        int nparts) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// stage 2 (one wavefront): box, overlap flag and cell grid (<= ncell_cap cells)
+__global__ __launch_bounds__(64) void k_cbox2(const EnvDev* __restrict__ envs, int slot, int cidx,
+                                              const double* __restrict__ part, int nparts) {
   const EnvDev& E = envs[slot];
   const ContactDev& C = E.contacts[cidx];
   double v[12];
-  for (int d = 0; d < 12; ++d) v[d] = part[d];
-  for (int b = 1; b < nparts; ++b)
-    for (int d = 0; d < 12; ++d)
-      v[d] = ((d / 3) % 2 == 0) ? fmin(v[d], part[b * 12 + d]) : fmax(v[d], part[b * 12 + d]);
+  for (int d = 0; d < 12; ++d) v[d] = ((d / 3) % 2 == 0) ? INFINITY : -INFINITY;
+  for (int b = threadIdx.x; b < nparts; b += 64)
+    for (int d = 0; d < 12; ++d) v[d] = red_op(d, v[d], part[b * 12 + d]);
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 12; ++d) v[d] = red_op(d, v[d], __shfl_down(v[d], o));
+  if (threadIdx.x != 0) return;
   const double dist = C.distance;
   double bmin[3], bmax[3];
   for (int d = 0; d < 3; ++d) {
@@ -84,7 +86,7 @@ __global__ void k_cbox2(const EnvDev* __restrict__ envs, int slot, int cidx, con
   const bool ok = !(vol < 0.0) && C.n1 > 0 && C.n2 > 0;
   box[6] = ok ? 1.0 : 0.0;
   double cw = dist * (1.0 + 9.313225746154785e-10);
-  int nc[3];
+  int nc[3] = {1, 1, 1};
   for (int it = 0; it < 64; ++it) {
     long long tot = 1;
     for (int d = 0; d < 3; ++d) {
@@ -136,30 +138,6 @@ __global__ void k_cbin(const EnvDev* __restrict__ envs, int slot, int cidx) {
   C.cell_of[j] = c;
 }
 
-// single-workgroup exclusive scan of n ints (chunked per thread)
-__global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ in, int32_t* __restrict__ out, int n) {
-  __shared__ int32_t sh[1024];
-  const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int b = t * per, e = min(n, b + per);
-  int32_t s = 0;
-  for (int i = b; i < e; ++i) s += in[i];
-  sh[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int32_t add = (t >= o) ? sh[t - o] : 0;
-    __syncthreads();
-    sh[t] += add;
-    __syncthreads();
-  }
-  int32_t run = sh[t] - s;
-  for (int i = b; i < e; ++i) {
-    const int32_t v = in[i];
-    out[i] = run;
-    run += v;
-  }
-}
-
 __global__ void k_cfill(const EnvDev* __restrict__ envs, int slot, int cidx, int32_t* __restrict__ cursor) {
   const EnvDev& E = envs[slot];
   const ContactDev& C = E.contacts[cidx];
@@ -170,8 +148,38 @@ __global__ void k_cfill(const EnvDev* __restrict__ envs, int slot, int cidx, int
   C.cell_items[C.cell_start[c] + atomicAdd(&cursor[c], 1)] = j;
 }
 
-// search: per block-1 member, top max_neighs largest block-2 member ids
-__global__ void k_csearch(const EnvDev* __restrict__ envs, int slot, int cidx) {
+// Visits every block-2 member within `distance` of p (27-cell stencil).
+template <typename F>
+__device__ __forceinline__ void for_candidates(const ContactDev& C, const double4* y, const double4& p,
+                                               const double* box, F&& f) {
+  int cc[3];
+  cell_id(p, box, cc);
+  const int nc[3] = {(int)box[8], (int)box[9], (int)box[10]};
+  const double d2max = C.distance * C.distance;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int zz = cc[2] + dz;
+    if (zz < 0 || zz >= nc[2]) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int yy = cc[1] + dy;
+      if (yy < 0 || yy >= nc[1]) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = cc[0] + dx;
+        if (xx < 0 || xx >= nc[0]) continue;
+        const int cell = xx + nc[0] * (yy + nc[1] * zz);
+        const int s = C.cell_start[cell], e = s + C.cell_count[cell];
+        for (int t = s; t < e; ++t) {
+          const int j = C.cell_items[t];
+          const double4 q = y[C.idx2[j]];
+          const double a = p.x - q.x, b = p.y - q.y, c = p.z - q.z;
+          if (a * a + b * b + c * c < d2max) f(j);
+        }
+      }
+    }
+  }
+}
+
+// search: per block-1 member, the max_neighs largest block-2 member ids
+__global__ void k_csearch(const EnvDev* __restrict__ envs, int slot, int cidx, int ordered) {
   const EnvDev& E = envs[slot];
   const ContactDev& C = E.contacts[cidx];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -179,43 +187,33 @@ __global__ void k_csearch(const EnvDev* __restrict__ envs, int slot, int cidx) {
   const double* box = C.box;
   const double4* y = cur_y(E);
   const double4 p = y[C.idx1[i]];
-  int cnt = 0;
   const int M = C.max_neighs;
   int32_t* own = C.own + (size_t)i * M;
+  int cnt = 0;
   if (box[6] != 0.0 && in_box(p, box)) {
-    int cc[3];
-    cell_id(p, box, cc);
-    const int nc[3] = {(int)box[8], (int)box[9], (int)box[10]};
-    const double d2max = C.distance * C.distance;
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int zz = cc[2] + dz;
-      if (zz < 0 || zz >= nc[2]) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = cc[1] + dy;
-        if (yy < 0 || yy >= nc[1]) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int xx = cc[0] + dx;
-          if (xx < 0 || xx >= nc[0]) continue;
-          const int cell = xx + nc[0] * (yy + nc[1] * zz);
-          const int s = C.cell_start[cell], e = s + C.cell_count[cell];
-          for (int t = s; t < e; ++t) {
-            const int j = C.cell_items[t];
-            const double4 q = y[C.idx2[j]];
-            const double a = p.x - q.x, b = p.y - q.y, c = p.z - q.z;
-            if (a * a + b * b + c * c < d2max) {
-              // keep the M largest ids, descending
-              if (cnt < M) {
-                int u = cnt++;
-                while (u > 0 && own[u - 1] < j) { own[u] = own[u - 1]; --u; }
-                own[u] = j;
-              } else if (j > own[M - 1]) {
-                int u = M - 1;
-                while (u > 0 && own[u - 1] < j) { own[u] = own[u - 1]; --u; }
-                own[u] = j;
-              }
-            }
-          }
-        }
+    int total = 0;
+    for_candidates(C, y, p, box, [&](int) { ++total; });
+    // keep ids >= thr: thr = 0 when everything fits, else the M-th largest id
+    int thr = 0;
+    if (total > M) {
+      int lo = 0, hi = C.n2;  // count(>= lo) >= M, count(>= hi) < M
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        int c = 0;
+        for_candidates(C, y, p, box, [&](int j) { c += (j >= mid); });
+        if (c >= M) lo = mid; else hi = mid;
+      }
+      thr = lo;
+    }
+    for_candidates(C, y, p, box, [&](int j) {
+      if (j >= thr) own[cnt++] = j;
+    });
+    if (ordered) {
+      for (int a = 1; a < cnt; ++a) {  // descending
+        const int v = own[a];
+        int b = a - 1;
+        while (b >= 0 && own[b] < v) { own[b + 1] = own[b]; --b; }
+        own[b + 1] = v;
       }
     }
   }
@@ -251,30 +249,99 @@ __global__ void k_crev_sort(const EnvDev* __restrict__ envs, int slot, int cidx)
   }
 }
 
+// ---- device-wide exclusive scan (tiles of 1024 = 256 threads x 4) ----------
+constexpr int kTile = 1024;
+
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* sh, int32_t& total) {
+  // 256-thread exclusive scan through LDS (wave scans + wave totals)
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  int32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wv] = x;
+  __syncthreads();
+  int32_t wofs = 0;
+  for (int q = 0; q < wv; ++q) wofs += sh[q];
+  total = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return wofs + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(const int32_t* __restrict__ in, int n,
+                                                    int32_t* __restrict__ tile_sum) {
+  const int base = blockIdx.x * kTile + threadIdx.x * 4;
+  int32_t s = 0;
+  for (int k = 0; k < 4; ++k) s += (base + k < n) ? in[base + k] : 0;
+  __shared__ int32_t sh[4];
+  int32_t tot;
+  block_excl_scan(s, sh, tot);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(int32_t* __restrict__ tile_sum, int ntiles) {
+  __shared__ int32_t sh[4];
+  int32_t carry = 0;
+  for (int b = 0; b < ntiles; b += 256) {
+    const int i = b + threadIdx.x;
+    const int32_t v = (i < ntiles) ? tile_sum[i] : 0;
+    int32_t tot;
+    const int32_t ex = block_excl_scan(v, sh, tot);
+    if (i < ntiles) tile_sum[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                    int n, const int32_t* __restrict__ tile_off) {
+  const int base = blockIdx.x * kTile + threadIdx.x * 4;
+  int32_t v[4];
+  int32_t s = 0;
+  for (int k = 0; k < 4; ++k) { v[k] = (base + k < n) ? in[base + k] : 0; s += v[k]; }
+  __shared__ int32_t sh[4];
+  int32_t tot;
+  int32_t run = block_excl_scan(s, sh, tot) + tile_off[blockIdx.x];
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+
 inline int nb(int n) { return n > 0 ? (n + 255) / 256 : 1; }
 
 }  // namespace
 
+int pd_scan_scratch_ints(int n) { return (n + kTile - 1) / kTile + 1; }
+
+void pd_scan_exclusive(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* scratch) {
+  if (n <= 0) return;
+  const int nt = (n + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, s, in, n, scratch);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, scratch, nt);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(256), 0, s, in, out, n, (const int32_t*)scratch);
+}
+
 // Rebuilds the contact lists of contact model `cidx` of env `slot`.  `c` is the
-// host copy of the ContactDev (device pointers, sizes); `part` scratch of
-// kRedBlocks*12 doubles lives in c.box + 16.
+// host copy of the ContactDev (device pointers, sizes).  Scratch inside c.box:
+// [16, 16 + 12*kRedBlocks) partial min/max; the scan scratch follows.
 void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDev& host_env,
-                              const ContactDev& c, int cidx) {
+                              const ContactDev& c, int cidx, bool ordered) {
   (void)host_env;
   double* part = c.box + 16;
+  int32_t* scan_tmp = reinterpret_cast<int32_t*>(c.box + 16 + 12 * kRedBlocks);
+  int32_t* cell_cursor = c.cell_start + c.ncell_cap;
   hipLaunchKernelGGL(k_cbox1, dim3(kRedBlocks), dim3(256), 0, s, envs, slot, cidx, part);
   hipLaunchKernelGGL(k_cbox2, dim3(1), dim3(64), 0, s, envs, slot, cidx, part, kRedBlocks);
   hipLaunchKernelGGL(k_izero, dim3(nb(c.ncell_cap)), dim3(256), 0, s, c.cell_count, c.ncell_cap);
+  hipLaunchKernelGGL(k_izero, dim3(nb(c.ncell_cap)), dim3(256), 0, s, cell_cursor, c.ncell_cap);
   hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cursor, c.n2);
   hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cnt, c.n2);
   hipLaunchKernelGGL(k_cbin, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c.cell_count, c.cell_start, c.ncell_cap);
-  // cell cursor: the second half of the cell_start allocation
-  int32_t* cell_cursor = c.cell_start + c.ncell_cap;
-  hipLaunchKernelGGL(k_izero, dim3(nb(c.ncell_cap)), dim3(256), 0, s, cell_cursor, c.ncell_cap);
+  pd_scan_exclusive(s, c.cell_count, c.cell_start, c.ncell_cap, scan_tmp);
   hipLaunchKernelGGL(k_cfill, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx, cell_cursor);
-  hipLaunchKernelGGL(k_csearch, dim3(nb(c.n1)), dim3(256), 0, s, envs, slot, cidx);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c.rev_cnt, c.rev_off, c.n2);
+  hipLaunchKernelGGL(k_csearch, dim3(nb(c.n1)), dim3(256), 0, s, envs, slot, cidx, ordered ? 1 : 0);
+  pd_scan_exclusive(s, c.rev_cnt, c.rev_off, c.n2, scan_tmp);
   hipLaunchKernelGGL(k_crev_fill, dim3(nb(c.n1)), dim3(256), 0, s, envs, slot, cidx);
-  hipLaunchKernelGGL(k_crev_sort, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
+  if (ordered) hipLaunchKernelGGL(k_crev_sort, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
 }

@@ -80,3 +80,10 @@ __device__ __forceinline__ int pd_xcd_block(int nblocks) {
   const int start = xcd * q + (xcd < r ? xcd : r);
   return start + idx;
 }
+
+// Slot of bond k of a particle whose bond 0 sits at `base` (layout in pd_internal.h).
+__host__ __device__ __forceinline__ uint32_t pd_slot(uint32_t base, int k, int lgG) {
+  const int G = 1 << lgG;
+  const int C = 64 >> lgG;
+  return base + 64u * (uint32_t)(k >> lgG) + (uint32_t)((k & (G - 1)) * C);
+}

@@ -45,7 +45,7 @@ template <bool FILL>
 __global__ void k_pairs(const double4* __restrict__ x, int first, int n, FamilyGrid g, double horizon,
                         const int32_t* __restrict__ cell_of, const int32_t* __restrict__ cell_start,
                         const int32_t* __restrict__ cell_count, const int32_t* __restrict__ items,
-                        const uint32_t* __restrict__ pslot0, int32_t* __restrict__ deg,
+                        const uint32_t* __restrict__ pslot0, int lgG, int32_t* __restrict__ deg,
                         int32_t* __restrict__ col, uint8_t* __restrict__ state) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -76,7 +76,7 @@ __global__ void k_pairs(const double4* __restrict__ x, int first, int n, FamilyG
           const double4 xq = x[q];
           const double a = xp.x - xq.x, b = xp.y - xq.y, cz2 = xp.z - xq.z;
           if ((a * a + b * b + cz2 * cz2) < hh) {
-            if (FILL) col[base + 64u * (uint32_t)k] = q;
+            if (FILL) col[pd_slot(base, k, lgG)] = q;
             ++k;
           }
         }
@@ -87,22 +87,23 @@ __global__ void k_pairs(const double4* __restrict__ x, int first, int n, FamilyG
     deg[p] = k;
     return;
   }
-  // insertion sort of the strided row, ascending
+  // insertion sort of the row, ascending in k
   for (int a = 1; a < k; ++a) {
-    const int v = col[base + 64u * (uint32_t)a];
+    const int v = col[pd_slot(base, a, lgG)];
     int b = a - 1;
-    while (b >= 0 && col[base + 64u * (uint32_t)b] > v) {
-      col[base + 64u * (uint32_t)(b + 1)] = col[base + 64u * (uint32_t)b];
+    while (b >= 0 && col[pd_slot(base, b, lgG)] > v) {
+      col[pd_slot(base, b + 1, lgG)] = col[pd_slot(base, b, lgG)];
       --b;
     }
-    col[base + 64u * (uint32_t)(b + 1)] = v;
+    col[pd_slot(base, b + 1, lgG)] = v;
   }
-  for (int a = 0; a < k; ++a) state[base + 64u * (uint32_t)a] = 1;
+  for (int a = 0; a < k; ++a) state[pd_slot(base, a, lgG)] = 1;
 }
 
 // weighted_volume (src/peri.jl:342-354), summed in the family's descending order
 __global__ void k_wvol(const double4* __restrict__ x, int first, int n, const uint32_t* __restrict__ pslot0,
-                       const int32_t* __restrict__ deg, const int32_t* __restrict__ col, double* __restrict__ m) {
+                       int lgG, const int32_t* __restrict__ deg, const int32_t* __restrict__ col,
+                       double* __restrict__ m) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int p = first + i;
@@ -110,7 +111,7 @@ __global__ void k_wvol(const double4* __restrict__ x, int first, int n, const ui
   const uint32_t base = pslot0[p];
   double acc = 0.0;
   for (int k = deg[p] - 1; k >= 0; --k) {
-    const int j = col[base + 64u * (uint32_t)k];
+    const int j = col[pd_slot(base, k, lgG)];
     const double4 xj = x[j];
     const double d1 = xj.x - xp.x, d2 = xj.y - xp.y, d3 = xj.z - xp.z;
     const double w = 1.0 / pd_mag(d1, d2, d3);
@@ -121,25 +122,26 @@ __global__ void k_wvol(const double4* __restrict__ x, int first, int n, const ui
 }
 
 // reverse-bond slot for every bond (rows sorted ascending: binary search)
-__global__ void k_rev(int first, int n, const uint32_t* __restrict__ pslot0, const int32_t* __restrict__ deg,
-                      const int32_t* __restrict__ col, uint32_t* __restrict__ rev) {
+__global__ void k_rev(int first, int n, const uint32_t* __restrict__ pslot0, int lgG,
+                      const int32_t* __restrict__ deg, const int32_t* __restrict__ col,
+                      uint32_t* __restrict__ rev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int p = first + i;
   const uint32_t base = pslot0[p];
   const int d = deg[p];
   for (int k = 0; k < d; ++k) {
-    const int j = col[base + 64u * (uint32_t)k];
+    const int j = col[pd_slot(base, k, lgG)];
     const uint32_t bj = pslot0[j];
     int lo = 0, hi = deg[j] - 1;
     int found = -1;
     while (lo <= hi) {
       const int mid = (lo + hi) >> 1;
-      const int v = col[bj + 64u * (uint32_t)mid];
+      const int v = col[pd_slot(bj, mid, lgG)];
       if (v == p) { found = mid; break; }
       if (v < p) lo = mid + 1; else hi = mid - 1;
     }
-    rev[base + 64u * (uint32_t)k] = (found >= 0) ? bj + 64u * (uint32_t)found : 0xffffffffu;
+    rev[pd_slot(base, k, lgG)] = (found >= 0) ? pd_slot(bj, found, lgG) : 0xffffffffu;
   }
 }
 
@@ -165,28 +167,26 @@ void pd_launch_degree(hipStream_t s, const double4* x, int first, int n, FamilyG
                       const int32_t* cell_items, int32_t* deg) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_pairs<false>, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
-                     cell_start, cell_count, cell_items, (const uint32_t*)nullptr, deg, (int32_t*)nullptr,
+                     cell_start, cell_count, cell_items, (const uint32_t*)nullptr, 0, deg, (int32_t*)nullptr,
                      (uint8_t*)nullptr);
 }
 
 void pd_launch_fill(hipStream_t s, const double4* x, int first, int n, FamilyGrid g, double horizon,
                     const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
-                    const int32_t* cell_items, const uint32_t* pslot0, const int32_t* deg,
-                    int32_t* col, uint8_t* state) {
+                    const int32_t* cell_items, const uint32_t* pslot0, int lgG, int32_t* col, uint8_t* state) {
   if (n <= 0) return;
-  (void)deg;
   hipLaunchKernelGGL(k_pairs<true>, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
-                     cell_start, cell_count, cell_items, pslot0, (int32_t*)nullptr, col, state);
+                     cell_start, cell_count, cell_items, pslot0, lgG, (int32_t*)nullptr, col, state);
 }
 
-void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0,
+void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0, int lgG,
                     const int32_t* deg, const int32_t* col, double* m) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_wvol, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, pslot0, deg, col, m);
+  hipLaunchKernelGGL(k_wvol, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, pslot0, lgG, deg, col, m);
 }
 
-void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, const int32_t* deg,
+void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* deg,
                    const int32_t* col, uint32_t* rev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_rev, dim3(blocks_for(n)), dim3(256), 0, s, first, n, pslot0, deg, col, rev);
+  hipLaunchKernelGGL(k_rev, dim3(blocks_for(n)), dim3(256), 0, s, first, n, pslot0, lgG, deg, col, rev);
 }

@@ -5,11 +5,18 @@
 //   y[2]   double4[N]  deformed positions, double-buffered across the fused
 //                      force->Verlet kernel (read y[cur], write y[cur^1])
 //   v, f   double4[N]  velocity, force density (acceleration) — .w unused
-//   bonds  SELL-64: particles are grouped in chunks of 64 consecutive ids of one
-//          block (one wavefront, one lane per particle); chunk c stores slot k of
-//          lane l at slot0[c] + 64*k + l, k < width[c] = max degree in the chunk.
-//          col int32 (neighbour id, rows sorted ascending), state uint8 (1 intact).
-//          Loads of col/state are fully coalesced (64 lanes x 4 B / 1 B).
+//   bonds  SELL-64 with G lanes per particle (G = 2^lgG, C = 64/G particles
+//          per wavefront chunk of consecutive ids of one block).  Lane
+//          L = g*C + q of a chunk serves particle first+q and its bonds
+//          k = g, g+G, g+2G, ...; bond k of that particle sits at
+//              slot0 + 64*(k/G) + (k%G)*C + q,
+//          so at every step of the bond loop the 64 lanes of a wavefront read 64
+//          consecutive slots.  col int32 (neighbour id; each particle's row is
+//          sorted ascending in k), state uint8 (1 intact).  G = 1 is one lane per
+//          particle (used by the reference-order form); G = 64 is one wavefront
+//          per particle.
+//   aux    double4[N] per-particle gather record of the fast forms:
+//          (theta, 1/m, td_norm2, C = sqrt(2 psi / td_norm2))
 //   EP     rev uint32 slot of the reverse bond, edp[2] double per slot.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -72,13 +79,17 @@ struct ContactDev {
 struct EnvDev {
   int32_t N, nblocks, ncontacts, nbc;
   int32_t precision, cur, edp_cur, has_ep;
+  int32_t lgG, pad0;         // log2(lanes per particle) of the bond layout
   double dt;
   int64_t time_step;         // completed steps (env.time_step)
   double4* x;
   double4* y[2];
   double4* v;
   double4* f;
-  float4* u32;               // FP32C: float displacement y - x (+ vol in .w)
+  // FP32C form (BB / OSB blocks): x32 = float(x - block minimum corner) with the
+  // volume in .w; aux32 = (theta, 1/m, 0, 0)
+  float4* x32;
+  float4* aux32;
   int32_t* pblock;           // block of each particle
   uint8_t* ptype;            // 0-based type-local index
   double* inv_den;           // 1 / env density (mass/volume), contacts.jl:231,235
@@ -86,6 +97,7 @@ struct EnvDev {
   int32_t* deg;              // degree per particle
   uint32_t* pslot0;          // slot of (p, k = 0); slot(p, k) = pslot0[p] + 64 k
   double* m;                 // weighted volume
+  double4* aux;              // (theta, 1/m, td_norm2, C) for the fast forms
   double* theta;
   double* td;                // EP td_norm2
   int32_t* col;

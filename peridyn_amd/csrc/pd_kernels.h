@@ -32,13 +32,20 @@ void pd_launch_degree(hipStream_t s, const double4* x, int first, int n, FamilyG
                       const int32_t* cell_items, int32_t* deg);
 void pd_launch_fill(hipStream_t s, const double4* x, int first, int n, FamilyGrid g, double horizon,
                     const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
-                    const int32_t* cell_items, const uint32_t* pslot0, const int32_t* deg,
-                    int32_t* col, uint8_t* state);
-void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0,
+                    const int32_t* cell_items, const uint32_t* pslot0, int lgG, int32_t* col, uint8_t* state);
+void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0, int lgG,
                     const int32_t* deg, const int32_t* col, double* m);
-void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, const int32_t* deg,
+void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* deg,
                    const int32_t* col, uint32_t* rev);
+void pd_launch_aux_init(hipStream_t s, const double* m, double4* aux, int n);
+// device-wide exclusive scan of n int32 (scratch: pd_scan_scratch_ints(n) ints)
+int pd_scan_scratch_ints(int n);
+void pd_scan_exclusive(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* scratch);
 
 // contacts (pd_contact.hip)
 void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDev& host_env,
-                              const ContactDev& c, int cidx);
+                              const ContactDev& c, int cidx, bool ordered);
+
+// FP32-compensated form (pd_step32.hip)
+void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
+void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags);

@@ -1,0 +1,172 @@
+// FP32-compensated form of the bond passes (PD_PREC_FP32C) for bond-based and
+// ordinary state-based blocks.
+//
+// Per bond:
+//   X  = x32_j - x32_i      reference bond vector in FP32; x32 holds positions
+//                           relative to the block's minimum corner, so lattice
+//                           offsets (and any FP32-representable ones) are exact
+//   U  = float((y_j - y_i) - X)   displacement difference formed in FP64 from the
+//                           resident FP64 positions, then rounded: its relative
+//                           error is that of FP32 on U itself, whatever the rigid
+//                           motion of the body
+//   |Y|^2 = |X|^2 + (2 X.U + U.U),  e = (2 X.U + U.U) / (|Y| + |X|),  s = e / |X|
+//                           (no |Y| - |X| cancellation), all FP32
+// Force and theta sums are Kahan-compensated FP32 per lane; the G lanes of a
+// particle are combined in FP64.  Positions, velocities, the Verlet update and
+// the weighted volume stay FP64.  Elastoplastic blocks run the FP64 passes in
+// this form: their return map uses ede = ed - edp, a small difference of
+// accumulated quantities that FP32 cannot resolve.
+#include "pd_wave.h"
+
+namespace {
+
+__device__ __forceinline__ void kahan(float& s, float& c, float v) {
+  const float yk = v - c;
+  const float t = s + yk;
+  c = (t - s) - yk;
+  s = t;
+}
+
+struct G32 {
+  float Y1, Y2, Y3, inv_x, inv_y, ext, s;
+};
+
+__device__ __forceinline__ void geom32(const float4& xp, const double4& yp, const float4& xj, const double4& yj,
+                                       G32& g) {
+  const float X1 = xj.x - xp.x, X2 = xj.y - xp.y, X3 = xj.z - xp.z;
+  const float U1 = (float)((yj.x - yp.x) - (double)X1);
+  const float U2 = (float)((yj.y - yp.y) - (double)X2);
+  const float U3 = (float)((yj.z - yp.z) - (double)X3);
+  g.Y1 = X1 + U1; g.Y2 = X2 + U2; g.Y3 = X3 + U3;
+  const float xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+  const float xu = 2.0f * (X1 * U1 + X2 * U2 + X3 * U3) + (U1 * U1 + U2 * U2 + U3 * U3);
+  const float yy2 = xx2 + xu;
+  g.inv_x = __builtin_amdgcn_rsqf(xx2);
+  const float xij = xx2 * g.inv_x;
+  g.inv_y = __builtin_amdgcn_rsqf(yy2);
+  const float yij = yy2 * g.inv_y;
+  g.ext = xu * __builtin_amdgcn_rcpf(yij + xij);
+  g.s = g.ext * g.inv_x;
+}
+
+__device__ __forceinline__ double comp_total(float s, float c, int C) {
+  return group_sum((double)s - (double)c, C);
+}
+
+}  // namespace
+
+// Dilatation theta (src/peri.jl:155-172) of OSB blocks.
+template <int DUMMY>
+__global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunks, int nchunks,
+                                                 EnvDev* __restrict__ envs) {
+  WaveCtx w;
+  if (!wave_chunk(chunks, nchunks, envs, w)) return;
+  const EnvDev& E = envs[w.ch.env];
+  const BlockDev& B = E.blocks[w.ch.block];
+  if (B.kind != PD_OSB || !B.deformed) return;
+  const float4* __restrict__ x = E.x32;
+  const double4* __restrict__ y = E.y[E.cur];
+  const int32_t* __restrict__ col = E.col;
+  const uint8_t* __restrict__ st = E.state;
+  const int p = w.active ? w.p : w.ch.first;
+  const float4 xp = x[p];
+  const double4 yp = y[p];
+  const int d = w.active ? E.deg[p] : 0;
+  const uint32_t base = w.ch.slot0 + w.lane;
+  float s = 0.0f, c = 0.0f;
+  for (int k = w.g, i = 0; k < d; k += w.G, ++i) {
+    const uint32_t slot = base + 64u * (uint32_t)i;
+    if (st[slot]) {
+      const int j = col[slot];
+      const float4 xj = x[j];
+      G32 g;
+      geom32(xp, yp, xj, y[j], g);
+      kahan(s, c, g.ext * xj.w);
+    }
+  }
+  const double tot = comp_total(s, c, w.C);
+  if (w.active && w.g == 0) {
+    const float th = (float)((3.0 * (double)E.aux32[p].y) * tot);
+    E.aux32[p].x = th;
+    E.theta[p] = (double)th;
+  }
+}
+
+// Fused force density (BB / OSB) + FP64 velocity-Verlet (finish_particle).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunks, int nchunks,
+                                                 EnvDev* __restrict__ envs, int flags) {
+  WaveCtx w;
+  if (!wave_chunk(chunks, nchunks, envs, w)) return;
+  EnvDev& E = envs[w.ch.env];
+  const BlockDev& B = E.blocks[w.ch.block];
+  const int p = w.active ? w.p : w.ch.first;
+  const double4* __restrict__ y = E.y[E.cur];
+  const double4 yp = y[p];
+  double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (B.deformed) {
+    const float4* __restrict__ x = E.x32;
+    const int32_t* __restrict__ col = E.col;
+    uint8_t* __restrict__ st = E.state;
+    const float4 xp = x[p];
+    const int d = w.active ? E.deg[p] : 0;
+    const int nt = B.ntype;
+    const int tp = E.ptype[p];
+    const float inv_den = (float)(1.0 / B.density[tp]);
+    const uint32_t base = w.ch.slot0 + w.lane;
+    float4 ap = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (KIND == PD_OSB) ap = E.aux32[p];
+    const float c11 = (float)B.c11;
+    float s1 = 0.f, s2 = 0.f, s3 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+    for (int k = w.g, i = 0; k < d; k += w.G, ++i) {
+      const uint32_t slot = base + 64u * (uint32_t)i;
+      if (!st[slot]) continue;
+      const int j = col[slot];
+      const float4 xj = x[j];
+      G32 g;
+      geom32(xp, yp, xj, y[j], g);
+      const int tj = (nt > 1) ? E.ptype[j] : 0;
+      const int ij = tp + nt * tj;
+      if ((double)g.s < B.s0[ij]) {
+        float t;
+        if (KIND == PD_BB) {
+          t = g.s * c11;
+        } else {
+          const float4 aj = E.aux32[j];
+          const float sm = ap.y + aj.y;
+          const float twm = ap.x * ap.y + aj.x * aj.y;
+          const float Kf = (float)B.K[ij], Gf = (float)B.G[ij];
+          t = (3.0f * Kf - 5.0f * Gf) * twm + 15.0f * Gf * g.ext * (g.inv_x * sm);
+        }
+        const float xx = t * g.inv_y * (xj.w * inv_den);
+        kahan(s1, c1, xx * g.Y1);
+        kahan(s2, c2, xx * g.Y2);
+        kahan(s3, c3, xx * g.Y3);
+      } else {
+        st[slot] = 0;
+      }
+    }
+    a1 = comp_total(s1, c1, w.C);
+    a2 = comp_total(s2, c2, w.C);
+    a3 = comp_total(s3, c3, w.C);
+  }
+  if (!w.active || w.g != 0) return;
+  add_contacts(E, p, yp, y, a1, a2, a3);
+  finish_particle(E, p, E.x[p], yp, a1, a2, a3, flags);
+}
+
+// ---- launch wrappers --------------------------------------------------------
+static inline int grid32(int nchunks) { return (nchunks + PD_WAVES_PER_BLOCK - 1) / PD_WAVES_PER_BLOCK; }
+
+void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_theta32<0>, dim3(grid32(n)), dim3(256), 0, s, chunks, n, envs);
+}
+
+void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags) {
+  if (n <= 0) return;
+  if (kind == PD_BB)
+    hipLaunchKernelGGL(k_force32<PD_BB>, dim3(grid32(n)), dim3(256), 0, s, chunks, n, envs, flags);
+  else
+    hipLaunchKernelGGL(k_force32<PD_OSB>, dim3(grid32(n)), dim3(256), 0, s, chunks, n, envs, flags);
+}

@@ -188,3 +188,82 @@ def test_batch_equals_alone(pd, orc):
         b = e.download()
         for u, w in zip(a, b):
             assert np.array_equal(u, w)
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+@pytest.mark.parametrize("lanes", [1, 4, 16, 64])
+def test_fp64_schedules_within_1e10(pd, orc, kind, lanes):
+    """Every bond-loop schedule (lanes per particle) of the FP64 form agrees with
+    the oracle to 1e-10 relative L2 and keeps the exact family."""
+    spec = W.tensile_bb(n=7, dt=1e-4, kind=kind, pull_per_step=0.002)
+    if kind == "ep":
+        spec["blocks"][0]["yield_stress"] = 20.0
+    o = orc.OracleEnv(spec)
+    o.run(30)
+    g = pd.Env.from_spec(spec, precision="fp64", lanes_per_particle=lanes)
+    assert g.lanes_per_particle == lanes
+    g.run(30)
+    y, v, f = g.download()
+    x = spec["blocks"][0]["x"]
+    assert rel_l2(f, o.f) <= 1e-10
+    assert rel_l2(y - x, o.y - x) <= 1e-10
+    fam, intact = g.family(0)
+    ofam, _ = orc.cal_family(x, 3.015, 200)
+    assert np.array_equal(fam, ofam)
+    assert np.array_equal(intact, o.blocks[0].intact)
+    if kind == "ep":
+        assert rel_l2(g.edp(0), o.blocks[0].edp) <= 1e-9
+
+
+def test_contact_plates_fp64_within_1e10(pd, orc):
+    spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=2)
+    o, g = run_both(pd, orc, spec, 30, nuf=5, prec="fp64")
+    y, v, f = g.download()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    assert rel_l2(f, o.f) <= 1e-10
+    assert rel_l2(y - x, o.y - x) <= 1e-10
+
+
+def test_contact_truncation_keeps_largest_ids(pd, orc):
+    """max_neighs smaller than the candidate count: the kept set is the
+    max_neighs largest block-2 ids (contacts.jl:468-469), in both forms."""
+    spec = W.plates_ep(nx=6, ny=6, nz=3, gap=-0.2, dt=1e-4, approach_per_step=0.0, exponent=1)
+    spec["contacts"][0]["distanceX"] = 3.0
+    spec["contacts"][0]["max_neighs"] = 4
+    for prec in ("exact", "fp64"):
+        o, g = run_both(pd, orc, spec, 3, nuf=1, prec=prec)
+        y, v, f = g.download()
+        if prec == "exact":
+            assert np.array_equal(f, o.f)
+        else:
+            assert rel_l2(f, o.f) <= 1e-10
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+def test_fp32c_error_vs_fp64_oracle(pd, orc, kind):
+    """FP32-compensated form against the FP64 oracle after 60 steps: relative
+    L2 displacement error <= 1e-5, force error <= 1e-4, identical bond masks
+    (tolerances stated here; measured values are printed)."""
+    spec = W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.003)
+    if kind == "ep":
+        spec["blocks"][0]["yield_stress"] = 20.0
+    o, g = run_both(pd, orc, spec, 60, prec="fp32c")
+    y, v, f = g.download()
+    x = spec["blocks"][0]["x"]
+    ed = rel_l2(y - x, o.y - x)
+    ef = rel_l2(f, o.f)
+    say(f"fp32c {kind}: displacement rel L2 {ed:.2e}, force rel L2 {ef:.2e}")
+    assert ed <= 1e-5
+    assert ef <= 1e-4
+    _, intact = g.family(0)
+    assert np.array_equal(intact, o.blocks[0].intact)
+
+
+def test_fp32c_contact_plates(pd, orc):
+    spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=2)
+    o, g = run_both(pd, orc, spec, 30, nuf=5, prec="fp32c")
+    y, v, f = g.download()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    ed = rel_l2(y - x, o.y - x)
+    say(f"fp32c plates: displacement rel L2 {ed:.2e}")
+    assert ed <= 1e-5
