@@ -7,6 +7,7 @@
 // kernel class (the batched / served form) and a single env is a set of one.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -52,6 +53,7 @@ struct Env {
   int64_t N = 0, nslots = 0, nbonds = 0;
   int64_t time_step = 0;
   int32_t cur = 0, edp_cur = 0;
+  std::vector<int32_t> perm, iperm;  // internal id -> reference index and back (empty: identity)
   std::vector<Chunk> chunks;
   Chunk* d_chunks = nullptr;
   int32_t* d_self = nullptr;  // this env's slot, for single-env bookkeeping launches
@@ -266,6 +268,68 @@ void ensure_slot(Ctx* c, Env& e) {
   c->free_slots.pop_back();
 }
 
+// ---- internal particle order ---------------------------------------------------
+// The fast forms store each block's particles along a Morton (Z-order) curve of
+// their reference positions, so that every chunk, every workgroup and every
+// XCD's contiguous share of chunks covers a compact region and its neighbour
+// gathers stay in L2.  The reference-order form keeps the reference order (its
+// summation order is defined by reference indices).  All host-facing arrays
+// stay in reference order; perm / iperm translate at the boundary.
+inline int64_t internal_of(const Env& e, int64_t r) { return e.iperm.empty() ? r : e.iperm[r]; }
+inline int64_t reference_of(const Env& e, int64_t p) { return e.perm.empty() ? p : e.perm[p]; }
+
+uint64_t morton_spread(uint64_t v) {
+  v &= 0x1fffffULL;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+void build_order(Env& e) {
+  e.perm.clear();
+  e.iperm.clear();
+  if (e.precision == PD_PREC_EXACT) return;
+  // Measured on lattice specimens the input (lattice row) order is faster: the
+  // k-th neighbours of consecutive lanes are consecutive particles, so neighbour
+  // gathers coalesce; PERIDYN_ORDER=morton selects the Z-order layout.
+  const char* ord = std::getenv("PERIDYN_ORDER");
+  if (!ord || std::string(ord) != "morton") return;
+  e.perm.resize(e.N);
+  e.iperm.resize(e.N);
+  for (auto& b : e.blocks) {
+    const int64_t n = b.d.n;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], b.x[3 * i + d]);
+        hi[d] = std::max(hi[d], b.x[3 * i + d]);
+      }
+    double cell = b.d.horizon / 2.0;
+    for (int d = 0; d < 3; ++d) cell = std::max(cell, (hi[d] - lo[d]) / 2097151.0);
+    std::vector<std::pair<uint64_t, int32_t>> key(n);
+    for (int64_t i = 0; i < n; ++i) {
+      uint64_t k = 0;
+      for (int d = 0; d < 3; ++d) {
+        const double q = std::floor((b.x[3 * i + d] - lo[d]) / cell);
+        const uint64_t qi = (uint64_t)std::min(std::max(q, 0.0), 2097151.0);
+        k |= morton_spread(qi) << d;
+      }
+      key[i] = {k, (int32_t)i};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& c) {
+                       return a.first < c.first;
+                     });
+    for (int64_t q = 0; q < n; ++q) {
+      e.perm[b.first + q] = b.first + key[q].second;
+      e.iperm[b.first + key[q].second] = (int32_t)(b.first + q);
+    }
+  }
+}
+
 void commit_env(Ctx* c, Env& e) {
   if (e.committed) throw PdError(PD_E_STATE, "env already committed");
   if (e.blocks.empty()) throw PdError(PD_E_ARG, "env has no material blocks");
@@ -282,7 +346,8 @@ void commit_env(Ctx* c, Env& e) {
     e.lgG = 0;
     while (e.lgG < 4 && (N << e.lgG) < (1LL << 19)) ++e.lgG;
   }
-  // ---- particle arrays (host staging)
+  build_order(e);
+  // ---- particle arrays (host staging, internal order)
   std::vector<double4> x4(N), y4(N), v4(N);
   std::vector<int32_t> pblock(N), bcid(N, -1);
   std::vector<uint8_t> ptype(N);
@@ -290,7 +355,7 @@ void commit_env(Ctx* c, Env& e) {
   for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
     HostBlock& b = e.blocks[bi];
     for (int64_t i = 0; i < b.d.n; ++i) {
-      const int64_t p = b.first + i;
+      const int64_t p = internal_of(e, b.first + i);
       x4[p] = make_double4(b.x[3 * i], b.x[3 * i + 1], b.x[3 * i + 2], b.vol[i]);
       y4[p] = make_double4(b.y[3 * i], b.y[3 * i + 1], b.y[3 * i + 2], 0.0);
       v4[p] = make_double4(b.v[3 * i], b.v[3 * i + 1], b.v[3 * i + 2], 0.0);
@@ -305,8 +370,8 @@ void commit_env(Ctx* c, Env& e) {
   }
   for (size_t k = 0; k < e.bcs.size(); ++k) {
     if ((int64_t)e.bcs[k].mask.size() != N) throw PdError(PD_E_ARG, "BC mask length must equal env particles");
-    for (int64_t p = 0; p < N; ++p)
-      if (e.bcs[k].mask[p]) bcid[p] = (int32_t)k;
+    for (int64_t r = 0; r < N; ++r)
+      if (e.bcs[k].mask[r]) bcid[internal_of(e, r)] = (int32_t)k;
   }
   EnvDev& D = e.dev;
   std::memset(&D, 0, sizeof(D));
@@ -402,13 +467,18 @@ void commit_env(Ctx* c, Env& e) {
     PD_HIP(hipMemcpyAsync(D.pslot0, pslot0.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s));
     D.col = dalloc<int32_t>(e, slots);
     D.state = dalloc<uint8_t>(e, slots);
+    // reference index of every internal particle: rows are ordered by it
+    std::vector<int32_t> refh(N);
+    for (int64_t p = 0; p < N; ++p) refh[p] = (int32_t)reference_of(e, p);
+    int32_t* d_ref = dalloc<int32_t>(e, N);
+    PD_HIP(hipMemcpyAsync(d_ref, refh.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
     PD_HIP(hipMemsetAsync(D.col, 0xff, sizeof(int32_t) * std::max<uint64_t>(slots, 1), s));
     PD_HIP(hipMemsetAsync(D.state, 0, std::max<uint64_t>(slots, 1), s));
     for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
       HostBlock& b = e.blocks[bi];
       FamTmp& t = tmp[bi];
       pd_launch_fill(s, D.x, b.first, (int)b.d.n, t.g, b.d.horizon, t.cell_of, t.cell_start, t.cell_count,
-                     t.items, D.pslot0, e.lgG, D.col, D.state);
+                     t.items, D.pslot0, e.lgG, d_ref, D.col, D.state);
       pd_launch_wvol(s, D.x, b.first, (int)b.d.n, D.pslot0, e.lgG, D.deg, D.col, D.m);
     }
     D.aux = dalloc<double4>(e, N);
@@ -420,7 +490,7 @@ void commit_env(Ctx* c, Env& e) {
       PD_HIP(hipMemsetAsync(D.edp[0], 0, sizeof(double) * std::max<uint64_t>(slots, 1), s));
       PD_HIP(hipMemsetAsync(D.edp[1], 0, sizeof(double) * std::max<uint64_t>(slots, 1), s));
       for (auto& b : e.blocks)
-        pd_launch_rev(s, b.first, (int)b.d.n, D.pslot0, e.lgG, D.deg, D.col, D.rev);
+        pd_launch_rev(s, b.first, (int)b.d.n, D.pslot0, e.lgG, d_ref, D.deg, D.col, D.rev);
     }
     if (e.precision == PD_PREC_FP32C) {
       // x32 relative to each block's minimum corner (bonds never leave a block)
@@ -433,7 +503,7 @@ void commit_env(Ctx* c, Env& e) {
         for (int64_t i = 0; i < b.d.n; ++i)
           for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], b.x[3 * i + d]);
         for (int64_t i = 0; i < b.d.n; ++i) {
-          const int64_t p = b.first + i;
+          const int64_t p = internal_of(e, b.first + i);
           x32[p] = make_float4((float)(b.x[3 * i] - lo[0]), (float)(b.x[3 * i + 1] - lo[1]),
                                (float)(b.x[3 * i + 2] - lo[2]), (float)b.vol[i]);
         }
@@ -509,12 +579,15 @@ void commit_env(Ctx* c, Env& e) {
     std::vector<int64_t> types(N);
     for (auto& b : e.blocks)
       for (int64_t i = 0; i < b.d.n; ++i) types[b.first + i] = b.type[i];
-    for (int64_t p = 0; p < N; ++p) {
-      const bool in1 = std::find(hc.t1.begin(), hc.t1.end(), types[p]) != hc.t1.end();
-      const bool in2 = std::find(hc.t2.begin(), hc.t2.end(), types[p]) != hc.t2.end();
+    // member ids follow the reference order (the type masks of contacts.jl:212-219);
+    // idx maps a member to its internal particle id
+    for (int64_t r = 0; r < N; ++r) {
+      const bool in1 = std::find(hc.t1.begin(), hc.t1.end(), types[r]) != hc.t1.end();
+      const bool in2 = std::find(hc.t2.begin(), hc.t2.end(), types[r]) != hc.t2.end();
       if (in1 && in2) throw PdError(PD_E_ARG, "contact pair type sets must be disjoint");
-      if (in1) { inv1[p] = (int32_t)idx1.size(); idx1.push_back((int32_t)p); }
-      if (in2) { inv2[p] = (int32_t)idx2.size(); idx2.push_back((int32_t)p); }
+      const int32_t p = (int32_t)internal_of(e, r);
+      if (in1) { inv1[p] = (int32_t)idx1.size(); idx1.push_back(p); }
+      if (in2) { inv2[p] = (int32_t)idx2.size(); idx2.push_back(p); }
     }
     cd.n1 = (int32_t)idx1.size();
     cd.n2 = (int32_t)idx2.size();
@@ -702,21 +775,28 @@ void force_pass(Ctx* c, Env& e, const std::vector<Chunk>& sub, int flags_extra) 
   refresh_latch(c, {e.id});
 }
 
-void download_xyz(Ctx* c, const double4* src, int64_t N, double* out) {
+// 3 x N reference-order host array <-> internal-order double4 device array
+void download_xyz(Ctx* c, const Env& e, const double4* src, double* out) {
+  const int64_t N = e.N;
   std::vector<double4> h(N);
   PD_HIP(hipMemcpyAsync(h.data(), src, sizeof(double4) * N, hipMemcpyDeviceToHost, c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
-  for (int64_t p = 0; p < N; ++p) { out[3 * p] = h[p].x; out[3 * p + 1] = h[p].y; out[3 * p + 2] = h[p].z; }
+  for (int64_t p = 0; p < N; ++p) {
+    const int64_t r = reference_of(e, p);
+    out[3 * r] = h[p].x; out[3 * r + 1] = h[p].y; out[3 * r + 2] = h[p].z;
+  }
 }
 
-void upload_xyz(Ctx* c, double4* dst, int64_t N, const double* in, bool keep_w) {
+void upload_xyz(Ctx* c, const Env& e, double4* dst, const double* in, bool keep_w) {
+  const int64_t N = e.N;
   std::vector<double4> h(N);
   if (keep_w) {
     PD_HIP(hipMemcpyAsync(h.data(), dst, sizeof(double4) * N, hipMemcpyDeviceToHost, c->stream));
     PD_HIP(hipStreamSynchronize(c->stream));
   }
   for (int64_t p = 0; p < N; ++p) {
-    h[p].x = in[3 * p]; h[p].y = in[3 * p + 1]; h[p].z = in[3 * p + 2];
+    const int64_t r = reference_of(e, p);
+    h[p].x = in[3 * r]; h[p].y = in[3 * r + 1]; h[p].z = in[3 * r + 2];
     if (!keep_w) h[p].w = 0.0;
   }
   PD_HIP(hipMemcpyAsync(dst, h.data(), sizeof(double4) * N, hipMemcpyHostToDevice, c->stream));
@@ -744,6 +824,20 @@ void download_bonds(Ctx* c, Env& e, BondsHost& h) {
 HostBlock& get_block(Env& e, int64_t b) {
   if (b < 0 || b >= (int64_t)e.blocks.size()) throw PdError(PD_E_ARG, "block index out of range");
   return e.blocks[b];
+}
+
+// Bonds of reference particle r as (reference neighbour index, slot), in the
+// family's descending order (general_material.jl:88-90).
+void row_of(const Env& e, const BondsHost& h, int64_t r, std::vector<std::pair<int64_t, uint32_t>>& out) {
+  const int64_t p = internal_of(e, r);
+  const int d = h.deg[p];
+  out.clear();
+  for (int k = 0; k < d; ++k) {
+    const uint32_t slot = pd_slot(h.pslot0[p], k, e.lgG);
+    out.push_back({reference_of(e, h.col[slot]), slot});
+  }
+  std::sort(out.begin(), out.end(),
+            [](const std::pair<int64_t, uint32_t>& a, const std::pair<int64_t, uint32_t>& b) { return a.first > b.first; });
 }
 
 thread_local std::string g_err;
@@ -968,16 +1062,16 @@ int pd_block_family_export(void* ctx, int64_t env, int64_t block, int64_t* famil
     if (rows < b.maxdeg + 1) throw PdError(PD_E_ARG, "rows must be >= max degree + 1");
     BondsHost h;
     download_bonds(c, e, h);
+    std::vector<std::pair<int64_t, uint32_t>> row;
     for (int64_t i = 0; i < b.d.n; ++i) {
-      const int64_t p = b.first + i;
-      const int d = h.deg[p];
+      row_of(e, h, b.first + i, row);
+      const int64_t d = (int64_t)row.size();
       for (int64_t r = 0; r < rows; ++r) {
         int64_t v = 0;
         uint8_t s = 0;
         if (r < d) {
-          const uint32_t slot = pd_slot(h.pslot0[p], d - 1 - r, e.lgG);
-          v = (int64_t)h.col[slot] - b.first + 1;
-          s = h.state[slot];
+          v = row[r].first - b.first + 1;
+          s = h.state[row[r].second];
         }
         if (family) family[i * rows + r] = v;
         if (intact) intact[i * rows + r] = s;
@@ -987,8 +1081,10 @@ int pd_block_family_export(void* ctx, int64_t env, int64_t block, int64_t* famil
 }
 
 static void download_block_vec(Ctx* c, Env& e, HostBlock& b, const double* src, double* out) {
-  PD_HIP(hipMemcpyAsync(out, src + b.first, sizeof(double) * b.d.n, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> h(e.N);
+  PD_HIP(hipMemcpyAsync(h.data(), src, sizeof(double) * e.N, hipMemcpyDeviceToHost, c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < b.d.n; ++i) out[i] = h[internal_of(e, b.first + i)];
 }
 
 int pd_block_weighted_volume(void* ctx, int64_t env, int64_t block, double* m) {
@@ -1023,11 +1119,11 @@ int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t row
     PD_HIP(hipMemcpyAsync(ed.data(), e.dev.edp[e.edp_cur], sizeof(double) * e.nslots, hipMemcpyDeviceToHost,
                           c->stream));
     PD_HIP(hipStreamSynchronize(c->stream));
+    std::vector<std::pair<int64_t, uint32_t>> row;
     for (int64_t i = 0; i < b.d.n; ++i) {
-      const int64_t p = b.first + i;
-      const int d = h.deg[p];
-      for (int64_t r = 0; r < rows; ++r)
-        edp[i * rows + r] = (r < d) ? ed[pd_slot(h.pslot0[p], d - 1 - r, e.lgG)] : 0.0;
+      row_of(e, h, b.first + i, row);
+      const int64_t d = (int64_t)row.size();
+      for (int64_t r = 0; r < rows; ++r) edp[i * rows + r] = (r < d) ? ed[row[r].second] : 0.0;
     }
   });
 }
@@ -1093,9 +1189,9 @@ int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f) {
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
-    if (y) upload_xyz(c, e.dev.y[e.cur], e.N, y, false);
+    if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
     force_pass(c, e, e.chunks, 0);
-    if (f) download_xyz(c, e.dev.f, e.N, f);
+    if (f) download_xyz(c, e, e.dev.f, f);
   });
 }
 
@@ -1107,7 +1203,7 @@ int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, dou
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     HostBlock& b = get_block(e, block);
     if (lo != b.first + 1 || hi != b.first + b.d.n) throw PdError(PD_E_ARG, "limits do not match the block");
-    if (y) upload_xyz(c, e.dev.y[e.cur], e.N, y, false);
+    if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
     std::vector<Chunk> sub(e.chunks.begin() + b.chunk0, e.chunks.begin() + b.chunk0 + b.nchunks);
     // contacts are not part of force_density_T!: zero them out of this pass
     EnvDev saved = e.dev;
@@ -1124,7 +1220,7 @@ int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, dou
     upload_env_table(c, e);
     PD_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> fb(3 * e.N);
-    download_xyz(c, e.dev.f, e.N, fb.data());
+    download_xyz(c, e, e.dev.f, fb.data());
     for (int64_t i = 0; i < b.d.n; ++i)
       for (int d = 0; d < 3; ++d) f[3 * (b.first + i) + d] += fb[3 * (b.first + i) + d];
   });
@@ -1146,9 +1242,9 @@ int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f) {
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     PD_HIP(hipStreamSynchronize(c->stream));
-    if (y) download_xyz(c, e.dev.y[e.cur], e.N, y);
-    if (v) download_xyz(c, e.dev.v, e.N, v);
-    if (f) download_xyz(c, e.dev.f, e.N, f);
+    if (y) download_xyz(c, e, e.dev.y[e.cur], y);
+    if (v) download_xyz(c, e, e.dev.v, v);
+    if (f) download_xyz(c, e, e.dev.f, f);
   });
 }
 
@@ -1158,9 +1254,9 @@ int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, cons
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     PD_HIP(hipStreamSynchronize(c->stream));
-    if (y) upload_xyz(c, e.dev.y[e.cur], e.N, y, false);
-    if (v) upload_xyz(c, e.dev.v, e.N, v, false);
-    if (f) upload_xyz(c, e.dev.f, e.N, f, false);
+    if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
+    if (v) upload_xyz(c, e, e.dev.v, v, false);
+    if (f) upload_xyz(c, e, e.dev.f, f, false);
   });
 }
 
@@ -1175,7 +1271,7 @@ int pd_env_damage(void* ctx, int64_t env, double* damage) {
       int64_t cnt = 0;
       for (int k = 0; k < h.deg[p]; ++k) cnt += h.state[pd_slot(h.pslot0[p], k, e.lgG)];
       // cal_damage: 1 .- intact ./ intact0 (intact0 = initial family size)
-      damage[p] = 1.0 - (double)cnt / (double)h.deg[p];
+      damage[reference_of(e, p)] = 1.0 - (double)cnt / (double)h.deg[p];
     }
   });
 }
@@ -1212,7 +1308,7 @@ int pd_env_check_finite(void* ctx, int64_t env) {
     const double4* srcs[3] = {e.dev.y[e.cur], e.dev.v, e.dev.f};
     const char* names[3] = {"y", "v", "f"};
     for (int q = 0; q < 3; ++q) {
-      download_xyz(c, srcs[q], e.N, a.data());
+      download_xyz(c, e, srcs[q], a.data());
       for (double x : a)
         if (!std::isfinite(x)) throw PdError(PD_E_NONFINITE, std::string("nan/inf in env.") + names[q]);
     }

@@ -1,7 +1,7 @@
 // Device helpers shared by the kernels.  The library is compiled with
 // -ffp-contract=off: every a*b+c below rounds twice, as in the reference's Julia
-// code (no @fastmath / @muladd there), and IEEE FP64 division and square root
-// are used throughout (no -ffast-math, no rcp/rsq shortcuts in the FP64 forms).
+// code (no @fastmath / @muladd there).  The reference-order (exact) form uses
+// IEEE FP64 division and square root only; the fast forms use pd_rsqrt/pd_rcp.
 #pragma once
 #include "pd_internal.h"
 
@@ -79,6 +79,55 @@ __device__ __forceinline__ int pd_xcd_block(int nblocks) {
   const int xcd = b % 8, idx = b / 8;
   const int start = xcd * q + (xcd < r ? xcd : r);
   return start + idx;
+}
+
+// ---- global-memory accessors ------------------------------------------------
+// The array pointers live in the device-side EnvDev table, so the compiler
+// cannot prove they address global memory and would emit generic (flat)
+// loads; these casts make the hot loops use global_load/global_store.
+typedef double pd_d4v __attribute__((ext_vector_type(4)));
+typedef float pd_f4v __attribute__((ext_vector_type(4)));
+#define PD_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ T gld(const T* p, uint32_t i) {
+  return ((const PD_GLOBAL T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, uint32_t i, T v) {
+  ((PD_GLOBAL T*)p)[i] = v;
+}
+__device__ __forceinline__ double4 gld4(const double4* p, uint32_t i) {
+  const pd_d4v v = ((const PD_GLOBAL pd_d4v*)p)[i];
+  return make_double4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 gld4(const float4* p, uint32_t i) {
+  const pd_f4v v = ((const PD_GLOBAL pd_f4v*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst4(double4* p, uint32_t i, double4 v) {
+  pd_d4v w;
+  w.x = v.x; w.y = v.y; w.z = v.z; w.w = v.w;
+  ((PD_GLOBAL pd_d4v*)p)[i] = w;
+}
+
+// ---- fast-form FP64 reciprocals (not used by the reference-order form) -------
+// Hardware estimates refined by two Newton-Raphson steps: ~1 ulp, far inside the
+// 1e-10 agreement of the FP64 form, at a fraction of the cost of the
+// correctly-rounded division and square-root sequences.
+__device__ __forceinline__ double pd_rsqrt(double a) {
+  double r = __builtin_amdgcn_rsq(a);
+  double e = __builtin_fma(-a * r, r, 1.0);
+  r = __builtin_fma(0.5 * r, e, r);
+  e = __builtin_fma(-a * r, r, 1.0);
+  return __builtin_fma(0.5 * r, e, r);
+}
+__device__ __forceinline__ double pd_rcp(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = __builtin_fma(-a, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-a, r, 1.0);
+  return __builtin_fma(r, e, r);
 }
 
 // Slot of bond k of a particle whose bond 0 sits at `base` (layout in pd_internal.h).

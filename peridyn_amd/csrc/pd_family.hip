@@ -45,8 +45,8 @@ template <bool FILL>
 __global__ void k_pairs(const double4* __restrict__ x, int first, int n, FamilyGrid g, double horizon,
                         const int32_t* __restrict__ cell_of, const int32_t* __restrict__ cell_start,
                         const int32_t* __restrict__ cell_count, const int32_t* __restrict__ items,
-                        const uint32_t* __restrict__ pslot0, int lgG, int32_t* __restrict__ deg,
-                        int32_t* __restrict__ col, uint8_t* __restrict__ state) {
+                        const uint32_t* __restrict__ pslot0, int lgG, const int32_t* __restrict__ ref,
+                        int32_t* __restrict__ deg, int32_t* __restrict__ col, uint8_t* __restrict__ state) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int p = first + i;
@@ -87,11 +87,13 @@ __global__ void k_pairs(const double4* __restrict__ x, int first, int n, FamilyG
     deg[p] = k;
     return;
   }
-  // insertion sort of the row, ascending in k
+  // insertion sort of the row by reference index, ascending in k (so that a
+  // descending walk over k visits the reference family order)
   for (int a = 1; a < k; ++a) {
     const int v = col[pd_slot(base, a, lgG)];
+    const int rv = ref[v];
     int b = a - 1;
-    while (b >= 0 && col[pd_slot(base, b, lgG)] > v) {
+    while (b >= 0 && ref[col[pd_slot(base, b, lgG)]] > rv) {
       col[pd_slot(base, b + 1, lgG)] = col[pd_slot(base, b, lgG)];
       --b;
     }
@@ -123,8 +125,8 @@ __global__ void k_wvol(const double4* __restrict__ x, int first, int n, const ui
 
 // reverse-bond slot for every bond (rows sorted ascending: binary search)
 __global__ void k_rev(int first, int n, const uint32_t* __restrict__ pslot0, int lgG,
-                      const int32_t* __restrict__ deg, const int32_t* __restrict__ col,
-                      uint32_t* __restrict__ rev) {
+                      const int32_t* __restrict__ ref, const int32_t* __restrict__ deg,
+                      const int32_t* __restrict__ col, uint32_t* __restrict__ rev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int p = first + i;
@@ -135,11 +137,12 @@ __global__ void k_rev(int first, int n, const uint32_t* __restrict__ pslot0, int
     const uint32_t bj = pslot0[j];
     int lo = 0, hi = deg[j] - 1;
     int found = -1;
+    const int rp = ref[p];
     while (lo <= hi) {
       const int mid = (lo + hi) >> 1;
-      const int v = col[pd_slot(bj, mid, lgG)];
-      if (v == p) { found = mid; break; }
-      if (v < p) lo = mid + 1; else hi = mid - 1;
+      const int v = ref[col[pd_slot(bj, mid, lgG)]];
+      if (v == rp) { found = mid; break; }
+      if (v < rp) lo = mid + 1; else hi = mid - 1;
     }
     rev[pd_slot(base, k, lgG)] = (found >= 0) ? pd_slot(bj, found, lgG) : 0xffffffffu;
   }
@@ -167,16 +170,17 @@ void pd_launch_degree(hipStream_t s, const double4* x, int first, int n, FamilyG
                       const int32_t* cell_items, int32_t* deg) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_pairs<false>, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
-                     cell_start, cell_count, cell_items, (const uint32_t*)nullptr, 0, deg, (int32_t*)nullptr,
-                     (uint8_t*)nullptr);
+                     cell_start, cell_count, cell_items, (const uint32_t*)nullptr, 0, (const int32_t*)nullptr, deg,
+                     (int32_t*)nullptr, (uint8_t*)nullptr);
 }
 
 void pd_launch_fill(hipStream_t s, const double4* x, int first, int n, FamilyGrid g, double horizon,
                     const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
-                    const int32_t* cell_items, const uint32_t* pslot0, int lgG, int32_t* col, uint8_t* state) {
+                    const int32_t* cell_items, const uint32_t* pslot0, int lgG, const int32_t* ref, int32_t* col,
+                    uint8_t* state) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_pairs<true>, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
-                     cell_start, cell_count, cell_items, pslot0, lgG, (int32_t*)nullptr, col, state);
+                     cell_start, cell_count, cell_items, pslot0, lgG, ref, (int32_t*)nullptr, col, state);
 }
 
 void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0, int lgG,
@@ -185,8 +189,8 @@ void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uin
   hipLaunchKernelGGL(k_wvol, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, pslot0, lgG, deg, col, m);
 }
 
-void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* deg,
-                   const int32_t* col, uint32_t* rev) {
+void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* ref,
+                   const int32_t* deg, const int32_t* col, uint32_t* rev) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_rev, dim3(blocks_for(n)), dim3(256), 0, s, first, n, pslot0, lgG, deg, col, rev);
+  hipLaunchKernelGGL(k_rev, dim3(blocks_for(n)), dim3(256), 0, s, first, n, pslot0, lgG, ref, deg, col, rev);
 }

@@ -32,11 +32,12 @@ void pd_launch_degree(hipStream_t s, const double4* x, int first, int n, FamilyG
                       const int32_t* cell_items, int32_t* deg);
 void pd_launch_fill(hipStream_t s, const double4* x, int first, int n, FamilyGrid g, double horizon,
                     const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
-                    const int32_t* cell_items, const uint32_t* pslot0, int lgG, int32_t* col, uint8_t* state);
+                    const int32_t* cell_items, const uint32_t* pslot0, int lgG, const int32_t* ref, int32_t* col,
+                    uint8_t* state);
 void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0, int lgG,
                     const int32_t* deg, const int32_t* col, double* m);
-void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* deg,
-                   const int32_t* col, uint32_t* rev);
+void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* ref,
+                   const int32_t* deg, const int32_t* col, uint32_t* rev);
 void pd_launch_aux_init(hipStream_t s, const double* m, double4* aux, int n);
 // device-wide exclusive scan of n int32 (scratch: pd_scan_scratch_ints(n) ints)
 int pd_scan_scratch_ints(int n);

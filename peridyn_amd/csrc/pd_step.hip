@@ -100,15 +100,33 @@ __global__ __launch_bounds__(256) void k_theta(const Chunk* __restrict__ chunks,
     E.aux[p].x = th;
   } else {
     double acc = 0.0;
-    for (int k = w.g, i = 0; k < d; k += w.G, ++i) {
-      const uint32_t slot = base + 64u * (uint32_t)i;
-      if (st[slot]) {
-        const int j = col[slot];
-        const double4 xj = x[j], yj = y[j];
-        const double X1 = xj.x - xp.x, X2 = xj.y - xp.y, X3 = xj.z - xp.z;
-        const double Y1 = yj.x - yp.x, Y2 = yj.y - yp.y, Y3 = yj.z - yp.z;
-        const double ext = pd_mag(Y1, Y2, Y3) - pd_mag(X1, X2, X3);
-        acc += ext * xj.w;
+    constexpr int U = 4;  // batched loads, as in material_fast
+    const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
+    for (int i0 = 0; i0 < ni; i0 += U) {
+      int jv[U];
+      uint8_t sv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = (i0 + u) < ni;
+        const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+        jv[u] = ok ? gld(col, slot) : p;
+        sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+      }
+      double4 xj[U], yj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xj[u] = gld4(x, jv[u]);
+        yj[u] = gld4(y, jv[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!sv[u]) continue;
+        const double X1 = xj[u].x - xp.x, X2 = xj[u].y - xp.y, X3 = xj[u].z - xp.z;
+        const double Y1 = yj[u].x - yp.x, Y2 = yj[u].y - yp.y, Y3 = yj[u].z - yp.z;
+        const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+        const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+        const double ext = yy2 * pd_rsqrt(yy2) - xx2 * pd_rsqrt(xx2);
+        acc += ext * xj[u].w;
       }
     }
     acc = group_sum(acc, w.C);
@@ -179,27 +197,49 @@ __global__ __launch_bounds__(256) void k_td(const Chunk* __restrict__ chunks, in
   } else {
     const double4 ap = E.aux[p];  // (theta, 1/m, ., .)
     double acc = 0.0;
-    for (int k = w.g, i = 0; k < d; k += w.G, ++i) {
-      const uint32_t slot = base + 64u * (uint32_t)i;
-      if (!st[slot]) continue;
-      const int j = col[slot];
-      const double4 xj = x[j], yj = y[j];
-      const double X1 = xj.x - xp.x, X2 = xj.y - xp.y, X3 = xj.z - xp.z;
-      const double Y1 = yj.x - yp.x, Y2 = yj.y - yp.y, Y3 = yj.z - yp.z;
-      const double xij = pd_mag(X1, X2, X3);
-      const double ext = pd_mag(Y1, Y2, Y3) - xij;
-      const double inv_x = 1.0 / xij;
-      const int tj = (nt > 1) ? E.ptype[j] : 0;
-      if (ext * inv_x < B.s0[tp + nt * tj]) {
-        const double4 aj = E.aux[j];
-        const double sm = ap.y + aj.y;
-        const double twm = ap.x * ap.y + aj.x * aj.y;
-        const double ed = ext - (twm / sm) * xij * (1.0 / 3.0);
-        const double td = (15.0 * B.G[tp + nt * tj]) * (ed - edp[slot]) * (inv_x * sm);
-        acc += td * td * xj.w;
-      } else {
-        st[slot] = 0;
-        edp_o[slot] = edp[slot];
+    constexpr int U = 2;  // batched loads, as in material_fast
+    const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
+    for (int i0 = 0; i0 < ni; i0 += U) {
+      int jv[U];
+      uint8_t sv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = (i0 + u) < ni;
+        const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+        jv[u] = ok ? gld(col, slot) : p;
+        sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+      }
+      double4 xj[U], yj[U], aj[U];
+      double eo[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xj[u] = gld4(x, jv[u]);
+        yj[u] = gld4(y, jv[u]);
+        aj[u] = gld4(E.aux, jv[u]);
+        eo[u] = sv[u] ? gld(edp, base + 64u * (uint32_t)(i0 + u)) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!sv[u]) continue;
+        const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+        const double X1 = xj[u].x - xp.x, X2 = xj[u].y - xp.y, X3 = xj[u].z - xp.z;
+        const double Y1 = yj[u].x - yp.x, Y2 = yj[u].y - yp.y, Y3 = yj[u].z - yp.z;
+        const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+        const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+        const double inv_x = pd_rsqrt(xx2);
+        const double xij = xx2 * inv_x;
+        const double ext = yy2 * pd_rsqrt(yy2) - xij;
+        const int tj = (nt > 1) ? gld(E.ptype, jv[u]) : 0;
+        if (ext * inv_x < B.s0[tp + nt * tj]) {
+          const double sm = ap.y + aj[u].y;
+          const double twm = ap.x * ap.y + aj[u].x * aj[u].y;
+          const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
+          const double td = (15.0 * B.G[tp + nt * tj]) * (ed - eo[u]) * (inv_x * sm);
+          acc += td * td * xj[u].w;
+        } else {
+          gst(st, slot, (uint8_t)0);
+          gst(edp_o, slot, eo[u]);
+        }
       }
     }
     acc = group_sum(acc, w.C);
@@ -357,58 +397,92 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
     edp_out = E.edp[E.edp_cur ^ 1];
   }
   const double c11 = B.c11;
-  for (int k = w.g, i = 0; k < d; k += w.G, ++i) {
-    const uint32_t slot = base + 64u * (uint32_t)i;
-    if (!st[slot]) continue;
-    const int j = col[slot];
-    const double4 xj = x[j], yj = y[j];
-    const double X1 = xj.x - xp.x, X2 = xj.y - xp.y, X3 = xj.z - xp.z;
-    const double Y1 = yj.x - yp.x, Y2 = yj.y - yp.y, Y3 = yj.z - yp.z;
-    const double xij = pd_mag(X1, X2, X3);
-    const double yij = pd_mag(Y1, Y2, Y3);
-    const double ext = yij - xij;
-    const double inv_x = 1.0 / xij;
-    const int tj = (nt > 1) ? E.ptype[j] : 0;
-    const int ij = tp + nt * tj;
-    if (ext * inv_x < B.s0[ij]) {
-      double t;
-      if (KIND == PD_BB) {
-        t = ext * inv_x * c11;
-      } else {
-        const double4 aj = E.aux[j];
-        const double sm = ap.y + aj.y;
-        const double twm = ap.x * ap.y + aj.x * aj.y;
-        if (KIND == PD_OSB) {
-          t = (3.0 * B.K[ij] - 5.0 * B.G[ij]) * twm + 15.0 * B.G[ij] * ext * (inv_x * sm);
-        } else {
-          const double ed = ext - (twm / sm) * xij * (1.0 / 3.0);
-          const double g15 = 15.0 * B.G[ij] * (inv_x * sm);
-          double Cp = ap.w, Cj = aj.w;
-          if (nt > 1) {
-            Cp = __builtin_sqrt((2.0 * B.psi[ij]) / ap.z);
-            Cj = __builtin_sqrt((2.0 * B.psi[ij]) / aj.z);
-          }
-          const double e_own = edp_in[slot];
-          const double ede = ed - e_own;
-          double td_own = g15 * ede;
-          double e_new = e_own;
-          if (!(Cp > 1.0)) {
-            td_own *= Cp;
-            e_new = e_own + ede * (1.0 - Cp);
-          }
-          edp_out[slot] = e_new;
-          double td_rev = g15 * (ed - edp_in[E.rev[slot]]);
-          if (!(Cj > 1.0)) td_rev *= Cj;
-          t = 3.0 * B.K[ij] * twm + 0.5 * (td_own + td_rev);
-        }
+  // Bonds in batches of U: all column/state (and reverse-slot) loads of the
+  // batch first, then all neighbour gathers, then the arithmetic, so that a
+  // lane keeps 2U independent memory requests in flight instead of walking a
+  // state -> column -> position dependency chain per bond.
+  constexpr int U = (KIND == PD_EP) ? 2 : 4;
+  const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
+  const double s0_pp = B.s0[tp + nt * tp];
+  for (int i0 = 0; i0 < ni; i0 += U) {
+    int jv[U];
+    uint8_t sv[U];
+    uint32_t rv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = (i0 + u) < ni;
+      const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+      jv[u] = ok ? gld(col, slot) : p;
+      sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+      if (KIND == PD_EP) rv[u] = (ok && sv[u]) ? gld(E.rev, slot) : slot;
+    }
+    double4 xj[U], yj[U], aj[U];
+    double eo[U], er[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      xj[u] = gld4(x, jv[u]);
+      yj[u] = gld4(y, jv[u]);
+      if (KIND != PD_BB) aj[u] = gld4(E.aux, jv[u]);
+      if (KIND == PD_EP) {
+        const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+        eo[u] = sv[u] ? gld(edp_in, slot) : 0.0;
+        er[u] = sv[u] ? gld(edp_in, rv[u]) : 0.0;
       }
-      const double xx = t * (1.0 / yij) * (xj.w * inv_den);
-      a1 += xx * Y1;
-      a2 += xx * Y2;
-      a3 += xx * Y3;
-    } else {
-      st[slot] = 0;
-      if (KIND == PD_EP) edp_out[slot] = edp_in[slot];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!sv[u]) continue;
+      const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+      const int j = jv[u];
+      const double X1 = xj[u].x - xp.x, X2 = xj[u].y - xp.y, X3 = xj[u].z - xp.z;
+      const double Y1 = yj[u].x - yp.x, Y2 = yj[u].y - yp.y, Y3 = yj[u].z - yp.z;
+      const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+      const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+      const double inv_x = pd_rsqrt(xx2);
+      const double inv_y = pd_rsqrt(yy2);
+      const double xij = xx2 * inv_x;
+      const double yij = yy2 * inv_y;
+      const double ext = yij - xij;
+      const int tj = (nt > 1) ? gld(E.ptype, j) : tp;
+      const int ij = tp + nt * tj;
+      if (ext * inv_x < ((nt > 1) ? B.s0[ij] : s0_pp)) {
+        double t;
+        if (KIND == PD_BB) {
+          t = ext * inv_x * c11;
+        } else {
+          const double sm = ap.y + aj[u].y;
+          const double twm = ap.x * ap.y + aj[u].x * aj[u].y;
+          if (KIND == PD_OSB) {
+            t = (3.0 * B.K[ij] - 5.0 * B.G[ij]) * twm + 15.0 * B.G[ij] * ext * (inv_x * sm);
+          } else {
+            const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
+            const double g15 = 15.0 * B.G[ij] * (inv_x * sm);
+            double Cp = ap.w, Cj = aj[u].w;
+            if (nt > 1) {
+              Cp = __builtin_sqrt((2.0 * B.psi[ij]) / ap.z);
+              Cj = __builtin_sqrt((2.0 * B.psi[ij]) / aj[u].z);
+            }
+            const double ede = ed - eo[u];
+            double td_own = g15 * ede;
+            double e_new = eo[u];
+            if (!(Cp > 1.0)) {
+              td_own *= Cp;
+              e_new = eo[u] + ede * (1.0 - Cp);
+            }
+            gst(edp_out, slot, e_new);
+            double td_rev = g15 * (ed - er[u]);
+            if (!(Cj > 1.0)) td_rev *= Cj;
+            t = 3.0 * B.K[ij] * twm + 0.5 * (td_own + td_rev);
+          }
+        }
+        const double xx = t * inv_y * (xj[u].w * inv_den);
+        a1 += xx * Y1;
+        a2 += xx * Y2;
+        a3 += xx * Y3;
+      } else {
+        gst(st, slot, (uint8_t)0);
+        if (KIND == PD_EP) gst(edp_out, slot, eo[u]);
+      }
     }
   }
 }

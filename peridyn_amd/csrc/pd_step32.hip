@@ -74,14 +74,31 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
   const int d = w.active ? E.deg[p] : 0;
   const uint32_t base = w.ch.slot0 + w.lane;
   float s = 0.0f, c = 0.0f;
-  for (int k = w.g, i = 0; k < d; k += w.G, ++i) {
-    const uint32_t slot = base + 64u * (uint32_t)i;
-    if (st[slot]) {
-      const int j = col[slot];
-      const float4 xj = x[j];
+  constexpr int U = 4;  // batched loads (column/state, then gathers, then math)
+  const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
+  for (int i0 = 0; i0 < ni; i0 += U) {
+    int jv[U];
+    uint8_t sv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = (i0 + u) < ni;
+      const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+      jv[u] = ok ? gld(col, slot) : p;
+      sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+    }
+    float4 xj[U];
+    double4 yj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      xj[u] = gld4(x, jv[u]);
+      yj[u] = gld4(y, jv[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!sv[u]) continue;
       G32 g;
-      geom32(xp, yp, xj, y[j], g);
-      kahan(s, c, g.ext * xj.w);
+      geom32(xp, yp, xj[u], yj[u], g);
+      kahan(s, c, g.ext * xj[u].w);
     }
   }
   const double tot = comp_total(s, c, w.C);
@@ -118,32 +135,51 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
     if (KIND == PD_OSB) ap = E.aux32[p];
     const float c11 = (float)B.c11;
     float s1 = 0.f, s2 = 0.f, s3 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
-    for (int k = w.g, i = 0; k < d; k += w.G, ++i) {
-      const uint32_t slot = base + 64u * (uint32_t)i;
-      if (!st[slot]) continue;
-      const int j = col[slot];
-      const float4 xj = x[j];
-      G32 g;
-      geom32(xp, yp, xj, y[j], g);
-      const int tj = (nt > 1) ? E.ptype[j] : 0;
-      const int ij = tp + nt * tj;
-      if ((double)g.s < B.s0[ij]) {
-        float t;
-        if (KIND == PD_BB) {
-          t = g.s * c11;
+    constexpr int U = 4;  // batched loads (column/state, then gathers, then math)
+    const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
+    for (int i0 = 0; i0 < ni; i0 += U) {
+      int jv[U];
+      uint8_t sv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = (i0 + u) < ni;
+        const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+        jv[u] = ok ? gld(col, slot) : p;
+        sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+      }
+      float4 xj[U], aj[U];
+      double4 yj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xj[u] = gld4(x, jv[u]);
+        yj[u] = gld4(y, jv[u]);
+        if (KIND == PD_OSB) aj[u] = gld4(E.aux32, jv[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!sv[u]) continue;
+        const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
+        G32 g;
+        geom32(xp, yp, xj[u], yj[u], g);
+        const int tj = (nt > 1) ? E.ptype[jv[u]] : 0;
+        const int ij = tp + nt * tj;
+        if ((double)g.s < B.s0[ij]) {
+          float t;
+          if (KIND == PD_BB) {
+            t = g.s * c11;
+          } else {
+            const float sm = ap.y + aj[u].y;
+            const float twm = ap.x * ap.y + aj[u].x * aj[u].y;
+            const float Kf = (float)B.K[ij], Gf = (float)B.G[ij];
+            t = (3.0f * Kf - 5.0f * Gf) * twm + 15.0f * Gf * g.ext * (g.inv_x * sm);
+          }
+          const float xx = t * g.inv_y * (xj[u].w * inv_den);
+          kahan(s1, c1, xx * g.Y1);
+          kahan(s2, c2, xx * g.Y2);
+          kahan(s3, c3, xx * g.Y3);
         } else {
-          const float4 aj = E.aux32[j];
-          const float sm = ap.y + aj.y;
-          const float twm = ap.x * ap.y + aj.x * aj.y;
-          const float Kf = (float)B.K[ij], Gf = (float)B.G[ij];
-          t = (3.0f * Kf - 5.0f * Gf) * twm + 15.0f * Gf * g.ext * (g.inv_x * sm);
+          gst(st, slot, (uint8_t)0);
         }
-        const float xx = t * g.inv_y * (xj.w * inv_den);
-        kahan(s1, c1, xx * g.Y1);
-        kahan(s2, c2, xx * g.Y2);
-        kahan(s3, c3, xx * g.Y3);
-      } else {
-        st[slot] = 0;
       }
     }
     a1 = comp_total(s1, c1, w.C);
