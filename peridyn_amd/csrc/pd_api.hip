@@ -85,6 +85,9 @@ struct Ctx {
   bool set_has_ep = false, set_has_theta = false;
   int2* d_latch = nullptr;
   size_t latch_cap = 0;
+  double4* d_latch_part = nullptr;  // per-slice partial sums of the latch test
+  size_t latch_part_cap = 0;
+  int latch_nparts = 1;
   // per-kernel-class HIP-event timing on the context stream (pd_ctx_profile)
   bool prof = false;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -339,12 +342,14 @@ void commit_env(Ctx* c, Env& e) {
   e.N = N;
   // bond layout: the reference-order form walks each row on one lane; the
   // other forms spread a particle over G lanes so that small specimens still
-  // fill the chip (>= ~512k lanes), unless a schedule was set explicitly
+  // fill the chip (>= 786,432 lanes = 48 waves per CU over 256 CUs; measured
+  // best on the 100k-particle EP plates (G=8) and the 721k-particle BB disk
+  // (G=2)), unless a schedule was set explicitly
   if (e.precision == PD_PREC_EXACT) {
     e.lgG = 0;
   } else if (e.lgG < 0) {
     e.lgG = 0;
-    while (e.lgG < 4 && (N << e.lgG) < (1LL << 19)) ++e.lgG;
+    while (e.lgG < 4 && (N << e.lgG) < 786432) ++e.lgG;
   }
   build_order(e);
   // ---- particle arrays (host staging, internal order)
@@ -702,13 +707,19 @@ int precision_of(Ctx* c, const std::vector<int64_t>& ids) {
 // deformed-latch work list for envs that may still hold undeformed blocks
 int prepare_latch(Ctx* c, const std::vector<int64_t>& ids) {
   std::vector<int2> items;
+  int64_t max_n = 1;
   for (int64_t id : ids) {
     Env& e = get_env(c, id);
     if (!e.maybe_undeformed) continue;
-    for (size_t b = 0; b < e.blocks.size(); ++b) items.push_back(make_int2(e.slot, (int)b));
+    for (size_t b = 0; b < e.blocks.size(); ++b) {
+      items.push_back(make_int2(e.slot, (int)b));
+      max_n = std::max<int64_t>(max_n, e.blocks[b].d.n);
+    }
   }
   if (items.empty()) return 0;
+  c->latch_nparts = pd_latch_parts(max_n);
   ensure_cap(c->d_latch, c->latch_cap, items.size());
+  ensure_cap(c->d_latch_part, c->latch_part_cap, items.size() * (size_t)c->latch_nparts);
   PD_HIP(hipMemcpyAsync(c->d_latch, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
   return (int)items.size();
@@ -754,7 +765,7 @@ void force_pass(Ctx* c, Env& e, const std::vector<Chunk>& sub, int flags_extra) 
     PD_HIP(hipMemcpyAsync(d, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
     std::vector<int64_t> one{e.id};
     const int nl = prepare_latch(c, one);
-    if (nl) pd_launch_latch(c->stream, c->d_envs, c->d_latch, nl);
+    if (nl) pd_launch_latch(c->stream, c->d_envs, c->d_latch, nl, c->latch_nparts, c->d_latch_part);
     pd_launch_theta(c->stream, e.precision, d, (int)th.size(), c->d_envs);
     pd_launch_td(c->stream, e.precision, d + o_td, (int)td.size(), c->d_envs);
     for (int k = 0; k < 3; ++k)
@@ -885,6 +896,7 @@ int pd_ctx_destroy(void* ctx) {
   if (c->d_set) (void)hipFree(c->d_set);
   if (c->d_slots) (void)hipFree(c->d_slots);
   if (c->d_latch) (void)hipFree(c->d_latch);
+  if (c->d_latch_part) (void)hipFree(c->d_latch_part);
   for (auto& r : c->pending) { c->pool.push_back(r.a); c->pool.push_back(r.b); }
   for (hipEvent_t ev : c->pool) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);
@@ -1153,7 +1165,7 @@ int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t
     pd_launch_book(s, T, c->d_slots, c->n_slots, PD_B_FLIP_Y);
     for (int64_t st = 1; st <= nsteps; ++st) {
       const bool last = (st == nsteps);
-      if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl); });
+      if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part); });
       if (c->set_has_theta)
         timed(c, PD_PROF_THETA, [&] { pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T); });
       if (c->set_has_ep) timed(c, PD_PROF_TD, [&] { pd_launch_td(s, prec, d + c->off_td, c->n_td, T); });

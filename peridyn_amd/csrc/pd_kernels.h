@@ -15,7 +15,9 @@ void pd_launch_td(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* e
 void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int n, EnvDev* envs, int flags);
 void pd_launch_prologue(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_book(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int flags);
-void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n);
+// deformed latch over n (env slot, block) items; part holds n * nparts double4
+int pd_latch_parts(int64_t max_n);
+void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n, int nparts, double4* part);
 
 // family build (pd_family.hip); all arrays device, env-local particle ids
 struct FamilyGrid {

@@ -19,6 +19,8 @@
 //                  once, both halves of the pair force are combined, and the
 //                  neighbour's (theta, 1/m, td_norm2, C) come from one 32-byte
 //                  gather of `aux`.  Agrees with the exact form to ~1e-15 per step.
+#include <algorithm>
+
 #include "pd_wave.h"
 
 namespace {
@@ -555,18 +557,43 @@ __global__ void k_book(EnvDev* __restrict__ envs, const int32_t* __restrict__ sl
 }
 
 // Deformed latch (general_material.jl:243-255): isapprox(y .- y[:,1] .+ x[:,1], x)
-// with rtol = sqrt(eps(Float64)); one workgroup per (env, block) entry.
-__global__ __launch_bounds__(1024) void k_latch(EnvDev* __restrict__ envs, const int2* __restrict__ items) {
-  const int2 it = items[blockIdx.x];
-  EnvDev& E = envs[it.x];
-  BlockDev& B = E.blocks[it.y];
-  if (B.deformed) return;
+// with rtol = sqrt(eps(Float64)).  Two kernels: k_latch_part sums |a - x|^2,
+// |a|^2 and |x|^2 over `nparts` slices of each (env, block) entry, one workgroup
+// per slice; k_latch_fin adds the slices of an entry in a fixed order and sets
+// the latch.  Latched blocks skip both.
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o);
+    b += __shfl_down(b, o);
+    c += __shfl_down(c, o);
+  }
+  __shared__ double red[3][4];
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][wv] = a; red[1][wv] = b; red[2][wv] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    b = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    c = (red[2][0] + red[2][1]) + (red[2][2] + red[2][3]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_latch_part(EnvDev* __restrict__ envs, const int2* __restrict__ items,
+                                                    int nparts, double4* __restrict__ part) {
+  const int item = blockIdx.x;  // entries on x (no 65535 limit), slices on y
+  const int2 it = items[item];
+  const EnvDev& E = envs[it.x];
+  const BlockDev& B = E.blocks[it.y];
+  if (B.deformed) return;  // uniform per workgroup
   const double4* __restrict__ y = E.y[E.cur];
   const double4* __restrict__ x = E.x;
   const double4 y1 = y[B.first], x1 = x[B.first];
+  const int per = (B.n + nparts - 1) / nparts;
+  const int lo = blockIdx.y * per;
+  const int hi = min(B.n, lo + per);
   double sd = 0.0, sa = 0.0, sx = 0.0;
-  for (int i = threadIdx.x; i < B.n; i += blockDim.x) {
-    const double4 yi = y[B.first + i], xi = x[B.first + i];
+  for (int i = lo + (int)threadIdx.x; i < hi; i += 256) {
+    const double4 yi = gld4(y, B.first + i), xi = gld4(x, B.first + i);
     const double a[3] = {(yi.x - y1.x) + x1.x, (yi.y - y1.y) + x1.y, (yi.z - y1.z) + x1.z};
     const double xx[3] = {xi.x, xi.y, xi.z};
 #pragma unroll
@@ -577,28 +604,36 @@ __global__ __launch_bounds__(1024) void k_latch(EnvDev* __restrict__ envs, const
       sx += xx[d] * xx[d];
     }
   }
-  __shared__ double red[3][1024 / 64];
+  block_sum3(sd, sa, sx);
+  if (threadIdx.x == 0) part[(size_t)item * nparts + blockIdx.y] = make_double4(sd, sa, sx, 0.0);
+}
+
+__global__ __launch_bounds__(64) void k_latch_fin(EnvDev* __restrict__ envs, const int2* __restrict__ items,
+                                                  int nparts, const double4* __restrict__ part) {
+  const int2 it = items[blockIdx.x];
+  EnvDev& E = envs[it.x];
+  BlockDev& B = E.blocks[it.y];
+  if (B.deformed) return;
+  double D = 0.0, A = 0.0, X = 0.0;
+  for (int q = threadIdx.x; q < nparts; q += 64) {
+    const double4 v = part[(size_t)blockIdx.x * nparts + q];
+    D += v.x; A += v.y; X += v.z;
+  }
   for (int o = 32; o > 0; o >>= 1) {
-    sd += __shfl_down(sd, o);
-    sa += __shfl_down(sa, o);
-    sx += __shfl_down(sx, o);
+    D += __shfl_down(D, o);
+    A += __shfl_down(A, o);
+    X += __shfl_down(X, o);
   }
-  const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) { red[0][wv] = sd; red[1][wv] = sa; red[2][wv] = sx; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double D = 0.0, A = 0.0, X = 0.0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) { D += red[0][q]; A += red[1][q]; X += red[2][q]; }
-    const double d = __builtin_sqrt(D);
-    bool approx;
-    if (__builtin_isfinite(d)) {
-      const double rtol = 1.4901161193847656e-08;  // sqrt(eps(Float64))
-      approx = d <= rtol * fmax(__builtin_sqrt(A), __builtin_sqrt(X));
-    } else {
-      approx = false;
-    }
-    if (!approx) B.deformed = 1;
+  if (threadIdx.x != 0) return;
+  const double d = __builtin_sqrt(D);
+  bool approx;
+  if (__builtin_isfinite(d)) {
+    const double rtol = 1.4901161193847656e-08;  // sqrt(eps(Float64))
+    approx = d <= rtol * fmax(__builtin_sqrt(A), __builtin_sqrt(X));
+  } else {
+    approx = false;
   }
+  if (!approx) B.deformed = 1;
 }
 
 // 1/m into aux.y after the weighted volume is built
@@ -667,9 +702,15 @@ void pd_launch_book(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, in
   hipLaunchKernelGGL(k_book, dim3((n + 255) / 256), dim3(256), 0, s, envs, slots, n, flags);
 }
 
-void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n) {
+int pd_latch_parts(int64_t max_n) {
+  const int64_t per = 256 * 16;  // particles per slice
+  return (int)std::min<int64_t>(1024, std::max<int64_t>(1, (max_n + per - 1) / per));
+}
+
+void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n, int nparts, double4* part) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_latch, dim3(n), dim3(1024), 0, s, envs, items);
+  hipLaunchKernelGGL(k_latch_part, dim3(n, nparts), dim3(256), 0, s, envs, items, nparts, part);
+  hipLaunchKernelGGL(k_latch_fin, dim3(n), dim3(64), 0, s, envs, items, nparts, (const double4*)part);
 }
 
 void pd_launch_aux_init(hipStream_t s, const double* m, double4* aux, int n) {
