@@ -1,16 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the PeriDyn explicit-dynamics hot path on one MI355X.
 
-Workload (default): two elastoplastic (ordinary state-based, Von Mises) plates of
+Default workload: two elastoplastic (ordinary state-based, Von Mises) plates of
 50x50x20 lattice particles (100,000 particles, 10,975,216 bonds) approaching at
 0.05 dx/step with a nonlinear spring contact (cutoff 0.9 dx, lists rebuilt every
 10 steps), 500 velocity-Verlet steps.  A "step" is one velocity_verlet_step!
 over the whole specimen pair; all state is resident in HBM before timing starts.
 
 Reported: bond-updates/s (initial bonds x steps / time), the HBM-roofline
-fraction of the dominant kernel (EP force pass, timed with HIP events on the
-library stream inside the timed region), and the CPU oracle timed on the host
-on a bounded sample of the same workload.
+fraction of the dominant kernel (timed with HIP events on the library stream
+inside the timed region), and the CPU oracle timed on the host on a bounded
+sample of the same workload.
+
+Other workloads (--workload):
+  disk_impact  BB disk r=150 t=10 + stiff sphere (721k particles), linear contact
+  tensile_bb   BB tensile block 20^3 (8k particles), MoveBC grips
+  batched      256 tensile specimens (BB / OSB 50/50, 2k-60k particles) stepped
+               together in one pd_run per step; compared with the same
+               specimens run one at a time in plain FP64
+  served       a Poisson stream of tensile / indentation requests through the
+               continuous-batching loop (peridyn_amd.serve); sims/s and p50/p99
+               time-to-result, compared with the same stream served one request
+               at a time in plain FP64
 """
 from __future__ import annotations
 
@@ -26,18 +37,30 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 METRIC = "bond-updates/s & % HBM roofline; served sims/s + p99 latency vs plain FP64"
+PROF_CLASSES = ("prologue", "latch", "theta", "td", "force_bb", "force_osb", "force_ep", "contact", "book")
 
-# algorithmic bytes per launch, per bond / per particle, of each kernel class
-# (FP64 forms; int32 column + uint8 state per bond, neighbour gathers counted
-# as cache hits, edp a per-bond stream; Verlet traffic folded into the fused
-# force kernel).
+# Algorithmic bytes per launch, (per bond, per particle), of each kernel class
+# and form: int32 column + uint8 state per bond, neighbour gathers counted as
+# cache hits, edp a per-bond stream; the Verlet update (read v, f, y; write v,
+# y: 120 B/particle) is fused into the force kernel.  FP32C reads a 16 B x32
+# record instead of the 32 B FP64 one (and 8 B of FP32 theta, 1/m for OSB);
+# its EP blocks run the FP64 passes.
 BYTES = {
     ("force_bb", "fp64"): (5, 204),
     ("force_osb", "fp64"): (5, 220),
     ("force_ep", "fp64"): (21, 228),
     ("theta", "fp64"): (5, 72),
     ("td", "fp64"): (13, 84),
+    ("force_bb", "fp32c"): (5, 188),
+    ("force_osb", "fp32c"): (5, 196),
+    ("force_ep", "fp32c"): (21, 228),
+    ("theta", "fp32c"): (5, 56),
+    ("td", "fp32c"): (13, 84),
 }
+
+
+def form_of(precision: str) -> str:
+    return "fp32c" if precision == "fp32c" else "fp64"
 
 
 def workload(name: str):
@@ -88,32 +111,32 @@ def traffic_for(kernel: str, workload_name: str):
     try:
         with open(path) as f:
             t = json.load(f)
-        e = t[workload_name][kernel]
-        return e["hbm_bytes_per_launch"]
+        return t[workload_name][kernel]["hbm_bytes_per_launch"]
     except Exception:
         return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="plates_ep")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "exact", "fp32c"])
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: chosen at upload)")
-    ap.add_argument("--autotune", action="store_true", help="time every schedule first and use the fastest")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run a plain 256 MiB device copy (x3) whose HBM counter readings "
-                         "scale FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
-              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def roofline(prof: dict, dom: str, form: str, nnz: int, N: int, workload_name: str):
+    """Roofline record of the dominant kernel class `dom` from the per-class
+    HIP-event totals of the timed region."""
+    ms_dom, n_dom = prof[dom]
+    bb, bp = BYTES[(dom, form)]
+    alg = bb * nnz + bp * N
+    avg_s = (ms_dom / max(n_dom, 1)) / 1e3
+    achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
+    kern = {"force_bb": "BB", "force_osb": "OSB", "force_ep": "EP"}.get(dom, dom)
+    return {"bound": "hbm", "kernel": f"{dom} ({kern}, {form})", "achieved": achieved, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_for(dom, workload_name),
+            "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3, "launches": n_dom}
 
+
+def read_profile(ctx):
+    prof = {k: ctx.profile_read(k) for k in PROF_CLASSES}
+    ctx.profile(False)
+    return prof
+
+
+def bench_single(args):
     import torch
     from peridyn_amd import core
 
@@ -138,34 +161,35 @@ def main():
         w.run(args.warmup, nuf)
         ctx.sync()
         w.close()
+    # timed region: the run as a caller gets it (steady steps replayed from the
+    # captured HIP graph, no per-kernel events)
     env = core.Env.from_spec(spec, precision=args.precision, ctx=ctx, lanes_per_particle=lanes)
     lanes = env.lanes_per_particle
     info = env.info()
     N, nnz = info["particles"], info["bonds"]
     ctx.sync()
     torch.cuda.synchronize()
-    ctx.profile(True)
     t0 = time.perf_counter()
     env.run(args.steps, nuf)
     ctx.sync()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    el = t1 - t0
-    prof = {k: ctx.profile_read(k) for k in ("prologue", "latch", "theta", "td", "force_bb", "force_osb",
-                                             "force_ep", "contact", "book")}
-    ctx.profile(False)
+    el = time.perf_counter() - t0
     env.check_finite()
-
+    env.close()
+    # kernel durations: the same run launched eagerly with HIP events around
+    # every launch on the library stream (graph replays carry no per-kernel
+    # events; a kernel's own duration does not depend on how it was launched)
+    env = core.Env.from_spec(spec, precision=args.precision, ctx=ctx, lanes_per_particle=lanes)
+    ctx.sync()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    env.run(args.steps, nuf)
+    ctx.sync()
+    el_prof = time.perf_counter() - t0
+    prof = read_profile(ctx)
+    env.close()
+    form = form_of(args.precision)
     value = nnz * args.steps / el
-    ms_step = 1e3 * el / args.steps
-    ms_dom, n_dom = prof[dom]
-    bb, bp = BYTES[(dom, "fp64")]
-    alg_bytes = bb * nnz + bp * N
-    avg_s = (ms_dom / max(n_dom, 1)) / 1e3
-    achieved = alg_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
-    # whole-step algorithmic bytes (all passes) against the step time
-    step_bytes = sum(BYTES[(k, "fp64")][0] * nnz + BYTES[(k, "fp64")][1] * N
-                     for k in ("theta", "td", "force_ep") if prof[k][1] > 0) if dom == "force_ep" else alg_bytes
     out = {
         "metric": METRIC,
         "value": value,
@@ -173,22 +197,19 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1e3 * el / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if args.precision == "fp32c" else "f64",
         "data": "synthetic",
         "config": {"workload": desc, "particles": N, "bonds": nnz, "precision": args.precision,
-                   "neigh_update_freq": nuf, "lanes_per_particle": lanes,
+                   "neigh_update_freq": nuf, "lanes_per_particle": lanes, "device_bytes": info["device_bytes"],
                    "autotune_ms_per_step": tuned["ms_per_step"] if tuned else None},
-        "roofline": {"bound": "hbm", "kernel": f"k_force<{dom[6:].upper()},{args.precision}>",
-                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_for(dom, args.workload),
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_s * 1e3,
-                     "launches": n_dom},
-        "step_hbm_frac": (step_bytes / (el / args.steps)) / 1e9 / PEAK_HBM_GBS,
+        "roofline": dict(roofline(prof, dom, form, nnz, N, args.workload),
+                         measured_in="eager run of the same workload, HIP events around every launch"),
         "kernel_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1] > 0},
+        "eager_profiled_run": {"value": nnz * args.steps / el_prof, "ms_per_step": 1e3 * el_prof / args.steps},
     }
     if not args.no_cpu_baseline:
         cb = cpu_baseline(spec, nuf, args.cpu_budget, nnz)
@@ -196,7 +217,203 @@ def main():
         out["speedup_vs_cpu"] = value / cb["value"]
     else:
         out["cpu_baseline"] = None
+    if args.served_requests > 0:
+        out["served"] = served_summary(ctx, args.served_requests, args.served_rate, seq=True)
     print(json.dumps(out))
+
+
+def bench_batched(args):
+    """Config-3 batch: specimens stepped together (one launch per kernel class
+    per step), FP32C by default, against the same specimens one at a time in
+    plain FP64."""
+    import torch
+    from peridyn_amd import core
+    from peridyn_amd import workloads as W
+
+    specs = W.batched_specimens(args.batch_count)
+    ctx = core.Context(0)
+    prec = args.precision if args.precision != "exact" else "fp32c"
+    # warm-up: code objects and allocator on a few specimens
+    w = [core.Env.from_spec(s, precision=prec, ctx=ctx) for s in specs[:4]]
+    core.run_batch(w, max(args.warmup, 1))
+    for e in w:
+        e.close()
+    lanes = args.lanes
+    tuned = None
+    if lanes == 0:
+        from peridyn_amd import autotune
+        tuned = autotune.tune_batch(specs, prec, ctx=ctx)
+        lanes = tuned["lanes_per_particle"]
+        print(f"batch schedule: {tuned}", file=sys.stderr, flush=True)
+    envs = [core.Env.from_spec(s, precision=prec, ctx=ctx, lanes_per_particle=lanes) for s in specs]
+    infos = [e.info() for e in envs]
+    N = sum(i["particles"] for i in infos)
+    nnz = sum(i["bonds"] for i in infos)
+    dev_bytes = sum(i["device_bytes"] for i in infos)
+    ctx.sync()
+    torch.cuda.synchronize()
+    # timed region 1: per-kernel HIP events (eager launches) for the roofline
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    core.run_batch(envs, args.steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    prof = read_profile(ctx)
+    for e in envs:
+        e.check_finite()
+    # timed region 2: steady steps replayed from the captured HIP graph
+    for e in envs:
+        e.close()
+    envs = [core.Env.from_spec(s, precision=prec, ctx=ctx, lanes_per_particle=lanes) for s in specs]
+    ctx.sync()
+    t0 = time.perf_counter()
+    core.run_batch(envs, args.steps)
+    ctx.sync()
+    el_g = time.perf_counter() - t0
+    for e in envs:
+        e.close()
+    dom = max(("force_bb", "force_osb", "force_ep"), key=lambda k: prof[k][0])
+    # nnz/N of the kernel class's own blocks
+    kind = {"force_bb": "bb", "force_osb": "osb", "force_ep": "ep"}[dom]
+    kn = sum(i["bonds"] for s, i in zip(specs, infos) if s["blocks"][0]["kind"] == kind)
+    kp = sum(i["particles"] for s, i in zip(specs, infos) if s["blocks"][0]["kind"] == kind)
+    # plain FP64, one specimen at a time (each its own pd_run)
+    seq_t = 0.0
+    seq_nnz = 0
+    ns = len(specs) if args.seq_sample <= 0 else min(args.seq_sample, len(specs))
+    for s in specs[:ns]:
+        e = core.Env.from_spec(s, precision="fp64", ctx=ctx)
+        seq_nnz += e.info()["bonds"]
+        ctx.sync()
+        t0 = time.perf_counter()
+        e.run(args.steps)
+        ctx.sync()
+        seq_t += time.perf_counter() - t0
+        e.close()
+    seq_value = seq_nnz * args.steps / seq_t
+    value = nnz * args.steps / min(el, el_g)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "bond-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * min(el, el_g) / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if prec == "fp32c" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(specs)} tensile specimens (BB/OSB 50/50, 2k-60k particles, seed 20261203), "
+                               f"{prec}, one pd_run over all", "particles": N, "bonds": nnz,
+                   "precision": prec, "device_bytes": dev_bytes, "lanes_per_particle": lanes,
+                   "batch_tune_ms_per_step": tuned["ms_per_step"] if tuned else None},
+        "eager_profiled_run": {"value": nnz * args.steps / el, "ms_per_step": 1e3 * el / args.steps},
+        "graph_run": {"value": nnz * args.steps / el_g, "ms_per_step": 1e3 * el_g / args.steps},
+        "roofline": roofline(prof, dom, form_of(prec), kn, kp, "batched"),
+        "kernel_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1] > 0},
+        "fp64_one_at_a_time": {"value": seq_value, "specimens_timed": ns,
+                               "note": "same specimens and steps, plain FP64, one pd_run per specimen"},
+        "vs_fp64_one_at_a_time": value / seq_value,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def _progress(label: str, total: int):
+    def cb(res):
+        print(f"{label}: request {res.id} done at {res.done:.2f} s (latency {res.latency:.3f} s)",
+              file=sys.stderr, flush=True)
+    return cb
+
+
+def served_summary(ctx, count: int, rate: float, seq: bool = True, seed: int = 20261204):
+    """Serve `count` requests (FP32C, continuous batching) and, if `seq`, the
+    same stream one request at a time in plain FP64."""
+    from peridyn_amd import serve
+    from peridyn_amd import workloads as W
+
+    reqs = W.served_requests(count=count, seed=seed, rate=rate)
+    for r in reqs:
+        r["spec"] = W.make_spec(r)
+    free, total = ctx.mem_info()
+    budget = int(0.8 * free)
+    # warm-up (code objects) on two small requests
+    warm = [dict(r, arrival=0.0) for r in sorted(reqs, key=lambda r: r["n"])[:2]]
+    serve.Server(serve.GpuBackend(ctx, "fp32c", keep_state=False), budget, chunk=50).serve(warm)
+    srv = serve.Server(serve.GpuBackend(ctx, "fp32c", keep_state=False), budget, chunk=50,
+                       on_retire=_progress("served fp32c", count))
+    _, st = srv.serve(reqs)
+    out = {"requests": count, "arrival_rate_per_s": rate, "precision": "fp32c", "chunk_steps": 50,
+           "budget_bytes": budget, **st.as_dict()}
+    if seq:
+        one = serve.Server(serve.GpuBackend(ctx, "fp64", keep_state=False), budget, chunk=None, max_active=1,
+                           on_retire=_progress("fp64 one at a time", count))
+        _, s1 = one.serve(reqs)
+        out["fp64_one_at_a_time"] = s1.as_dict()
+        out["sims_per_s_vs_fp64_one_at_a_time"] = st.sims_per_s / s1.sims_per_s
+        out["p99_vs_fp64_one_at_a_time"] = st.p99 / s1.p99
+    return out
+
+
+def bench_served(args):
+    from peridyn_amd import core
+
+    ctx = core.Context(0)
+    s = served_summary(ctx, args.served_requests or 64, args.served_rate, seq=True)
+    out = {
+        "metric": METRIC,
+        "value": s["sims_per_s"],
+        "unit": "sims/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 2,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{s['requests']} Poisson requests at {args.served_rate}/s: tensile / indentation "
+                               "50/50, BB / OSB 50/50, 1k-200k particles, 200-2000 steps, horizon 2.5-4.0 dx; "
+                               "FP32C continuous batching, 50-step chunks"},
+        "served": s,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="plates_ep")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "exact", "fp32c"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: chosen at upload)")
+    ap.add_argument("--autotune", action="store_true", help="time every schedule first and use the fastest")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--seq-sample", type=int, default=64,
+                    help="batched: specimens timed one at a time in FP64 (0: all)")
+    ap.add_argument("--served-requests", type=int, default=0,
+                    help="served: number of requests; with other workloads, add a served summary")
+    ap.add_argument("--served-rate", type=float, default=40.0, help="served: Poisson arrivals per second")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy (x3) whose HBM counter readings "
+                         "scale FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "batched":
+        bench_batched(args)
+    elif args.workload == "served":
+        bench_served(args)
+    else:
+        bench_single(args)
 
 
 if __name__ == "__main__":

@@ -103,6 +103,9 @@ int pd_ctx_create(int device, void** ctx);
 int pd_ctx_destroy(void* ctx);
 const char* pd_last_error(void* ctx);
 int pd_sync(void* ctx);
+/* free / total device memory of the context's GPU (hipMemGetInfo); no
+ * reference counterpart: the serving loop admits simulations by footprint */
+int pd_ctx_mem_info(void* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 /* ---- environment (Env, src/environment.jl:166-235) ------------------- */
 int pd_env_create(void* ctx, double dt, int32_t precision, int64_t* env);
@@ -132,9 +135,10 @@ int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t row
 
 /* ---- stepping ----------------------------------------------------------- */
 /* run! physics cadence (solvers.jl:97-187) for the listed envs, stepped
- * together: at an env's first call update_neighs! + apply_bc_at0; then nsteps
+ * together: update_neighs! (as every run! call does); then nsteps
  * velocity_verlet_step! (velocity_verlet.jl:57-89) with contact lists rebuilt
- * after every step i with i % nuf == 0.  Asynchronous on the context stream. */
+ * after every step i with i % nuf == 0, i counted from 1 in each call (run!'s
+ * default start_at = 0).  Returns when the steps have completed. */
 int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf);
 /* update_acc! (solvers.jl:320-336) at the current positions; y may be NULL
  * (resident state) or 3 x N positions to upload first.  f receives 3 x N. */

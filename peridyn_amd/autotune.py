@@ -5,9 +5,12 @@ lane per particle, G = 64 one wavefront per particle).  Which G is fastest
 depends on the specimen size (enough wavefronts to hide gather latency), the
 degree distribution and the material passes.  `tune(spec, precision)` builds a
 replica of the env for each candidate G, times a few steps of the real run
-loop with HIP events, and returns the fastest G together with the timings;
-results are cached per (particle-count bucket, material kinds, precision) in
-a JSON file so later envs of similar shape reuse them.
+loop (wall clock around pd_run, which returns when its steps are done), and
+returns the fastest G together with the timings; results are cached per
+(particle-count bucket, material kinds, precision) in a JSON file so later envs
+of similar shape reuse them.  `tune_batch(specs, precision)` does the same for
+many specimens stepped together by one pd_run, where the batch as a whole, not
+each specimen, has to fill the chip.
 """
 from __future__ import annotations
 
@@ -50,12 +53,9 @@ def _store(db: Dict[str, dict]):
         pass
 
 
-def time_schedule(spec: dict, precision: str, lanes: int, steps: int = 20, nuf: int = 10,
-                  ctx: Optional[core.Context] = None) -> float:
-    """Mean device time per step (ms) of `steps` steps with the given schedule."""
-    ctx = ctx or core.Context.default()
-    # every block deformed from the first step (the deformed latch would
-    # otherwise skip the bond passes of a rigidly moving specimen)
+def _deformed_replica(spec: dict) -> dict:
+    """Copy of spec whose blocks are deformed from the first step (the deformed
+    latch would otherwise skip the bond passes of a rigidly moving specimen)."""
     rep = dict(spec)
     rep["blocks"] = []
     for b in spec["blocks"]:
@@ -64,7 +64,14 @@ def time_schedule(spec: dict, precision: str, lanes: int, steps: int = 20, nuf: 
         y[:, 0] += 1e-7 * np.sin(np.arange(y.shape[0]))
         b2["y"] = y
         rep["blocks"].append(b2)
-    env = core.Env.from_spec(rep, precision=precision, ctx=ctx, lanes_per_particle=lanes)
+    return rep
+
+
+def time_schedule(spec: dict, precision: str, lanes: int, steps: int = 20, nuf: int = 10,
+                  ctx: Optional[core.Context] = None) -> float:
+    """Mean wall time per step (ms) of `steps` steps with the given schedule."""
+    ctx = ctx or core.Context.default()
+    env = core.Env.from_spec(_deformed_replica(spec), precision=precision, ctx=ctx, lanes_per_particle=lanes)
     try:
         env.run(2, nuf)  # warm code objects and caches
         ctx.sync()
@@ -92,3 +99,37 @@ def tune(spec: dict, precision: str = "fp64", candidates: Iterable[int] = CANDID
         db[key] = res
         _store(db)
     return res
+
+
+def time_batch(specs: list, precision: str, lanes: int, steps: int = 10, nuf: int = 10,
+               ctx: Optional[core.Context] = None) -> float:
+    """Mean wall time per step (ms) of the specimens stepped together."""
+    ctx = ctx or core.Context.default()
+    envs = []
+    try:
+        for s in specs:
+            envs.append(core.Env.from_spec(_deformed_replica(s), precision=precision, ctx=ctx,
+                                           lanes_per_particle=lanes))
+        core.run_batch(envs, 2, nuf)
+        ctx.sync()
+        t0 = time.perf_counter()
+        core.run_batch(envs, steps, nuf)
+        ctx.sync()
+        return 1e3 * (time.perf_counter() - t0) / steps
+    finally:
+        for e in envs:
+            e.close()
+
+
+def tune_batch(specs: list, precision: str = "fp32c", candidates: Iterable[int] = (1, 2, 4, 8, 16),
+               steps: int = 10, nuf: int = 10, sample: int = 128, ctx: Optional[core.Context] = None) -> dict:
+    """Lanes per particle for a batch: every candidate G timed on an evenly
+    strided sample of the specimens.  Returns {"lanes_per_particle", "ms_per_step",
+    "sample"}."""
+    if precision == "exact":
+        return {"lanes_per_particle": 1, "ms_per_step": {}, "sample": 0}
+    stride = max(1, len(specs) // max(sample, 1))
+    sub = specs[::stride][:sample]
+    times = {int(g): time_batch(sub, precision, int(g), steps, nuf, ctx) for g in candidates}
+    best = min(times, key=times.get)
+    return {"lanes_per_particle": best, "ms_per_step": times, "sample": len(sub)}

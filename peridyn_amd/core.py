@@ -52,6 +52,12 @@ class Context:
     def sync(self):
         self.check(self.lib.pd_sync(self.h))
 
+    def mem_info(self):
+        """(free, total) device bytes of the context's GPU."""
+        fr, tot = ctypes.c_int64(0), ctypes.c_int64(0)
+        self.check(self.lib.pd_ctx_mem_info(self.h, ctypes.byref(fr), ctypes.byref(tot)))
+        return fr.value, tot.value
+
     def profile(self, enable: bool = True):
         """Start (and reset) or stop per-kernel-class HIP-event timing."""
         self.check(self.lib.pd_ctx_profile(self.h, 1 if enable else 0))

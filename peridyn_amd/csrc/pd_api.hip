@@ -49,6 +49,9 @@ struct Env {
   std::vector<HostBc> bcs;
   bool committed = false;
   bool maybe_undeformed = true;
+  // the contact lists were built from the current positions (nothing has moved
+  // them since), so an update_neighs! now would rebuild the same sorted lists
+  bool lists_current = false;
   bool has_ep = false;
   int64_t N = 0, nslots = 0, nbonds = 0;
   int64_t time_step = 0;
@@ -740,8 +743,8 @@ void refresh_latch(Ctx* c, const std::vector<int64_t>& ids) {
 
 void contact_update(Ctx* c, Env& e) {
   for (size_t k = 0; k < e.cdev.size(); ++k)
-    pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k,
-                             e.precision == PD_PREC_EXACT);
+    pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k);
+  e.lists_current = true;
 }
 
 // one force evaluation (update_acc!) for a chunk subset of one env
@@ -853,6 +856,40 @@ void row_of(const Env& e, const BondsHost& h, int64_t r, std::vector<std::pair<i
 
 thread_local std::string g_err;
 
+// One captured step sequence, replayed with hipGraphLaunch; destroyed with the
+// pd_run call that made it.
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  template <typename F>
+  void capture(hipStream_t s, F&& enqueue) {
+    hipGraph_t g = nullptr;
+    PD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue();
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    PD_HIP(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      exec = nullptr;
+      throw PdError(PD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  ~StepGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+// PERIDYN_GRAPH=0 launches every step eagerly (for comparison)
+bool graphs_enabled() {
+  const char* v = std::getenv("PERIDYN_GRAPH");
+  return !(v && std::string(v) == "0");
+}
+
 }  // namespace
 
 // =============================================================================
@@ -908,6 +945,16 @@ const char* pd_last_error(void* ctx) { return ctx ? C(ctx)->err.c_str() : g_err.
 
 int pd_sync(void* ctx) {
   return run_guard(ctx, [&] { PD_HIP(hipStreamSynchronize(C(ctx)->stream)); });
+}
+
+int pd_ctx_mem_info(void* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+  return run_guard(ctx, [&] {
+    PD_HIP(hipSetDevice(C(ctx)->device));
+    size_t fr = 0, tot = 0;
+    PD_HIP(hipMemGetInfo(&fr, &tot));
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    if (total_bytes) *total_bytes = (int64_t)tot;
+  });
 }
 
 int pd_env_create(void* ctx, double dt, int32_t precision, int64_t* env) {
@@ -1152,10 +1199,12 @@ int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t
     build_set(c, ids);
     hipStream_t s = c->stream;
     // run! start: update_neighs! (solvers.jl:114); apply_bc_at0 records
-    // BC.last = y, which the resident state already holds
+    // BC.last = y, which the resident state already holds.  Contact lists are
+    // a deterministic function of the positions (sorted), so lists built from
+    // the current positions are kept rather than rebuilt identically.
     for (int64_t id : ids) {
       Env& e = get_env(c, id);
-      if (!e.contacts.empty()) contact_update(c, e);
+      if (!e.contacts.empty() && !e.lists_current) contact_update(c, e);
     }
     if (nsteps == 0) return;
     const int nl = prepare_latch(c, ids);
@@ -1163,8 +1212,8 @@ int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t
     Chunk* d = c->d_set;
     timed(c, PD_PROF_PROLOGUE, [&] { pd_launch_prologue(s, d, c->n_all, T); });
     pd_launch_book(s, T, c->d_slots, c->n_slots, PD_B_FLIP_Y);
-    for (int64_t st = 1; st <= nsteps; ++st) {
-      const bool last = (st == nsteps);
+    // the kernels of one velocity_verlet_step! (all envs of the set)
+    auto step_kernels = [&](bool last) {
       if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part); });
       if (c->set_has_theta)
         timed(c, PD_PROF_THETA, [&] { pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T); });
@@ -1174,22 +1223,52 @@ int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t
         if (c->n_force[k])
           timed(c, PD_PROF_FORCE_BB + k,
                 [&] { pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl); });
-      for (int64_t id : ids) {
-        Env& e = get_env(c, id);
-        if (!e.contacts.empty() && ((e.time_step + st) % nuf) == 0)
-          timed(c, PD_PROF_CONTACT, [&] { contact_update(c, e); });
-      }
+    };
+    auto book = [&](bool last) {
       timed(c, PD_PROF_BOOK, [&] {
         pd_launch_book(s, T, c->d_slots, c->n_slots,
                        PD_B_STEP | (last ? 0 : PD_B_FLIP_Y) | (c->set_has_ep ? PD_B_FLIP_EDP : 0));
       });
+    };
+    // Steps that are neither the last nor followed by a contact rebuild are
+    // identical launch sequences (all step-dependent state lives on the
+    // device), so they replay one captured HIP graph.  Profiled runs launch
+    // eagerly so that every kernel class keeps its own events.
+    StepGraph graph;
+    const bool use_graph = graphs_enabled() && !c->prof && nsteps >= 3;
+    for (int64_t st = 1; st <= nsteps; ++st) {
+      const bool last = (st == nsteps);
+      // run! (solvers.jl:133,177) updates neighbours after loop step i when
+      // i % neigh_update_freq == 0, i counted from 1 in each call (start_at = 0)
+      bool rebuild = false;
+      if (st % nuf == 0)
+        for (int64_t id : ids) rebuild = rebuild || !get_env(c, id).contacts.empty();
+      if (use_graph && !last && !rebuild) {
+        if (!graph.exec) {
+          graph.capture(s, [&] {
+            step_kernels(false);
+            book(false);
+          });
+        }
+        PD_HIP(hipGraphLaunch(graph.exec, s));
+        continue;
+      }
+      step_kernels(last);
+      if (rebuild)
+        for (int64_t id : ids) {
+          Env& e = get_env(c, id);
+          if (!e.contacts.empty()) timed(c, PD_PROF_CONTACT, [&] { contact_update(c, e); });
+        }
+      book(last);
     }
     PD_HIP(hipGetLastError());
+    PD_HIP(hipStreamSynchronize(s));  // also before the step graph is destroyed
     for (int64_t id : ids) {
       Env& e = get_env(c, id);
       e.time_step += nsteps;
       e.cur ^= (int32_t)(nsteps & 1);
       if (e.has_ep) e.edp_cur ^= (int32_t)(nsteps & 1);
+      e.lists_current = (nsteps % nuf) == 0;  // rebuilt after the last step
     }
     if (nl) refresh_latch(c, ids);
   });
@@ -1201,7 +1280,10 @@ int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f) {
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
-    if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
+    if (y) {
+      upload_xyz(c, e, e.dev.y[e.cur], y, false);
+      e.lists_current = false;
+    }
     force_pass(c, e, e.chunks, 0);
     if (f) download_xyz(c, e, e.dev.f, f);
   });
@@ -1215,7 +1297,10 @@ int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, dou
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     HostBlock& b = get_block(e, block);
     if (lo != b.first + 1 || hi != b.first + b.d.n) throw PdError(PD_E_ARG, "limits do not match the block");
-    if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
+    if (y) {
+      upload_xyz(c, e, e.dev.y[e.cur], y, false);
+      e.lists_current = false;
+    }
     std::vector<Chunk> sub(e.chunks.begin() + b.chunk0, e.chunks.begin() + b.chunk0 + b.nchunks);
     // contacts are not part of force_density_T!: zero them out of this pass
     EnvDev saved = e.dev;
@@ -1266,7 +1351,10 @@ int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, cons
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     PD_HIP(hipStreamSynchronize(c->stream));
-    if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
+    if (y) {
+      upload_xyz(c, e, e.dev.y[e.cur], y, false);
+      e.lists_current = false;
+    }
     if (v) upload_xyz(c, e, e.dev.v, v, false);
     if (f) upload_xyz(c, e, e.dev.f, f, false);
   });

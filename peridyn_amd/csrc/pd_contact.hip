@@ -8,10 +8,11 @@
 // LARGEST block-2 member indices (contacts.jl:468-469).  The brute-force
 // O(n1 n2) scan is replaced by a cell list (cells >= distance wide) over the
 // in-box block-2 members.  For the gather-form force kernel the lists are also
-// transposed: each block-2 member gets its block-1 partners.  In the
-// reference-order form (`ordered`) the own lists are sorted descending and the
-// transposed lists ascending, the order in which the reference's loop over i
-// scatters into f1[:, i] and f2[:, j]; the other forms skip both sorts.
+// transposed: each block-2 member gets its block-1 partners.  The own lists are
+// sorted descending and the transposed lists ascending, the order in which the
+// reference's loop over i scatters into f1[:, i] and f2[:, j]; the cell-list
+// fill uses atomics, so without the sorts the summation order, and the last bits
+// of a run, would vary between rebuilds.
 #include "pd_device.h"
 #include "pd_kernels.h"
 
@@ -148,10 +149,38 @@ __global__ void k_cfill(const EnvDev* __restrict__ envs, int slot, int cidx, int
   C.cell_items[C.cell_start[c] + atomicAdd(&cursor[c], 1)] = j;
 }
 
-// Visits every block-2 member within `distance` of p (27-cell stencil).
+// Wavefront-level LDS ordering: every lane's earlier LDS writes are visible to
+// every lane of the wavefront after this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_add(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+// number of set bits of m below this lane
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Visits every block-2 member within `distance` of p (27-cell stencil), the
+// members of a cell spread over the 64 lanes of the calling wavefront: f(hit, j)
+// is called by all lanes together (wave-uniform trip count) with hit = lane's
+// member j is a contact.
 template <typename F>
-__device__ __forceinline__ void for_candidates(const ContactDev& C, const double4* y, const double4& p,
-                                               const double* box, F&& f) {
+__device__ __forceinline__ void wave_candidates(const ContactDev& C, const double4* y, const double4& p,
+                                                const double* box, int lane, F&& f) {
   int cc[3];
   cell_id(p, box, cc);
   const int nc[3] = {(int)box[8], (int)box[9], (int)box[10]};
@@ -167,85 +196,156 @@ __device__ __forceinline__ void for_candidates(const ContactDev& C, const double
         if (xx < 0 || xx >= nc[0]) continue;
         const int cell = xx + nc[0] * (yy + nc[1] * zz);
         const int s = C.cell_start[cell], e = s + C.cell_count[cell];
-        for (int t = s; t < e; ++t) {
-          const int j = C.cell_items[t];
-          const double4 q = y[C.idx2[j]];
-          const double a = p.x - q.x, b = p.y - q.y, c = p.z - q.z;
-          if (a * a + b * b + c * c < d2max) f(j);
+        for (int t0 = s; t0 < e; t0 += 64) {
+          const int t = t0 + lane;
+          bool hit = false;
+          int j = -1;
+          if (t < e) {
+            j = C.cell_items[t];
+            const double4 q = y[C.idx2[j]];
+            const double a = p.x - q.x, b = p.y - q.y, c = p.z - q.z;
+            hit = (a * a + b * b + c * c < d2max);
+          }
+          f(hit, j);
         }
       }
     }
   }
 }
 
-// search: per block-1 member, the max_neighs largest block-2 member ids
-__global__ void k_csearch(const EnvDev* __restrict__ envs, int slot, int cidx, int ordered) {
+// Descending bitonic sort of buf[0, n) (n <= kCandCap) by one wavefront; the
+// tail up to the next power of two (>= 64) is padded with -1.
+constexpr int kCandCap = 2048;
+__device__ void wave_sort_desc(int32_t* buf, int n, int lane) {
+  int P = 64;
+  while (P < n) P <<= 1;
+  for (int t = n + lane; t < P; t += 64) buf[t] = -1;
+  wave_sync();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int t = lane; t < P; t += 64) {
+        const int l = t ^ jj;
+        if (l > t) {
+          const int32_t a = buf[t], b = buf[l];
+          const bool lead = (t & k) == 0;
+          if (lead ? (a < b) : (a > b)) { buf[t] = b; buf[l] = a; }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+// One wavefront per block-1 member i: the contact list of i is the max_neighs
+// largest block-2 member ids within `distance`, stored descending.  Candidates
+// are compacted in scan order into the wavefront's LDS buffer and sorted there;
+// with more than kCandCap candidates the max_neighs-th largest id is found by
+// bisection over the candidates' id range first.
+__global__ __launch_bounds__(256) void k_csearch(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  __shared__ int32_t cand[4][kCandCap];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const EnvDev& E = envs[slot];
   const ContactDev& C = E.contacts[cidx];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.n1) return;
+  const int i = blockIdx.x * 4 + wv;
+  if (i >= C.n1) return;  // whole wavefront; the kernel uses no workgroup barrier
+  int32_t* buf = cand[wv];
   const double* box = C.box;
   const double4* y = cur_y(E);
   const double4 p = y[C.idx1[i]];
   const int M = C.max_neighs;
   int32_t* own = C.own + (size_t)i * M;
   int cnt = 0;
-  if (box[6] != 0.0 && in_box(p, box)) {
-    int total = 0;
-    for_candidates(C, y, p, box, [&](int) { ++total; });
-    // keep ids >= thr: thr = 0 when everything fits, else the M-th largest id
-    int thr = 0;
-    if (total > M) {
-      int lo = 0, hi = C.n2;  // count(>= lo) >= M, count(>= hi) < M
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        int c = 0;
-        for_candidates(C, y, p, box, [&](int j) { c += (j >= mid); });
-        if (c >= M) lo = mid; else hi = mid;
+  if (M > 0 && box[6] != 0.0 && in_box(p, box)) {
+    int total = 0, lo_id = INT_MAX, hi_id = -1;
+    wave_candidates(C, y, p, box, lane, [&](bool hit, int j) {
+      const uint64_t m = __ballot(hit);
+      if (hit) {
+        const int pos = total + lanes_below(m);
+        if (pos < kCandCap) buf[pos] = j;
+        lo_id = min(lo_id, j);
+        hi_id = max(hi_id, j);
       }
-      thr = lo;
-    }
-    for_candidates(C, y, p, box, [&](int j) {
-      if (j >= thr) own[cnt++] = j;
+      total += __popcll(m);
     });
-    if (ordered) {
-      for (int a = 1; a < cnt; ++a) {  // descending
-        const int v = own[a];
-        int b = a - 1;
-        while (b >= 0 && own[b] < v) { own[b + 1] = own[b]; --b; }
-        own[b + 1] = v;
+    if (total <= kCandCap) {
+      wave_sort_desc(buf, total, lane);
+      cnt = min(total, M);
+      for (int k = lane; k < cnt; k += 64) own[k] = buf[k];
+    } else {
+      // count(>= lo) >= M > count(>= hi); ids are distinct, so lo converges to
+      // the M-th largest id and exactly M candidates are >= lo
+      int lo = wave_min(lo_id), hi = wave_max(hi_id) + 1;
+      if (total > M) {
+        while (hi - lo > 1) {
+          const int mid = lo + ((hi - lo) >> 1);
+          int c = 0;
+          wave_candidates(C, y, p, box, lane, [&](bool hit, int j) { c += (hit && j >= mid); });
+          if (wave_add(c) >= M) lo = mid; else hi = mid;
+        }
+      }
+      wave_candidates(C, y, p, box, lane, [&](bool hit, int j) {
+        const bool keep = hit && j >= lo;
+        const uint64_t m = __ballot(keep);
+        if (keep) own[cnt + lanes_below(m)] = j;
+        cnt += __popcll(m);
+      });
+      if (cnt <= kCandCap) {
+        for (int k = lane; k < cnt; k += 64) buf[k] = own[k];
+        wave_sort_desc(buf, cnt, lane);
+        for (int k = lane; k < cnt; k += 64) own[k] = buf[k];
+      } else if (lane == 0) {
+        for (int a = 1; a < cnt; ++a) {  // descending insertion sort (max_neighs > kCandCap)
+          const int v = own[a];
+          int b = a - 1;
+          while (b >= 0 && own[b] < v) { own[b + 1] = own[b]; --b; }
+          own[b + 1] = v;
+        }
       }
     }
   }
-  C.own_cnt[i] = cnt;
-  for (int k = 0; k < cnt; ++k) atomicAdd(&C.rev_cnt[own[k]], 1);
+  if (lane == 0) C.own_cnt[i] = cnt;
+  wave_sync();  // own[] written by other lanes (or lane 0 alone) above
+  for (int k = lane; k < cnt; k += 64) atomicAdd(&C.rev_cnt[own[k]], 1);
 }
 
-__global__ void k_crev_fill(const EnvDev* __restrict__ envs, int slot, int cidx) {
+// One wavefront per block-1 member: append i to the transposed list of each of
+// its contacts (order fixed afterwards by k_crev_sort).
+__global__ __launch_bounds__(256) void k_crev_fill(const EnvDev* __restrict__ envs, int slot, int cidx) {
   const EnvDev& E = envs[slot];
   const ContactDev& C = E.contacts[cidx];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (i >= C.n1) return;
   const int cnt = C.own_cnt[i];
   const int32_t* own = C.own + (size_t)i * C.max_neighs;
-  for (int k = 0; k < cnt; ++k) {
+  for (int k = lane; k < cnt; k += 64) {
     const int j = own[k];
     C.rev[C.rev_off[j] + atomicAdd(&C.rev_cursor[j], 1)] = i;
   }
 }
 
-__global__ void k_crev_sort(const EnvDev* __restrict__ envs, int slot, int cidx) {
+// One wavefront per block-2 member: its transposed list sorted ascending.
+__global__ __launch_bounds__(256) void k_crev_sort(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  __shared__ int32_t cand[4][kCandCap];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const EnvDev& E = envs[slot];
   const ContactDev& C = E.contacts[cidx];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= C.n2) return;
+  const int j = blockIdx.x * 4 + wv;
+  if (j >= C.n2) return;  // whole wavefront; no workgroup barrier below
   int32_t* r = C.rev + C.rev_off[j];
   const int n = C.rev_cnt[j];
-  for (int a = 1; a < n; ++a) {
-    const int v = r[a];
-    int b = a - 1;
-    while (b >= 0 && r[b] > v) { r[b + 1] = r[b]; --b; }
-    r[b + 1] = v;
+  if (n <= 1) return;
+  if (n <= kCandCap) {
+    int32_t* buf = cand[wv];
+    for (int k = lane; k < n; k += 64) buf[k] = r[k];
+    wave_sort_desc(buf, n, lane);
+    for (int k = lane; k < n; k += 64) r[k] = buf[n - 1 - k];
+  } else if (lane == 0) {
+    for (int a = 1; a < n; ++a) {
+      const int v = r[a];
+      int b = a - 1;
+      while (b >= 0 && r[b] > v) { r[b + 1] = r[b]; --b; }
+      r[b + 1] = v;
+    }
   }
 }
 
@@ -326,7 +426,7 @@ void pd_scan_exclusive(hipStream_t s, const int32_t* in, int32_t* out, int n, in
 // host copy of the ContactDev (device pointers, sizes).  Scratch inside c.box:
 // [16, 16 + 12*kRedBlocks) partial min/max; the scan scratch follows.
 void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDev& host_env,
-                              const ContactDev& c, int cidx, bool ordered) {
+                              const ContactDev& c, int cidx) {
   (void)host_env;
   double* part = c.box + 16;
   int32_t* scan_tmp = reinterpret_cast<int32_t*>(c.box + 16 + 12 * kRedBlocks);
@@ -340,8 +440,9 @@ void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDe
   hipLaunchKernelGGL(k_cbin, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
   pd_scan_exclusive(s, c.cell_count, c.cell_start, c.ncell_cap, scan_tmp);
   hipLaunchKernelGGL(k_cfill, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx, cell_cursor);
-  hipLaunchKernelGGL(k_csearch, dim3(nb(c.n1)), dim3(256), 0, s, envs, slot, cidx, ordered ? 1 : 0);
+  const int wgrid = c.n1 > 0 ? (c.n1 + 3) / 4 : 1;  // one wavefront per block-1 member
+  hipLaunchKernelGGL(k_csearch, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
   pd_scan_exclusive(s, c.rev_cnt, c.rev_off, c.n2, scan_tmp);
-  hipLaunchKernelGGL(k_crev_fill, dim3(nb(c.n1)), dim3(256), 0, s, envs, slot, cidx);
-  if (ordered) hipLaunchKernelGGL(k_crev_sort, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
+  hipLaunchKernelGGL(k_crev_fill, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
+  hipLaunchKernelGGL(k_crev_sort, dim3(c.n2 > 0 ? (c.n2 + 3) / 4 : 1), dim3(256), 0, s, envs, slot, cidx);
 }

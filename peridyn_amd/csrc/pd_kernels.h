@@ -47,7 +47,7 @@ void pd_scan_exclusive(hipStream_t s, const int32_t* in, int32_t* out, int n, in
 
 // contacts (pd_contact.hip)
 void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDev& host_env,
-                              const ContactDev& c, int cidx, bool ordered);
+                              const ContactDev& c, int cidx);
 
 // FP32-compensated form (pd_step32.hip)
 void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);

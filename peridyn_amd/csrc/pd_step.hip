@@ -538,7 +538,9 @@ __global__ __launch_bounds__(256) void k_prologue(const Chunk* __restrict__ chun
     if (bc.kind == PD_BC_FIX) {
       yn = yp;
     } else {
-      const double ddir = dt * pd_bc_dir(bc, E.time_step + 1);
+      // this drift belongs to step time_step + 1 (the fused kernels drift the
+      // following step and use time_step + 1 there)
+      const double ddir = dt * pd_bc_dir(bc, E.time_step);
       yn.x = yp.x + ddir * bc.rate[0]; yn.y = yp.y + ddir * bc.rate[1]; yn.z = yp.z + ddir * bc.rate[2];
     }
   }

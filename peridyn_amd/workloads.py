@@ -161,3 +161,96 @@ def batched_specimens(count: int = 256, seed: int = 20261203, nmin: int = 2000, 
         b = block(kind, x, DELTA, **params)
         specs.append({"dt": dt, "blocks": [b], "contacts": [], "bcs": grips_z(x, 3, pull / dt)})
     return specs
+
+
+def lattice_ball_count(horizon: float) -> int:
+    """Number of unit-lattice offsets xi != 0 with |xi| < horizon: the degree of
+    an interior particle of a unit lattice, the maximum degree of any lattice
+    specimen with that horizon (cal_family!'s strict test, src/peri.jl:545)."""
+    r = int(math.ceil(horizon))
+    ax = np.arange(-r, r + 1)
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    d2 = X * X + Y * Y + Z * Z
+    return int(np.count_nonzero((d2 > 0) & (d2 < horizon * horizon)))
+
+
+def indentation(n: int = 20000, horizon: float = DELTA, kind: str = "bb", steps: int = 1000,
+                dt: float = 1e-5, E: float = E0, s0: float = 0.1) -> dict:
+    """Block indentation in the shape of examples/indentation/block_indentor.jl:22-99:
+    a 2a x 2a x a block (about n particles) held by FixBC on its 3 bottom
+    layers, a sphere of radius a/4 placed 2 dx above it and driven down by
+    ToFroBC at 10 dx over the run, reversing after 80% of the steps; 100x
+    stiffer bond-based indenter; linear contact k = 18K/(pi delta^4) with the
+    contact model's default distances (distanceD 1, distanceX 3, max_neighs 50,
+    src/contacts/contacts.jl:76-91).  The example's self-contact models (RM1,
+    RM2) are not part of this workload."""
+    a = max(8, int(round((n / 4.0) ** (1.0 / 3.0))))
+    xb = lattice_box(2 * a, 2 * a, a)
+    top = float(a - 1)
+    rs = max(2.0, a / 4.0)
+    xs = lattice_sphere(rs, center=(a - 0.5, a - 0.5, top + 2.0 + rs))
+    K = bulk_from_E(E, NU0)
+    maxn = lattice_ball_count(horizon) + 8
+    p = dict(bulk_modulus=K, critical_stretch=s0, density=RHO0)
+    if kind != "bb":
+        p["shear_modulus"] = 3.0 * K * (1 - 2 * NU0) / (2 * (1 + NU0))
+    bb = block(kind, xb, horizon, type_id=1, max_neigh=maxn, **p)
+    bs = block("bb", xs, horizon, type_id=2, max_neigh=maxn, bulk_modulus=100.0 * K, critical_stretch=1.0,
+               density=RHO0)
+    N = xb.shape[0] + xs.shape[0]
+    fix = np.zeros(N, dtype=bool)
+    fix[: xb.shape[0]] = xb[:, 2] <= 2.0 + 1e-9
+    ind = np.zeros(N, dtype=bool)
+    ind[xb.shape[0]:] = True
+    speed = 10.0 / (steps * dt)
+    bcs = [{"kind": "fix", "mask": fix},
+           {"kind": "tofro", "mask": ind, "rate": np.array([0.0, 0.0, -speed]), "freq": float(int(steps * 0.8)),
+            "applyafter": 0}]
+    contact = {"pair": ([2], [1]), "exponent": 1, "stiffness": 18.0 * K / (math.pi * horizon ** 4),
+               "distanceD": 1.0, "distanceX": 3.0, "max_neighs": 50}
+    return {"dt": dt, "blocks": [bb, bs], "contacts": [contact], "bcs": bcs}
+
+
+def tensile(n: int, horizon: float = DELTA, kind: str = "bb", pull_per_step: float = 0.01, dt: float = 1e-4,
+            E: float = E0, s0: float = S00) -> dict:
+    """Near-cubic tensile box of about n particles (>= 7 z-layers), MoveBC grips
+    on 3 z-layers each end."""
+    side = max(4, int(round(n ** (1.0 / 3.0))))
+    nz = max(7, int(round(n / (side * side))))
+    x = lattice_box(side, side, nz)
+    K = bulk_from_E(E, NU0)
+    p = dict(bulk_modulus=K, critical_stretch=s0, density=RHO0)
+    if kind != "bb":
+        p["shear_modulus"] = 3.0 * K * (1 - 2 * NU0) / (2 * (1 + NU0))
+    b = block(kind, x, horizon, max_neigh=lattice_ball_count(horizon) + 8, **p)
+    return {"dt": dt, "blocks": [b], "contacts": [], "bcs": grips_z(x, 3, pull_per_step / dt)}
+
+
+def served_requests(count: int = 64, seed: int = 20261204, nmin: int = 1000, nmax: int = 200000,
+                    smin: int = 200, smax: int = 2000, rate: float = 10.0, step_quantum: int = 10) -> list:
+    """Request stream: Poisson arrivals at `rate` per second; particle counts
+    log-uniform in [nmin, nmax]; step counts uniform in [smin, smax] rounded to
+    multiples of `step_quantum` (the contact-rebuild period, so that a request
+    stepped in chunks rebuilds at the same steps as alone); horizon uniform in
+    [2.5, 4.0] dx; material bond-based / ordinary state-based 50/50; tensile or
+    indentation 50/50.  Returns dicts {id, arrival, steps, nuf, kind, shape,
+    n, horizon, spec_args}; make_spec(r) builds the spec."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        steps = int(round(rng.uniform(smin, smax) / step_quantum)) * step_quantum
+        h = float(rng.uniform(2.5, 4.0))
+        kind = "bb" if rng.uniform() < 0.5 else "osb"
+        shape = "tensile" if rng.uniform() < 0.5 else "indentation"
+        out.append({"id": i, "arrival": t, "steps": steps, "nuf": step_quantum, "kind": kind, "shape": shape,
+                    "n": n, "horizon": h})
+    return out
+
+
+def make_spec(r: dict) -> dict:
+    if r["shape"] == "tensile":
+        return tensile(r["n"], horizon=r["horizon"], kind=r["kind"])
+    return indentation(r["n"], horizon=r["horizon"], kind=r["kind"], steps=r["steps"])

@@ -102,6 +102,28 @@ def test_bb_exact_split_runs_equal_single_run(pd, orc):
     assert np.array_equal(y, o.y) and np.array_equal(v, o.v) and np.array_equal(f, o.f)
 
 
+@pytest.mark.parametrize("prec", ["exact", "fp64"])
+def test_tofro_reversal_on_run_boundaries(pd, orc, prec):
+    """ToFroBC grips reversing every 5 steps (ToFroBC.jl:107-111), stepped in
+    runs whose boundaries fall on and next to the reversal steps: equal to one
+    oracle run (bit-identical in the reference-order form)."""
+    spec = W.tensile_bb(n=7, dt=1e-4, pull_per_step=0.002)
+    for bc in spec["bcs"]:
+        bc["freq"] = 5.0
+        bc["applyafter"] = 0
+    o = orc.OracleEnv(spec)
+    o.run(27)
+    g = pd.Env.from_spec(spec, precision=prec)
+    for n in (5, 1, 4, 10, 2, 5):
+        g.run(n)
+    y, v, f = g.download()
+    x = spec["blocks"][0]["x"]
+    if prec == "exact":
+        assert np.array_equal(y, o.y) and np.array_equal(v, o.v) and np.array_equal(f, o.f)
+    else:
+        assert rel_l2(y - x, o.y - x) <= 1e-10 and rel_l2(v, o.v) <= 1e-10
+
+
 def test_bb_fp64_within_1e10(pd, orc):
     say("bb fp64")
     spec = W.tensile_bb(n=10, dt=1e-4, pull_per_step=0.002)
@@ -239,6 +261,23 @@ def test_contact_truncation_keeps_largest_ids(pd, orc):
             assert rel_l2(f, o.f) <= 1e-10
 
 
+@pytest.mark.parametrize("max_neighs", [60, 2500])
+def test_contact_search_large_candidate_sets(pd, orc, max_neighs):
+    """Contact radius 11 dx between 24x24x12 plates: central particles have more
+    than 2048 candidates (the device search's in-LDS capacity), so the kept set
+    comes from the threshold search; with max_neighs = 2500 the kept lists are
+    also longer than that capacity.  Exact form, bit-identical to the oracle."""
+    spec = W.plates_ep(nx=24, ny=24, nz=12, gap=0.5, dt=1e-4, approach_per_step=0.0, exponent=1)
+    c = spec["contacts"][0]
+    c["distanceX"] = 11.0 / 0.9
+    c["max_neighs"] = max_neighs
+    say(f"contact search max_neighs={max_neighs}")
+    o, g = run_both(pd, orc, spec, 2, nuf=1, prec="exact")
+    y, v, f = g.download()
+    assert np.array_equal(f, o.f)
+    assert np.array_equal(y, o.y)
+
+
 @pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
 def test_fp32c_error_vs_fp64_oracle(pd, orc, kind):
     """FP32-compensated form against the FP64 oracle after 60 steps: relative
@@ -267,3 +306,31 @@ def test_fp32c_contact_plates(pd, orc):
     ed = rel_l2(y - x, o.y - x)
     say(f"fp32c plates: displacement rel L2 {ed:.2e}")
     assert ed <= 1e-5
+
+
+def test_graph_replay_equals_eager(pd, orc, monkeypatch):
+    """Steps replayed from the captured HIP graph equal eagerly launched steps
+    bit for bit, over several pd_run calls (each capturing and replaying its own
+    graph) with eager contact rebuilds between replays; and both equal the
+    oracle in the reference-order form."""
+    spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=2)
+    for prec in ("exact", "fp64"):
+        runs = {}
+        for mode in ("0", "1"):
+            monkeypatch.setenv("PERIDYN_GRAPH", mode)
+            g = pd.Env.from_spec(spec, precision=prec)
+            snaps = []
+            for n in (7, 12, 11):
+                g.run(n, 5)
+                snaps.append(g.download())
+            runs[mode] = snaps
+        for a, b in zip(runs["0"], runs["1"]):
+            for u, w in zip(a, b):
+                assert np.array_equal(u, w)
+        if prec == "exact":
+            o = orc.OracleEnv(spec)
+            o.run(7, 5)
+            o.run(12, 5, start=True)
+            o.run(11, 5, start=True)
+            assert np.array_equal(runs["1"][-1][0], o.y)
+            assert np.array_equal(runs["1"][-1][2], o.f)

@@ -1,0 +1,103 @@
+"""Served and batched forms on the GPU: every request's result is the same
+when it is served in a mix (continuous batching, chunked pd_run calls over
+changing sets of resident simulations) as when it runs alone in one call, and
+the FP32-compensated results stay within the stated error of plain FP64.
+
+Bars (stated here):
+  * mixed vs alone, same form: bit-identical y, v, f and damage (the specimens
+    share no data, chunks end on contact-rebuild steps, contact lists are
+    sorted, so nothing in a result depends on its batch mates);
+  * FP32C served vs FP64 alone: displacement relative L2 <= 1e-5, identical
+    damage.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from peridyn_amd import serve
+from peridyn_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def core():
+    from peridyn_amd import core
+    return core
+
+
+def requests(count=10, seed=11):
+    reqs = W.served_requests(count=count, seed=seed, nmin=400, nmax=4000, smin=30, smax=150, rate=400.0)
+    for r in reqs:
+        r["spec"] = W.make_spec(r)
+    return reqs
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_served_mix_equals_alone(core, prec):
+    reqs = requests()
+    assert {r["shape"] for r in reqs} == {"tensile", "indentation"}
+    ctx = core.Context(0)
+    srv = serve.Server(serve.GpuBackend(ctx, precision=prec), budget_bytes=4 << 30, chunk=20)
+    say(f"serve {len(reqs)} requests ({prec})")
+    res, st = srv.serve(reqs)
+    say(f"  done: {st.completed} sims, max resident {st.max_active}, {st.chunks} chunks")
+    assert st.completed == len(reqs) and st.max_active > 1
+    for r in reqs:
+        e = core.Env.from_spec(r["spec"], precision=prec, ctx=ctx)
+        e.run(r["steps"], r["nuf"])
+        y, v, f = e.download()
+        d = e.damage()
+        e.close()
+        got = res[r["id"]]
+        assert np.array_equal(got.y, y), r["id"]
+        assert np.array_equal(got.v, v), r["id"]
+        assert np.array_equal(got.f, f), r["id"]
+        assert np.array_equal(got.damage, d), r["id"]
+
+
+def test_served_fp32c_within_bar_of_fp64(core):
+    reqs = requests(count=6, seed=12)
+    ctx = core.Context(0)
+    srv = serve.Server(serve.GpuBackend(ctx, precision="fp32c"), budget_bytes=4 << 30, chunk=20)
+    res, st = srv.serve(reqs)
+    worst = 0.0
+    for r in reqs:
+        e = core.Env.from_spec(r["spec"], precision="fp64", ctx=ctx)
+        e.run(r["steps"], r["nuf"])
+        y, v, f = e.download()
+        d = e.damage()
+        e.close()
+        x = np.concatenate([b["x"] for b in r["spec"]["blocks"]])
+        err = rel_l2(res[r["id"]].y - x, y - x)
+        worst = max(worst, err)
+        say(f"  request {r['id']} {r['shape']} {r['kind']} n={x.shape[0]}: fp32c displacement rel L2 {err:.2e}")
+        assert err <= 1e-5
+        assert np.array_equal(res[r["id"]].damage, d)
+    say(f"worst fp32c displacement rel L2 {worst:.2e}")
+
+
+def test_batched_specimens_equal_alone(core):
+    """Config-3 style batch (mixed BB / OSB boxes of uneven sizes) in one
+    pd_run equals each specimen alone, FP32C form."""
+    specs = W.batched_specimens(count=6, seed=3, nmin=500, nmax=4000)
+    ctx = core.Context(0)
+    envs = [core.Env.from_spec(s, precision="fp32c", ctx=ctx) for s in specs]
+    core.run_batch(envs, 40)
+    for s, e in zip(specs, envs):
+        a = core.Env.from_spec(s, precision="fp32c", ctx=ctx)
+        a.run(40)
+        for u, w in zip(e.download(), a.download()):
+            assert np.array_equal(u, w)
+        a.close()
+        e.close()
