@@ -86,6 +86,7 @@ struct Ctx {
   int n_slots = 0;
   size_t slots_cap = 0;
   bool set_has_ep = false, set_has_theta = false;
+  bool set_gpow = false;  // some contact model of the set has a non-integer exponent
   int2* d_latch = nullptr;
   size_t latch_cap = 0;
   double4* d_latch_part = nullptr;  // per-slice partial sums of the latch test
@@ -659,15 +660,22 @@ void ensure_cap(T*& p, size_t& cap, size_t n) {
   PD_HIP(hipMalloc(&p, sizeof(T) * cap));
 }
 
+bool env_gpow(const Env& e) {
+  for (const ContactDev& cd : e.cdev)
+    if (!cd.expo_is_int) return true;
+  return false;
+}
+
 void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   if (c->set_valid && ids == c->set_ids) return;
   std::vector<Chunk> all, th, td, fk[3];
   std::vector<int32_t> slots;
-  bool has_ep = false, has_th = false;
+  bool has_ep = false, has_th = false, gpow = false;
   for (int64_t id : ids) {
     Env& e = get_env(c, id);
     if (!e.committed) throw PdError(PD_E_STATE, "env " + std::to_string(id) + " is not committed");
     slots.push_back(e.slot);
+    gpow = gpow || env_gpow(e);
     for (const Chunk& ch : e.chunks) {
       all.push_back(ch);
       const int kind = e.blocks[ch.block].d.kind;
@@ -693,6 +701,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   c->n_slots = (int)slots.size();
   c->set_has_ep = has_ep;
   c->set_has_theta = has_th;
+  c->set_gpow = gpow;
   c->set_ids = ids;
   c->set_valid = true;
 }
@@ -772,7 +781,8 @@ void force_pass(Ctx* c, Env& e, const std::vector<Chunk>& sub, int flags_extra) 
     pd_launch_theta(c->stream, e.precision, d, (int)th.size(), c->d_envs);
     pd_launch_td(c->stream, e.precision, d + o_td, (int)td.size(), c->d_envs);
     for (int k = 0; k < 3; ++k)
-      pd_launch_force(c->stream, k, e.precision, d + o_f[k], (int)fk[k].size(), c->d_envs, flags_extra);
+      pd_launch_force(c->stream, k, e.precision, d + o_f[k], (int)fk[k].size(), c->d_envs, flags_extra,
+                      env_gpow(e));
     if (e.has_ep && !td.empty()) {
       // the force pass wrote the updated plastic strain into the other buffer
       pd_launch_book(c->stream, c->d_envs, e.d_self, 1, PD_B_FLIP_EDP);
@@ -1222,7 +1232,7 @@ int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t
       for (int k = 0; k < 3; ++k)
         if (c->n_force[k])
           timed(c, PD_PROF_FORCE_BB + k,
-                [&] { pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl); });
+                [&] { pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl, c->set_gpow); });
     };
     auto book = [&](bool last) {
       timed(c, PD_PROF_BOOK, [&] {

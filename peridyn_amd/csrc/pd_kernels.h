@@ -12,7 +12,9 @@
 
 void pd_launch_theta(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_td(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* envs);
-void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int n, EnvDev* envs, int flags);
+// gpow: some contact model of the envs has a non-integer exponent
+void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int n, EnvDev* envs, int flags,
+                     bool gpow);
 void pd_launch_prologue(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_book(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int flags);
 // deformed latch over n (env slot, block) items; part holds n * nparts double4
@@ -51,4 +53,4 @@ void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDe
 
 // FP32-compensated form (pd_step32.hip)
 void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
-void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags);
+void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags, bool gpow);

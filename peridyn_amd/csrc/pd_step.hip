@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_theta(const Chunk* __restrict__ chunks,
     E.aux[p].x = th;
   } else {
     double acc = 0.0;
-    constexpr int U = 4;  // batched loads, as in material_fast
+    constexpr int U = 1;  // batched loads, as in material_fast
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
     for (int i0 = 0; i0 < ni; i0 += U) {
       int jv[U];
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_td(const Chunk* __restrict__ chunks, in
   } else {
     const double4 ap = E.aux[p];  // (theta, 1/m, ., .)
     double acc = 0.0;
-    constexpr int U = 2;  // batched loads, as in material_fast
+    constexpr int U = 1;  // batched loads, as in material_fast
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
     for (int i0 = 0; i0 < ni; i0 += U) {
       int jv[U];
@@ -403,7 +403,7 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
   // batch first, then all neighbour gathers, then the arithmetic, so that a
   // lane keeps 2U independent memory requests in flight instead of walking a
   // state -> column -> position dependency chain per bond.
-  constexpr int U = (KIND == PD_EP) ? 2 : 4;
+  constexpr int U = (KIND == PD_EP) ? 1 : 2;
   const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
   const double s0_pp = B.s0[tp + nt * tp];
   for (int i0 = 0; i0 < ni; i0 += U) {
@@ -489,9 +489,9 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
   }
 }
 
-template <int KIND, int PREC>
-__global__ __launch_bounds__(256) void k_force(const Chunk* __restrict__ chunks, int nchunks,
-                                               EnvDev* __restrict__ envs, int flags) {
+template <int KIND, int PREC, bool GPOW>
+__device__ __forceinline__ void force_body(const Chunk* __restrict__ chunks, int nchunks, EnvDev* __restrict__ envs,
+                                           int flags) {
   WaveCtx w;
   if (!wave_chunk(chunks, nchunks, envs, w)) return;
   EnvDev& E = envs[w.ch.env];
@@ -513,8 +513,23 @@ __global__ __launch_bounds__(256) void k_force(const Chunk* __restrict__ chunks,
     }
     if (!w.active || w.g != 0) return;
   }
-  add_contacts(E, p, yp, y, a1, a2, a3);
+  add_contacts<GPOW>(E, p, yp, y, a1, a2, a3);
   finish_particle(E, p, xp, yp, a1, a2, a3, flags);
+}
+
+template <int KIND, int PREC, bool GPOW>
+__global__ __launch_bounds__(256) void k_force(const Chunk* __restrict__ chunks, int nchunks,
+                                               EnvDev* __restrict__ envs, int flags) {
+  force_body<KIND, PREC, GPOW>(chunks, nchunks, envs, flags);
+}
+
+// FP64 form with integer contact exponents: held to at most 96 VGPRs (5 waves
+// per SIMD), which the elastoplastic bond loop reaches without spilling (it
+// would take 98 unconstrained, 4 waves); the other kinds use fewer anyway.
+template <int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_force_lean(
+    const Chunk* __restrict__ chunks, int nchunks, EnvDev* __restrict__ envs, int flags) {
+  force_body<KIND, PD_PREC_FP64, false>(chunks, nchunks, envs, flags);
 }
 
 // Start of a run call: kick + drift + position BCs of step time_step + 1 from
@@ -673,25 +688,30 @@ void pd_launch_td(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* e
 }
 
 template <int KIND>
-static void launch_force_kind(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* envs, int flags) {
+static void launch_force_kind(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* envs, int flags,
+                              bool gpow) {
+  const dim3 g(grid_for(n)), b(256);
   if (prec == PD_PREC_EXACT)
-    hipLaunchKernelGGL((k_force<KIND, PD_PREC_EXACT>), dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, flags);
+    hipLaunchKernelGGL((k_force<KIND, PD_PREC_EXACT, true>), g, b, 0, s, chunks, n, envs, flags);
+  else if (gpow)
+    hipLaunchKernelGGL((k_force<KIND, PD_PREC_FP64, true>), g, b, 0, s, chunks, n, envs, flags);
   else
-    hipLaunchKernelGGL((k_force<KIND, PD_PREC_FP64>), dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, flags);
+    hipLaunchKernelGGL((k_force_lean<KIND>), g, b, 0, s, chunks, n, envs, flags);
 }
 
-void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int n, EnvDev* envs, int flags) {
+void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int n, EnvDev* envs, int flags,
+                     bool gpow) {
   if (n <= 0) return;
   if (prec == PD_PREC_FP32C) {
     if (kind != PD_EP) {
-      pd_launch_force32(s, kind, chunks, n, envs, flags);
+      pd_launch_force32(s, kind, chunks, n, envs, flags, gpow);
       return;
     }
     prec = PD_PREC_FP64;
   }
-  if (kind == PD_BB) launch_force_kind<PD_BB>(s, prec, chunks, n, envs, flags);
-  else if (kind == PD_OSB) launch_force_kind<PD_OSB>(s, prec, chunks, n, envs, flags);
-  else launch_force_kind<PD_EP>(s, prec, chunks, n, envs, flags);
+  if (kind == PD_BB) launch_force_kind<PD_BB>(s, prec, chunks, n, envs, flags, gpow);
+  else if (kind == PD_OSB) launch_force_kind<PD_OSB>(s, prec, chunks, n, envs, flags, gpow);
+  else launch_force_kind<PD_EP>(s, prec, chunks, n, envs, flags, gpow);
 }
 
 void pd_launch_prologue(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) {

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
   const int d = w.active ? E.deg[p] : 0;
   const uint32_t base = w.ch.slot0 + w.lane;
   float s = 0.0f, c = 0.0f;
-  constexpr int U = 4;  // batched loads (column/state, then gathers, then math)
+  constexpr int U = 2;  // batched loads (column/state, then gathers, then math)
   const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
   for (int i0 = 0; i0 < ni; i0 += U) {
     int jv[U];
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
 }
 
 // Fused force density (BB / OSB) + FP64 velocity-Verlet (finish_particle).
-template <int KIND>
+template <int KIND, bool GPOW>
 __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunks, int nchunks,
                                                  EnvDev* __restrict__ envs, int flags) {
   WaveCtx w;
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
     if (KIND == PD_OSB) ap = E.aux32[p];
     const float c11 = (float)B.c11;
     float s1 = 0.f, s2 = 0.f, s3 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
-    constexpr int U = 4;  // batched loads (column/state, then gathers, then math)
+    constexpr int U = 2;  // batched loads (column/state, then gathers, then math)
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
     for (int i0 = 0; i0 < ni; i0 += U) {
       int jv[U];
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
     a3 = comp_total(s3, c3, w.C);
   }
   if (!w.active || w.g != 0) return;
-  add_contacts(E, p, yp, y, a1, a2, a3);
+  add_contacts<GPOW>(E, p, yp, y, a1, a2, a3);
   finish_particle(E, p, E.x[p], yp, a1, a2, a3, flags);
 }
 
@@ -199,10 +199,14 @@ void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) 
   hipLaunchKernelGGL(k_theta32<0>, dim3(grid32(n)), dim3(256), 0, s, chunks, n, envs);
 }
 
-void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags) {
+void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags, bool gpow) {
   if (n <= 0) return;
-  if (kind == PD_BB)
-    hipLaunchKernelGGL(k_force32<PD_BB>, dim3(grid32(n)), dim3(256), 0, s, chunks, n, envs, flags);
-  else
-    hipLaunchKernelGGL(k_force32<PD_OSB>, dim3(grid32(n)), dim3(256), 0, s, chunks, n, envs, flags);
+  const dim3 g(grid32(n)), b(256);
+  if (kind == PD_BB) {
+    if (gpow) hipLaunchKernelGGL((k_force32<PD_BB, true>), g, b, 0, s, chunks, n, envs, flags);
+    else hipLaunchKernelGGL((k_force32<PD_BB, false>), g, b, 0, s, chunks, n, envs, flags);
+  } else {
+    if (gpow) hipLaunchKernelGGL((k_force32<PD_OSB, true>), g, b, 0, s, chunks, n, envs, flags);
+    else hipLaunchKernelGGL((k_force32<PD_OSB, false>), g, b, 0, s, chunks, n, envs, flags);
+  }
 }

@@ -36,7 +36,11 @@ __device__ __forceinline__ double group_sum(double v, int C) {
   return v;
 }
 
-// NonLinearSpring force (nonlinearspring.jl:133-143) on dr.
+// NonLinearSpring force (nonlinearspring.jl:133-143) on dr.  GPOW admits
+// non-integer exponents (Julia's Float64 ^ through the device pow); without it
+// every exponent is integer (the host launches the GPOW kernels otherwise) and
+// the kernel carries no pow code, whose registers would cap its occupancy.
+template <bool GPOW>
 __device__ __forceinline__ void contact_fn(const ContactDev& C, double d1, double d2, double d3,
                                            double& F1, double& F2, double& F3) {
   const double mag = pd_mag(d1, d2, d3);
@@ -46,7 +50,9 @@ __device__ __forceinline__ void contact_fn(const ContactDev& C, double d1, doubl
     F1 = 0.0; F2 = 0.0; F3 = 0.0;
     return;
   }
-  const double pw = C.expo_is_int ? pd_pow_int(strain, (int64_t)C.expo) : pow(strain, C.expo);
+  double pw;
+  if (GPOW && !C.expo_is_int) pw = pow(strain, C.expo);
+  else pw = pd_pow_int(strain, (int64_t)C.expo);
   const double s = (C.K * pw) / (mag + 1e-10);
   F1 = s * d1; F2 = s * d2; F3 = s * d3;
 }
@@ -54,6 +60,7 @@ __device__ __forceinline__ void contact_fn(const ContactDev& C, double d1, doubl
 // Contact terms of particle p, models in order (update_contact_acc!,
 // solvers.jl:257-269; short_range_contact! RM12, contacts.jl:230-245), added to
 // the material force as the reference's f1/f2 updates are.
+template <bool GPOW>
 __device__ __forceinline__ void add_contacts(const EnvDev& E, int p, const double4& yp, const double4* y,
                                              double& a1, double& a2, double& a3) {
   const double4* __restrict__ x = E.x;
@@ -67,7 +74,7 @@ __device__ __forceinline__ void add_contacts(const EnvDev& E, int p, const doubl
         const int q = C.idx2[C.own[(size_t)i1 * C.max_neighs + k]];
         const double4 yq = y[q];
         double F1, F2, F3;
-        contact_fn(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+        contact_fn<GPOW>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
         const double a = x[q].w * inv_den_p;
         a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
       }
@@ -80,7 +87,7 @@ __device__ __forceinline__ void add_contacts(const EnvDev& E, int p, const doubl
         const int q = C.idx1[C.rev[off + k]];
         const double4 yq = y[q];
         double F1, F2, F3;
-        contact_fn(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+        contact_fn<GPOW>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
         const double b = x[q].w * inv_den_p;
         a1 -= F1 * b; a2 -= F2 * b; a3 -= F3 * b;
       }

@@ -246,6 +246,21 @@ def test_contact_plates_fp64_within_1e10(pd, orc):
     assert rel_l2(y - x, o.y - x) <= 1e-10
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_contact_noninteger_exponent(pd, orc, prec):
+    """Nonlinear spring with exponent 1.5 (the kernels carrying the general
+    power): within the FP64 bar of the oracle (the oracle's pow is the C
+    library's, the device's the ROCm math library's; both round to ~1 ulp)."""
+    spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=2)
+    spec["contacts"][0]["exponent"] = 1.5
+    o, g = run_both(pd, orc, spec, 30, nuf=5, prec=prec)
+    y, v, f = g.download()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    assert np.abs(o.f).max() > 0
+    assert rel_l2(f, o.f) <= 1e-10
+    assert rel_l2(y - x, o.y - x) <= 1e-10
+
+
 def test_contact_truncation_keeps_largest_ids(pd, orc):
     """max_neighs smaller than the candidate count: the kept set is the
     max_neighs largest block-2 ids (contacts.jl:468-469), in both forms."""
