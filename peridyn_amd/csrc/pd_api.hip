@@ -342,7 +342,8 @@ void commit_env(Ctx* c, Env& e) {
   if (e.blocks.empty()) throw PdError(PD_E_ARG, "env has no material blocks");
   int64_t N = 0;
   for (auto& b : e.blocks) { b.first = (int32_t)N; N += b.d.n; }
-  if (N >= (1LL << 31) - 64) throw PdError(PD_E_ARG, "env too large (particles)");
+  if (N >= (1LL << 31) - 64 || (uint64_t)N * sizeof(double4) >= PD_MAX_ARRAY_BYTES)
+    throw PdError(PD_E_ARG, "env too large: its particle arrays would exceed 4 GiB; split it into several envs");
   e.N = N;
   // bond layout: the reference-order form walks each row on one lane; the
   // other forms spread a particle over G lanes so that small specimens still
@@ -473,6 +474,9 @@ void commit_env(Ctx* c, Env& e) {
       if (b.d.kind == PD_EP) e.has_ep = true;
     }
     e.nslots = (int64_t)slots;
+    // kernels address every array with 32-bit byte offsets
+    if ((uint64_t)slots * (e.has_ep ? sizeof(double) : sizeof(int32_t)) >= PD_MAX_ARRAY_BYTES)
+      throw PdError(PD_E_ARG, "env too large: its bond arrays would exceed 4 GiB; split it into several envs");
     PD_HIP(hipMemcpyAsync(D.pslot0, pslot0.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s));
     D.col = dalloc<int32_t>(e, slots);
     D.state = dalloc<uint8_t>(e, slots);

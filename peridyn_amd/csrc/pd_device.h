@@ -84,31 +84,43 @@ __device__ __forceinline__ int pd_xcd_block(int nblocks) {
 // ---- global-memory accessors ------------------------------------------------
 // The array pointers live in the device-side EnvDev table, so the compiler
 // cannot prove they address global memory and would emit generic (flat)
-// loads; these casts make the hot loops use global_load/global_store.
+// loads; these casts make the hot loops use global_load/global_store.  The
+// byte offset is formed in 32 bits and added to the (wave-uniform) base, so the
+// loads take the scalar-base form (one VGPR of offset, no 64-bit address
+// arithmetic per lane).  Every array of an env therefore spans < 4 GiB, which
+// pd_env_commit checks (PD_MAX_ARRAY_BYTES).
 typedef double pd_d4v __attribute__((ext_vector_type(4)));
 typedef float pd_f4v __attribute__((ext_vector_type(4)));
 #define PD_GLOBAL __attribute__((address_space(1)))
 
 template <typename T>
+__device__ __forceinline__ const PD_GLOBAL T* gaddr(const T* p, uint32_t i) {
+  return (const PD_GLOBAL T*)((const PD_GLOBAL char*)p + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ PD_GLOBAL T* gaddr_w(T* p, uint32_t i) {
+  return (PD_GLOBAL T*)((PD_GLOBAL char*)p + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
 __device__ __forceinline__ T gld(const T* p, uint32_t i) {
-  return ((const PD_GLOBAL T*)p)[i];
+  return *gaddr(p, i);
 }
 template <typename T>
 __device__ __forceinline__ void gst(T* p, uint32_t i, T v) {
-  ((PD_GLOBAL T*)p)[i] = v;
+  *gaddr_w(p, i) = v;
 }
 __device__ __forceinline__ double4 gld4(const double4* p, uint32_t i) {
-  const pd_d4v v = ((const PD_GLOBAL pd_d4v*)p)[i];
+  const pd_d4v v = *(const PD_GLOBAL pd_d4v*)gaddr(p, i);
   return make_double4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ float4 gld4(const float4* p, uint32_t i) {
-  const pd_f4v v = ((const PD_GLOBAL pd_f4v*)p)[i];
+  const pd_f4v v = *(const PD_GLOBAL pd_f4v*)gaddr(p, i);
   return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ void gst4(double4* p, uint32_t i, double4 v) {
   pd_d4v w;
   w.x = v.x; w.y = v.y; w.z = v.z; w.w = v.w;
-  ((PD_GLOBAL pd_d4v*)p)[i] = w;
+  *(PD_GLOBAL pd_d4v*)gaddr_w(p, i) = w;
 }
 
 // ---- fast-form FP64 reciprocals (not used by the reference-order form) -------

@@ -27,6 +27,7 @@
 #define PD_WAVE 64
 #define PD_WAVES_PER_BLOCK 4
 #define PD_MAX_TYPES 8
+#define PD_MAX_ARRAY_BYTES (1ull << 32)  // per device array of an env (32-bit offsets)
 #define PD_MAX_CONTACTS 8
 
 struct BlockDev {

@@ -102,24 +102,32 @@ __global__ __launch_bounds__(256) void k_theta(const Chunk* __restrict__ chunks,
     E.aux[p].x = th;
   } else {
     double acc = 0.0;
-    constexpr int U = 1;  // batched loads, as in material_fast
+    constexpr int U = 1;  // bonds per batch, streamed data one batch ahead (material_fast)
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
-    for (int i0 = 0; i0 < ni; i0 += U) {
-      int jv[U];
-      uint8_t sv[U];
+    int jn[U];
+    uint8_t sn[U];
+    auto load_stream = [&](int i0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const bool ok = (i0 + u) < ni;
         const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
-        jv[u] = ok ? gld(col, slot) : p;
-        sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+        jn[u] = ok ? gld(col, slot) : p;
+        sn[u] = ok ? gld(st, slot) : (uint8_t)0;
       }
+    };
+    load_stream(0);
+    for (int i0 = 0; i0 < ni; i0 += U) {
+      int jv[U];
+      uint8_t sv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) { jv[u] = jn[u]; sv[u] = sn[u]; }
       double4 xj[U], yj[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         xj[u] = gld4(x, jv[u]);
         yj[u] = gld4(y, jv[u]);
       }
+      if (i0 + U < ni) load_stream(i0 + U);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (!sv[u]) continue;
@@ -199,27 +207,36 @@ __global__ __launch_bounds__(256) void k_td(const Chunk* __restrict__ chunks, in
   } else {
     const double4 ap = E.aux[p];  // (theta, 1/m, ., .)
     double acc = 0.0;
-    constexpr int U = 1;  // batched loads, as in material_fast
+    constexpr int U = 1;  // bonds per batch, streamed data one batch ahead (material_fast)
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
-    for (int i0 = 0; i0 < ni; i0 += U) {
-      int jv[U];
-      uint8_t sv[U];
+    int jn[U];
+    uint8_t sn[U];
+    double en[U];
+    auto load_stream = [&](int i0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const bool ok = (i0 + u) < ni;
         const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
-        jv[u] = ok ? gld(col, slot) : p;
-        sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+        jn[u] = ok ? gld(col, slot) : p;
+        sn[u] = ok ? gld(st, slot) : (uint8_t)0;
+        en[u] = ok ? gld(edp, slot) : 0.0;
       }
-      double4 xj[U], yj[U], aj[U];
+    };
+    load_stream(0);
+    for (int i0 = 0; i0 < ni; i0 += U) {
+      int jv[U];
+      uint8_t sv[U];
       double eo[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) { jv[u] = jn[u]; sv[u] = sn[u]; eo[u] = en[u]; }
+      double4 xj[U], yj[U], aj[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         xj[u] = gld4(x, jv[u]);
         yj[u] = gld4(y, jv[u]);
         aj[u] = gld4(E.aux, jv[u]);
-        eo[u] = sv[u] ? gld(edp, base + 64u * (uint32_t)(i0 + u)) : 0.0;
       }
+      if (i0 + U < ni) load_stream(i0 + U);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (!sv[u]) continue;
@@ -400,9 +417,9 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
   }
   const double c11 = B.c11;
   // Bonds in batches of U: all column/state (and reverse-slot) loads of the
-  // batch first, then all neighbour gathers, then the arithmetic, so that a
-  // lane keeps 2U independent memory requests in flight instead of walking a
-  // state -> column -> position dependency chain per bond.
+  // batch first, then all neighbour gathers, then the arithmetic.  Loading the
+  // next batch's streamed data ahead (as the theta / td passes do) measured
+  // slower here: its registers cost these kernels a wave per SIMD.
   constexpr int U = (KIND == PD_EP) ? 1 : 2;
   const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
   const double s0_pp = B.s0[tp + nt * tp];
@@ -410,26 +427,25 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
     int jv[U];
     uint8_t sv[U];
     uint32_t rv[U];
+    double eo[U], er[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const bool ok = (i0 + u) < ni;
       const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
       jv[u] = ok ? gld(col, slot) : p;
       sv[u] = ok ? gld(st, slot) : (uint8_t)0;
-      if (KIND == PD_EP) rv[u] = (ok && sv[u]) ? gld(E.rev, slot) : slot;
+      if (KIND == PD_EP) {
+        rv[u] = ok ? gld(E.rev, slot) : slot;
+        eo[u] = ok ? gld(edp_in, slot) : 0.0;
+      }
     }
     double4 xj[U], yj[U], aj[U];
-    double eo[U], er[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       xj[u] = gld4(x, jv[u]);
       yj[u] = gld4(y, jv[u]);
       if (KIND != PD_BB) aj[u] = gld4(E.aux, jv[u]);
-      if (KIND == PD_EP) {
-        const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
-        eo[u] = sv[u] ? gld(edp_in, slot) : 0.0;
-        er[u] = sv[u] ? gld(edp_in, rv[u]) : 0.0;
-      }
+      if (KIND == PD_EP) er[u] = sv[u] ? gld(edp_in, rv[u]) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {

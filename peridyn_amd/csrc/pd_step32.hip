@@ -74,18 +74,25 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
   const int d = w.active ? E.deg[p] : 0;
   const uint32_t base = w.ch.slot0 + w.lane;
   float s = 0.0f, c = 0.0f;
-  constexpr int U = 2;  // batched loads (column/state, then gathers, then math)
+  constexpr int U = 2;  // bonds per batch; column/state loaded one batch ahead
   const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
-  for (int i0 = 0; i0 < ni; i0 += U) {
-    int jv[U];
-    uint8_t sv[U];
+  int jn[U];
+  uint8_t sn[U];
+  auto load_stream = [&](int i0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const bool ok = (i0 + u) < ni;
       const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
-      jv[u] = ok ? gld(col, slot) : p;
-      sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+      jn[u] = ok ? gld(col, slot) : p;
+      sn[u] = ok ? gld(st, slot) : (uint8_t)0;
     }
+  };
+  load_stream(0);
+  for (int i0 = 0; i0 < ni; i0 += U) {
+    int jv[U];
+    uint8_t sv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { jv[u] = jn[u]; sv[u] = sn[u]; }
     float4 xj[U];
     double4 yj[U];
 #pragma unroll
@@ -93,6 +100,7 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
       xj[u] = gld4(x, jv[u]);
       yj[u] = gld4(y, jv[u]);
     }
+    if (i0 + U < ni) load_stream(i0 + U);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (!sv[u]) continue;
@@ -135,18 +143,25 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
     if (KIND == PD_OSB) ap = E.aux32[p];
     const float c11 = (float)B.c11;
     float s1 = 0.f, s2 = 0.f, s3 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
-    constexpr int U = 2;  // batched loads (column/state, then gathers, then math)
+    constexpr int U = 2;  // bonds per batch; column/state loaded one batch ahead
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
-    for (int i0 = 0; i0 < ni; i0 += U) {
-      int jv[U];
-      uint8_t sv[U];
+    int jn[U];
+    uint8_t sn[U];
+    auto load_stream = [&](int i0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const bool ok = (i0 + u) < ni;
         const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
-        jv[u] = ok ? gld(col, slot) : p;
-        sv[u] = ok ? gld(st, slot) : (uint8_t)0;
+        jn[u] = ok ? gld(col, slot) : p;
+        sn[u] = ok ? gld(st, slot) : (uint8_t)0;
       }
+    };
+    load_stream(0);
+    for (int i0 = 0; i0 < ni; i0 += U) {
+      int jv[U];
+      uint8_t sv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) { jv[u] = jn[u]; sv[u] = sn[u]; }
       float4 xj[U], aj[U];
       double4 yj[U];
 #pragma unroll
@@ -155,6 +170,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
         yj[u] = gld4(y, jv[u]);
         if (KIND == PD_OSB) aj[u] = gld4(E.aux32, jv[u]);
       }
+      if (i0 + U < ni) load_stream(i0 + U);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (!sv[u]) continue;
