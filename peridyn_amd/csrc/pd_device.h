@@ -142,6 +142,13 @@ __device__ __forceinline__ double pd_rcp(double a) {
   return __builtin_fma(r, e, r);
 }
 
+// Fast forms, elastoplastic blocks: the td pass keeps a sticky "has yielded"
+// flag in the sign bit of aux.z (|aux.z| = td_norm2 >= 0).  A particle that has
+// never yielded (C > 1 at every step so far) has never changed its plastic
+// extensions, which are therefore exactly 0, so their loads and stores can be
+// skipped without changing any result.
+__device__ __forceinline__ bool pd_has_yielded(const double4& aux) { return __builtin_signbit(aux.z); }
+
 // Slot of bond k of a particle whose bond 0 sits at `base` (layout in pd_internal.h).
 __host__ __device__ __forceinline__ uint32_t pd_slot(uint32_t base, int k, int lgG) {
   const int G = 1 << lgG;

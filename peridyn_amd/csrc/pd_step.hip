@@ -205,7 +205,9 @@ __global__ __launch_bounds__(256) void k_td(const Chunk* __restrict__ chunks, in
     E.aux[p].z = acc;
     E.aux[p].w = __builtin_sqrt((2.0 * B.psi[tp + nt * tp]) / acc);
   } else {
-    const double4 ap = E.aux[p];  // (theta, 1/m, ., .)
+    const double4 ap = E.aux[p];  // (theta, 1/m, +-td_norm2 of the last step, C)
+    // p's plastic extensions are all exactly 0 until it first yields
+    const bool yielded = pd_has_yielded(ap);
     double acc = 0.0;
     constexpr int U = 1;  // bonds per batch, streamed data one batch ahead (material_fast)
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) void k_td(const Chunk* __restrict__ chunks, in
         const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
         jn[u] = ok ? gld(col, slot) : p;
         sn[u] = ok ? gld(st, slot) : (uint8_t)0;
-        en[u] = ok ? gld(edp, slot) : 0.0;
+        en[u] = (ok && yielded) ? gld(edp, slot) : 0.0;
       }
     };
     load_stream(0);
@@ -257,15 +259,19 @@ __global__ __launch_bounds__(256) void k_td(const Chunk* __restrict__ chunks, in
           acc += td * td * xj[u].w;
         } else {
           gst(st, slot, (uint8_t)0);
-          gst(edp_o, slot, eo[u]);
+          if (yielded) gst(edp_o, slot, eo[u]);  // else 0 in both buffers already
         }
       }
     }
     acc = group_sum(acc, w.C);
     if (w.active && w.g == 0) {
+      const double C = __builtin_sqrt((2.0 * B.psi[tp + nt * tp]) / acc);
+      // sticky yield flag in the sign of aux.z (multi-type blocks yield per type
+      // pair: flagged from the start)
+      const bool flag = yielded || !(C > 1.0) || nt > 1;
       E.td[p] = acc;
-      E.aux[p].z = acc;
-      E.aux[p].w = __builtin_sqrt((2.0 * B.psi[tp + nt * tp]) / acc);
+      E.aux[p].z = flag ? -acc : acc;
+      E.aux[p].w = C;
     }
   }
 }
@@ -411,9 +417,11 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
   const double* __restrict__ edp_in = nullptr;
   double* __restrict__ edp_out = nullptr;
   if (KIND != PD_BB) ap = E.aux[p];
+  bool yielded_p = false;  // p's plastic extensions are exactly 0 until it yields
   if (KIND == PD_EP) {
     edp_in = E.edp[E.edp_cur];
     edp_out = E.edp[E.edp_cur ^ 1];
+    yielded_p = pd_has_yielded(ap);
   }
   const double c11 = B.c11;
   // Bonds in batches of U: all column/state (and reverse-slot) loads of the
@@ -434,10 +442,7 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
       const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
       jv[u] = ok ? gld(col, slot) : p;
       sv[u] = ok ? gld(st, slot) : (uint8_t)0;
-      if (KIND == PD_EP) {
-        rv[u] = ok ? gld(E.rev, slot) : slot;
-        eo[u] = ok ? gld(edp_in, slot) : 0.0;
-      }
+      if (KIND == PD_EP) eo[u] = (ok && yielded_p) ? gld(edp_in, slot) : 0.0;
     }
     double4 xj[U], yj[U], aj[U];
 #pragma unroll
@@ -445,7 +450,17 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
       xj[u] = gld4(x, jv[u]);
       yj[u] = gld4(y, jv[u]);
       if (KIND != PD_BB) aj[u] = gld4(E.aux, jv[u]);
-      if (KIND == PD_EP) er[u] = sv[u] ? gld(edp_in, rv[u]) : 0.0;
+    }
+    if (KIND == PD_EP) {
+      // the reverse bond's plastic extension, 0 unless j has yielded
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        er[u] = 0.0;
+        if (sv[u] && pd_has_yielded(aj[u])) {
+          rv[u] = gld(E.rev, base + 64u * (uint32_t)(i0 + u));
+          er[u] = gld(edp_in, rv[u]);
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -477,8 +492,8 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
             const double g15 = 15.0 * B.G[ij] * (inv_x * sm);
             double Cp = ap.w, Cj = aj[u].w;
             if (nt > 1) {
-              Cp = __builtin_sqrt((2.0 * B.psi[ij]) / ap.z);
-              Cj = __builtin_sqrt((2.0 * B.psi[ij]) / aj[u].z);
+              Cp = __builtin_sqrt((2.0 * B.psi[ij]) / __builtin_fabs(ap.z));
+              Cj = __builtin_sqrt((2.0 * B.psi[ij]) / __builtin_fabs(aj[u].z));
             }
             const double ede = ed - eo[u];
             double td_own = g15 * ede;
@@ -487,7 +502,7 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
               td_own *= Cp;
               e_new = eo[u] + ede * (1.0 - Cp);
             }
-            gst(edp_out, slot, e_new);
+            if (yielded_p) gst(edp_out, slot, e_new);  // else e_new = 0 = stored
             double td_rev = g15 * (ed - er[u]);
             if (!(Cj > 1.0)) td_rev *= Cj;
             t = 3.0 * B.K[ij] * twm + 0.5 * (td_own + td_rev);
@@ -499,7 +514,7 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
         a3 += xx * Y3;
       } else {
         gst(st, slot, (uint8_t)0);
-        if (KIND == PD_EP) gst(edp_out, slot, eo[u]);
+        if (KIND == PD_EP && yielded_p) gst(edp_out, slot, eo[u]);
       }
     }
   }
