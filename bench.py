@@ -188,6 +188,18 @@ def bench_single(args):
     el_prof = time.perf_counter() - t0
     prof = read_profile(ctx)
     env.close()
+    logged = None
+    if args.log_every > 0:
+        # the same run logging the printdata columns every log_every steps
+        env = core.Env.from_spec(spec, precision=args.precision, ctx=ctx, lanes_per_particle=lanes)
+        ctx.sync()
+        t0 = time.perf_counter()
+        rows = env.run_logged(args.steps, nuf, args.log_every)
+        ctx.sync()
+        el_log = time.perf_counter() - t0
+        env.close()
+        logged = {"every": args.log_every, "rows": int(rows.shape[0]), "ms_per_step": 1e3 * el_log / args.steps,
+                  "last_row": [float(v) for v in rows[-1]]}
     form = form_of(args.precision)
     value = nnz * args.steps / el
     out = {
@@ -211,6 +223,8 @@ def bench_single(args):
         "kernel_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1] > 0},
         "eager_profiled_run": {"value": nnz * args.steps / el_prof, "ms_per_step": 1e3 * el_prof / args.steps},
     }
+    if logged:
+        out["logged_run"] = logged
     if not args.no_cpu_baseline:
         cb = cpu_baseline(spec, nuf, args.cpu_budget, nnz)
         out["cpu_baseline"] = cb
@@ -328,9 +342,10 @@ def _progress(label: str, total: int):
     return cb
 
 
-def served_summary(ctx, count: int, rate: float, seq: bool = True, seed: int = 20261204):
+def served_summary(ctx, count: int, rate: float, seq: bool = True, seed: int = 20261204, lanes: int = 0):
     """Serve `count` requests (FP32C, continuous batching) and, if `seq`, the
-    same stream one request at a time in plain FP64."""
+    same stream one request at a time in plain FP64.  `lanes`: lanes per
+    particle of the served envs (0: each env's own choice)."""
     from peridyn_amd import serve
     from peridyn_amd import workloads as W
 
@@ -342,10 +357,11 @@ def served_summary(ctx, count: int, rate: float, seq: bool = True, seed: int = 2
     # warm-up (code objects) on two small requests
     warm = [dict(r, arrival=0.0) for r in sorted(reqs, key=lambda r: r["n"])[:2]]
     serve.Server(serve.GpuBackend(ctx, "fp32c", keep_state=False), budget, chunk=50).serve(warm)
-    srv = serve.Server(serve.GpuBackend(ctx, "fp32c", keep_state=False), budget, chunk=50,
-                       on_retire=_progress("served fp32c", count))
+    srv = serve.Server(serve.GpuBackend(ctx, "fp32c", keep_state=False, lanes_per_particle=lanes), budget,
+                       chunk=50, on_retire=_progress("served fp32c", count))
     _, st = srv.serve(reqs)
     out = {"requests": count, "arrival_rate_per_s": rate, "precision": "fp32c", "chunk_steps": 50,
+           "lanes_per_particle": lanes,
            "budget_bytes": budget, **st.as_dict()}
     if seq:
         one = serve.Server(serve.GpuBackend(ctx, "fp64", keep_state=False), budget, chunk=None, max_active=1,
@@ -361,7 +377,8 @@ def bench_served(args):
     from peridyn_amd import core
 
     ctx = core.Context(0)
-    s = served_summary(ctx, args.served_requests or 64, args.served_rate, seq=True)
+    s = served_summary(ctx, args.served_requests or 64, args.served_rate, seq=not args.no_seq,
+                       lanes=args.served_lanes)
     out = {
         "metric": METRIC,
         "value": s["sims_per_s"],
@@ -395,12 +412,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: chosen at upload)")
     ap.add_argument("--autotune", action="store_true", help="time every schedule first and use the fastest")
+    ap.add_argument("--log-every", type=int, default=0,
+                    help="also time the run logging the printdata columns every K steps")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--seq-sample", type=int, default=64,
                     help="batched: specimens timed one at a time in FP64 (0: all)")
     ap.add_argument("--served-requests", type=int, default=0,
                     help="served: number of requests; with other workloads, add a served summary")
     ap.add_argument("--served-rate", type=float, default=40.0, help="served: Poisson arrivals per second")
+    ap.add_argument("--served-lanes", type=int, default=4,
+                    help="served: lanes per particle of the served envs (0: each env's own choice; "
+                         "4 measured best for the default stream)")
+    ap.add_argument("--no-seq", action="store_true", help="served: skip the one-at-a-time FP64 baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (x3) whose HBM counter readings "
                          "scale FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc")

@@ -140,6 +140,19 @@ int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t row
  * after every step i with i % nuf == 0, i counted from 1 in each call (run!'s
  * default start_at = 0).  Returns when the steps have completed. */
 int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf);
+/* pd_run plus the printdata columns (environment.jl:308-320) that run! logs
+ * after step 1 and after every `every`-th step (solvers.jl:158-167, every =
+ * average_prop_freq): per logged step and env 8 doubles, time (time_step * dt),
+ * px, py, pz (sum of p = mass .* v as update_misc! sets it inside update_acc!,
+ * solvers.jl:289-291, i.e. from the step's half-step velocities), Fx, Fy, Fz
+ * (sum of f) and the mean damage (cal_damage, peri.jl:587-594).  log holds
+ * pd_log_rows(nsteps, every) x nenv x 8 doubles, rows in step order. */
+int pd_run_logged(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf, int64_t every,
+                  double* log);
+int64_t pd_log_rows(int64_t nsteps, int64_t every);
+/* The same 8 columns for the env's current state, momentum from the current
+ * velocities (as after update_acc!, which calls update_misc!). */
+int pd_env_observables(void* ctx, int64_t env, double* out);
 /* update_acc! (solvers.jl:320-336) at the current positions; y may be NULL
  * (resident state) or 3 x N positions to upload first.  f receives 3 x N. */
 int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f);

@@ -357,6 +357,15 @@ class OracleEnv:
             b.force_density(self.y[lo:hi], self.f[lo:hi])
         for c in self.contacts:
             self._contact_force(c)
+        # update_misc! (solvers.jl:289-291): momentum from the current velocities
+        self.p = self.mass[:, None] * self.v
+
+    def printdata(self) -> np.ndarray:
+        """printdata columns (environment.jl:308-320): time, sum(p), sum(f),
+        mean(cal_damage).  Sums in numpy's pairwise order, as Julia's sum."""
+        p = getattr(self, "p", np.zeros_like(self.v))
+        d = self.damage()
+        return np.array([self.time_step * self.dt, *p.sum(axis=0), *self.f.sum(axis=0), d.sum() / d.size])
 
     # -- boundary conditions -------------------------------------------------
     def apply_bc_at0(self):
@@ -401,15 +410,22 @@ class OracleEnv:
         self._bc_check()
         self.time_step += 1
 
-    def run(self, nsteps: int, nuf: int = 1, start: bool = True):
-        """run! physics cadence (solvers.jl:97-187) without file output."""
+    def run(self, nsteps: int, nuf: int = 1, start: bool = True, log_every: int = 0):
+        """run! physics cadence (solvers.jl:97-187) without file output; with
+        log_every > 0 returns the printdata rows of step 1 and of every
+        log_every-th step (solvers.jl:158-167), taken before that step's
+        neighbour update."""
         if start:
             self.update_neighs()
             self.apply_bc_at0()
+        rows = []
         for i in range(1, nsteps + 1):
             self.step()
+            if log_every > 0 and (i % log_every == 0 or i == 1):
+                rows.append(self.printdata())
             if i % nuf == 0:
                 self.update_neighs()
+        return np.array(rows) if log_every > 0 else None
 
     def damage(self) -> np.ndarray:
         counts = np.concatenate([b.intact.sum(axis=1) for b in self.blocks]).astype(np.float64)

@@ -212,6 +212,18 @@ class Env:
         ids = np.array([self.id], dtype=np.int64)
         self.ctx.check(self.ctx.lib.pd_run(self.ctx.h, _ptr(ids), 1, int(nsteps), int(nuf)))
 
+    def run_logged(self, nsteps: int, nuf: int = 1, every: int = 1) -> np.ndarray:
+        """run() plus the printdata rows (time, px, py, pz, Fx, Fy, Fz, mean
+        damage) of step 1 and every `every`-th step: array (rows, 8)."""
+        return run_batch_logged([self], nsteps, nuf, every)[:, 0, :]
+
+    def observables(self) -> np.ndarray:
+        """The 8 printdata columns of the current state (momentum from the
+        current velocities, as after update_acc!)."""
+        out = np.zeros(8)
+        self.ctx.check(self.ctx.lib.pd_env_observables(self.ctx.h, self.id, _ptr(out)))
+        return out
+
     def update_acc(self, y: Optional[np.ndarray] = None) -> np.ndarray:
         f = np.zeros((self.N, 3))
         yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
@@ -296,3 +308,14 @@ def run_batch(envs: Iterable[Env], nsteps: int, nuf: int = 1):
     ctx = envs[0].ctx
     ids = np.array([e.id for e in envs], dtype=np.int64)
     ctx.check(ctx.lib.pd_run(ctx.h, _ptr(ids), len(envs), int(nsteps), int(nuf)))
+
+
+def run_batch_logged(envs: Iterable[Env], nsteps: int, nuf: int = 1, every: int = 1) -> np.ndarray:
+    """run_batch plus the printdata rows of every env: array (rows, nenv, 8)."""
+    envs = list(envs)
+    ctx = envs[0].ctx
+    ids = np.array([e.id for e in envs], dtype=np.int64)
+    rows = int(ctx.lib.pd_log_rows(int(nsteps), int(every)))
+    log = np.zeros((max(rows, 1), len(envs), 8))
+    ctx.check(ctx.lib.pd_run_logged(ctx.h, _ptr(ids), len(envs), int(nsteps), int(nuf), int(every), _ptr(log)))
+    return log[:rows]

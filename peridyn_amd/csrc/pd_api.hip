@@ -87,6 +87,14 @@ struct Ctx {
   size_t slots_cap = 0;
   bool set_has_ep = false, set_has_theta = false;
   bool set_gpow = false;  // some contact model of the set has a non-integer exponent
+  // observables: chunk range (first, count) of each env of the set in its
+  // chunk list, per-chunk partial sums, one 8-column row per env and logged step
+  int2* d_obs_ranges = nullptr;
+  size_t obs_ranges_cap = 0;
+  double4* d_obs_part = nullptr;
+  size_t obs_part_cap = 0;
+  double* d_obs_rows = nullptr;
+  size_t obs_rows_cap = 0;
   int2* d_latch = nullptr;
   size_t latch_cap = 0;
   double4* d_latch_part = nullptr;  // per-slice partial sums of the latch test
@@ -674,12 +682,14 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   if (c->set_valid && ids == c->set_ids) return;
   std::vector<Chunk> all, th, td, fk[3];
   std::vector<int32_t> slots;
+  std::vector<int2> ranges;
   bool has_ep = false, has_th = false, gpow = false;
   for (int64_t id : ids) {
     Env& e = get_env(c, id);
     if (!e.committed) throw PdError(PD_E_STATE, "env " + std::to_string(id) + " is not committed");
     slots.push_back(e.slot);
     gpow = gpow || env_gpow(e);
+    ranges.push_back(make_int2((int)all.size(), (int)e.chunks.size()));
     for (const Chunk& ch : e.chunks) {
       all.push_back(ch);
       const int kind = e.blocks[ch.block].d.kind;
@@ -701,6 +711,10 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   PD_HIP(hipMemcpyAsync(c->d_set, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
   ensure_cap(c->d_slots, c->slots_cap, slots.size());
   PD_HIP(hipMemcpyAsync(c->d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, c->stream));
+  ensure_cap(c->d_obs_ranges, c->obs_ranges_cap, ranges.size());
+  PD_HIP(hipMemcpyAsync(c->d_obs_ranges, ranges.data(), sizeof(int2) * ranges.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  ensure_cap(c->d_obs_part, c->obs_part_cap, std::max<size_t>(all.size(), 1));
   PD_HIP(hipStreamSynchronize(c->stream));
   c->n_slots = (int)slots.size();
   c->set_has_ep = has_ep;
@@ -904,6 +918,119 @@ bool graphs_enabled() {
   return !(v && std::string(v) == "0");
 }
 
+// run! for the listed envs (pd_run); with every > 0 the printdata columns of
+// step 1 and of every every-th step (solvers.jl:158) go to log_host (rows x
+// nenv x 8, rows in step order).
+void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t nuf, int64_t every,
+              double* log_host) {
+  PD_HIP(hipSetDevice(c->device));
+  const int prec = precision_of(c, ids);
+  build_set(c, ids);
+  hipStream_t s = c->stream;
+  const int nenv = (int)ids.size();
+  const int64_t nrows = every > 0 ? pd_log_rows(nsteps, every) : 0;
+  if (nrows > 0) ensure_cap(c->d_obs_rows, c->obs_rows_cap, (size_t)(nrows * nenv * 8));
+  // run! start: update_neighs! (solvers.jl:114); apply_bc_at0 records
+  // BC.last = y, which the resident state already holds.  Contact lists are
+  // a deterministic function of the positions (sorted), so lists built from
+  // the current positions are kept rather than rebuilt identically.
+  for (int64_t id : ids) {
+    Env& e = get_env(c, id);
+    if (!e.contacts.empty() && !e.lists_current) contact_update(c, e);
+  }
+  if (nsteps == 0) return;
+  const int nl = prepare_latch(c, ids);
+  EnvDev* T = c->d_envs;
+  Chunk* d = c->d_set;
+  timed(c, PD_PROF_PROLOGUE, [&] { pd_launch_prologue(s, d, c->n_all, T); });
+  pd_launch_book(s, T, c->d_slots, c->n_slots, PD_B_FLIP_Y);
+  int64_t row = 0;
+  // the kernels of one velocity_verlet_step! (all envs of the set); a logged
+  // step takes the momentum from the half-step velocities just before the
+  // force pass (update_misc! runs inside update_acc!) and the force and damage
+  // sums after it
+  auto step_kernels = [&](bool last, bool logged) {
+    if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part); });
+    if (c->set_has_theta)
+      timed(c, PD_PROF_THETA, [&] { pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T); });
+    if (c->set_has_ep) timed(c, PD_PROF_TD, [&] { pd_launch_td(s, prec, d + c->off_td, c->n_td, T); });
+    double* rows = logged ? c->d_obs_rows + (size_t)row * nenv * 8 : nullptr;
+    if (logged) pd_launch_observables(s, 0, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
+    const int fl = PD_F_KICK | (last ? 0 : PD_F_DRIFT);
+    for (int k = 0; k < 3; ++k)
+      if (c->n_force[k])
+        timed(c, PD_PROF_FORCE_BB + k,
+              [&] { pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl, c->set_gpow); });
+    if (logged) pd_launch_observables(s, 1, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
+  };
+  auto book = [&](bool last) {
+    timed(c, PD_PROF_BOOK, [&] {
+      pd_launch_book(s, T, c->d_slots, c->n_slots,
+                     PD_B_STEP | (last ? 0 : PD_B_FLIP_Y) | (c->set_has_ep ? PD_B_FLIP_EDP : 0));
+    });
+  };
+  // Steps that are neither the last, nor logged, nor followed by a contact
+  // rebuild are identical launch sequences (all step-dependent state lives on
+  // the device), so they replay one captured HIP graph.  Profiled runs launch
+  // eagerly so that every kernel class keeps its own events.
+  StepGraph graph;
+  const bool use_graph = graphs_enabled() && !c->prof && nsteps >= 3;
+  for (int64_t st = 1; st <= nsteps; ++st) {
+    const bool last = (st == nsteps);
+    const bool logged = every > 0 && (st == 1 || st % every == 0);
+    // run! (solvers.jl:133,177) updates neighbours after loop step i when
+    // i % neigh_update_freq == 0, i counted from 1 in each call (start_at = 0)
+    bool rebuild = false;
+    if (st % nuf == 0)
+      for (int64_t id : ids) rebuild = rebuild || !get_env(c, id).contacts.empty();
+    if (use_graph && !last && !rebuild && !logged) {
+      if (!graph.exec) {
+        graph.capture(s, [&] {
+          step_kernels(false, false);
+          book(false);
+        });
+      }
+      PD_HIP(hipGraphLaunch(graph.exec, s));
+      continue;
+    }
+    step_kernels(last, logged);
+    if (logged) ++row;
+    if (rebuild)
+      for (int64_t id : ids) {
+        Env& e = get_env(c, id);
+        if (!e.contacts.empty()) timed(c, PD_PROF_CONTACT, [&] { contact_update(c, e); });
+      }
+    book(last);
+  }
+  PD_HIP(hipGetLastError());
+  PD_HIP(hipStreamSynchronize(s));  // also before the step graph is destroyed
+  if (nrows > 0) {
+    PD_HIP(hipMemcpyAsync(log_host, c->d_obs_rows, sizeof(double) * (size_t)(nrows * nenv * 8),
+                          hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    // time column (time_step * dt after the logged step) and mean damage
+    int64_t r = 0;
+    for (int64_t st = 1; st <= nsteps; ++st) {
+      if (!(st == 1 || st % every == 0)) continue;
+      for (int k = 0; k < nenv; ++k) {
+        const Env& e = get_env(c, ids[k]);
+        double* out = log_host + ((size_t)r * nenv + k) * 8;
+        out[0] = (double)(e.time_step + st) * e.dt;
+        out[7] = out[7] / (double)e.N;
+      }
+      ++r;
+    }
+  }
+  for (int64_t id : ids) {
+    Env& e = get_env(c, id);
+    e.time_step += nsteps;
+    e.cur ^= (int32_t)(nsteps & 1);
+    if (e.has_ep) e.edp_cur ^= (int32_t)(nsteps & 1);
+    e.lists_current = (nsteps % nuf) == 0;  // rebuilt after the last step
+  }
+  if (nl) refresh_latch(c, ids);
+}
+
 }  // namespace
 
 // =============================================================================
@@ -948,6 +1075,9 @@ int pd_ctx_destroy(void* ctx) {
   if (c->d_slots) (void)hipFree(c->d_slots);
   if (c->d_latch) (void)hipFree(c->d_latch);
   if (c->d_latch_part) (void)hipFree(c->d_latch_part);
+  if (c->d_obs_ranges) (void)hipFree(c->d_obs_ranges);
+  if (c->d_obs_part) (void)hipFree(c->d_obs_part);
+  if (c->d_obs_rows) (void)hipFree(c->d_obs_rows);
   for (auto& r : c->pending) { c->pool.push_back(r.a); c->pool.push_back(r.b); }
   for (hipEvent_t ev : c->pool) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);
@@ -1201,90 +1331,49 @@ int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t row
   });
 }
 
+int64_t pd_log_rows(int64_t nsteps, int64_t every) {
+  if (nsteps < 1 || every < 1) return 0;
+  return nsteps / every + (every > 1 ? 1 : 0);
+}
+
 int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
     if (nenv <= 0 || !envs) throw PdError(PD_E_ARG, "empty env list");
     if (nsteps < 0) throw PdError(PD_E_ARG, "nsteps must be >= 0");
     if (nuf < 1) throw PdError(PD_E_ARG, "neigh_update_freq must be >= 1");
+    run_impl(C(ctx), std::vector<int64_t>(envs, envs + nenv), nsteps, nuf, 0, nullptr);
+  });
+}
+
+int pd_run_logged(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf, int64_t every,
+                  double* log) {
+  return run_guard(ctx, [&] {
+    if (nenv <= 0 || !envs) throw PdError(PD_E_ARG, "empty env list");
+    if (nsteps < 0) throw PdError(PD_E_ARG, "nsteps must be >= 0");
+    if (nuf < 1) throw PdError(PD_E_ARG, "neigh_update_freq must be >= 1");
+    if (every < 1) throw PdError(PD_E_ARG, "average_prop_freq must be >= 1");
+    if (!log && pd_log_rows(nsteps, every) > 0) throw PdError(PD_E_ARG, "null log buffer");
+    run_impl(C(ctx), std::vector<int64_t>(envs, envs + nenv), nsteps, nuf, every, log);
+  });
+}
+
+int pd_env_observables(void* ctx, int64_t env, double* out) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    if (!out) throw PdError(PD_E_ARG, "null output");
     PD_HIP(hipSetDevice(c->device));
-    std::vector<int64_t> ids(envs, envs + nenv);
-    const int prec = precision_of(c, ids);
-    build_set(c, ids);
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    build_set(c, {env});
+    ensure_cap(c->d_obs_rows, c->obs_rows_cap, 8);
     hipStream_t s = c->stream;
-    // run! start: update_neighs! (solvers.jl:114); apply_bc_at0 records
-    // BC.last = y, which the resident state already holds.  Contact lists are
-    // a deterministic function of the positions (sorted), so lists built from
-    // the current positions are kept rather than rebuilt identically.
-    for (int64_t id : ids) {
-      Env& e = get_env(c, id);
-      if (!e.contacts.empty() && !e.lists_current) contact_update(c, e);
-    }
-    if (nsteps == 0) return;
-    const int nl = prepare_latch(c, ids);
-    EnvDev* T = c->d_envs;
-    Chunk* d = c->d_set;
-    timed(c, PD_PROF_PROLOGUE, [&] { pd_launch_prologue(s, d, c->n_all, T); });
-    pd_launch_book(s, T, c->d_slots, c->n_slots, PD_B_FLIP_Y);
-    // the kernels of one velocity_verlet_step! (all envs of the set)
-    auto step_kernels = [&](bool last) {
-      if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part); });
-      if (c->set_has_theta)
-        timed(c, PD_PROF_THETA, [&] { pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T); });
-      if (c->set_has_ep) timed(c, PD_PROF_TD, [&] { pd_launch_td(s, prec, d + c->off_td, c->n_td, T); });
-      const int fl = PD_F_KICK | (last ? 0 : PD_F_DRIFT);
-      for (int k = 0; k < 3; ++k)
-        if (c->n_force[k])
-          timed(c, PD_PROF_FORCE_BB + k,
-                [&] { pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl, c->set_gpow); });
-    };
-    auto book = [&](bool last) {
-      timed(c, PD_PROF_BOOK, [&] {
-        pd_launch_book(s, T, c->d_slots, c->n_slots,
-                       PD_B_STEP | (last ? 0 : PD_B_FLIP_Y) | (c->set_has_ep ? PD_B_FLIP_EDP : 0));
-      });
-    };
-    // Steps that are neither the last nor followed by a contact rebuild are
-    // identical launch sequences (all step-dependent state lives on the
-    // device), so they replay one captured HIP graph.  Profiled runs launch
-    // eagerly so that every kernel class keeps its own events.
-    StepGraph graph;
-    const bool use_graph = graphs_enabled() && !c->prof && nsteps >= 3;
-    for (int64_t st = 1; st <= nsteps; ++st) {
-      const bool last = (st == nsteps);
-      // run! (solvers.jl:133,177) updates neighbours after loop step i when
-      // i % neigh_update_freq == 0, i counted from 1 in each call (start_at = 0)
-      bool rebuild = false;
-      if (st % nuf == 0)
-        for (int64_t id : ids) rebuild = rebuild || !get_env(c, id).contacts.empty();
-      if (use_graph && !last && !rebuild) {
-        if (!graph.exec) {
-          graph.capture(s, [&] {
-            step_kernels(false);
-            book(false);
-          });
-        }
-        PD_HIP(hipGraphLaunch(graph.exec, s));
-        continue;
-      }
-      step_kernels(last);
-      if (rebuild)
-        for (int64_t id : ids) {
-          Env& e = get_env(c, id);
-          if (!e.contacts.empty()) timed(c, PD_PROF_CONTACT, [&] { contact_update(c, e); });
-        }
-      book(last);
-    }
+    pd_launch_observables(s, 0, c->d_set, c->n_all, c->d_envs, c->d_obs_part, c->d_obs_ranges, 1, c->d_obs_rows);
+    pd_launch_observables(s, 1, c->d_set, c->n_all, c->d_envs, c->d_obs_part, c->d_obs_ranges, 1, c->d_obs_rows);
     PD_HIP(hipGetLastError());
-    PD_HIP(hipStreamSynchronize(s));  // also before the step graph is destroyed
-    for (int64_t id : ids) {
-      Env& e = get_env(c, id);
-      e.time_step += nsteps;
-      e.cur ^= (int32_t)(nsteps & 1);
-      if (e.has_ep) e.edp_cur ^= (int32_t)(nsteps & 1);
-      e.lists_current = (nsteps % nuf) == 0;  // rebuilt after the last step
-    }
-    if (nl) refresh_latch(c, ids);
+    PD_HIP(hipMemcpyAsync(out, c->d_obs_rows, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    out[0] = (double)e.time_step * e.dt;
+    out[7] = out[7] / (double)e.N;
   });
 }
 

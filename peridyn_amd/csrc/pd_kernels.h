@@ -43,6 +43,11 @@ void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uin
 void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* ref,
                    const int32_t* deg, const int32_t* col, uint32_t* rev);
 void pd_launch_aux_init(hipStream_t s, const double* m, double4* aux, int n);
+// printdata columns: mode 0 momentum into rows[8 e + 1..3], mode 1 force and
+// damage sums into rows[8 e + 4..7] for the nenv envs whose chunks (in `chunks`,
+// n in all) are the ranges (first, count); part: n double4 of scratch
+void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, EnvDev* envs, double4* part,
+                           const int2* ranges, int nenv, double* rows);
 // device-wide exclusive scan of n int32 (scratch: pd_scan_scratch_ints(n) ints)
 int pd_scan_scratch_ints(int n);
 void pd_scan_exclusive(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* scratch);

@@ -684,6 +684,69 @@ __global__ __launch_bounds__(64) void k_latch_fin(EnvDev* __restrict__ envs, con
   if (!approx) B.deformed = 1;
 }
 
+// ---- observables (printdata, src/environment.jl:308-320) ---------------------
+// Partial sums per chunk, then per env in chunk order (deterministic):
+//   MODE 0: p = mass .* v (update_misc!, solvers.jl:289-291), mass = volume *
+//           density[type] (environment.jl:173-180);
+//   MODE 1: f, and damage = 1 - intact / intact0 (cal_damage, peri.jl:587-594).
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_obs_part(const Chunk* __restrict__ chunks, int nchunks,
+                                                  const EnvDev* __restrict__ envs, double4* __restrict__ part) {
+  WaveCtx w;
+  if (!wave_chunk(chunks, nchunks, envs, w)) return;
+  const EnvDev& E = envs[w.ch.env];
+  const BlockDev& B = E.blocks[w.ch.block];
+  const int p = w.active ? w.p : w.ch.first;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (MODE == 0) {
+    if (w.active && w.g == 0) {
+      const double4 v = E.v[p];
+      const double mass = E.x[p].w * B.density[E.ptype[p]];
+      s0 = mass * v.x; s1 = mass * v.y; s2 = mass * v.z;
+    }
+  } else {
+    const int d = w.active ? E.deg[p] : 0;
+    const uint32_t base = w.ch.slot0 + w.lane;
+    int cnt = 0;
+    const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
+    for (int i = 0; i < ni; ++i) cnt += E.state[base + 64u * (uint32_t)i];
+    for (int off = w.C; off < 64; off <<= 1) cnt += __shfl_xor(cnt, off);
+    if (w.active && w.g == 0) {
+      const double4 f = E.f[p];
+      s0 = f.x; s1 = f.y; s2 = f.z;
+      s3 = 1.0 - (double)cnt / (double)d;
+    }
+  }
+  s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
+  if (w.lane == 0) part[w.cid] = make_double4(s0, s1, s2, s3);
+}
+
+// One wavefront per env: the env's chunk partials [first, first + count) in
+// order; MODE 0 writes row[1..3], MODE 1 row[4..7] of the env's 8-column row.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_obs_fin(const double4* __restrict__ part, const int2* __restrict__ ranges,
+                                                double* __restrict__ rows) {
+  const int2 r = ranges[blockIdx.x];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int i = threadIdx.x; i < r.y; i += 64) {
+    const double4 v = part[r.x + i];
+    s0 += v.x; s1 += v.y; s2 += v.z; s3 += v.w;
+  }
+  s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
+  if (threadIdx.x != 0) return;
+  double* row = rows + 8 * (size_t)blockIdx.x;
+  if (MODE == 0) {
+    row[1] = s0; row[2] = s1; row[3] = s2;
+  } else {
+    row[4] = s0; row[5] = s1; row[6] = s2; row[7] = s3;
+  }
+}
+
 // 1/m into aux.y after the weighted volume is built
 __global__ void k_aux_init(const double* __restrict__ m, double4* __restrict__ aux, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -764,6 +827,18 @@ void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n, int 
   if (n <= 0) return;
   hipLaunchKernelGGL(k_latch_part, dim3(n, nparts), dim3(256), 0, s, envs, items, nparts, part);
   hipLaunchKernelGGL(k_latch_fin, dim3(n), dim3(64), 0, s, envs, items, nparts, (const double4*)part);
+}
+
+void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, EnvDev* envs, double4* part,
+                           const int2* ranges, int nenv, double* rows) {
+  if (n <= 0 || nenv <= 0) return;
+  if (mode == 0) {
+    hipLaunchKernelGGL(k_obs_part<0>, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, part);
+    hipLaunchKernelGGL(k_obs_fin<0>, dim3(nenv), dim3(64), 0, s, (const double4*)part, ranges, rows);
+  } else {
+    hipLaunchKernelGGL(k_obs_part<1>, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, part);
+    hipLaunchKernelGGL(k_obs_fin<1>, dim3(nenv), dim3(64), 0, s, (const double4*)part, ranges, rows);
+  }
 }
 
 void pd_launch_aux_init(hipStream_t s, const double* m, double4* aux, int n) {

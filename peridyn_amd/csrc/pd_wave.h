@@ -6,6 +6,7 @@
 
 struct WaveCtx {
   Chunk ch;
+  int cid;  // index of the chunk in the launch's chunk list
   int lane, lgG, G, C, q, g, p;
   bool active;
 };
@@ -17,6 +18,7 @@ __device__ __forceinline__ bool wave_chunk(const Chunk* __restrict__ chunks, int
   const int lb = pd_xcd_block(gridDim.x);
   const int cid = lb * PD_WAVES_PER_BLOCK + pd_wave_id();
   if (cid >= nchunks) return false;
+  w.cid = cid;
   w.ch = chunks[cid];
   w.lgG = envs[w.ch.env].lgG;
   w.G = 1 << w.lgG;

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     src = open(os.path.join(ROOT, "include", "peridyn.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(pd_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(pd_\w+)\s*\(", src, re.M)))
 
 
 def test_header_and_binding_agree():

@@ -346,3 +346,31 @@ def test_oracle_run_is_deterministic(orc):
     a, b = orc.OracleEnv(spec), orc.OracleEnv(spec)
     a.run(20); b.run(20)
     assert np.array_equal(a.y, b.y) and np.array_equal(a.f, b.f)
+
+
+def test_printdata_closed_form_and_log_schedule(orc):
+    """printdata columns (environment.jl:308-320) for a force-free block in
+    uniform motion: time = n dt, p = (sum of vol*rho) v0, f = 0, damage 0; rows
+    at step 1 and every 10th step (solvers.jl:158)."""
+    spec = W.tensile_bb(n=5, dt=1e-3)
+    spec["bcs"] = []
+    b = spec["blocks"][0]
+    v0 = np.array([0.5, -0.25, 2.0])
+    b["v"] = np.broadcast_to(v0, b["x"].shape).copy()
+    o = orc.OracleEnv(spec)
+    rows = o.run(25, 1, log_every=10)
+    assert rows.shape == (3, 8)
+    M = float(np.sum(b["vol"] * b["density"]))
+    for r, n in zip(rows, (1, 10, 20)):
+        assert r[0] == n * 1e-3
+        assert np.allclose(r[1:4], M * v0, rtol=1e-12, atol=0)
+        assert np.all(r[4:7] == 0.0) and r[7] == 0.0
+
+
+def test_printdata_damage_is_mean_of_cal_damage(orc):
+    spec = W.tensile_bb(n=7, dt=1e-3, pull_per_step=0.05)
+    o = orc.OracleEnv(spec)
+    rows = o.run(12, 1, log_every=4)
+    d = o.damage()
+    assert d.max() > 0
+    assert rows[-1][7] == d.sum() / d.size
