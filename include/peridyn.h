@@ -167,7 +167,17 @@ int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f);
 int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, const double* f);
 int pd_env_damage(void* ctx, int64_t env, double* damage);        /* cal_damage, peri.jl:587-594 */
 int pd_env_time_step(void* ctx, int64_t env, int64_t* time_step);
-int pd_env_check_finite(void* ctx, int64_t env);                  /* @check_nan, const_macros.jl:60-74 */
+int pd_env_check_finite(void* ctx, int64_t env);
+/* Resident-state snapshot (the restart role of save_to_file / load_from_file,
+ * src/io/io.jl:170-187): every piece of state a run changes (positions,
+ * velocities, forces, bond states, theta, td_norm2 and their per-particle
+ * records, plastic extensions, deformed latches, time step) as an opaque blob
+ * of pd_env_state_size bytes.  pd_env_state_load restores it into an env of the
+ * same shape, form and schedule (e.g. built again from the same description);
+ * a run resumed from it is bit-identical to the uninterrupted run. */
+int pd_env_state_size(void* ctx, int64_t env, int64_t* bytes);
+int pd_env_state_save(void* ctx, int64_t env, void* buf, int64_t bytes);
+int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes);                  /* @check_nan, const_macros.jl:60-74 */
 
 /* ---- measurement ------------------------------------------------------------
  * No reference counterpart (the reference's only timing hook is TimerOutputs,

@@ -217,6 +217,16 @@ class Env:
         damage) of step 1 and every `every`-th step: array (rows, 8)."""
         return run_batch_logged([self], nsteps, nuf, every)[:, 0, :]
 
+    def checkpoint(self) -> bytes:
+        """Resident device state as an opaque blob (pd_env_state_save)."""
+        from . import snapshot
+        return snapshot.checkpoint(self)
+
+    def restore(self, blob: bytes):
+        """Loads a checkpoint of an env of the same shape, form and schedule."""
+        from . import snapshot
+        snapshot.restore(self, blob)
+
     def observables(self) -> np.ndarray:
         """The 8 printdata columns of the current state (momentum from the
         current velocities, as after update_acc!)."""

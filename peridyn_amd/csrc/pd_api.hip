@@ -261,6 +261,11 @@ void build_cells(Ctx* c, Env& e, HostBlock& b, FamTmp& t) {
 }
 
 void upload_env_table(Ctx* c, Env& e) {
+  // the device table's buffer indices and step counter are advanced on the
+  // device (k_book); the host tracks the same values in Env
+  e.dev.cur = e.cur;
+  e.dev.edp_cur = e.edp_cur;
+  e.dev.time_step = e.time_step;
   PD_HIP(hipMemcpyAsync(c->d_envs + e.slot, &e.dev, sizeof(EnvDev), hipMemcpyHostToDevice, c->stream));
 }
 
@@ -1031,6 +1036,24 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
   if (nl) refresh_latch(c, ids);
 }
 
+// ---- resident-state snapshot ----------------------------------------------
+// Blob: 16 x int64 header (magic, version, N, nslots, nblocks, precision, lgG,
+// has_ep, time_step, 7 reserved), nblocks x int32 deformed latches, then y, v,
+// f, aux (N double4 each), theta, td (N double each), bond state (nslots
+// bytes) and, for elastoplastic envs, edp (nslots double); all in the env's
+// internal particle / slot order.
+constexpr int64_t kStateMagic = 0x5453445045524550LL;  // "PEREPDST"
+constexpr int64_t kStateVersion = 1;
+
+int64_t state_bytes(const Env& e) {
+  int64_t b = 16 * 8 + 4 * (int64_t)e.blocks.size();
+  b = (b + 7) / 8 * 8;
+  b += 4 * 32 * e.N + 2 * 8 * e.N;
+  b += (e.nslots + 7) / 8 * 8;
+  if (e.has_ep) b += 8 * e.nslots;
+  return b;
+}
+
 }  // namespace
 
 // =============================================================================
@@ -1500,6 +1523,97 @@ int pd_ctx_profile_read(void* ctx, int32_t cls, double* total_ms, int64_t* launc
 
 int pd_env_time_step(void* ctx, int64_t env, int64_t* ts) {
   return run_guard(ctx, [&] { *ts = get_env(C(ctx), env).time_step; });
+}
+
+int pd_env_state_size(void* ctx, int64_t env, int64_t* bytes) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    if (!bytes) throw PdError(PD_E_ARG, "null output");
+    *bytes = state_bytes(e);
+  });
+}
+
+int pd_env_state_save(void* ctx, int64_t env, void* buf, int64_t bytes) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    if (!buf || bytes < state_bytes(e)) throw PdError(PD_E_ARG, "state buffer too small");
+    char* p = static_cast<char*>(buf);
+    int64_t* hdr = reinterpret_cast<int64_t*>(p);
+    std::memset(hdr, 0, 16 * 8);
+    hdr[0] = kStateMagic; hdr[1] = kStateVersion; hdr[2] = e.N; hdr[3] = e.nslots;
+    hdr[4] = (int64_t)e.blocks.size(); hdr[5] = e.precision; hdr[6] = e.lgG; hdr[7] = e.has_ep ? 1 : 0;
+    hdr[8] = e.time_step;
+    p += 16 * 8;
+    std::vector<BlockDev> bd(e.blocks.size());
+    hipStream_t s = c->stream;
+    PD_HIP(hipMemcpyAsync(bd.data(), e.dev.blocks, sizeof(BlockDev) * bd.size(), hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    for (size_t b = 0; b < bd.size(); ++b) reinterpret_cast<int32_t*>(p)[b] = bd[b].deformed;
+    p += ((4 * (int64_t)bd.size() + 16 * 8 + 7) / 8 * 8) - 16 * 8;
+    const size_t n4 = sizeof(double4) * (size_t)e.N, n1 = sizeof(double) * (size_t)e.N;
+    PD_HIP(hipMemcpyAsync(p, e.dev.y[e.cur], n4, hipMemcpyDeviceToHost, s)); p += n4;
+    PD_HIP(hipMemcpyAsync(p, e.dev.v, n4, hipMemcpyDeviceToHost, s)); p += n4;
+    PD_HIP(hipMemcpyAsync(p, e.dev.f, n4, hipMemcpyDeviceToHost, s)); p += n4;
+    PD_HIP(hipMemcpyAsync(p, e.dev.aux, n4, hipMemcpyDeviceToHost, s)); p += n4;
+    PD_HIP(hipMemcpyAsync(p, e.dev.theta, n1, hipMemcpyDeviceToHost, s)); p += n1;
+    PD_HIP(hipMemcpyAsync(p, e.dev.td, n1, hipMemcpyDeviceToHost, s)); p += n1;
+    PD_HIP(hipMemcpyAsync(p, e.dev.state, (size_t)e.nslots, hipMemcpyDeviceToHost, s));
+    p += (e.nslots + 7) / 8 * 8;
+    if (e.has_ep) PD_HIP(hipMemcpyAsync(p, e.dev.edp[e.edp_cur], 8 * (size_t)e.nslots, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    if (!buf || bytes < 16 * 8) throw PdError(PD_E_ARG, "state buffer too small");
+    const char* p = static_cast<const char*>(buf);
+    const int64_t* hdr = reinterpret_cast<const int64_t*>(p);
+    if (hdr[0] != kStateMagic || hdr[1] != kStateVersion) throw PdError(PD_E_ARG, "not a state snapshot of this library");
+    if (hdr[2] != e.N || hdr[3] != e.nslots || hdr[4] != (int64_t)e.blocks.size() || hdr[5] != e.precision ||
+        hdr[6] != e.lgG || hdr[7] != (e.has_ep ? 1 : 0))
+      throw PdError(PD_E_ARG, "snapshot was taken from an env of another shape, form or schedule");
+    if (bytes < state_bytes(e)) throw PdError(PD_E_ARG, "state buffer too small");
+    p += 16 * 8;
+    hipStream_t s = c->stream;
+    std::vector<BlockDev> bd(e.blocks.size());
+    PD_HIP(hipMemcpyAsync(bd.data(), e.dev.blocks, sizeof(BlockDev) * bd.size(), hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    bool any_undeformed = false;
+    for (size_t b = 0; b < bd.size(); ++b) {
+      bd[b].deformed = reinterpret_cast<const int32_t*>(p)[b];
+      any_undeformed = any_undeformed || !bd[b].deformed;
+    }
+    PD_HIP(hipMemcpyAsync(e.dev.blocks, bd.data(), sizeof(BlockDev) * bd.size(), hipMemcpyHostToDevice, s));
+    p += ((4 * (int64_t)bd.size() + 16 * 8 + 7) / 8 * 8) - 16 * 8;
+    const size_t n4 = sizeof(double4) * (size_t)e.N, n1 = sizeof(double) * (size_t)e.N;
+    PD_HIP(hipMemcpyAsync(e.dev.y[e.cur], p, n4, hipMemcpyHostToDevice, s)); p += n4;
+    PD_HIP(hipMemcpyAsync(e.dev.v, p, n4, hipMemcpyHostToDevice, s)); p += n4;
+    PD_HIP(hipMemcpyAsync(e.dev.f, p, n4, hipMemcpyHostToDevice, s)); p += n4;
+    PD_HIP(hipMemcpyAsync(e.dev.aux, p, n4, hipMemcpyHostToDevice, s)); p += n4;
+    PD_HIP(hipMemcpyAsync(e.dev.theta, p, n1, hipMemcpyHostToDevice, s)); p += n1;
+    PD_HIP(hipMemcpyAsync(e.dev.td, p, n1, hipMemcpyHostToDevice, s)); p += n1;
+    PD_HIP(hipMemcpyAsync(e.dev.state, p, (size_t)e.nslots, hipMemcpyHostToDevice, s));
+    p += (e.nslots + 7) / 8 * 8;
+    if (e.has_ep) {
+      // both buffers: the fast forms skip writing plastic extensions that stay 0
+      PD_HIP(hipMemcpyAsync(e.dev.edp[0], p, 8 * (size_t)e.nslots, hipMemcpyHostToDevice, s));
+      PD_HIP(hipMemcpyAsync(e.dev.edp[1], p, 8 * (size_t)e.nslots, hipMemcpyHostToDevice, s));
+    }
+    e.time_step = hdr[8];
+    e.maybe_undeformed = any_undeformed;
+    e.lists_current = false;
+    upload_env_table(c, e);
+    PD_HIP(hipStreamSynchronize(s));
+  });
 }
 
 int pd_env_check_finite(void* ctx, int64_t env) {
