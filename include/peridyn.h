@@ -168,6 +168,16 @@ int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, cons
 int pd_env_damage(void* ctx, int64_t env, double* damage);        /* cal_damage, peri.jl:587-594 */
 int pd_env_time_step(void* ctx, int64_t env, int64_t* time_step);
 int pd_env_check_finite(void* ctx, int64_t env);
+/* Quasi-static relaxation, QSDrag minimize! (src/solvers/minimize.jl:105-231):
+ * BCs applied once (position, velocity), then up to max_iter iterations of
+ * update_acc! followed by the drag update of the particles held by no BC
+ *   force = f - drag*opt*(1 + drag/10 |opt|);  dy = clip(0.5 force step^2 +
+ *   opt step, ps/10);  opt += step force;  y += dy
+ * (ps: particle size of the first block), stopping at the first iteration with
+ * max|f| < f_tol and max|dy| < x_tol; then check! and time_step += 1.  iters,
+ * f_max, x_max (may be NULL) receive the iteration count and the last maxima. */
+int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64_t max_iter, double x_tol,
+                    double f_tol, int64_t* iters, double* f_max, double* x_max);
 /* Resident-state snapshot (the restart role of save_to_file / load_from_file,
  * src/io/io.jl:170-187): every piece of state a run changes (positions,
  * velocities, forces, bond states, theta, td_norm2 and their per-particle

@@ -427,6 +427,40 @@ class OracleEnv:
                 self.update_neighs()
         return np.array(rows) if log_every > 0 else None
 
+    def minimize(self, step_size: float, drag: float, max_iter: int = 1000, x_tol: float = 1e-3,
+                 f_tol: float = 1e-3):
+        """QSDrag minimize! (minimize.jl:105-231): BCs once, then update_acc!
+        and the drag update of the free particles until max|f| < f_tol and
+        max|dy| < x_tol; the clipped velocity of apply_! is not stored back
+        (it rebinds a local), so opt keeps its unclipped value.  Returns
+        (iterations, max|f|, max|dy|) of the last iteration."""
+        self._bc_position()
+        self._bc_velocity()
+        dt = step_size
+        a = self.blocks[0].particle_size / 10.0
+        held = np.zeros(self.N, dtype=bool)
+        for bc in self.bcs:
+            held |= bc["mask"]
+        free = ~held
+        opt = np.zeros((int(free.sum()), 3))
+        k = drag / 10.0
+        it, fm, xm = 0, math.nan, math.nan
+        for i in range(1, max_iter + 1):
+            self.update_acc()
+            gs = self.f[free]
+            force = gs - (drag * opt) * (1.0 + k * np.abs(opt))
+            dy = (0.5 * force) * (dt * dt) + opt * dt
+            opt = opt + dt * force
+            dy = np.maximum(np.minimum(dy, a), -a)
+            self.y[free] += dy
+            it, fm, xm = i, float(np.abs(gs).max()), float(np.abs(dy).max())
+            self.qs_opt = opt
+            if fm < f_tol and xm < x_tol:
+                break
+        self._bc_check()
+        self.time_step += 1
+        return it, fm, xm
+
     def damage(self) -> np.ndarray:
         counts = np.concatenate([b.intact.sum(axis=1) for b in self.blocks]).astype(np.float64)
         return 1.0 - counts / self.intact0.astype(np.float64)

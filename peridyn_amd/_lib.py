@@ -25,6 +25,7 @@ EXPORTS = [
     "pd_env_time_step", "pd_env_check_finite", "pd_ctx_profile", "pd_ctx_profile_read",
     "pd_env_set_schedule", "pd_env_schedule", "pd_ctx_mem_info", "pd_run_logged", "pd_log_rows",
     "pd_env_observables", "pd_env_state_size", "pd_env_state_save", "pd_env_state_load",
+    "pd_env_minimize",
 ]
 PROF_CLASSES = {"prologue": 0, "latch": 1, "theta": 2, "td": 3, "force_bb": 4, "force_osb": 5,
                 "force_ep": 6, "contact": 7, "book": 8}
@@ -88,6 +89,8 @@ def load() -> ctypes.CDLL:
     L.pd_env_state_size.argtypes = [P, I64, ctypes.POINTER(I64)]
     L.pd_env_state_save.argtypes = [P, I64, P, I64]
     L.pd_env_state_load.argtypes = [P, I64, P, I64]
+    L.pd_env_minimize.argtypes = [P, I64, D, D, I64, D, D, ctypes.POINTER(I64), ctypes.POINTER(D),
+                                  ctypes.POINTER(D)]
     L.pd_env_create.argtypes = [P, D, I32, ctypes.POINTER(I64)]
     L.pd_env_add_block.argtypes = [P, I64, ctypes.POINTER(BlockDesc), ctypes.POINTER(I64)]
     L.pd_env_add_contact.argtypes = [P, I64, ctypes.POINTER(ContactDesc)]

@@ -217,6 +217,16 @@ class Env:
         damage) of step 1 and every `every`-th step: array (rows, 8)."""
         return run_batch_logged([self], nsteps, nuf, every)[:, 0, :]
 
+    def minimize(self, step_size: float, drag: float, max_iter: int = 1000, x_tol: float = 1e-3,
+                 f_tol: float = 1e-3):
+        """QSDrag minimize! (minimize.jl:105-231).  Returns (iterations, last
+        max |f|, last max |dy|)."""
+        it, fm, xm = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(self.ctx.lib.pd_env_minimize(self.ctx.h, self.id, float(step_size), float(drag),
+                                                    int(max_iter), float(x_tol), float(f_tol), ctypes.byref(it),
+                                                    ctypes.byref(fm), ctypes.byref(xm)))
+        return it.value, fm.value, xm.value
+
     def checkpoint(self) -> bytes:
         """Resident device state as an opaque blob (pd_env_state_save)."""
         from . import snapshot

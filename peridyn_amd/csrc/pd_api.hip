@@ -1616,6 +1616,80 @@ int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes) {
   });
 }
 
+int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64_t max_iter, double x_tol,
+                    double f_tol, int64_t* iters, double* f_max, double* x_max) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    if (max_iter < 1) throw PdError(PD_E_ARG, "max_iter must be >= 1");
+    if (!(step_size > 0.0) || !std::isfinite(step_size)) throw PdError(PD_E_ARG, "step_size must be > 0");
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    {
+      std::vector<uint8_t> held(e.N, 0);
+      for (const HostBc& b : e.bcs)
+        for (int64_t r = 0; r < e.N; ++r) held[r] |= b.mask[r];
+      bool any_free = false;
+      for (uint8_t h : held) any_free = any_free || !h;
+      if (!any_free) throw PdError(PD_E_ARG, "no particle is free of boundary conditions");
+    }
+    const std::vector<int64_t> ids{env};
+    const int prec = e.precision;
+    build_set(c, ids);
+    hipStream_t s = c->stream;
+    const int nl = prepare_latch(c, ids);
+    EnvDev* T = c->d_envs;
+    Chunk* d = c->d_set;
+    const double a = e.blocks[0].d.particle_size / 10.0;  // ps of material block 1 (minimize.jl:127-136)
+    char* buf = nullptr;
+    PD_HIP(hipMalloc(&buf, sizeof(double4) * (size_t)e.N + 16));
+    double4* opt = reinterpret_cast<double4*>(buf);
+    auto* maxes = reinterpret_cast<unsigned long long*>(buf + sizeof(double4) * (size_t)e.N);
+    try {
+      PD_HIP(hipMemsetAsync(opt, 0, sizeof(double4) * (size_t)e.N, s));
+      pd_launch_qs_bc(s, d, c->n_all, T);
+      int64_t it = 0;
+      unsigned long long hm[2] = {0, 0};
+      for (int64_t i = 1; i <= max_iter; ++i) {
+        // update_acc! (solvers.jl:320-336): the env's own force passes, no Verlet
+        if (nl) pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part);
+        if (c->set_has_theta) pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T);
+        if (c->set_has_ep) pd_launch_td(s, prec, d + c->off_td, c->n_td, T);
+        for (int k = 0; k < 3; ++k)
+          if (c->n_force[k]) pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, 0, c->set_gpow);
+        if (c->set_has_ep) {
+          pd_launch_book(s, T, e.d_self, 1, PD_B_FLIP_EDP);
+          e.edp_cur ^= 1;
+        }
+        PD_HIP(hipMemsetAsync(maxes, 0, 16, s));
+        pd_launch_qs_update(s, d, c->n_all, T, opt, step_size, drag, a, maxes);
+        PD_HIP(hipMemcpyAsync(hm, maxes, 16, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        it = i;
+        double fm, xm;
+        std::memcpy(&fm, &hm[0], 8);
+        std::memcpy(&xm, &hm[1], 8);
+        if (f_max) *f_max = fm;
+        if (x_max) *x_max = xm;
+        if ((fm < f_tol) && (xm < x_tol)) break;
+      }
+      if (iters) *iters = it;
+      // check! of every BC, then time_step += 1 (minimize.jl:225-229)
+      pd_launch_book(s, T, e.d_self, 1, PD_B_STEP);
+      PD_HIP(hipGetLastError());
+      PD_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(buf);
+      throw;
+    }
+    PD_HIP(hipFree(buf));
+    e.time_step += 1;
+    e.lists_current = false;
+    if (nl) refresh_latch(c, ids);
+  });
+}
+
 int pd_env_check_finite(void* ctx, int64_t env) {
   Ctx* c = C(ctx);
   int rc = run_guard(ctx, [&] {

@@ -43,6 +43,11 @@ void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uin
 void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* ref,
                    const int32_t* deg, const int32_t* col, uint32_t* rev);
 void pd_launch_aux_init(hipStream_t s, const double* m, double4* aux, int n);
+// quasi-static minimiser: BCs once, then per iteration the drag update of the
+// free particles (maxes: 2 x u64, max |f| and max |dy| as non-negative bits)
+void pd_launch_qs_bc(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
+void pd_launch_qs_update(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs, double4* opt, double dt,
+                         double lambda, double a, unsigned long long* maxes);
 // printdata columns: mode 0 momentum into rows[8 e + 1..3], mode 1 force and
 // damage sums into rows[8 e + 4..7] for the nenv envs whose chunks (in `chunks`,
 // n in all) are the ranges (first, count); part: n double4 of scratch

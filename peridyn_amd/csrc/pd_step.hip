@@ -747,6 +747,87 @@ __global__ __launch_bounds__(64) void k_obs_fin(const double4* __restrict__ part
   }
 }
 
+// ---- quasi-static minimiser (QSDrag minimize!, src/solvers/minimize.jl:105-231) --
+// apply_bc!(:position) and (:velocity) of every BC once, as minimize! does
+// before iterating (the last BC listed for a particle wins, as in list order).
+__global__ __launch_bounds__(256) void k_qs_bc(const Chunk* __restrict__ chunks, int nchunks,
+                                               EnvDev* __restrict__ envs) {
+  WaveCtx w;
+  if (!wave_chunk(chunks, nchunks, envs, w)) return;
+  EnvDev& E = envs[w.ch.env];
+  if (!w.active || w.g != 0) return;
+  const int p = w.p;
+  const int bci = E.bcid[p];
+  if (bci < 0) return;
+  const BcDev& bc = E.bcs[bci];
+  double4 v = E.v[p];
+  if (bc.kind == PD_BC_FIX) {
+    v.x = 0.0; v.y = 0.0; v.z = 0.0;           // y = last = y
+  } else {
+    const double dir = pd_bc_dir(bc, E.time_step);
+    const double ddir = E.dt * dir;             // last + dt * direction * rate
+    double4 y = E.y[E.cur][p];
+    y.x = y.x + ddir * bc.rate[0]; y.y = y.y + ddir * bc.rate[1]; y.z = y.z + ddir * bc.rate[2];
+    E.y[E.cur][p] = y;
+    v.x = dir * bc.rate[0]; v.y = dir * bc.rate[1]; v.z = dir * bc.rate[2];
+  }
+  E.v[p] = v;
+}
+
+// Julia's min / max on Float64 (NaN if either argument is NaN); a is finite.
+__device__ __forceinline__ double jl_clip(double x, double a) {
+  if (__builtin_isnan(x)) return x;
+  const double lo = x < a ? x : a;              // min.(x, a)
+  return lo > -a ? lo : -a;                     // max.(., -a)
+}
+__device__ __forceinline__ unsigned long long abs_bits(double x) {
+  return (unsigned long long)__double_as_longlong(__builtin_fabs(x));  // orders like |x|, NaN above all
+}
+
+// One minimize! iteration on the free particles (no BC) after update_acc!:
+//   force = f - (lambda * opt) * (1 + k |opt|),  k = lambda / 10
+//   dy = clip((0.5 * force) * dt^2 + opt * dt, ps/10);  opt += dt * force;  y += dy
+// (the reference's clipv result is not stored back into opt: apply_! rebinds
+// a local).  maxes[0] / [1] take max |f| and max |dy| as non-negative bits.
+__global__ __launch_bounds__(256) void k_qs_update(const Chunk* __restrict__ chunks, int nchunks,
+                                                   EnvDev* __restrict__ envs, double4* __restrict__ opt,
+                                                   double dt, double dt2, double lambda, double kk, double a,
+                                                   unsigned long long* __restrict__ maxes) {
+  WaveCtx w;
+  if (!wave_chunk(chunks, nchunks, envs, w)) return;
+  EnvDev& E = envs[w.ch.env];
+  unsigned long long mf = 0, mx = 0;
+  if (w.active && w.g == 0 && E.bcid[w.p] < 0) {
+    const int p = w.p;
+    const double4 f = E.f[p];
+    double4 o = opt[p];
+    double4 y = E.y[E.cur][p];
+    const double fc[3] = {f.x, f.y, f.z};
+    double oc[3] = {o.x, o.y, o.z};
+    double yc[3] = {y.x, y.y, y.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double force = fc[c] - (lambda * oc[c]) * (1.0 + kk * __builtin_fabs(oc[c]));
+      double dy = (0.5 * force) * dt2 + oc[c] * dt;
+      oc[c] = oc[c] + dt * force;
+      dy = jl_clip(dy, a);
+      yc[c] = yc[c] + dy;
+      mf = max(mf, abs_bits(fc[c]));
+      mx = max(mx, abs_bits(dy));
+    }
+    opt[p] = make_double4(oc[0], oc[1], oc[2], 0.0);
+    E.y[E.cur][p] = make_double4(yc[0], yc[1], yc[2], y.w);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    mf = max(mf, (unsigned long long)__shfl_xor((long long)mf, off));
+    mx = max(mx, (unsigned long long)__shfl_xor((long long)mx, off));
+  }
+  if (w.lane == 0) {
+    atomicMax(&maxes[0], mf);
+    atomicMax(&maxes[1], mx);
+  }
+}
+
 // 1/m into aux.y after the weighted volume is built
 __global__ void k_aux_init(const double* __restrict__ m, double4* __restrict__ aux, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -839,6 +920,18 @@ void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, 
     hipLaunchKernelGGL(k_obs_part<1>, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, part);
     hipLaunchKernelGGL(k_obs_fin<1>, dim3(nenv), dim3(64), 0, s, (const double4*)part, ranges, rows);
   }
+}
+
+void pd_launch_qs_bc(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_qs_bc, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs);
+}
+
+void pd_launch_qs_update(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs, double4* opt, double dt,
+                         double lambda, double a, unsigned long long* maxes) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_qs_update, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, opt, dt, dt * dt, lambda,
+                     lambda / 10.0, a, maxes);
 }
 
 void pd_launch_aux_init(hipStream_t s, const double* m, double4* aux, int n) {

@@ -374,3 +374,50 @@ def test_printdata_damage_is_mean_of_cal_damage(orc):
     d = o.damage()
     assert d.max() > 0
     assert rows[-1][7] == d.sum() / d.size
+
+
+def _pair_spec(stretch):
+    x = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    b = W.block("bb", x, 3.015, bulk_modulus=W.bulk_from_E(70e3), critical_stretch=1.0, density=2.7e-3)
+    b["y"] = x.copy()
+    b["y"][1, 0] += stretch
+    return {"dt": 1e-3, "blocks": [b], "contacts": [], "bcs": []}
+
+
+def test_qsdrag_first_iteration_closed_form(orc):
+    """First minimize! iteration from opt = 0: force = f, dy = clip(0.5 f dt^2,
+    ps/10), opt = dt f, left unclipped even where |opt| > ps/10/dt."""
+    spec = _pair_spec(0.2)
+    o = orc.OracleEnv(spec)
+    o.update_acc()
+    f0 = o.f.copy()
+    y0 = o.y.copy()
+    o2 = orc.OracleEnv(spec)
+    dt = 0.5
+    o2.minimize(dt, 1.0, max_iter=1)
+    dy = np.clip(0.5 * f0 * (dt * dt), -0.1, 0.1)
+    assert np.array_equal(o2.y, y0 + dy)
+    assert np.abs(0.5 * f0 * dt * dt).max() > 0.1           # the clip was active
+    assert np.array_equal(o2.qs_opt, dt * f0)                # opt not clipped
+    assert np.abs(o2.qs_opt).max() > 0.1 / dt
+
+
+def test_qsdrag_relaxes_pair_to_rest_length(orc):
+    """Drag relaxation of a stretched bond pair ends at the rest length: with
+    max|f| < f_tol the stretch left is below f_tol / (bond stiffness) ~ 1e-9."""
+    o = orc.OracleEnv(_pair_spec(0.004))
+    it, fm, xm = o.minimize(1e-4, 300.0, max_iter=20000, x_tol=1e-9, f_tol=1e-3)
+    assert fm < 1e-3 and xm < 1e-9 and it < 20000
+    sep = o.y[1] - o.y[0]
+    assert abs(np.linalg.norm(sep) - 1.0) < 1e-8
+    assert o.time_step == 1
+
+
+def test_qsdrag_bc_particles_move_once(orc):
+    spec = W.tensile_bb(n=8, dt=1e-3, pull_per_step=0.01)
+    o = orc.OracleEnv(spec)
+    o.apply_bc_at0()
+    y0 = o.y.copy()
+    o.minimize(1e-3, 10.0, max_iter=5)
+    top = spec["bcs"][1]["mask"]
+    assert np.allclose(o.y[top] - y0[top], [0.0, 0.0, 1e-3 * 10.0], rtol=0, atol=1e-15)
