@@ -186,6 +186,103 @@ def dilatation(fam, intact, x, y, vol, m) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+# Self contact, ContactModel11 (src/contacts/contacts.jl:94-119, 357-380,
+# 539-591), restated literally in Python (small test sizes only)
+# ---------------------------------------------------------------------------
+def jl_fld(x: float, y: float) -> float:
+    """Julia fld for Float64: round((x - mod(x, y)) / y)."""
+    r = math.fmod(x, y)
+    if r != 0.0 and ((r < 0.0) != (y < 0.0)):
+        r += y
+    return float(round((x - r) / y))
+
+
+def jl_get_cells(x: np.ndarray, horizon: float, max_part: int = 30):
+    """get_cells (src/peri.jl:500-520) with neigh_cells / cell_number
+    (:459-486): 1-based cell lists of 1-based particle ids, 27-stencils."""
+    lo, hi = x.min(axis=0), x.max(axis=0)
+    N = [int(max(1, min(max_part, jl_fld(hi[d] - lo[d], horizon)))) for d in range(3)]
+    num = lambda i, j, k: i + (j - 1) * N[0] + (k - 1) * N[0] * N[1]
+    cells = [[] for _ in range(N[0] * N[1] * N[2])]
+    neighs = [None] * len(cells)
+    for k in range(1, N[2] + 1):
+        for j in range(1, N[1] + 1):
+            for i in range(1, N[0] + 1):
+                a = []
+                for kk in (k - 1, k, k + 1):
+                    for jj in (j - 1, j, j + 1):
+                        for ii in (i - 1, i, i + 1):
+                            if 0 < ii * jj * kk and ii <= N[0] and jj <= N[1] and kk <= N[2]:
+                                a.append(num(ii, jj, kk))
+                neighs[num(i, j, k) - 1] = a
+    for p in range(x.shape[0]):
+        c = [int(max(1, min(max_part, jl_fld(x[p, d] - lo[d], horizon)))) for d in range(3)]
+        cells[num(*c) - 1].append(p + 1)
+    return cells, neighs
+
+
+def rm11_neighs(x, fam, intact, search_distance, equi_dist, max_neighs, max_part=30) -> np.ndarray:
+    """update_contact_neighs! for ContactModel11 (contacts.jl:539-591) as
+    written: particles with a broken bond, cells of width search_distance,
+    and for every stencil cell a merge walk of its sorted ids against the
+    particle's family column, which the walk expects ascending but which
+    GeneralMaterial stores descending (general_material.jl:88-90).  fam, intact:
+    (n, M') rows.  Returns neighbours as (n, max_neighs), 1-based, 0-padded."""
+    n = x.shape[0]
+    FS = fam.shape[1]
+    mask = np.any((fam > 0) & (intact == 0), axis=1)
+    cells, cell_neighs = jl_get_cells(x, search_distance, max_part)
+    out = np.zeros((n, max_neighs), dtype=np.int64)
+    e2 = equi_dist * equi_dist
+    for cell_i in range(len(cells)):
+        for ca in cells[cell_i]:
+            ind = 1
+            if not mask[ca - 1]:
+                continue
+            a1, b1, c1 = x[ca - 1]
+            fms = fam[ca - 1]
+            for nc in cell_neighs[cell_i]:
+                nbs = cells[nc - 1]
+                nbs.sort()
+                fm_ind = 1
+                for neigh in nbs:
+                    while fms[fm_ind - 1] < neigh and fm_ind < FS:
+                        fm_ind += 1
+                    if fms[fm_ind - 1] == neigh and not intact[ca - 1, fm_ind - 1]:
+                        a2, b2, c2 = x[neigh - 1]
+                        dr2 = ((a1 - a2) * (a1 - a2) + (b1 - b2) * (b1 - b2)) + (c1 - c2) * (c1 - c2)
+                        if dr2 < e2:
+                            out[ca - 1, ind - 1] = neigh
+                            ind += 1
+    return out
+
+
+def rm11_force(y, f, vol, den, neighs, equi, K, expo, expo_is_int):
+    """short_range_contact! for ContactModel11 (contacts.jl:357-380): i in
+    order, f_i += F (vol_j / den_i), f_j -= F (vol_j / den_j) (vol_j on both
+    sides, as written), F = NonLinearSpring (nonlinearspring.jl:124-145) of
+    dr = y_i - y_j."""
+    n, M = neighs.shape
+    for i in range(n):
+        for k in range(M):
+            j = int(neighs[i, k])
+            if j == 0:
+                continue
+            j -= 1
+            d = y[i] - y[j]
+            mag = math.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+            dl = equi - mag
+            if dl < 0.0:
+                F = np.zeros(3)
+            else:
+                st = dl / equi
+                pw = lib().orc_julia_pow_int(st, int(expo)) if expo_is_int else st ** expo
+                F = ((K * pw) / (mag + 1e-10)) * d
+            f[i] += F * (vol[j] / den[i])
+            f[j] -= F * (vol[j] / den[j])
+
+
+# ---------------------------------------------------------------------------
 # Blocks and environment
 # ---------------------------------------------------------------------------
 class OracleBlock:
@@ -297,6 +394,8 @@ class OracleEnv:
 
     # -- contacts ----------------------------------------------------------
     def _make_contact(self, c: dict) -> dict:
+        if len(c["pair"]) == 1 or len(c["pair"][1]) == 0:
+            return self._make_self_contact(c)
         t1, t2 = [set(int(t) for t in ts) for ts in c["pair"]]
         mask1 = np.isin(self.type, list(t1))
         mask2 = np.isin(self.type, list(t2))
@@ -315,15 +414,41 @@ class OracleEnv:
                     expo=float(expo), expo_is_int=int(isinstance(expo, (int, np.integer))),
                     neighs=np.zeros((int(mask1.sum()), max_neighs), dtype=np.int64))
 
+    def _make_self_contact(self, c: dict) -> dict:
+        """ContactModel11_gcal (contacts.jl:105-119): one block, equi_dist =
+        particle_size * distanceD, distance = distanceX * equi_dist."""
+        t1 = set(int(t) for t in c["pair"][0])
+        bi = next(i for i, b in enumerate(self.blocks) if b.type_start in t1)
+        b = self.blocks[bi]
+        equi = b.particle_size * float(c.get("distanceD", 1.0))
+        max_neighs = min(int(c.get("max_neighs", 50)), b.n)
+        expo = c.get("exponent", 1)
+        return dict(self_block=bi, equi=equi, distance=float(c.get("distanceX", 5.0)) * equi,
+                    max_neighs=max_neighs, K=float(c["stiffness"]), expo=float(expo),
+                    expo_is_int=int(isinstance(expo, (int, np.integer))),
+                    neighs=np.zeros((b.n, max_neighs), dtype=np.int64))
+
     def update_neighs(self):
-        """update_neighs! (solvers.jl:351-361) -> update_contact_neighs! RM12."""
+        """update_neighs! (solvers.jl:351-361) -> update_contact_neighs!."""
         for c in self.contacts:
+            if "self_block" in c:
+                b = self.blocks[c["self_block"]]
+                lo, hi = self.limits[c["self_block"]]
+                c["neighs"] = rm11_neighs(self.y[lo:hi], b.fam, b.intact, c["distance"], c["equi"],
+                                          c["max_neighs"])
+                continue
             y1 = np.ascontiguousarray(self.y[c["mask1"]])
             y2 = np.ascontiguousarray(self.y[c["mask2"]])
             lib().orc_contact_neighs(y1.shape[0], _p(y1), y2.shape[0], _p(y2), c["distance"],
                                      c["max_neighs"], _p(c["neighs"]))
 
     def _contact_force(self, c: dict):
+        if "self_block" in c:
+            lo, hi = self.limits[c["self_block"]]
+            f = self.f[lo:hi]
+            rm11_force(self.y[lo:hi], f, self.volume[lo:hi], self.density[lo:hi], c["neighs"], c["equi"],
+                       c["K"], c["expo"], c["expo_is_int"])
+            return
         m1, m2 = c["mask1"], c["mask2"]
         y1 = np.ascontiguousarray(self.y[m1]); y2 = np.ascontiguousarray(self.y[m2])
         f1 = np.ascontiguousarray(self.f[m1]); f2 = np.ascontiguousarray(self.f[m2])

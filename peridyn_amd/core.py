@@ -159,18 +159,22 @@ class Env:
                 psizes.append(ps)
                 type_ranges.append((t0, t1))
             for cspec in spec.get("contacts", []):
-                t1 = np.ascontiguousarray(np.asarray(cspec["pair"][0], dtype=np.int64).ravel())
-                t2 = np.ascontiguousarray(np.asarray(cspec["pair"][1], dtype=np.int64).ravel())
+                pair = cspec["pair"]
+                # one type list: self contact of one block (ContactModel11,
+                # contacts.jl:105-119), else a pair (ContactModel12, :76-91)
+                self_contact = len(pair) == 1 or len(pair[1]) == 0
+                t1 = np.ascontiguousarray(np.asarray(pair[0], dtype=np.int64).ravel())
+                t2 = np.ascontiguousarray(np.asarray([] if self_contact else pair[1], dtype=np.int64).ravel())
                 b1 = next(i for i, r in enumerate(type_ranges) if r[0] <= t1[0] <= r[1])
-                b2 = next(i for i, r in enumerate(type_ranges) if r[0] <= t2[0] <= r[1])
+                b2 = b1 if self_contact else next(i for i, r in enumerate(type_ranges) if r[0] <= t2[0] <= r[1])
                 dD = float(cspec.get("distanceD", 1.0))
-                dX = float(cspec.get("distanceX", 3.0))
-                equi = (psizes[b1] + psizes[b2]) / 2.0 * dD
+                dX = float(cspec.get("distanceX", 5.0 if self_contact else 3.0))
+                equi = psizes[b1] * dD if self_contact else (psizes[b1] + psizes[b2]) / 2.0 * dD
                 n2 = ns[b2]
                 expo = cspec.get("exponent", 1)
                 cd = _lib.ContactDesc()
                 cd.types1, cd.ntypes1 = _ptr(t1), t1.size
-                cd.types2, cd.ntypes2 = _ptr(t2), t2.size
+                cd.types2, cd.ntypes2 = (_ptr(t2), t2.size) if t2.size else (None, 0)
                 cd.exponent = float(expo)
                 cd.exponent_is_int = int(isinstance(expo, (int, np.integer)))
                 cd.stiffness = float(cspec["stiffness"])

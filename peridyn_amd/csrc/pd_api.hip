@@ -607,6 +607,7 @@ void commit_env(Ctx* c, Env& e) {
       for (int64_t i = 0; i < b.d.n; ++i) types[b.first + i] = b.type[i];
     // member ids follow the reference order (the type masks of contacts.jl:212-219);
     // idx maps a member to its internal particle id
+    const bool self = hc.t2.empty();  // ContactModel11 (contacts.jl:105-119)
     for (int64_t r = 0; r < N; ++r) {
       const bool in1 = std::find(hc.t1.begin(), hc.t1.end(), types[r]) != hc.t1.end();
       const bool in2 = std::find(hc.t2.begin(), hc.t2.end(), types[r]) != hc.t2.end();
@@ -615,6 +616,22 @@ void commit_env(Ctx* c, Env& e) {
       if (in1) { inv1[p] = (int32_t)idx1.size(); idx1.push_back(p); }
       if (in2) { inv2[p] = (int32_t)idx2.size(); idx2.push_back(p); }
     }
+    if (self) {
+      // one material block, members in the block's own order: short_range_contact!
+      // and update_contact_neighs! of ContactModel11 index the particles of the
+      // block with its family (contacts.jl:361-365, 574-591)
+      int64_t nb_match = 0;
+      for (auto& b : e.blocks) {
+        int64_t in = 0;
+        for (int64_t i = 0; i < b.d.n; ++i)
+          in += std::find(hc.t1.begin(), hc.t1.end(), b.type[i]) != hc.t1.end();
+        if (in == b.d.n && in == (int64_t)idx1.size()) ++nb_match;
+      }
+      if (nb_match != 1) throw PdError(PD_E_ARG, "a self contact must cover exactly the particles of one block");
+      idx2 = idx1;
+      inv2 = inv1;
+    }
+    cd.self_contact = self ? 1 : 0;
     cd.n1 = (int32_t)idx1.size();
     cd.n2 = (int32_t)idx2.size();
     cd.max_neighs = (int32_t)std::max<int64_t>(1, std::min<int64_t>(hc.d.max_neighs, cd.n2));
@@ -775,7 +792,8 @@ void refresh_latch(Ctx* c, const std::vector<int64_t>& ids) {
 
 void contact_update(Ctx* c, Env& e) {
   for (size_t k = 0; k < e.cdev.size(); ++k)
-    pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k);
+    if (e.cdev[k].self_contact) pd_launch_self_contact_update(c->stream, c->d_envs, e.slot, e.cdev[k], (int)k);
+    else pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k);
   e.lists_current = true;
 }
 
@@ -1182,15 +1200,15 @@ int pd_env_add_contact(void* ctx, int64_t env, const pd_contact_desc* d) {
   return run_guard(ctx, [&] {
     Env& e = get_env(C(ctx), env);
     if (e.committed) throw PdError(PD_E_STATE, "env already committed");
-    if (!d || !d->types1 || !d->types2 || d->ntypes1 <= 0 || d->ntypes2 <= 0)
-      throw PdError(PD_E_ARG, "contact descriptor needs both type lists");
+    if (!d || !d->types1 || d->ntypes1 <= 0 || d->ntypes2 < 0 || (d->ntypes2 > 0 && !d->types2))
+      throw PdError(PD_E_ARG, "contact descriptor needs a first type list (and a second one unless self contact)");
     if ((int)e.contacts.size() >= PD_MAX_CONTACTS) throw PdError(PD_E_ARG, "too many contact models");
     if (d->exponent_is_int && d->exponent != std::floor(d->exponent))
       throw PdError(PD_E_ARG, "integer exponent must be integer-valued");
     HostContact hc;
     hc.d = *d;
     hc.t1.assign(d->types1, d->types1 + d->ntypes1);
-    hc.t2.assign(d->types2, d->types2 + d->ntypes2);
+    if (d->ntypes2 > 0) hc.t2.assign(d->types2, d->types2 + d->ntypes2);
     e.contacts.push_back(std::move(hc));
   });
 }

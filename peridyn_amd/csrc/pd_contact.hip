@@ -349,6 +349,42 @@ __global__ __launch_bounds__(256) void k_crev_sort(const EnvDev* __restrict__ en
   }
 }
 
+// Self contact (update_contact_neighs! for ContactModel11, contacts.jl:539-591).
+// Its merge walk advances through the family column while the column entry is
+// smaller than the candidate, expecting the column ascending; GeneralMaterial
+// stores it descending with zero padding (general_material.jl:88-90), so the
+// walk can only ever match the column's first entry, the largest-index family
+// member.  A member therefore gets at most one contact: that particle, when
+// the bond to it is broken and the pair is closer than equi_dist (such a pair
+// always lies in neighbouring cells of width >= distance >= equi_dist).
+// Bond rows are sorted by reference index, so the largest member is bond
+// deg - 1.  One thread per member.
+__global__ void k_cself(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  const EnvDev& E = envs[slot];
+  const ContactDev& C = E.contacts[cidx];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.n1) return;
+  const int p = C.idx1[i];
+  const int d = E.deg[p];
+  int cnt = 0, jm = -1;
+  if (d > 0) {
+    const uint32_t sl = pd_slot(E.pslot0[p], d - 1, E.lgG);
+    if (!E.state[sl]) {
+      const int q = E.col[sl];
+      const double4* y = cur_y(E);
+      const double4 a = y[p], b = y[q];
+      const double dr2 = (a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) + (a.z - b.z) * (a.z - b.z);
+      if (dr2 < C.equi * C.equi) {
+        jm = C.inv1[q];
+        cnt = 1;
+      }
+    }
+  }
+  if (cnt) C.own[(size_t)i * C.max_neighs] = jm;
+  C.own_cnt[i] = cnt;
+  if (cnt) atomicAdd(&C.rev_cnt[jm], 1);
+}
+
 // ---- device-wide exclusive scan (tiles of 1024 = 256 threads x 4) ----------
 constexpr int kTile = 1024;
 
@@ -443,6 +479,17 @@ void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDe
   const int wgrid = c.n1 > 0 ? (c.n1 + 3) / 4 : 1;  // one wavefront per block-1 member
   hipLaunchKernelGGL(k_csearch, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
   pd_scan_exclusive(s, c.rev_cnt, c.rev_off, c.n2, scan_tmp);
+  hipLaunchKernelGGL(k_crev_fill, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
+  hipLaunchKernelGGL(k_crev_sort, dim3(c.n2 > 0 ? (c.n2 + 3) / 4 : 1), dim3(256), 0, s, envs, slot, cidx);
+}
+
+void pd_launch_self_contact_update(hipStream_t s, EnvDev* envs, int slot, const ContactDev& c, int cidx) {
+  int32_t* scan_tmp = reinterpret_cast<int32_t*>(c.box + 16 + 12 * kRedBlocks);
+  hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cursor, c.n2);
+  hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cnt, c.n2);
+  hipLaunchKernelGGL(k_cself, dim3(nb(c.n1)), dim3(256), 0, s, envs, slot, cidx);
+  pd_scan_exclusive(s, c.rev_cnt, c.rev_off, c.n2, scan_tmp);
+  const int wgrid = c.n1 > 0 ? (c.n1 + 3) / 4 : 1;
   hipLaunchKernelGGL(k_crev_fill, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
   hipLaunchKernelGGL(k_crev_sort, dim3(c.n2 > 0 ? (c.n2 + 3) / 4 : 1), dim3(256), 0, s, envs, slot, cidx);
 }

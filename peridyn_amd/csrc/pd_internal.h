@@ -54,9 +54,11 @@ struct BcDev {
 };
 
 struct ContactDev {
-  int32_t n1, n2;            // members of pair[1] / pair[2]
+  int32_t n1, n2;            // members of pair[1] / pair[2] (self contact: n2 = n1)
   int32_t max_neighs;
   int32_t expo_is_int;
+  int32_t self_contact;      // ContactModel11 of one block (idx2 = idx1)
+  int32_t pad_;
   double expo, K, equi, distance;
   const int32_t* idx1;       // member -> env particle id
   const int32_t* idx2;

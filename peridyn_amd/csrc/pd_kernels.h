@@ -60,6 +60,8 @@ void pd_scan_exclusive(hipStream_t s, const int32_t* in, int32_t* out, int n, in
 // contacts (pd_contact.hip)
 void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDev& host_env,
                               const ContactDev& c, int cidx);
+// self contact (ContactModel11): own lists from the bond states, transposed lists
+void pd_launch_self_contact_update(hipStream_t s, EnvDev* envs, int slot, const ContactDev& c, int cidx);
 
 // FP32-compensated form (pd_step32.hip)
 void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);

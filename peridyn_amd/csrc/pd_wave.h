@@ -59,6 +59,42 @@ __device__ __forceinline__ void contact_fn(const ContactDev& C, double d1, doubl
   F1 = s * d1; F2 = s * d2; F3 = s * d3;
 }
 
+// Self-contact terms of particle p (short_range_contact! for ContactModel11,
+// contacts.jl:357-380), in the order of the reference's loop over members i:
+// the terms p receives as neighbour of members i < p, its own term (i = p),
+// then those of members i > p.  Own: f_p += F(y_p - y_j) (vol_j / den_p);
+// as neighbour of i: f_p -= F(y_i - y_p) (vol_p / den_p) -- the reference uses
+// the neighbour's volume on both sides.  den = mass / volume (environment.jl:214).
+template <bool GPOW>
+__device__ __forceinline__ void add_self_contact(const EnvDev& E, const ContactDev& C, int p, const double4& yp,
+                                                 const double4* y, double& a1, double& a2, double& a3) {
+  const int i = C.inv1[p];
+  if (i < 0) return;
+  const double4* __restrict__ x = E.x;
+  const double vol_p = x[p].w;
+  const double den_p = (vol_p * E.blocks[E.pblock[p]].density[E.ptype[p]]) / vol_p;
+  const int off = C.rev_off[i], cnt = C.rev_cnt[i];
+  const double b = vol_p / den_p;
+  int k = 0;
+  auto rev_term = [&](int kk) {
+    const int q = C.idx1[C.rev[off + kk]];
+    const double4 yq = y[q];
+    double F1, F2, F3;
+    contact_fn<GPOW>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+    a1 -= F1 * b; a2 -= F2 * b; a3 -= F3 * b;
+  };
+  for (; k < cnt && C.rev[off + k] < i; ++k) rev_term(k);
+  if (C.own_cnt[i]) {
+    const int q = C.idx1[C.own[(size_t)i * C.max_neighs]];
+    const double4 yq = y[q];
+    double F1, F2, F3;
+    contact_fn<GPOW>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+    const double a = x[q].w / den_p;
+    a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
+  }
+  for (; k < cnt; ++k) rev_term(k);
+}
+
 // Contact terms of particle p, models in order (update_contact_acc!,
 // solvers.jl:257-269; short_range_contact! RM12, contacts.jl:230-245), added to
 // the material force as the reference's f1/f2 updates are.
@@ -68,6 +104,10 @@ __device__ __forceinline__ void add_contacts(const EnvDev& E, int p, const doubl
   const double4* __restrict__ x = E.x;
   for (int cm = 0; cm < E.ncontacts; ++cm) {
     const ContactDev& C = E.contacts[cm];
+    if (C.self_contact) {
+      add_self_contact<GPOW>(E, C, p, yp, y, a1, a2, a3);
+      continue;
+    }
     const int i1 = C.inv1[p];
     if (i1 >= 0) {
       const double inv_den_p = E.inv_den[p];

@@ -421,3 +421,46 @@ def test_qsdrag_bc_particles_move_once(orc):
     o.minimize(1e-3, 10.0, max_iter=5)
     top = spec["bcs"][1]["mask"]
     assert np.allclose(o.y[top] - y0[top], [0.0, 0.0, 1e-3 * 10.0], rtol=0, atol=1e-15)
+
+
+def _derived_rm11(y, fam, intact, equi):
+    """The one-contact rule implied by the merge walk over descending columns:
+    the largest family member, if that bond is broken and the pair is closer
+    than equi (a different algorithm from the walk it is checked against)."""
+    n = y.shape[0]
+    out = np.zeros(n, dtype=np.int64)
+    for i in range(n):
+        j = fam[i, 0]
+        if j > 0 and not intact[i, 0]:
+            d = y[i] - y[j - 1]
+            if ((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]) < equi * equi:
+                out[i] = j
+    return out
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_rm11_merge_walk_keeps_only_the_largest_family_member(orc, seed):
+    rng = np.random.default_rng(seed)
+    x = W.lattice_box(6, 6, 6)
+    fam, intact = orc.cal_family(x, 1.1, 40)  # faces: the largest member is +z at 1 dx
+    intact = (intact.astype(bool) & (rng.uniform(size=intact.shape) < 0.5)).astype(np.uint8)
+    y = x + rng.normal(scale=0.25, size=x.shape)
+    nb = orc.rm11_neighs(y, fam, intact, 5.0, 1.0, 50)
+    assert np.all(nb[:, 1:] == 0)                      # at most one contact each
+    want = _derived_rm11(y, fam, intact, 1.0)
+    assert np.array_equal(nb[:, 0], want)
+    assert np.count_nonzero(want) > 5                  # the case occurs
+
+
+def test_rm11_force_two_particle_closed_form(orc):
+    y = np.array([[0.0, 0.0, 0.0], [0.6, 0.0, 0.0]])
+    f = np.zeros_like(y)
+    vol = np.array([1.0, 2.0])
+    den = np.array([2.7e-3, 3.1e-3])
+    nb = np.array([[2], [0]])
+    K, equi = 100.0, 1.0
+    orc.rm11_force(y, f, vol, den, nb, equi, K, 2.0, 1)
+    strain = (equi - 0.6) / equi
+    F = (K * strain * strain) / (0.6 + 1e-10) * (y[0] - y[1])
+    assert np.allclose(f[0], F * (vol[1] / den[0]), rtol=1e-15, atol=0)
+    assert np.allclose(f[1], -F * (vol[1] / den[1]), rtol=1e-15, atol=0)  # vol_j on both sides
