@@ -1,0 +1,82 @@
+"""The benchmark workloads at full size on the GPU: family sizes against the
+committed lattice fixture, and the tensile block of the bond-based config
+(20^3, dt = 1e-3, 200 steps) against the oracle.
+
+Bars: bond counts exact; reference-order form bit-identical to the oracle over
+the whole literal run; FP64 form: the literal run (dt = 1e-3 is close to the
+stability limit, so rounding differences grow) is reported as the first step
+past 1e-10 relative L2, and the stable twin (dt = 1e-4) stays within 1e-10."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from peridyn_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "lattice_counts.json")))
+
+
+@pytest.fixture(scope="module")
+def core():
+    from peridyn_amd import core
+    return core
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def test_workload_bond_counts(core):
+    g = core.Env.from_spec(W.tensile_bb(), precision="fp64")
+    assert g.info()["bonds"] == GOLD["tensile_box_20"]["nnz"]
+    g.close()
+    g = core.Env.from_spec(W.plates_ep(), precision="fp64")
+    assert g.info()["bonds"] == 2 * GOLD["plate_50x50x20"]["nnz"]
+    g.close()
+    g = core.Env.from_spec(W.disk_impact(), precision="fp64")
+    assert g.info()["bonds"] == GOLD["disk_r150_t10"]["nnz"] + GOLD["sphere_r15"]["nnz"]
+    g.close()
+
+
+def test_tensile_literal_config_exact(core, orc):
+    spec = W.tensile_bb()
+    say("oracle: tensile 20^3, 200 steps")
+    o = orc.OracleEnv(spec)
+    o.run(200, 1 << 30)
+    g = core.Env.from_spec(spec, precision="exact")
+    g.run(200, 1 << 30)
+    y, v, f = g.download()
+    assert np.array_equal(y, o.y) and np.array_equal(v, o.v) and np.array_equal(f, o.f)
+    assert np.array_equal(g.damage(), o.damage())
+
+
+def test_tensile_fp64_divergence_step_and_stable_twin(core):
+    spec = W.tensile_bb()
+    x = spec["blocks"][0]["x"]
+    e = core.Env.from_spec(spec, precision="exact")
+    g = core.Env.from_spec(spec, precision="fp64")
+    first = None
+    for st in range(10, 201, 10):
+        e.run(10, 1 << 30)
+        g.run(10, 1 << 30)
+        err = rel_l2(g.download()[0] - x, e.download()[0] - x)
+        if first is None and err > 1e-10:
+            first = st
+    say(f"literal tensile config (dt = 1e-3): FP64 form first past 1e-10 at step {first} "
+        f"(final rel L2 {err:.2e})")
+    twin = W.tensile_bb(dt=1e-4)
+    e = core.Env.from_spec(twin, precision="exact")
+    g = core.Env.from_spec(twin, precision="fp64")
+    e.run(200, 1 << 30)
+    g.run(200, 1 << 30)
+    err = rel_l2(g.download()[0] - x, e.download()[0] - x)
+    say(f"stable twin (dt = 1e-4): FP64 rel L2 {err:.2e} after 200 steps")
+    assert err <= 1e-10
