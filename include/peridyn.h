@@ -74,12 +74,17 @@ typedef struct {
 } pd_block_desc;
 
 /* NonLinearSpringContactModel12 / LinearSpringContactModel (exponent 1) /
- * ShortRangeContactModel (contacts.jl:76-91, nonlinearspring.jl:63-76,124-145). */
+ * ShortRangeContactModel (contacts.jl:76-91, nonlinearspring.jl:63-76,124-145);
+ * ntypes2 == 0 describes NonLinearSpringContactModel11, the self contact of one
+ * block (contacts.jl:105-119, nonlinearspring.jl:78-122): types1 = mat.type,
+ * equi_dist = particle_size * distanceD, types2 unused. */
 typedef struct {
-  const int64_t* types1; int64_t ntypes1; /* pair[1] type ids */
-  const int64_t* types2; int64_t ntypes2; /* pair[2] type ids */
+  const int64_t* types1; int64_t ntypes1; /* pair[1] type ids (self contact: the block's types) */
+  const int64_t* types2; int64_t ntypes2; /* pair[2] type ids; 0 for a self contact */
   double exponent;
-  int32_t exponent_is_int;      /* Julia Int exponent (power by squaring) vs Float64 */
+  int32_t exponent_is_int;      /* nonzero: integer-valued exponent, evaluated by Julia's compensated
+                                   power by squaring (base/math.jl pow_body), as Julia 1.10's
+                                   Float64^Float64 does for integer-valued y; 0: general pow */
   int32_t reserved;
   double stiffness;
   double equi_dist;             /* (ps1 + ps2)/2 * distanceD */
@@ -167,7 +172,7 @@ int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f);
 int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, const double* f);
 int pd_env_damage(void* ctx, int64_t env, double* damage);        /* cal_damage, peri.jl:587-594 */
 int pd_env_time_step(void* ctx, int64_t env, int64_t* time_step);
-int pd_env_check_finite(void* ctx, int64_t env);
+int pd_env_check_finite(void* ctx, int64_t env);                  /* @check_nan, const_macros.jl:60-74 */
 /* Quasi-static relaxation, QSDrag minimize! (src/solvers/minimize.jl:105-231):
  * BCs applied once (position, velocity), then up to max_iter iterations of
  * update_acc! followed by the drag update of the particles held by no BC
@@ -187,7 +192,7 @@ int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64
  * a run resumed from it is bit-identical to the uninterrupted run. */
 int pd_env_state_size(void* ctx, int64_t env, int64_t* bytes);
 int pd_env_state_save(void* ctx, int64_t env, void* buf, int64_t bytes);
-int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes);                  /* @check_nan, const_macros.jl:60-74 */
+int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes);
 
 /* ---- measurement ------------------------------------------------------------
  * No reference counterpart (the reference's only timing hook is TimerOutputs,

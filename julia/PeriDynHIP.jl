@@ -8,8 +8,11 @@ seams:
   * force_density_T!(f, y, limits, mat, ::Type{Val{:hip}})   (general_material.jl:274-296)
   * short_range_contact!(y, f, type, bid, vol, den, RM, ::Type{Val{:hip}}) (contacts.jl:210)
   * update_contact_neighs!(y, type, RM::ContactModel12; device=:hip)       (contacts.jl:434)
-  * run_resident!(env, N; neigh_update_freq)  — the resident, fused run loop
-    (run!'s physics cadence, solvers.jl:97-187, without file output)
+  * run_resident!(env, N; neigh_update_freq, average_prop_freq) — the resident,
+    fused run loop (run!'s physics cadence, solvers.jl:97-187, with its printdata
+    rows but without file output)
+  * minimize_resident!(env, solver::QSDrag)                 (minimize.jl:105-231)
+  * checkpoint(id) / restore!(id, bytes) — resident restart state (io.jl:170-187)
 
 Julia is not installed in the build container, so this file is written against
 the C ABI in include/peridyn.h and the Python binding (peridyn_amd/_lib.py,
@@ -128,14 +131,20 @@ function PeriDyn.force_density_T!(f, y, limits, mat::PeriDyn.PeridynamicsMateria
 end
 
 """
-    run_resident!(env, N; neigh_update_freq=1, precision=PD_PREC_FP64)
+    upload_env(env; precision=PD_PREC_FP64) -> Int64
 
-Uploads the whole Env once (blocks, contacts, FixBC/ToFroBC/MoveBC) and runs N
-velocity_verlet_step!s on the device with run!'s contact-update cadence; y, v
-and f are copied back into env at the end.
+Uploads a whole Env once (environment.jl:166-235): its material blocks, its
+short-range contacts (ContactModel12 pairs and ContactModel11 self contacts) and
+its FixBC / ToFroBC / MoveBC conditions.  Returns the device env id; the state
+then stays resident until `release(id)`.
 """
-function run_resident!(env, N::Integer; neigh_update_freq::Integer=1, precision=PD_PREC_FP64)
+function upload_env(env; precision=PD_PREC_FP64)
     ctx = context()
+    # the device counts steps from 0 at upload; a ToFroBC direction depends on
+    # the absolute step (ToFroBC.jl:107-111)
+    if env.time_step != 0 && any(bc -> !(bc isa PeriDyn.FixBC) && isfinite(bc.freq), env.boundary_conditions)
+        error("upload_env: env.time_step = $(env.time_step) with a reversing ToFroBC; upload before stepping")
+    end
     id = Ref{Int64}(0)
     check(ccall((:pd_env_create, LIB), Cint, (Ptr{Cvoid}, Float64, Int32, Ref{Int64}),
                 ctx.h, env.dt, precision, id), ctx)
@@ -161,10 +170,14 @@ function run_resident!(env, N::Integer; neigh_update_freq::Integer=1, precision=
                                       (Ptr{Cvoid}, Int64, Ref{BlockDesc}, Ptr{Int64}), ctx.h, id[], d, C_NULL), ctx)
     end
     for RM in env.short_range_contact
-        t1 = collect(Int64, RM.pair[1]); t2 = collect(Int64, RM.pair[2])
+        # ContactModel11 (contacts.jl:105-119): one block, no second type list
+        selfc = RM isa PeriDyn.ContactModel11
+        t1 = selfc ? collect(Int64, RM.type) : collect(Int64, RM.pair[1])
+        t2 = selfc ? Int64[] : collect(Int64, RM.pair[2])
         push!(keep, (t1, t2))
-        cd = ContactDesc(pointer(t1), length(t1), pointer(t2), length(t2), Float64(RM.exponent),
-                         Int32(RM.exponent isa Integer), Int32(0), RM.stifness, RM.equi_dist,
+        e = Float64(RM.exponent)        # stored as Float64 (nonlinearspring.jl:16,33)
+        cd = ContactDesc(pointer(t1), length(t1), selfc ? Ptr{Int64}(C_NULL) : pointer(t2), length(t2), e,
+                         Int32(isinteger(e) && abs(e) <= 4096), Int32(0), RM.stifness, RM.equi_dist,
                          RM.distance, RM.max_neighs)
         GC.@preserve t1 t2 check(ccall((:pd_env_add_contact, LIB), Cint,
                                        (Ptr{Cvoid}, Int64, Ref{ContactDesc}), ctx.h, id[], cd), ctx)
@@ -182,16 +195,88 @@ function run_resident!(env, N::Integer; neigh_update_freq::Integer=1, precision=
                                       ctx.h, id[], bd), ctx)
     end
     check(ccall((:pd_env_commit, LIB), Cint, (Ptr{Cvoid}, Int64), ctx.h, id[]), ctx)
-    ids = [id[]]
-    check(ccall((:pd_run, LIB), Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64, Int64, Int64),
-                ctx.h, ids, 1, N, neigh_update_freq), ctx)
+    id[]
+end
+
+"Copies the resident y, v, f of device env `id` into env; env.time_step = t0 + device steps."
+function download!(env, id::Int64, t0::Integer)
+    ctx = context()
     y = similar(env.y, Float64); v = similar(env.v, Float64); f = similar(env.f, Float64)
     check(ccall((:pd_env_download, LIB), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-                ctx.h, id[], y, v, f), ctx)
+                ctx.h, id, y, v, f), ctx)
     env.y .= y; env.v .= v; env.f .= f
-    env.time_step += N
-    check(ccall((:pd_env_destroy, LIB), Cint, (Ptr{Cvoid}, Int64), ctx.h, id[]), ctx)
+    ts = Ref{Int64}(0)
+    check(ccall((:pd_env_time_step, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{Int64}), ctx.h, id, ts), ctx)
+    env.time_step = t0 + ts[]
     env
 end
+
+release(id::Int64) = check(ccall((:pd_env_destroy, LIB), Cint, (Ptr{Cvoid}, Int64), context().h, id), context())
+
+"""
+    run_resident!(env, N; neigh_update_freq=1, precision=PD_PREC_FP64, average_prop_freq=0)
+
+Uploads the Env, runs N velocity_verlet_step!s on the device with run!'s
+contact-update cadence (solvers.jl:97-187), copies y, v, f and time_step back.
+With average_prop_freq > 0 it also returns the printdata rows that run! logs
+(time, momentum, force sum, mean damage; environment.jl:308-320) after step 1
+and every average_prop_freq-th step, as a (rows, 8) matrix.
+"""
+function run_resident!(env, N::Integer; neigh_update_freq::Integer=1, precision=PD_PREC_FP64,
+                       average_prop_freq::Integer=0)
+    ctx = context()
+    t0 = env.time_step
+    id = upload_env(env; precision=precision)
+    ids = [id]
+    rows = nothing
+    if average_prop_freq > 0
+        nrow = ccall((:pd_log_rows, LIB), Int64, (Int64, Int64), N, average_prop_freq)
+        log = zeros(Float64, 8, nrow)
+        check(ccall((:pd_run_logged, LIB), Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64, Int64, Int64, Int64, Ptr{Float64}),
+                    ctx.h, ids, 1, N, neigh_update_freq, average_prop_freq, log), ctx)
+        rows = permutedims(log)
+        rows[:, 1] .+= t0 * env.dt
+    else
+        check(ccall((:pd_run, LIB), Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64, Int64, Int64),
+                    ctx.h, ids, 1, N, neigh_update_freq), ctx)
+    end
+    download!(env, id, t0)
+    release(id)
+    average_prop_freq > 0 ? (env, rows) : env
+end
+
+"""
+    minimize_resident!(env, solver::PeriDyn.QSDrag; precision=PD_PREC_FP64)
+
+QSDrag minimize! (minimize.jl:105-231) on the device: BCs once, then drag
+iterations of update_acc! until max|f| < f_tol and max|dy| < x_tol or max_iter.
+Returns (env, iterations).
+"""
+function minimize_resident!(env, solver; precision=PD_PREC_FP64)
+    ctx = context()
+    t0 = env.time_step
+    id = upload_env(env; precision=precision)
+    it = Ref{Int64}(0); fm = Ref{Float64}(0); xm = Ref{Float64}(0)
+    check(ccall((:pd_env_minimize, LIB), Cint,
+                (Ptr{Cvoid}, Int64, Float64, Float64, Int64, Float64, Float64, Ref{Int64}, Ref{Float64}, Ref{Float64}),
+                ctx.h, id, solver.step_size, solver.drag, solver.max_iter, solver.x_tol, solver.f_tol, it, fm, xm), ctx)
+    download!(env, id, t0)
+    release(id)
+    env, it[]
+end
+
+"Resident state of device env `id` as bytes (pd_env_state_save); `restore!` loads it back."
+function checkpoint(id::Int64)
+    ctx = context()
+    n = Ref{Int64}(0)
+    check(ccall((:pd_env_state_size, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{Int64}), ctx.h, id, n), ctx)
+    buf = Vector{UInt8}(undef, n[])
+    check(ccall((:pd_env_state_save, LIB), Cint, (Ptr{Cvoid}, Int64, Ptr{UInt8}, Int64), ctx.h, id, buf, n[]), ctx)
+    buf
+end
+
+restore!(id::Int64, buf::Vector{UInt8}) =
+    check(ccall((:pd_env_state_load, LIB), Cint, (Ptr{Cvoid}, Int64, Ptr{UInt8}, Int64),
+                context().h, id, buf, length(buf)), context())
 
 end # module

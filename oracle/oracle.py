@@ -27,7 +27,7 @@ FP64 results are the reference's.
 
 Parity status of every function is listed in the test files; functions without a
 pinning check are marked "parity unpinned" there: Julia's float `^` (contact
-exponent given as a float), `volume[1]^(1/3)` for non-cube volumes, and the tie
+exponent that is not integer-valued), `volume[1]^(1/3)` for non-cube volumes, and the tie
 cases of Julia's `isapprox` in the deformed latch.
 """
 from __future__ import annotations
@@ -59,6 +59,15 @@ def build(force: bool = False) -> str:
         os.replace(_SO + ".tmp", _SO)
     return _SO
 
+
+
+def _julia_int_pow(exponent) -> bool:
+    """`strain^expo` with expo::Float64 (nonlinearspring.jl:16,33,140) in Julia
+    1.10 (Manifest.toml: julia_version 1.10.4): `^(x::Float64, y::Float64)`
+    returns `x^Int(y)` (pow_body's compensated power by squaring, restated in
+    orc_julia_pow_int) when y is integer-valued, else its general power."""
+    e = float(exponent)
+    return math.isfinite(e) and e == math.floor(e) and abs(e) <= 4096.0
 
 class _OrcBlock(ctypes.Structure):
     _fields_ = [
@@ -411,7 +420,7 @@ class OracleEnv:
         expo = c.get("exponent", 1)
         return dict(mask1=mask1, mask2=mask2, equi=equi, distance=distanceX * equi,
                     max_neighs=max_neighs, K=float(c["stiffness"]),
-                    expo=float(expo), expo_is_int=int(isinstance(expo, (int, np.integer))),
+                    expo=float(expo), expo_is_int=int(_julia_int_pow(expo)),
                     neighs=np.zeros((int(mask1.sum()), max_neighs), dtype=np.int64))
 
     def _make_self_contact(self, c: dict) -> dict:
@@ -425,7 +434,7 @@ class OracleEnv:
         expo = c.get("exponent", 1)
         return dict(self_block=bi, equi=equi, distance=float(c.get("distanceX", 5.0)) * equi,
                     max_neighs=max_neighs, K=float(c["stiffness"]), expo=float(expo),
-                    expo_is_int=int(isinstance(expo, (int, np.integer))),
+                    expo_is_int=int(_julia_int_pow(expo)),
                     neighs=np.zeros((b.n, max_neighs), dtype=np.int64))
 
     def update_neighs(self):

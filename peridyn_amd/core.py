@@ -28,6 +28,17 @@ def _ptr(a: Optional[np.ndarray]):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+
+def pow_by_squaring(exponent) -> bool:
+    """Whether the reference evaluates `strain^exponent` by power by squaring.
+    The contact models store the exponent as Float64 (nonlinearspring.jl:16,33),
+    and Julia 1.10's `^(::Float64, ::Float64)` (the reference's Manifest pins
+    1.10.4) returns `x^Int(y)`, i.e. pow_body's compensated squaring, whenever y
+    is integer-valued; Int exponents are converted to Float64 on construction.
+    Exponents beyond +-4096 are sent to the general power."""
+    e = float(exponent)
+    return math.isfinite(e) and e == math.floor(e) and abs(e) <= 4096.0
+
 class Context:
     """One HIP device + stream (pd_ctx)."""
 
@@ -176,7 +187,7 @@ class Env:
                 cd.types1, cd.ntypes1 = _ptr(t1), t1.size
                 cd.types2, cd.ntypes2 = (_ptr(t2), t2.size) if t2.size else (None, 0)
                 cd.exponent = float(expo)
-                cd.exponent_is_int = int(isinstance(expo, (int, np.integer)))
+                cd.exponent_is_int = int(pow_by_squaring(expo))
                 cd.stiffness = float(cspec["stiffness"])
                 cd.equi_dist = equi
                 cd.distance = dX * equi

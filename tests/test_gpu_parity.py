@@ -168,10 +168,11 @@ def test_state_based_fp64_within_1e10(pd, orc, kind):
     assert rel_l2(y - x, o.y - x) <= 1e-10
 
 
-@pytest.mark.parametrize("expo", [1, 2])
+@pytest.mark.parametrize("expo", [1, 2, 3.0])
 def test_contact_plates_exact_bit_identical(pd, orc, expo):
     """Two small EP plates colliding with a (non)linear spring contact, contact
-    lists rebuilt every 5 steps."""
+    lists rebuilt every 5 steps.  3.0: an integer-valued Float64 exponent takes
+    Julia's power by squaring, as an Int one does."""
     say("contact plates exact", expo)
     spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=expo)
     o, g = run_both(pd, orc, spec, 30, nuf=5)

@@ -296,6 +296,37 @@ def test_contact_two_particle_closed_form(orc):
         assert np.allclose(env.f[1], -F * (1.0 / 4.0), rtol=1e-14, atol=0)
 
 
+def test_contact_exponent_value_selects_the_power(orc):
+    """The contact models store the exponent as Float64 and Julia 1.10's
+    Float64^Float64 takes power by squaring for integer-valued exponents: 3 and
+    3.0 give the same bits (x*x*x), 2.5 takes the general power.  The binding's
+    rule agrees with the oracle's."""
+    from peridyn_amd import core
+    for e in (1, 2, 3, 2.0, 3.0, -2.0, 4096.0):
+        assert core.pow_by_squaring(e) and orc._julia_int_pow(e)
+    for e in (1.5, 2.5, 4097.0, float("inf"), float("nan")):
+        assert not core.pow_by_squaring(e) and not orc._julia_int_pow(e)
+    forces = {}
+    for expo in (3, 3.0, 2.5):
+        xa = np.array([[0.0, 0.0, 0.0]])
+        xb = np.array([[0.0, 0.0, 0.7]])
+        ba = W.block("bb", xa, 3.0, type_id=1, bulk_modulus=1.0, critical_stretch=1.0, density=2.0)
+        bb = W.block("bb", xb, 3.0, type_id=2, bulk_modulus=1.0, critical_stretch=1.0, density=4.0)
+        spec = {"dt": 1.0, "blocks": [ba, bb], "bcs": [],
+                "contacts": [{"pair": ([1], [2]), "exponent": expo, "stiffness": 10.0,
+                              "distanceD": 0.9, "distanceX": 2.0, "max_neighs": 5}]}
+        env = orc.OracleEnv(spec)
+        env.update_neighs()
+        env.update_acc()
+        forces[expo] = env.f.copy()
+    assert np.array_equal(forces[3], forces[3.0])
+    strain = (0.9 - 0.7) / 0.9
+    s3 = 10.0 * (strain * strain * strain) / (0.7 + 1e-10)
+    assert forces[3][0, 2] == (s3 * -0.7) * (1.0 / 2.0)
+    s25 = 10.0 * strain ** 2.5 / (0.7 + 1e-10)
+    assert np.isclose(forces[2.5][0, 2], (s25 * -0.7) * (1.0 / 2.0), rtol=1e-14, atol=0)
+
+
 def test_contact_search_matches_brute_force(orc):
     rng = np.random.default_rng(5)
     x1 = rng.uniform(0, 6, size=(300, 3))
