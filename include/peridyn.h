@@ -118,12 +118,19 @@ int pd_env_add_block(void* ctx, int64_t env, const pd_block_desc* b, int64_t* bl
 int pd_env_add_contact(void* ctx, int64_t env, const pd_contact_desc* c);
 int pd_env_add_bc(void* ctx, int64_t env, const pd_bc_desc* bc);
 /* Bond-loop schedule of the env: lanes per particle (power of two 1..64; 1 =
- * lane per particle, 64 = wavefront per particle; 0 = choose from the particle
- * count at commit).  Set before commit; the reference-order form uses 1.  No
+ * lane per particle, 64 = wavefront per particle; 0 = choose at commit: 4 for
+ * envs that run tiled (pd_env_layout), else from the particle count).  Set
+ * before commit; the reference-order form uses 1.  No
  * reference counterpart: it selects among layouts of the same computation
  * (peridyn_amd/autotune.py picks it by timing). */
 int pd_env_set_schedule(void* ctx, int64_t env, int32_t lanes_per_particle);
 int pd_env_schedule(void* ctx, int64_t env, int32_t* lanes_per_particle);
+/* Bond layout of a committed env (no reference counterpart): lanes per
+ * particle, whether the FP64 passes run tiled (halos staged in LDS; the
+ * default for FP64 envs of single-type blocks with integer contact exponents
+ * and 2..8 lanes per particle whose halos fit; PERIDYN_TILES=0 at commit turns
+ * it off) and the largest tile halo.  Any pointer may be NULL. */
+int pd_env_layout(void* ctx, int64_t env, int32_t* lanes_per_particle, int32_t* tiled, int32_t* max_halo);
 /* Uploads the env, builds every block's family on the GPU (cell list -> CSR),
  * the weighted volume m and the per-particle BC map.  PD_E_OVERFLOW where
  * cal_family/GeneralMaterial would raise BoundsError. */

@@ -23,7 +23,7 @@ EXPORTS = [
     "pd_block_weighted_volume", "pd_block_theta", "pd_block_edp", "pd_run", "pd_env_update_acc",
     "pd_force_density", "pd_env_update_neighs", "pd_env_download", "pd_env_upload", "pd_env_damage",
     "pd_env_time_step", "pd_env_check_finite", "pd_ctx_profile", "pd_ctx_profile_read",
-    "pd_env_set_schedule", "pd_env_schedule", "pd_ctx_mem_info", "pd_run_logged", "pd_log_rows",
+    "pd_env_set_schedule", "pd_env_schedule", "pd_env_layout", "pd_ctx_mem_info", "pd_run_logged", "pd_log_rows",
     "pd_env_observables", "pd_env_state_size", "pd_env_state_save", "pd_env_state_load",
     "pd_env_minimize",
 ]
@@ -114,6 +114,7 @@ def load() -> ctypes.CDLL:
     L.pd_env_check_finite.argtypes = [P, I64]
     L.pd_env_set_schedule.argtypes = [P, I64, I32]
     L.pd_env_schedule.argtypes = [P, I64, ctypes.POINTER(I32)]
+    L.pd_env_layout.argtypes = [P, I64, ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I32)]
     L.pd_ctx_profile.argtypes = [P, I32]
     L.pd_ctx_profile_read.argtypes = [P, I32, ctypes.POINTER(D), ctypes.POINTER(I64)]
     _lib = L

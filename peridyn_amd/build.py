@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libperidyn_hip.so")
 BUILD_DIR = os.path.join(ROOT, "_build")
-SOURCES = ["pd_api.hip", "pd_step.hip", "pd_step32.hip", "pd_family.hip", "pd_contact.hip"]
+SOURCES = ["pd_api.hip", "pd_step.hip", "pd_step32.hip", "pd_family.hip", "pd_contact.hip", "pd_tile.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

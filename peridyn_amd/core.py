@@ -321,6 +321,13 @@ class Env:
                                                 ctypes.byref(by)))
         return {"particles": n.value, "bonds": b.value, "device_bytes": by.value}
 
+    def layout(self) -> dict:
+        """Bond layout: lanes per particle, tiled FP64 passes, largest tile halo."""
+        g, t, h = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.pd_env_layout(self.ctx.h, self.id, ctypes.byref(g), ctypes.byref(t),
+                                                  ctypes.byref(h)))
+        return {"lanes_per_particle": g.value, "tiled": bool(t.value), "max_halo": h.value}
+
     @property
     def lanes_per_particle(self) -> int:
         g = ctypes.c_int32()

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -53,10 +54,16 @@ struct Env {
   // them since), so an update_neighs! now would rebuild the same sorted lists
   bool lists_current = false;
   bool has_ep = false;
-  int64_t N = 0, nslots = 0, nbonds = 0;
+  int64_t N = 0, nslots = 0, nbonds = 0, npk = 0;
   int64_t time_step = 0;
   int32_t cur = 0, edp_cur = 0;
   std::vector<int32_t> perm, iperm;  // internal id -> reference index and back (empty: identity)
+  // tiled FP64 passes: brick order requested at commit (want_tiles), tiles
+  // built (tiled: the halos fit the LDS budget); brick of each internal particle
+  bool want_tiles = false, tiled = false;
+  std::vector<int32_t> brick;
+  std::vector<Tile> tiles;
+  int32_t hmax = 0;  // largest tile halo
   std::vector<Chunk> chunks;
   Chunk* d_chunks = nullptr;
   int32_t* d_self = nullptr;  // this env's slot, for single-env bookkeeping launches
@@ -78,6 +85,16 @@ struct Ctx {
   // cached run set
   std::vector<int64_t> set_ids;
   bool set_valid = false;
+  // stepping lists: chunks of untiled envs per pass, tiles of tiled envs per
+  // pass and lanes per particle (lgG 1..3)
+  int n_theta_u = 0, n_td_u = 0, n_force_u[3] = {0, 0, 0};
+  int off_theta_u = 0, off_td_u = 0, off_force_u[3] = {0, 0, 0};
+  Tile* d_tset = nullptr;
+  size_t tset_cap = 0;
+  int nt_theta[4] = {}, nt_td[4] = {}, nt_force[3][4] = {};
+  int ot_theta[4] = {}, ot_td[4] = {}, ot_force[3][4] = {};
+  int set_hcap = 1;
+  bool set_tiled = false;
   Chunk* d_set = nullptr;
   size_t set_cap = 0;
   int n_all = 0, n_theta = 0, n_td = 0, n_force[3] = {0, 0, 0};
@@ -289,12 +306,11 @@ void ensure_slot(Ctx* c, Env& e) {
 }
 
 // ---- internal particle order ---------------------------------------------------
-// The fast forms store each block's particles along a Morton (Z-order) curve of
-// their reference positions, so that every chunk, every workgroup and every
-// XCD's contiguous share of chunks covers a compact region and its neighbour
-// gathers stay in L2.  The reference-order form keeps the reference order (its
-// summation order is defined by reference indices).  All host-facing arrays
-// stay in reference order; perm / iperm translate at the boundary.
+// Device arrays may hold a block's particles in an internal order (perm:
+// internal id -> reference index, iperm: back; empty = identity).  All
+// host-facing arrays stay in reference order; perm / iperm translate at the
+// boundary.  The reference-order form always keeps the reference order (its
+// summation order is defined by reference indices).
 inline int64_t internal_of(const Env& e, int64_t r) { return e.iperm.empty() ? r : e.iperm[r]; }
 inline int64_t reference_of(const Env& e, int64_t p) { return e.perm.empty() ? p : e.perm[p]; }
 
@@ -308,10 +324,80 @@ uint64_t morton_spread(uint64_t v) {
   return v;
 }
 
+// Tiled passes for the FP64 form (PERIDYN_TILES=0 turns them off).
+bool tiles_enabled() {
+  const char* v = std::getenv("PERIDYN_TILES");
+  return !(v && std::string(v) == "0");
+}
+
+// Brick of a tile in lattice cells for lgG: 4 chunks of C = 64/G particles,
+// each chunk one bx x by layer of the brick.
+void brick_dims(int lgG, int& bx, int& by, int& bz) {
+  if (lgG == 1) { bx = 8; by = 4; bz = 4; }
+  else if (lgG == 2) { bx = 4; by = 4; bz = 4; }
+  else { bx = 4; by = 2; bz = 4; }
+}
+
+// Brick order of every block: particles binned into lattice cells of the
+// block's particle size, cells grouped into bricks (brick_dims), bricks in
+// z-y-x order and cells inside a brick likewise; ties keep the input order.
+// e.brick numbers each particle's brick (unique over the env).
+void build_brick_order(Env& e) {
+  e.perm.resize(e.N);
+  e.iperm.resize(e.N);
+  e.brick.assign(e.N, 0);
+  int bx, by, bz;
+  brick_dims(e.lgG, bx, by, bz);
+  int32_t next_brick = 0;
+  for (auto& b : e.blocks) {
+    const int64_t n = b.d.n;
+    if (n == 0) continue;
+    double lo[3] = {INFINITY, INFINITY, INFINITY};
+    for (int64_t i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], b.x[3 * i + d]);
+    const double ps = b.d.particle_size > 0.0 ? b.d.particle_size : b.d.horizon / 3.0;
+    std::vector<int64_t> q(3 * n);
+    int64_t nb[3] = {1, 1, 1};
+    const int dims[3] = {bx, by, bz};
+    for (int64_t i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        const double v = std::floor((b.x[3 * i + d] - lo[d]) / ps + 0.5);
+        const int64_t qi = (int64_t)std::min(std::max(v, 0.0), 1048575.0);
+        q[3 * i + d] = qi;
+        nb[d] = std::max(nb[d], qi / dims[d] + 1);
+      }
+    struct K { int64_t brick, cell; int32_t i; };
+    std::vector<K> key(n);
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t* qq = &q[3 * i];
+      const int64_t br = ((qq[2] / bz) * nb[1] + qq[1] / by) * nb[0] + qq[0] / bx;
+      const int64_t cl = ((qq[2] % bz) * by + qq[1] % by) * bx + qq[0] % bx;
+      key[i] = K{br, cl, (int32_t)i};
+    }
+    std::sort(key.begin(), key.end(), [](const K& a, const K& c) {
+      if (a.brick != c.brick) return a.brick < c.brick;
+      if (a.cell != c.cell) return a.cell < c.cell;
+      return a.i < c.i;
+    });
+    for (int64_t k = 0; k < n; ++k) {
+      if (k > 0 && key[k].brick != key[k - 1].brick) ++next_brick;
+      e.perm[b.first + k] = b.first + key[k].i;
+      e.iperm[b.first + key[k].i] = (int32_t)(b.first + k);
+      e.brick[b.first + k] = next_brick;
+    }
+    ++next_brick;
+  }
+}
+
 void build_order(Env& e) {
   e.perm.clear();
   e.iperm.clear();
+  e.brick.clear();
   if (e.precision == PD_PREC_EXACT) return;
+  if (e.want_tiles) {
+    build_brick_order(e);
+    return;
+  }
   // Measured on lattice specimens the input (lattice row) order is faster: the
   // k-th neighbours of consecutive lanes are consecutive particles, so neighbour
   // gathers coalesce; PERIDYN_ORDER=morton selects the Z-order layout.
@@ -350,6 +436,105 @@ void build_order(Env& e) {
   }
 }
 
+// LDS a tile may use for its largest pass (2 workgroups fit a CU's 160 KiB)
+constexpr size_t kTileLdsBudget = 80 * 1024;
+
+// Tiles of a brick-ordered env: consecutive chunks of one brick, up to
+// PD_TILE_CHUNKS per tile.  Each tile's halo is the sorted set of neighbour ids
+// of its particles; pk receives every bond's halo-local index with the intact
+// bit (every bond is intact after the family build).  Leaves e.tiled false
+// (untiled passes) when a halo exceeds 15-bit indices or the LDS budget.
+void build_tiles(Ctx* c, Env& e) {
+  hipStream_t s = c->stream;
+  std::vector<int32_t> col((size_t)std::max<int64_t>(e.nslots, 1));
+  PD_HIP(hipMemcpyAsync(col.data(), e.dev.col, sizeof(int32_t) * (size_t)e.nslots, hipMemcpyDeviceToHost, s));
+  PD_HIP(hipStreamSynchronize(s));
+  // tiles: chunk ranges
+  std::vector<std::pair<int, int>> tr;
+  for (int k = 0; k < (int)e.chunks.size();) {
+    const int br = e.brick[e.chunks[k].first];
+    int n = 1;
+    while (n < PD_TILE_CHUNKS && k + n < (int)e.chunks.size() && e.chunks[k + n].block == e.chunks[k].block &&
+           e.brick[e.chunks[k + n].first] == br)
+      ++n;
+    tr.emplace_back(k, n);
+    k += n;
+  }
+  const int nt = (int)tr.size();
+  std::vector<std::vector<int32_t>> halos(nt);
+  std::vector<uint64_t> pk((size_t)std::max<int64_t>(e.npk, 1), 0ull);
+  const int64_t N = e.N;
+  auto work = [&](int t0, int t1) {
+    std::vector<int32_t> stamp(N, -1), pos(N, 0);
+    for (int t = t0; t < t1; ++t) {
+      std::vector<int32_t>& h = halos[t];
+      for (int k = tr[t].first; k < tr[t].first + tr[t].second; ++k) {
+        const Chunk& ch = e.chunks[k];
+        const size_t nsl = (size_t)PD_WAVE * ch.width;
+        for (size_t i = 0; i < nsl; ++i) {
+          const int32_t j = col[ch.slot0 + i];
+          if (j >= 0 && stamp[j] != t) {
+            stamp[j] = t;
+            h.push_back(j);
+          }
+        }
+      }
+      std::sort(h.begin(), h.end());
+      if (h.size() > 0x7fff) continue;  // rejected below
+      for (size_t i = 0; i < h.size(); ++i) pos[h[i]] = (int32_t)i;
+      for (int k = tr[t].first; k < tr[t].first + tr[t].second; ++k) {
+        const Chunk& ch = e.chunks[k];
+        for (int st = 0; st < ch.width; ++st)
+          for (int l = 0; l < PD_WAVE; ++l) {
+            const int32_t j = col[ch.slot0 + (size_t)PD_WAVE * st + l];
+            if (j < 0) continue;
+            const uint64_t v = 0x8000u | (uint32_t)pos[j];
+            pk[ch.pk0 + (size_t)PD_WAVE * (st >> 2) + l] |= v << (16 * (st & 3));
+          }
+      }
+    }
+  };
+  const int nth = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), nt}));
+  std::vector<std::thread> pool;
+  for (int w = 0; w < nth; ++w) pool.emplace_back(work, (int)((int64_t)nt * w / nth), (int)((int64_t)nt * (w + 1) / nth));
+  for (auto& th : pool) th.join();
+  size_t hmax = 1, total = 0;
+  for (auto& h : halos) {
+    hmax = std::max(hmax, h.size());
+    total += h.size();
+  }
+  int nf = 7;
+  for (auto& b : e.blocks) nf = std::max(nf, pd_tile_fields(2, b.d.kind));
+  if (e.has_ep) nf = std::max(nf, pd_tile_fields(1, PD_EP));
+  if (hmax > 0x7fff || sizeof(double) * nf * hmax > kTileLdsBudget ||
+      total * sizeof(int32_t) >= PD_MAX_ARRAY_BYTES) {
+    e.tiles.clear();
+    e.tiled = false;
+    return;
+  }
+  std::vector<int32_t> hall;
+  hall.reserve(total);
+  e.tiles.assign(nt, Tile{});
+  for (int t = 0; t < nt; ++t) {
+    Tile& T = e.tiles[t];
+    T.env = 0;
+    T.block = e.chunks[tr[t].first].block;
+    T.nch = tr[t].second;
+    T.hn = (int32_t)halos[t].size();
+    T.hoff = (uint32_t)hall.size();
+    for (int k = 0; k < T.nch; ++k) T.ch[k] = e.chunks[tr[t].first + k];
+    hall.insert(hall.end(), halos[t].begin(), halos[t].end());
+  }
+  e.hmax = (int32_t)hmax;
+  e.dev.halo = dalloc<int32_t>(e, std::max<size_t>(hall.size(), 1));
+  e.dev.pk = dalloc<uint64_t>(e, pk.size());
+  if (!hall.empty())
+    PD_HIP(hipMemcpyAsync(e.dev.halo, hall.data(), sizeof(int32_t) * hall.size(), hipMemcpyHostToDevice, s));
+  PD_HIP(hipMemcpyAsync(e.dev.pk, pk.data(), sizeof(uint64_t) * pk.size(), hipMemcpyHostToDevice, s));
+  PD_HIP(hipStreamSynchronize(s));
+  e.tiled = true;
+}
+
 void commit_env(Ctx* c, Env& e) {
   if (e.committed) throw PdError(PD_E_STATE, "env already committed");
   if (e.blocks.empty()) throw PdError(PD_E_ARG, "env has no material blocks");
@@ -359,16 +544,27 @@ void commit_env(Ctx* c, Env& e) {
     throw PdError(PD_E_ARG, "env too large: its particle arrays would exceed 4 GiB; split it into several envs");
   e.N = N;
   // bond layout: the reference-order form walks each row on one lane; the
-  // other forms spread a particle over G lanes so that small specimens still
+  // untiled forms spread a particle over G lanes so that small specimens still
   // fill the chip (>= 786,432 lanes = 48 waves per CU over 256 CUs; measured
-  // best on the 100k-particle EP plates (G=8) and the 721k-particle BB disk
-  // (G=2)), unless a schedule was set explicitly
+  // best untiled on the 100k-particle EP plates (G=8) and the 721k-particle BB
+  // disk (G=2)), unless a schedule was set explicitly
+  // tiled passes: FP64 form, 2..8 lanes per particle, single-type blocks and
+  // integer contact exponents (the tiled force kernel carries no general pow);
+  // 4 lanes per particle unless a schedule was set (measured best on the EP
+  // plates, equal to 2 on the BB disk)
+  bool tileable = e.precision == PD_PREC_FP64 && tiles_enabled();
+  for (auto& b : e.blocks) tileable = tileable && b.d.ntype == 1;
+  for (auto& hc : e.contacts) tileable = tileable && hc.d.exponent_is_int;
   if (e.precision == PD_PREC_EXACT) {
     e.lgG = 0;
+  } else if (e.lgG < 0 && tileable) {
+    e.lgG = 2;
   } else if (e.lgG < 0) {
     e.lgG = 0;
     while (e.lgG < 4 && (N << e.lgG) < 786432) ++e.lgG;
   }
+  e.want_tiles = tileable && e.lgG >= 1 && e.lgG <= 3;
+  e.tiled = false;
   build_order(e);
   // ---- particle arrays (host staging, internal order)
   std::vector<double4> x4(N), y4(N), v4(N);
@@ -449,6 +645,7 @@ void commit_env(Ctx* c, Env& e) {
     std::vector<uint32_t> pslot0(N);
     uint64_t slots = 0;
     e.chunks.clear();
+    e.npk = 0;
     e.nbonds = 0;
     for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
       HostBlock& b = e.blocks[bi];
@@ -464,12 +661,20 @@ void commit_env(Ctx* c, Env& e) {
                                          std::to_string(b.d.max_neigh) + " must exceed the maximum degree");
       b.chunk0 = (int32_t)e.chunks.size();
       const int G = 1 << e.lgG, Cp = PD_WAVE >> e.lgG;
-      for (int64_t c0 = 0; c0 < b.d.n; c0 += Cp) {
+      for (int64_t c0 = 0; c0 < b.d.n;) {
+        int64_t cnt = std::min<int64_t>(Cp, b.d.n - c0);
+        if (e.want_tiles) {  // a chunk never spans two bricks
+          const int32_t br = e.brick[b.first + c0];
+          int64_t k = 1;
+          while (k < cnt && e.brick[b.first + c0 + k] == br) ++k;
+          cnt = k;
+        }
         Chunk ch{};
         ch.env = 0;
         ch.block = (int32_t)bi;
         ch.first = (int32_t)(b.first + c0);
-        ch.count = (int32_t)std::min<int64_t>(Cp, b.d.n - c0);
+        ch.count = (int32_t)cnt;
+        c0 += cnt;
         int32_t md = 0;
         for (int l = 0; l < ch.count; ++l) md = std::max(md, deg[ch.first + l]);
         const int32_t wdt = (md + G - 1) / G;  // slots per lane
@@ -481,6 +686,8 @@ void commit_env(Ctx* c, Env& e) {
         }
         slots += (uint64_t)PD_WAVE * wdt;
         if (slots >= 0xfffffff0ULL) throw PdError(PD_E_ARG, "env too large (bond slots exceed 2^32)");
+        ch.pk0 = (uint32_t)e.npk;
+        if (e.want_tiles) e.npk += (int64_t)PD_WAVE * ((wdt + 3) / 4);
         e.chunks.push_back(ch);
       }
       b.nchunks = (int32_t)e.chunks.size() - b.chunk0;
@@ -490,6 +697,7 @@ void commit_env(Ctx* c, Env& e) {
     // kernels address every array with 32-bit byte offsets
     if ((uint64_t)slots * (e.has_ep ? sizeof(double) : sizeof(int32_t)) >= PD_MAX_ARRAY_BYTES)
       throw PdError(PD_E_ARG, "env too large: its bond arrays would exceed 4 GiB; split it into several envs");
+    if ((uint64_t)e.npk * sizeof(uint64_t) >= PD_MAX_ARRAY_BYTES) e.want_tiles = false;
     PD_HIP(hipMemcpyAsync(D.pslot0, pslot0.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s));
     D.col = dalloc<int32_t>(e, slots);
     D.state = dalloc<uint8_t>(e, slots);
@@ -518,6 +726,7 @@ void commit_env(Ctx* c, Env& e) {
       for (auto& b : e.blocks)
         pd_launch_rev(s, b.first, (int)b.d.n, D.pslot0, e.lgG, d_ref, D.deg, D.col, D.rev);
     }
+    if (e.want_tiles) build_tiles(c, e);
     if (e.precision == PD_PREC_FP32C) {
       // x32 relative to each block's minimum corner (bonds never leave a block)
       std::vector<float4> x32(N), a32(N);
@@ -550,6 +759,7 @@ void commit_env(Ctx* c, Env& e) {
   }
   D.has_ep = e.has_ep ? 1 : 0;
   D.lgG = e.lgG;
+  D.tiled = e.tiled ? 1 : 0;
 
   // ---- blocks
   std::vector<BlockDev> bd(e.blocks.size());
@@ -674,6 +884,10 @@ void commit_env(Ctx* c, Env& e) {
   }
   ensure_slot(c, e);
   for (auto& ch : e.chunks) ch.env = e.slot;
+  for (auto& t : e.tiles) {
+    t.env = e.slot;
+    for (int k = 0; k < t.nch; ++k) t.ch[k].env = e.slot;
+  }
   e.d_self = dalloc<int32_t>(e, 1);
   PD_HIP(hipMemcpyAsync(e.d_self, &e.slot, sizeof(int32_t), hipMemcpyHostToDevice, s));
   e.d_chunks = dalloc<Chunk>(e, e.chunks.size());
@@ -702,10 +916,14 @@ bool env_gpow(const Env& e) {
 
 void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   if (c->set_valid && ids == c->set_ids) return;
-  std::vector<Chunk> all, th, td, fk[3];
+  // chunk lists of every env (all passes), chunk lists of the untiled envs for
+  // stepping, tile lists of the tiled envs per pass and lanes per particle
+  std::vector<Chunk> all, th, td, fk[3], thu, tdu, fku[3];
+  std::vector<Tile> tth[4], ttd[4], tfk[3][4];
   std::vector<int32_t> slots;
   std::vector<int2> ranges;
-  bool has_ep = false, has_th = false, gpow = false;
+  bool has_ep = false, has_th = false, gpow = false, tiled = false;
+  int hcap = 1;
   for (int64_t id : ids) {
     Env& e = get_env(c, id);
     if (!e.committed) throw PdError(PD_E_STATE, "env " + std::to_string(id) + " is not committed");
@@ -718,6 +936,21 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
       fk[kind].push_back(ch);
       if (kind != PD_BB) { th.push_back(ch); has_th = true; }
       if (kind == PD_EP) { td.push_back(ch); has_ep = true; }
+      if (!e.tiled) {
+        fku[kind].push_back(ch);
+        if (kind != PD_BB) thu.push_back(ch);
+        if (kind == PD_EP) tdu.push_back(ch);
+      }
+    }
+    if (e.tiled) {
+      tiled = true;
+      hcap = std::max(hcap, (int)e.hmax);
+      for (const Tile& t : e.tiles) {
+        const int kind = e.blocks[t.block].d.kind;
+        tfk[kind][e.lgG].push_back(t);
+        if (kind != PD_BB) tth[e.lgG].push_back(t);
+        if (kind == PD_EP) ttd[e.lgG].push_back(t);
+      }
     }
   }
   std::vector<Chunk> cat;
@@ -725,12 +958,38 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   c->off_theta = (int)cat.size(); cat.insert(cat.end(), th.begin(), th.end());
   c->off_td = (int)cat.size(); cat.insert(cat.end(), td.begin(), td.end());
   for (int k = 0; k < 3; ++k) { c->off_force[k] = (int)cat.size(); cat.insert(cat.end(), fk[k].begin(), fk[k].end()); }
+  c->off_theta_u = (int)cat.size(); cat.insert(cat.end(), thu.begin(), thu.end());
+  c->off_td_u = (int)cat.size(); cat.insert(cat.end(), tdu.begin(), tdu.end());
+  for (int k = 0; k < 3; ++k) {
+    c->off_force_u[k] = (int)cat.size();
+    cat.insert(cat.end(), fku[k].begin(), fku[k].end());
+  }
   c->n_all = (int)all.size();
   c->n_theta = (int)th.size();
   c->n_td = (int)td.size();
   for (int k = 0; k < 3; ++k) c->n_force[k] = (int)fk[k].size();
+  c->n_theta_u = (int)thu.size();
+  c->n_td_u = (int)tdu.size();
+  for (int k = 0; k < 3; ++k) c->n_force_u[k] = (int)fku[k].size();
   ensure_cap(c->d_set, c->set_cap, cat.size());
   PD_HIP(hipMemcpyAsync(c->d_set, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
+  std::vector<Tile> tcat;
+  for (int lg = 1; lg <= 3; ++lg) {
+    c->ot_theta[lg] = (int)tcat.size(); c->nt_theta[lg] = (int)tth[lg].size();
+    tcat.insert(tcat.end(), tth[lg].begin(), tth[lg].end());
+    c->ot_td[lg] = (int)tcat.size(); c->nt_td[lg] = (int)ttd[lg].size();
+    tcat.insert(tcat.end(), ttd[lg].begin(), ttd[lg].end());
+    for (int k = 0; k < 3; ++k) {
+      c->ot_force[k][lg] = (int)tcat.size(); c->nt_force[k][lg] = (int)tfk[k][lg].size();
+      tcat.insert(tcat.end(), tfk[k][lg].begin(), tfk[k][lg].end());
+    }
+  }
+  if (!tcat.empty()) {
+    ensure_cap(c->d_tset, c->tset_cap, tcat.size());
+    PD_HIP(hipMemcpyAsync(c->d_tset, tcat.data(), sizeof(Tile) * tcat.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  c->set_hcap = hcap;
+  c->set_tiled = tiled;
   ensure_cap(c->d_slots, c->slots_cap, slots.size());
   PD_HIP(hipMemcpyAsync(c->d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, c->stream));
   ensure_cap(c->d_obs_ranges, c->obs_ranges_cap, ranges.size());
@@ -829,6 +1088,8 @@ void force_pass(Ctx* c, Env& e, const std::vector<Chunk>& sub, int flags_extra) 
       pd_launch_book(c->stream, c->d_envs, e.d_self, 1, PD_B_FLIP_EDP);
       e.edp_cur ^= 1;
     }
+    // bonds broken by these (untiled) passes: refresh the tiles' intact bits
+    if (e.tiled) pd_launch_pk_sync(c->stream, e.d_chunks, (int)e.chunks.size(), c->d_envs);
     PD_HIP(hipGetLastError());
     PD_HIP(hipStreamSynchronize(c->stream));
   } catch (...) {
@@ -974,16 +1235,32 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
   // sums after it
   auto step_kernels = [&](bool last, bool logged) {
     if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part); });
+    Tile* dt = c->d_tset;
+    const int hc = c->set_hcap;
     if (c->set_has_theta)
-      timed(c, PD_PROF_THETA, [&] { pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T); });
-    if (c->set_has_ep) timed(c, PD_PROF_TD, [&] { pd_launch_td(s, prec, d + c->off_td, c->n_td, T); });
+      timed(c, PD_PROF_THETA, [&] {
+        if (c->n_theta_u) pd_launch_theta(s, prec, d + c->off_theta_u, c->n_theta_u, T);
+        for (int lg = 1; lg <= 3; ++lg)
+          if (c->nt_theta[lg]) pd_launch_tile_theta(s, lg, dt + c->ot_theta[lg], c->nt_theta[lg], T, hc);
+      });
+    if (c->set_has_ep)
+      timed(c, PD_PROF_TD, [&] {
+        if (c->n_td_u) pd_launch_td(s, prec, d + c->off_td_u, c->n_td_u, T);
+        for (int lg = 1; lg <= 3; ++lg)
+          if (c->nt_td[lg]) pd_launch_tile_td(s, lg, dt + c->ot_td[lg], c->nt_td[lg], T, hc);
+      });
     double* rows = logged ? c->d_obs_rows + (size_t)row * nenv * 8 : nullptr;
     if (logged) pd_launch_observables(s, 0, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
     const int fl = PD_F_KICK | (last ? 0 : PD_F_DRIFT);
     for (int k = 0; k < 3; ++k)
       if (c->n_force[k])
-        timed(c, PD_PROF_FORCE_BB + k,
-              [&] { pd_launch_force(s, k, prec, d + c->off_force[k], c->n_force[k], T, fl, c->set_gpow); });
+        timed(c, PD_PROF_FORCE_BB + k, [&] {
+          if (c->n_force_u[k])
+            pd_launch_force(s, k, prec, d + c->off_force_u[k], c->n_force_u[k], T, fl, c->set_gpow);
+          for (int lg = 1; lg <= 3; ++lg)
+            if (c->nt_force[k][lg])
+              pd_launch_tile_force(s, k, lg, dt + c->ot_force[k][lg], c->nt_force[k][lg], T, fl, hc);
+        });
     if (logged) pd_launch_observables(s, 1, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
   };
   auto book = [&](bool last) {
@@ -1056,7 +1333,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
 
 // ---- resident-state snapshot ----------------------------------------------
 // Blob: 16 x int64 header (magic, version, N, nslots, nblocks, precision, lgG,
-// has_ep, time_step, 7 reserved), nblocks x int32 deformed latches, then y, v,
+// has_ep, time_step, tiled, 6 reserved), nblocks x int32 deformed latches, then y, v,
 // f, aux (N double4 each), theta, td (N double each), bond state (nslots
 // bytes) and, for elastoplastic envs, edp (nslots double); all in the env's
 // internal particle / slot order.
@@ -1101,6 +1378,7 @@ int pd_ctx_create(int device, void** out) {
     c->device = device;
     hipError_t r = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (r != hipSuccess) { delete c; PD_HIP(r); }
+    pd_tile_allow_lds(kTileLdsBudget);
     *out = c;
   });
 }
@@ -1113,6 +1391,7 @@ int pd_ctx_destroy(void* ctx) {
   for (auto& kv : c->envs) { free_env_device(*kv.second); delete kv.second; }
   if (c->d_envs) (void)hipFree(c->d_envs);
   if (c->d_set) (void)hipFree(c->d_set);
+  if (c->d_tset) (void)hipFree(c->d_tset);
   if (c->d_slots) (void)hipFree(c->d_slots);
   if (c->d_latch) (void)hipFree(c->d_latch);
   if (c->d_latch_part) (void)hipFree(c->d_latch_part);
@@ -1250,6 +1529,16 @@ int pd_env_schedule(void* ctx, int64_t env, int32_t* lanes_per_particle) {
     Env& e = get_env(C(ctx), env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     *lanes_per_particle = 1 << e.lgG;
+  });
+}
+
+int pd_env_layout(void* ctx, int64_t env, int32_t* lanes_per_particle, int32_t* tiled, int32_t* max_halo) {
+  return run_guard(ctx, [&] {
+    Env& e = get_env(C(ctx), env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    if (lanes_per_particle) *lanes_per_particle = 1 << e.lgG;
+    if (tiled) *tiled = e.tiled ? 1 : 0;
+    if (max_halo) *max_halo = e.tiled ? e.hmax : 0;
   });
 }
 
@@ -1565,6 +1854,7 @@ int pd_env_state_save(void* ctx, int64_t env, void* buf, int64_t bytes) {
     hdr[0] = kStateMagic; hdr[1] = kStateVersion; hdr[2] = e.N; hdr[3] = e.nslots;
     hdr[4] = (int64_t)e.blocks.size(); hdr[5] = e.precision; hdr[6] = e.lgG; hdr[7] = e.has_ep ? 1 : 0;
     hdr[8] = e.time_step;
+    hdr[9] = e.tiled ? 1 : 0;
     p += 16 * 8;
     std::vector<BlockDev> bd(e.blocks.size());
     hipStream_t s = c->stream;
@@ -1597,7 +1887,7 @@ int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes) {
     const int64_t* hdr = reinterpret_cast<const int64_t*>(p);
     if (hdr[0] != kStateMagic || hdr[1] != kStateVersion) throw PdError(PD_E_ARG, "not a state snapshot of this library");
     if (hdr[2] != e.N || hdr[3] != e.nslots || hdr[4] != (int64_t)e.blocks.size() || hdr[5] != e.precision ||
-        hdr[6] != e.lgG || hdr[7] != (e.has_ep ? 1 : 0))
+        hdr[6] != e.lgG || hdr[7] != (e.has_ep ? 1 : 0) || hdr[9] != (e.tiled ? 1 : 0))
       throw PdError(PD_E_ARG, "snapshot was taken from an env of another shape, form or schedule");
     if (bytes < state_bytes(e)) throw PdError(PD_E_ARG, "state buffer too small");
     p += 16 * 8;
@@ -1630,6 +1920,7 @@ int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes) {
     e.maybe_undeformed = any_undeformed;
     e.lists_current = false;
     upload_env_table(c, e);
+    if (e.tiled) pd_launch_pk_sync(s, e.d_chunks, (int)e.chunks.size(), c->d_envs);
     PD_HIP(hipStreamSynchronize(s));
   });
 }
@@ -1694,6 +1985,7 @@ int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64
       if (iters) *iters = it;
       // check! of every BC, then time_step += 1 (minimize.jl:225-229)
       pd_launch_book(s, T, e.d_self, 1, PD_B_STEP);
+      if (e.tiled) pd_launch_pk_sync(s, e.d_chunks, (int)e.chunks.size(), T);
       PD_HIP(hipGetLastError());
       PD_HIP(hipStreamSynchronize(s));
     } catch (...) {

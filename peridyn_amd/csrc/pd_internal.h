@@ -18,6 +18,13 @@
 //   aux    double4[N] per-particle gather record of the fast forms:
 //          (theta, 1/m, td_norm2, C = sqrt(2 psi / td_norm2))
 //   EP     rev uint32 slot of the reverse bond, edp[2] double per slot.
+//   tiles  (FP64 form, optional) a tile is up to 4 chunks of one brick of the
+//          block's internal (brick) order, run by one workgroup that first
+//          stages its halo -- the sorted, unique neighbour ids of its particles
+//          (halo[hoff .. hoff+hn)) -- into LDS.  pk holds, per chunk, per group
+//          of 4 bond steps t and lane L, one uint64 at pk0 + 64 t + L: bond step
+//          4t+u in bits 16u..16u+15 as (intact << 15) | halo-local index.  The
+//          intact bits mirror `state` (the master copy).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -82,7 +89,7 @@ struct ContactDev {
 struct EnvDev {
   int32_t N, nblocks, ncontacts, nbc;
   int32_t precision, cur, edp_cur, has_ep;
-  int32_t lgG, pad0;         // log2(lanes per particle) of the bond layout
+  int32_t lgG, tiled;        // log2(lanes per particle) of the bond layout; tiles built
   double dt;
   int64_t time_step;         // completed steps (env.time_step)
   double4* x;
@@ -107,6 +114,8 @@ struct EnvDev {
   uint8_t* state;
   uint32_t* rev;             // EP only
   double* edp[2];            // EP only
+  int32_t* halo;             // tiled: halo lists of all tiles
+  uint64_t* pk;              // tiled: packed local indices + intact bits
   BlockDev* blocks;
   BcDev* bcs;
   ContactDev* contacts;
@@ -116,7 +125,18 @@ struct Chunk {
   int32_t env, block;
   int32_t first, count;      // env-local particle range of the chunk
   uint32_t slot0;
-  int32_t width;
+  int32_t width;             // bond steps per lane
+  uint32_t pk0;              // tiled: first pk word
+  int32_t pad;
+};
+
+#define PD_TILE_CHUNKS 4
+struct Tile {
+  int32_t env, block;
+  int32_t nch, hn;           // chunks (<= PD_TILE_CHUNKS), halo records
+  uint32_t hoff;             // first halo entry
+  int32_t pad;
+  Chunk ch[PD_TILE_CHUNKS];
 };
 
 // ---- host-side helpers ----------------------------------------------------

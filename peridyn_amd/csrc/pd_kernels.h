@@ -66,3 +66,14 @@ void pd_launch_self_contact_update(hipStream_t s, EnvDev* envs, int slot, const 
 // FP32-compensated form (pd_step32.hip)
 void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags, bool gpow);
+
+// tiled FP64 passes (pd_tile.hip): LDS fields per pass (0 theta, 1 td, 2 force)
+// and kind; hcap = largest halo of the launched tiles
+int pd_tile_fields(int pass, int kind);
+void pd_tile_allow_lds(size_t bytes);
+void pd_launch_tile_theta(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap);
+void pd_launch_tile_td(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap);
+void pd_launch_tile_force(hipStream_t s, int kind, int lgG, const Tile* tiles, int n, EnvDev* envs, int flags,
+                          int hcap);
+// pk intact bits := state for the listed chunks of tiled envs
+void pd_launch_pk_sync(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);

@@ -1,0 +1,434 @@
+// Tiled FP64 passes: one workgroup per tile (up to 4 chunks of one brick of a
+// block's internal order).  The workgroup first stages the fields a pass needs
+// of every particle in the tile's halo -- the sorted, unique neighbour ids of
+// its particles -- into LDS (structure of arrays, one double array per field),
+// then runs the bond loop of the untiled FP64 kernels with the neighbour
+// records read from LDS: one bulk gather per halo particle instead of one
+// gather per bond.  The neighbour of bond step s of a lane comes from the
+// packed stream pk (one uint64 per lane and group of 4 bond steps; layout in
+// pd_internal.h), so a lane reads its next 4 bonds with one load.
+//
+// Every bond is evaluated with the same expressions, in the same per-lane order
+// and with the same G-lane combination as the untiled FP64 kernels (pd_step.hip),
+// so the results are bit-identical to theirs for the same lanes per particle.
+// Tiled envs hold single-type blocks only (the host checks), so type pairs
+// reduce to index 0 and no neighbour type is read.
+#include "pd_wave.h"
+
+namespace {
+
+enum { TP_THETA = 0, TP_TD = 1, TP_FORCE = 2 };
+
+// LDS fields: 0..2 x, 3 volume, 4..6 y, 7 theta, 8 1/m, 9 signed td_norm2, 10 C
+template <int PASS, int KIND>
+__host__ __device__ constexpr int tile_fields() {
+  return PASS == TP_THETA ? 7 : PASS == TP_TD ? 9 : (KIND == PD_BB ? 7 : KIND == PD_OSB ? 9 : 11);
+}
+
+struct TileWave {
+  Chunk ch;
+  int lane, q, g, p;
+  bool active;
+};
+
+// Stages the halo of tile T into LDS; every thread of the workgroup takes part.
+template <int NF>
+__device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap) {
+  const double4* __restrict__ x = E.x;
+  const double4* __restrict__ y = E.y[E.cur];
+  const int32_t* __restrict__ halo = E.halo;
+  for (int i = threadIdx.x; i < T.hn; i += 256) {
+    const uint32_t id = (uint32_t)gld(halo, T.hoff + (uint32_t)i);
+    const double4 a = gld4(x, id);
+    const double4 b = gld4(y, id);
+    sh[i] = a.x;
+    sh[hcap + i] = a.y;
+    sh[2 * hcap + i] = a.z;
+    sh[3 * hcap + i] = a.w;
+    sh[4 * hcap + i] = b.x;
+    sh[5 * hcap + i] = b.y;
+    sh[6 * hcap + i] = b.z;
+    if (NF > 7) {
+      const double4 c = gld4(E.aux, id);
+      sh[7 * hcap + i] = c.x;
+      sh[8 * hcap + i] = c.y;
+      if (NF > 9) {
+        sh[9 * hcap + i] = c.z;
+        sh[10 * hcap + i] = c.w;
+      }
+    }
+  }
+}
+
+template <int LG>
+__device__ __forceinline__ bool tile_wave(const Tile& T, TileWave& w) {
+  constexpr int C = 64 >> LG;
+  const int wv = pd_wave_id();
+  if (wv >= T.nch) return false;
+  w.ch = T.ch[wv];
+  w.lane = pd_lane();
+  w.q = w.lane & (C - 1);
+  w.g = w.lane >> (6 - LG);
+  w.p = w.ch.first + w.q;
+  w.active = w.q < w.ch.count;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// dilatation theta (src/peri.jl:155-172), OSB and EP blocks
+template <int LG>
+__global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ tiles, int ntiles,
+                                                    EnvDev* __restrict__ envs, int hcap) {
+  extern __shared__ double sh[];
+  constexpr int C = 64 >> LG;
+  const int t = pd_xcd_block(gridDim.x);
+  if (t >= ntiles) return;
+  const Tile& T = tiles[t];
+  EnvDev& E = envs[T.env];
+  const BlockDev& B = E.blocks[T.block];
+  if (B.kind == PD_BB || !B.deformed) return;
+  stage_halo<7>(E, T, sh, hcap);
+  __syncthreads();
+  TileWave w;
+  if (!tile_wave<LG>(T, w)) return;
+  const uint64_t* __restrict__ pk = E.pk;
+  const int p = w.active ? w.p : w.ch.first;
+  const double4 xp = gld4(E.x, p), yp = gld4(E.y[E.cur], p);
+  const int nt4 = (w.ch.width + 3) >> 2;
+  const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
+  double acc = 0.0;
+  uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+  for (int t4 = 0; t4 < nt4; ++t4) {
+    const uint64_t cw = nxt;
+    if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+      if (!(e & 0x8000u)) continue;
+      const int j = (int)(e & 0x7fffu);
+      const double X1 = sh[j] - xp.x, X2 = sh[hcap + j] - xp.y, X3 = sh[2 * hcap + j] - xp.z;
+      const double Y1 = sh[4 * hcap + j] - yp.x, Y2 = sh[5 * hcap + j] - yp.y, Y3 = sh[6 * hcap + j] - yp.z;
+      const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+      const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+      const double ext = yy2 * pd_rsqrt(yy2) - xx2 * pd_rsqrt(xx2);
+      acc += ext * sh[3 * hcap + j];
+    }
+  }
+  acc = group_sum(acc, C);
+  if (w.active && w.g == 0) {
+    const double th = (3.0 * E.aux[p].y) * acc;
+    E.theta[p] = th;
+    E.aux[p].x = th;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// td_norm2 (elasto_plastic.jl:168-172,282-295; src/peri.jl:411-445), EP
+// blocks; breaks bonds with s >= s0 (state and the pk intact bit)
+template <int LG>
+__global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles, int ntiles,
+                                                 EnvDev* __restrict__ envs, int hcap) {
+  extern __shared__ double sh[];
+  constexpr int C = 64 >> LG;
+  const int t = pd_xcd_block(gridDim.x);
+  if (t >= ntiles) return;
+  const Tile& T = tiles[t];
+  EnvDev& E = envs[T.env];
+  const BlockDev& B = E.blocks[T.block];
+  if (B.kind != PD_EP || !B.deformed) return;
+  stage_halo<9>(E, T, sh, hcap);
+  __syncthreads();
+  TileWave w;
+  if (!tile_wave<LG>(T, w)) return;
+  const uint64_t* __restrict__ pk = E.pk;
+  uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
+  uint8_t* __restrict__ st = E.state;
+  const double* __restrict__ edp = E.edp[E.edp_cur];
+  double* __restrict__ edp_o = E.edp[E.edp_cur ^ 1];
+  const int p = w.active ? w.p : w.ch.first;
+  const double4 xp = gld4(E.x, p), yp = gld4(E.y[E.cur], p);
+  const double4 ap = E.aux[p];  // (theta, 1/m, +-td_norm2 of the last step, C)
+  const bool yielded = pd_has_yielded(ap);
+  const double s0 = B.s0[0];
+  const double g15 = 15.0 * B.G[0];
+  const int nt4 = (w.ch.width + 3) >> 2;
+  const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
+  const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
+  double acc = 0.0;
+  uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+  for (int t4 = 0; t4 < nt4; ++t4) {
+    const uint64_t cw = nxt;
+    if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+    double eo[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool on = (cw >> (16 * u + 15)) & 1ull;
+      eo[u] = (on && yielded) ? gld(edp, base + 64u * (uint32_t)(4 * t4 + u)) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+      if (!(e & 0x8000u)) continue;
+      const int j = (int)(e & 0x7fffu);
+      const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
+      const double X1 = sh[j] - xp.x, X2 = sh[hcap + j] - xp.y, X3 = sh[2 * hcap + j] - xp.z;
+      const double Y1 = sh[4 * hcap + j] - yp.x, Y2 = sh[5 * hcap + j] - yp.y, Y3 = sh[6 * hcap + j] - yp.z;
+      const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+      const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+      const double inv_x = pd_rsqrt(xx2);
+      const double xij = xx2 * inv_x;
+      const double ext = yy2 * pd_rsqrt(yy2) - xij;
+      if (ext * inv_x < s0) {
+        const double th_j = sh[7 * hcap + j], rm_j = sh[8 * hcap + j];
+        const double sm = ap.y + rm_j;
+        const double twm = ap.x * ap.y + th_j * rm_j;
+        const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
+        const double td = g15 * (ed - eo[u]) * (inv_x * sm);
+        acc += td * td * sh[3 * hcap + j];
+      } else {
+        gst(st, slot, (uint8_t)0);
+        gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
+        if (yielded) gst(edp_o, slot, eo[u]);  // else 0 in both buffers already
+      }
+    }
+  }
+  acc = group_sum(acc, C);
+  if (w.active && w.g == 0) {
+    const double Cp = __builtin_sqrt((2.0 * B.psi[0]) / acc);
+    const bool flag = yielded || !(Cp > 1.0);
+    E.td[p] = acc;
+    E.aux[p].z = flag ? -acc : acc;
+    E.aux[p].w = Cp;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused force density + velocity-Verlet (material part of
+// general_material.jl:296-369 with the plugins' t_ij, then contacts and the
+// step epilogue of pd_wave.h)
+template <int KIND, int LG>
+__global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
+                                                    EnvDev* __restrict__ envs, int flags, int hcap) {
+  extern __shared__ double sh[];
+  constexpr int C = 64 >> LG;
+  const int t = pd_xcd_block(gridDim.x);
+  if (t >= ntiles) return;
+  const Tile& T = tiles[t];
+  EnvDev& E = envs[T.env];
+  const BlockDev& B = E.blocks[T.block];
+  const bool deformed = B.deformed != 0;  // uniform over the workgroup
+  if (deformed) stage_halo<tile_fields<TP_FORCE, KIND>()>(E, T, sh, hcap);
+  __syncthreads();
+  TileWave w;
+  if (!tile_wave<LG>(T, w)) return;
+  const int p = w.active ? w.p : w.ch.first;
+  const double4* __restrict__ y = E.y[E.cur];
+  const double4 xp = gld4(E.x, p), yp = gld4(y, p);
+  double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (deformed) {
+    const uint64_t* __restrict__ pk = E.pk;
+    uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
+    uint8_t* __restrict__ st = E.state;
+    const double inv_den = 1.0 / B.density[0];
+    double4 ap = make_double4(0.0, 0.0, 0.0, 0.0);
+    const double* __restrict__ edp_in = nullptr;
+    double* __restrict__ edp_out = nullptr;
+    if (KIND != PD_BB) ap = E.aux[p];
+    bool yielded_p = false;
+    if (KIND == PD_EP) {
+      edp_in = E.edp[E.edp_cur];
+      edp_out = E.edp[E.edp_cur ^ 1];
+      yielded_p = pd_has_yielded(ap);
+    }
+    const double c11 = B.c11;
+    const double s0_pp = B.s0[0];
+    const double K0 = B.K[0], G0 = B.G[0];
+    const int nt4 = (w.ch.width + 3) >> 2;
+    const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
+    const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
+    uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+    for (int t4 = 0; t4 < nt4; ++t4) {
+      const uint64_t cw = nxt;
+      if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+      double eo[4], er[4];
+      if (KIND == PD_EP) {
+        // own plastic extensions (0 until p yields) and the reverse bonds' (0
+        // until the neighbour yields), all loads of the group issued first
+        uint32_t rv[4];
+        bool need[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+          const bool on = (e & 0x8000u) != 0;
+          const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
+          eo[u] = (on && yielded_p) ? gld(edp_in, slot) : 0.0;
+          need[u] = on && __builtin_signbit(sh[9 * hcap + (int)(e & 0x7fffu)]);
+          rv[u] = need[u] ? gld(E.rev, slot) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) er[u] = need[u] ? gld(edp_in, rv[u]) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+        if (!(e & 0x8000u)) continue;
+        const int j = (int)(e & 0x7fffu);
+        const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
+        const double X1 = sh[j] - xp.x, X2 = sh[hcap + j] - xp.y, X3 = sh[2 * hcap + j] - xp.z;
+        const double Y1 = sh[4 * hcap + j] - yp.x, Y2 = sh[5 * hcap + j] - yp.y, Y3 = sh[6 * hcap + j] - yp.z;
+        const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+        const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+        const double inv_x = pd_rsqrt(xx2);
+        const double inv_y = pd_rsqrt(yy2);
+        const double xij = xx2 * inv_x;
+        const double yij = yy2 * inv_y;
+        const double ext = yij - xij;
+        if (ext * inv_x < s0_pp) {
+          double t;
+          if (KIND == PD_BB) {
+            t = ext * inv_x * c11;
+          } else {
+            const double th_j = sh[7 * hcap + j], rm_j = sh[8 * hcap + j];
+            const double sm = ap.y + rm_j;
+            const double twm = ap.x * ap.y + th_j * rm_j;
+            if (KIND == PD_OSB) {
+              t = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
+            } else {
+              const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
+              const double g15 = 15.0 * G0 * (inv_x * sm);
+              const double Cp = ap.w, Cj = sh[10 * hcap + j];
+              const double ede = ed - eo[u];
+              double td_own = g15 * ede;
+              double e_new = eo[u];
+              if (!(Cp > 1.0)) {
+                td_own *= Cp;
+                e_new = eo[u] + ede * (1.0 - Cp);
+              }
+              if (yielded_p) gst(edp_out, slot, e_new);  // else e_new = 0 = stored
+              double td_rev = g15 * (ed - er[u]);
+              if (!(Cj > 1.0)) td_rev *= Cj;
+              t = 3.0 * K0 * twm + 0.5 * (td_own + td_rev);
+            }
+          }
+          const double xx = t * inv_y * (sh[3 * hcap + j] * inv_den);
+          a1 += xx * Y1;
+          a2 += xx * Y2;
+          a3 += xx * Y3;
+        } else {
+          gst(st, slot, (uint8_t)0);
+          gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
+          if (KIND == PD_EP && yielded_p) gst(edp_out, slot, eo[u]);
+        }
+      }
+    }
+    a1 = group_sum(a1, C);
+    a2 = group_sum(a2, C);
+    a3 = group_sum(a3, C);
+  }
+  if (!w.active || w.g != 0) return;
+  add_contacts<false>(E, p, yp, y, a1, a2, a3);
+  finish_particle(E, p, xp, yp, a1, a2, a3, flags);
+}
+
+// pk intact bits := state (after bonds were broken by the untiled passes or the
+// state was loaded); one wavefront per chunk
+__global__ __launch_bounds__(256) void k_pk_sync(const Chunk* __restrict__ chunks, int nchunks,
+                                                 EnvDev* __restrict__ envs) {
+  const int cid = blockIdx.x * PD_WAVES_PER_BLOCK + pd_wave_id();
+  if (cid >= nchunks) return;
+  const Chunk ch = chunks[cid];
+  EnvDev& E = envs[ch.env];
+  const int lane = pd_lane();
+  const int nt4 = (ch.width + 3) >> 2;
+  for (int t4 = 0; t4 < nt4; ++t4) {
+    const uint32_t wi = ch.pk0 + 64u * (uint32_t)t4 + (uint32_t)lane;
+    uint64_t wd = gld(E.pk, wi);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int s = 4 * t4 + u;
+      const uint64_t bit = 1ull << (16 * u + 15);
+      const bool on = s < ch.width && gld(E.state, ch.slot0 + 64u * (uint32_t)s + (uint32_t)lane);
+      wd = on ? (wd | bit) : (wd & ~bit);
+    }
+    gst(E.pk, wi, wd);
+  }
+}
+
+template <int LG>
+void launch_theta_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  const size_t lds = sizeof(double) * tile_fields<TP_THETA, 0>() * (size_t)hcap;
+  hipLaunchKernelGGL(k_tile_theta<LG>, dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+}
+template <int LG>
+void launch_td_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  const size_t lds = sizeof(double) * tile_fields<TP_TD, 0>() * (size_t)hcap;
+  hipLaunchKernelGGL(k_tile_td<LG>, dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+}
+template <int KIND, int LG>
+void launch_force_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
+  const size_t lds = sizeof(double) * tile_fields<TP_FORCE, KIND>() * (size_t)hcap;
+  hipLaunchKernelGGL((k_tile_force<KIND, LG>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
+}
+
+// dynamic LDS beyond the default 64 KiB per workgroup must be enabled per kernel
+template <int LG>
+void allow_lds_lg(size_t bytes) {
+  (void)hipFuncSetAttribute((const void*)k_tile_theta<LG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_tile_td<LG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_tile_force<PD_BB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_tile_force<PD_OSB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_tile_force<PD_EP, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
+}
+
+}  // namespace
+
+int pd_tile_fields(int pass, int kind) {
+  if (pass == 0) return tile_fields<TP_THETA, 0>();
+  if (pass == 1) return tile_fields<TP_TD, 0>();
+  return kind == PD_BB ? tile_fields<TP_FORCE, PD_BB>()
+                       : kind == PD_OSB ? tile_fields<TP_FORCE, PD_OSB>() : tile_fields<TP_FORCE, PD_EP>();
+}
+
+void pd_tile_allow_lds(size_t bytes) {
+  allow_lds_lg<1>(bytes);
+  allow_lds_lg<2>(bytes);
+  allow_lds_lg<3>(bytes);
+}
+
+void pd_launch_tile_theta(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  if (n <= 0) return;
+  if (lgG == 1) launch_theta_lg<1>(s, tiles, n, envs, hcap);
+  else if (lgG == 2) launch_theta_lg<2>(s, tiles, n, envs, hcap);
+  else launch_theta_lg<3>(s, tiles, n, envs, hcap);
+}
+
+void pd_launch_tile_td(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  if (n <= 0) return;
+  if (lgG == 1) launch_td_lg<1>(s, tiles, n, envs, hcap);
+  else if (lgG == 2) launch_td_lg<2>(s, tiles, n, envs, hcap);
+  else launch_td_lg<3>(s, tiles, n, envs, hcap);
+}
+
+void pd_launch_tile_force(hipStream_t s, int kind, int lgG, const Tile* tiles, int n, EnvDev* envs, int flags,
+                          int hcap) {
+  if (n <= 0) return;
+#define PD_TF(KIND)                                                              \
+  do {                                                                           \
+    if (lgG == 1) launch_force_lg<KIND, 1>(s, tiles, n, envs, flags, hcap);      \
+    else if (lgG == 2) launch_force_lg<KIND, 2>(s, tiles, n, envs, flags, hcap); \
+    else launch_force_lg<KIND, 3>(s, tiles, n, envs, flags, hcap);               \
+  } while (0)
+  if (kind == PD_BB) PD_TF(PD_BB);
+  else if (kind == PD_OSB) PD_TF(PD_OSB);
+  else PD_TF(PD_EP);
+#undef PD_TF
+}
+
+void pd_launch_pk_sync(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pk_sync, dim3((n + PD_WAVES_PER_BLOCK - 1) / PD_WAVES_PER_BLOCK), dim3(256), 0, s, chunks,
+                     n, envs);
+}

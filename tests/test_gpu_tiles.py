@@ -1,0 +1,190 @@
+"""Tiled FP64 passes (halos staged in LDS) against the untiled FP64 kernels and
+the oracle.
+
+The tiled kernels evaluate every bond with the untiled kernels' expressions in
+the same per-lane order, so for the same lanes per particle the results are
+bit-identical (== on y, v, f, bond states, damage).  The internal particle
+order differs (bricks vs. the input order), which the boundary translates: the
+comparisons are in reference order.  Against the oracle the FP64 bar holds
+(relative L2 <= 1e-10).
+"""
+import contextlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from peridyn_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pd():
+    from peridyn_amd import core
+    return core
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+@contextlib.contextmanager
+def tiles(on: bool):
+    old = os.environ.get("PERIDYN_TILES")
+    os.environ["PERIDYN_TILES"] = "1" if on else "0"
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["PERIDYN_TILES"]
+        else:
+            os.environ["PERIDYN_TILES"] = old
+
+
+def make(pd, spec, tiled, lanes):
+    with tiles(tiled):
+        env = pd.Env.from_spec(spec, precision="fp64", lanes_per_particle=lanes)
+    lay = env.layout()
+    assert lay["tiled"] == tiled, lay
+    assert lay["lanes_per_particle"] == lanes
+    return env
+
+
+def same_state(a, b):
+    ya, va, fa = a.download()
+    yb, vb, fb = b.download()
+    assert np.array_equal(ya, yb)
+    assert np.array_equal(va, vb)
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(a.damage(), b.damage())
+    for blk in range(len(a.spec["blocks"])):
+        fa_, ia = a.family(blk)
+        fb_, ib = b.family(blk)
+        assert np.array_equal(fa_, fb_)
+        assert np.array_equal(ia, ib)
+
+
+def ep_tensile(n=9):
+    spec = W.tensile_bb(n=n, dt=1e-4, kind="ep", pull_per_step=0.002)
+    spec["blocks"][0]["yield_stress"] = 20.0
+    return spec
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_tiled_equals_untiled(pd, kind, lanes):
+    say("tiled vs untiled", kind, lanes)
+    spec = ep_tensile() if kind == "ep" else W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.002)
+    a = make(pd, spec, True, lanes)
+    b = make(pd, spec, False, lanes)
+    a.run(40, 1)
+    b.run(40, 1)
+    same_state(a, b)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+def test_tiled_within_fp64_bar_of_oracle(pd, orc, kind):
+    say("tiled vs oracle", kind)
+    spec = ep_tensile() if kind == "ep" else W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.002)
+    o = orc.OracleEnv(spec)
+    o.run(40, 1)
+    g = make(pd, spec, True, 4)
+    g.run(40, 1)
+    y, v, f = g.download()
+    x = spec["blocks"][0]["x"]
+    assert rel_l2(f, o.f) <= 1e-10
+    assert rel_l2(y - x, o.y - x) <= 1e-10
+    g.close()
+
+
+def test_tiled_bond_breaking_equals_untiled(pd):
+    """Fast pull: bonds break during the run (the tiled kernels clear the packed
+    intact bits together with the bond states)."""
+    say("tiled breaking")
+    spec = W.tensile_bb(n=10, dt=1e-3, kind="bb", pull_per_step=0.05)
+    a = make(pd, spec, True, 4)
+    b = make(pd, spec, False, 4)
+    a.run(60, 1)
+    b.run(60, 1)
+    assert a.damage().max() > 0.0
+    same_state(a, b)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_tiled_contact_plates_equal_untiled(pd, lanes):
+    say("tiled contact plates", lanes)
+    spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=2)
+    a = make(pd, spec, True, lanes)
+    b = make(pd, spec, False, lanes)
+    a.run(30, 5)
+    b.run(30, 5)
+    same_state(a, b)
+    assert np.abs(a.download()[2]).max() > 0
+    a.close()
+    b.close()
+
+
+def test_tiled_seams_then_run_equal_untiled(pd):
+    """update_acc! through the untiled passes on a tiled env (bonds break
+    there), then tiled steps: the packed intact bits follow the bond states."""
+    say("tiled seams")
+    spec = W.tensile_bb(n=10, dt=1e-3, kind="bb", pull_per_step=0.05)
+    rng = np.random.default_rng(3)
+    y0 = spec["blocks"][0]["x"] * 1.02 + rng.normal(scale=2e-2, size=spec["blocks"][0]["x"].shape)
+    envs = []
+    for tiled in (True, False):
+        e = make(pd, spec, tiled, 4)
+        e.update_acc(np.ascontiguousarray(y0))
+        e.run(20, 1)
+        envs.append(e)
+    assert envs[0].damage().max() > 0.0
+    same_state(envs[0], envs[1])
+    for e in envs:
+        e.close()
+
+
+def test_tiled_checkpoint_resume_bit_identical(pd):
+    say("tiled checkpoint")
+    spec = ep_tensile()
+    a = make(pd, spec, True, 4)
+    a.run(30, 1)
+    blob = a.checkpoint()
+    a.run(20, 1)
+    b = make(pd, spec, True, 4)
+    b.restore(blob)
+    b.run(20, 1)
+    same_state(a, b)
+    # a snapshot of the untiled layout does not load into a tiled env
+    c = make(pd, spec, False, 4)
+    from peridyn_amd._lib import PeriDynError
+    with pytest.raises(PeriDynError):
+        b.restore(c.checkpoint())
+    for e in (a, b, c):
+        e.close()
+
+
+def test_tiled_and_untiled_batched_equal_alone(pd):
+    say("tiled batch")
+    s1 = ep_tensile()
+    s2 = W.tensile_bb(n=8, dt=1e-4, kind="osb", pull_per_step=0.002)
+    alone = [make(pd, s1, True, 4), make(pd, s2, False, 2)]
+    for e in alone:
+        e.run(25, 1)
+    mix = [make(pd, s1, True, 4), make(pd, s2, False, 2)]
+    pd.run_batch(mix, 25, 1)
+    for a, b in zip(alone, mix):
+        same_state(a, b)
+    for e in alone + mix:
+        e.close()
