@@ -91,8 +91,9 @@ struct Ctx {
   int off_theta_u = 0, off_td_u = 0, off_force_u[3] = {0, 0, 0};
   Tile* d_tset = nullptr;
   size_t tset_cap = 0;
-  int nt_theta[4] = {}, nt_td[4] = {}, nt_force[3][4] = {};
-  int ot_theta[4] = {}, ot_td[4] = {}, ot_force[3][4] = {};
+  // [form class: 0 FP64 kernels, 1 FP32C kernels][lgG]
+  int nt_theta[2][4] = {}, nt_td[4] = {}, nt_force[3][2][4] = {};
+  int ot_theta[2][4] = {}, ot_td[4] = {}, ot_force[3][2][4] = {};
   int set_hcap = 1;
   bool set_tiled = false;
   Chunk* d_set = nullptr;
@@ -503,10 +504,13 @@ void build_tiles(Ctx* c, Env& e) {
     hmax = std::max(hmax, h.size());
     total += h.size();
   }
-  int nf = 7;
-  for (auto& b : e.blocks) nf = std::max(nf, pd_tile_fields(2, b.d.kind));
-  if (e.has_ep) nf = std::max(nf, pd_tile_fields(1, PD_EP));
-  if (hmax > 0x7fff || sizeof(double) * nf * hmax > kTileLdsBudget ||
+  size_t rb = 0;  // LDS bytes per halo record of the env's largest pass
+  for (auto& b : e.blocks) {
+    rb = std::max<size_t>(rb, pd_tile_record_bytes(2, b.d.kind, e.precision));
+    if (b.d.kind != PD_BB) rb = std::max<size_t>(rb, pd_tile_record_bytes(0, b.d.kind, e.precision));
+    if (b.d.kind == PD_EP) rb = std::max<size_t>(rb, pd_tile_record_bytes(1, PD_EP, e.precision));
+  }
+  if (hmax > 0x7fff || rb * hmax > kTileLdsBudget ||
       total * sizeof(int32_t) >= PD_MAX_ARRAY_BYTES) {
     e.tiles.clear();
     e.tiled = false;
@@ -552,7 +556,7 @@ void commit_env(Ctx* c, Env& e) {
   // integer contact exponents (the tiled force kernel carries no general pow);
   // 4 lanes per particle unless a schedule was set (measured best on the EP
   // plates, equal to 2 on the BB disk)
-  bool tileable = e.precision == PD_PREC_FP64 && tiles_enabled();
+  bool tileable = (e.precision == PD_PREC_FP64 || e.precision == PD_PREC_FP32C) && tiles_enabled();
   for (auto& b : e.blocks) tileable = tileable && b.d.ntype == 1;
   for (auto& hc : e.contacts) tileable = tileable && hc.d.exponent_is_int;
   if (e.precision == PD_PREC_EXACT) {
@@ -919,7 +923,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   // chunk lists of every env (all passes), chunk lists of the untiled envs for
   // stepping, tile lists of the tiled envs per pass and lanes per particle
   std::vector<Chunk> all, th, td, fk[3], thu, tdu, fku[3];
-  std::vector<Tile> tth[4], ttd[4], tfk[3][4];
+  std::vector<Tile> tth[2][4], ttd[4], tfk[3][2][4];
   std::vector<int32_t> slots;
   std::vector<int2> ranges;
   bool has_ep = false, has_th = false, gpow = false, tiled = false;
@@ -947,8 +951,9 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
       hcap = std::max(hcap, (int)e.hmax);
       for (const Tile& t : e.tiles) {
         const int kind = e.blocks[t.block].d.kind;
-        tfk[kind][e.lgG].push_back(t);
-        if (kind != PD_BB) tth[e.lgG].push_back(t);
+        const int pc = (e.precision == PD_PREC_FP32C && kind != PD_EP) ? 1 : 0;  // EP runs FP64
+        tfk[kind][pc][e.lgG].push_back(t);
+        if (kind != PD_BB) tth[pc][e.lgG].push_back(t);
         if (kind == PD_EP) ttd[e.lgG].push_back(t);
       }
     }
@@ -975,14 +980,16 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   PD_HIP(hipMemcpyAsync(c->d_set, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
   std::vector<Tile> tcat;
   for (int lg = 1; lg <= 3; ++lg) {
-    c->ot_theta[lg] = (int)tcat.size(); c->nt_theta[lg] = (int)tth[lg].size();
-    tcat.insert(tcat.end(), tth[lg].begin(), tth[lg].end());
+    for (int pc = 0; pc < 2; ++pc) {
+      c->ot_theta[pc][lg] = (int)tcat.size(); c->nt_theta[pc][lg] = (int)tth[pc][lg].size();
+      tcat.insert(tcat.end(), tth[pc][lg].begin(), tth[pc][lg].end());
+      for (int k = 0; k < 3; ++k) {
+        c->ot_force[k][pc][lg] = (int)tcat.size(); c->nt_force[k][pc][lg] = (int)tfk[k][pc][lg].size();
+        tcat.insert(tcat.end(), tfk[k][pc][lg].begin(), tfk[k][pc][lg].end());
+      }
+    }
     c->ot_td[lg] = (int)tcat.size(); c->nt_td[lg] = (int)ttd[lg].size();
     tcat.insert(tcat.end(), ttd[lg].begin(), ttd[lg].end());
-    for (int k = 0; k < 3; ++k) {
-      c->ot_force[k][lg] = (int)tcat.size(); c->nt_force[k][lg] = (int)tfk[k][lg].size();
-      tcat.insert(tcat.end(), tfk[k][lg].begin(), tfk[k][lg].end());
-    }
   }
   if (!tcat.empty()) {
     ensure_cap(c->d_tset, c->tset_cap, tcat.size());
@@ -1240,8 +1247,11 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
     if (c->set_has_theta)
       timed(c, PD_PROF_THETA, [&] {
         if (c->n_theta_u) pd_launch_theta(s, prec, d + c->off_theta_u, c->n_theta_u, T);
-        for (int lg = 1; lg <= 3; ++lg)
-          if (c->nt_theta[lg]) pd_launch_tile_theta(s, lg, dt + c->ot_theta[lg], c->nt_theta[lg], T, hc);
+        for (int lg = 1; lg <= 3; ++lg) {
+          if (c->nt_theta[0][lg]) pd_launch_tile_theta(s, lg, dt + c->ot_theta[0][lg], c->nt_theta[0][lg], T, hc);
+          if (c->nt_theta[1][lg])
+            pd_launch_tile_theta32(s, lg, dt + c->ot_theta[1][lg], c->nt_theta[1][lg], T, hc);
+        }
       });
     if (c->set_has_ep)
       timed(c, PD_PROF_TD, [&] {
@@ -1257,9 +1267,12 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
         timed(c, PD_PROF_FORCE_BB + k, [&] {
           if (c->n_force_u[k])
             pd_launch_force(s, k, prec, d + c->off_force_u[k], c->n_force_u[k], T, fl, c->set_gpow);
-          for (int lg = 1; lg <= 3; ++lg)
-            if (c->nt_force[k][lg])
-              pd_launch_tile_force(s, k, lg, dt + c->ot_force[k][lg], c->nt_force[k][lg], T, fl, hc);
+          for (int lg = 1; lg <= 3; ++lg) {
+            if (c->nt_force[k][0][lg])
+              pd_launch_tile_force(s, k, lg, dt + c->ot_force[k][0][lg], c->nt_force[k][0][lg], T, fl, hc);
+            if (c->nt_force[k][1][lg])
+              pd_launch_tile_force32(s, k, lg, dt + c->ot_force[k][1][lg], c->nt_force[k][1][lg], T, fl, hc);
+          }
         });
     if (logged) pd_launch_observables(s, 1, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
   };

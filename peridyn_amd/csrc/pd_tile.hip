@@ -13,7 +13,7 @@
 // so the results are bit-identical to theirs for the same lanes per particle.
 // Tiled envs hold single-type blocks only (the host checks), so type pairs
 // reduce to index 0 and no neighbour type is read.
-#include "pd_wave.h"
+#include "pd_fp32c.h"
 
 namespace {
 
@@ -330,6 +330,160 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
   finish_particle(E, p, xp, yp, a1, a2, a3, flags);
 }
 
+// ---------------------------------------------------------------------------
+// FP32-compensated form (BB / OSB blocks of PD_PREC_FP32C envs; the bond
+// arithmetic of pd_step32.hip).  LDS: y (3 doubles) first, then x32 (4 floats:
+// offsets from the block corner, volume) and for OSB aux32 (theta, 1/m).
+template <int NFF>
+__device__ __forceinline__ void stage_halo32(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap) {
+  const float4* __restrict__ x = E.x32;
+  const double4* __restrict__ y = E.y[E.cur];
+  const int32_t* __restrict__ halo = E.halo;
+  float* __restrict__ fs = (float*)(sh + 3 * hcap);
+  for (int i = threadIdx.x; i < T.hn; i += 256) {
+    const uint32_t id = (uint32_t)gld(halo, T.hoff + (uint32_t)i);
+    const float4 a = gld4(x, id);
+    const double4 b = gld4(y, id);
+    sh[i] = b.x;
+    sh[hcap + i] = b.y;
+    sh[2 * hcap + i] = b.z;
+    fs[i] = a.x;
+    fs[hcap + i] = a.y;
+    fs[2 * hcap + i] = a.z;
+    fs[3 * hcap + i] = a.w;
+    if (NFF > 4) {
+      const float4 c = gld4(E.aux32, id);
+      fs[4 * hcap + i] = c.x;
+      fs[5 * hcap + i] = c.y;
+    }
+  }
+}
+
+// dilatation theta (src/peri.jl:155-172) of OSB blocks
+template <int LG>
+__global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ tiles, int ntiles,
+                                                      EnvDev* __restrict__ envs, int hcap) {
+  extern __shared__ double sh[];
+  constexpr int C = 64 >> LG;
+  const int t = pd_xcd_block(gridDim.x);
+  if (t >= ntiles) return;
+  const Tile& T = tiles[t];
+  EnvDev& E = envs[T.env];
+  const BlockDev& B = E.blocks[T.block];
+  if (B.kind != PD_OSB || !B.deformed) return;
+  stage_halo32<4>(E, T, sh, hcap);
+  __syncthreads();
+  TileWave w;
+  if (!tile_wave<LG>(T, w)) return;
+  const float* __restrict__ fs = (const float*)(sh + 3 * hcap);
+  const uint64_t* __restrict__ pk = E.pk;
+  const int p = w.active ? w.p : w.ch.first;
+  const float4 xp = gld4(E.x32, p);
+  const double4 yp = gld4(E.y[E.cur], p);
+  const int nt4 = (w.ch.width + 3) >> 2;
+  const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
+  float s = 0.0f, c = 0.0f;
+  uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+  for (int t4 = 0; t4 < nt4; ++t4) {
+    const uint64_t cw = nxt;
+    if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+      if (!(e & 0x8000u)) continue;
+      const int j = (int)(e & 0x7fffu);
+      const float4 xj = make_float4(fs[j], fs[hcap + j], fs[2 * hcap + j], fs[3 * hcap + j]);
+      const double4 yj = make_double4(sh[j], sh[hcap + j], sh[2 * hcap + j], 0.0);
+      G32 g;
+      geom32(xp, yp, xj, yj, g);
+      kahan(s, c, g.ext * xj.w);
+    }
+  }
+  const double tot = comp_total(s, c, C);
+  if (w.active && w.g == 0) {
+    const float th = (float)((3.0 * (double)E.aux32[p].y) * tot);
+    E.aux32[p].x = th;
+    E.theta[p] = (double)th;
+  }
+}
+
+// fused force density (BB / OSB) + FP64 velocity-Verlet
+template <int KIND, int LG>
+__global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
+                                                      EnvDev* __restrict__ envs, int flags, int hcap) {
+  extern __shared__ double sh[];
+  constexpr int C = 64 >> LG;
+  const int t = pd_xcd_block(gridDim.x);
+  if (t >= ntiles) return;
+  const Tile& T = tiles[t];
+  EnvDev& E = envs[T.env];
+  const BlockDev& B = E.blocks[T.block];
+  const bool deformed = B.deformed != 0;  // uniform over the workgroup
+  if (deformed) stage_halo32<KIND == PD_OSB ? 6 : 4>(E, T, sh, hcap);
+  __syncthreads();
+  TileWave w;
+  if (!tile_wave<LG>(T, w)) return;
+  const int p = w.active ? w.p : w.ch.first;
+  const double4* __restrict__ y = E.y[E.cur];
+  const double4 yp = gld4(y, p);
+  double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (deformed) {
+    const float* __restrict__ fs = (const float*)(sh + 3 * hcap);
+    const uint64_t* __restrict__ pk = E.pk;
+    uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
+    uint8_t* __restrict__ st = E.state;
+    const float4 xp = gld4(E.x32, p);
+    const float inv_den = (float)(1.0 / B.density[0]);
+    float4 ap = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (KIND == PD_OSB) ap = E.aux32[p];
+    const float c11 = (float)B.c11;
+    const double s0 = B.s0[0];
+    const float Kf = (float)B.K[0], Gf = (float)B.G[0];
+    float s1 = 0.f, s2 = 0.f, s3 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+    const int nt4 = (w.ch.width + 3) >> 2;
+    const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
+    const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
+    uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+    for (int t4 = 0; t4 < nt4; ++t4) {
+      const uint64_t cw = nxt;
+      if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+        if (!(e & 0x8000u)) continue;
+        const int j = (int)(e & 0x7fffu);
+        const float4 xj = make_float4(fs[j], fs[hcap + j], fs[2 * hcap + j], fs[3 * hcap + j]);
+        const double4 yj = make_double4(sh[j], sh[hcap + j], sh[2 * hcap + j], 0.0);
+        G32 g;
+        geom32(xp, yp, xj, yj, g);
+        if ((double)g.s < s0) {
+          float tt;
+          if (KIND == PD_BB) {
+            tt = g.s * c11;
+          } else {
+            const float sm = ap.y + fs[5 * hcap + j];
+            const float twm = ap.x * ap.y + fs[4 * hcap + j] * fs[5 * hcap + j];
+            tt = (3.0f * Kf - 5.0f * Gf) * twm + 15.0f * Gf * g.ext * (g.inv_x * sm);
+          }
+          const float xx = tt * g.inv_y * (xj.w * inv_den);
+          kahan(s1, c1, xx * g.Y1);
+          kahan(s2, c2, xx * g.Y2);
+          kahan(s3, c3, xx * g.Y3);
+        } else {
+          gst(st, base + 64u * (uint32_t)(4 * t4 + u), (uint8_t)0);
+          gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
+        }
+      }
+    }
+    a1 = comp_total(s1, c1, C);
+    a2 = comp_total(s2, c2, C);
+    a3 = comp_total(s3, c3, C);
+  }
+  if (!w.active || w.g != 0) return;
+  add_contacts<false>(E, p, yp, y, a1, a2, a3);
+  finish_particle(E, p, gld4(E.x, p), yp, a1, a2, a3, flags);
+}
+
 // pk intact bits := state (after bonds were broken by the untiled passes or the
 // state was loaded); one wavefront per chunk
 __global__ __launch_bounds__(256) void k_pk_sync(const Chunk* __restrict__ chunks, int nchunks,
@@ -370,9 +524,25 @@ void launch_force_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int 
   hipLaunchKernelGGL((k_tile_force<KIND, LG>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
 }
 
+template <int LG>
+void launch_theta32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  const size_t lds = (3 * sizeof(double) + 4 * sizeof(float)) * (size_t)hcap;
+  hipLaunchKernelGGL(k_tile_theta32<LG>, dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+}
+template <int KIND, int LG>
+void launch_force32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
+  const size_t lds = (3 * sizeof(double) + (KIND == PD_OSB ? 6 : 4) * sizeof(float)) * (size_t)hcap;
+  hipLaunchKernelGGL((k_tile_force32<KIND, LG>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
+}
+
 // dynamic LDS beyond the default 64 KiB per workgroup must be enabled per kernel
 template <int LG>
 void allow_lds_lg(size_t bytes) {
+  (void)hipFuncSetAttribute((const void*)k_tile_theta32<LG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_tile_force32<PD_BB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_tile_force32<PD_OSB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
   (void)hipFuncSetAttribute((const void*)k_tile_theta<LG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   (void)hipFuncSetAttribute((const void*)k_tile_td<LG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   (void)hipFuncSetAttribute((const void*)k_tile_force<PD_BB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -385,11 +555,13 @@ void allow_lds_lg(size_t bytes) {
 
 }  // namespace
 
-int pd_tile_fields(int pass, int kind) {
-  if (pass == 0) return tile_fields<TP_THETA, 0>();
-  if (pass == 1) return tile_fields<TP_TD, 0>();
-  return kind == PD_BB ? tile_fields<TP_FORCE, PD_BB>()
-                       : kind == PD_OSB ? tile_fields<TP_FORCE, PD_OSB>() : tile_fields<TP_FORCE, PD_EP>();
+int pd_tile_record_bytes(int pass, int kind, int prec) {
+  if (prec == PD_PREC_FP32C && kind != PD_EP && pass != 1)
+    return (int)(3 * sizeof(double) + ((pass == 2 && kind == PD_OSB) ? 6 : 4) * sizeof(float));
+  if (pass == 0) return 8 * tile_fields<TP_THETA, 0>();
+  if (pass == 1) return 8 * tile_fields<TP_TD, 0>();
+  return 8 * (kind == PD_BB ? tile_fields<TP_FORCE, PD_BB>()
+                            : kind == PD_OSB ? tile_fields<TP_FORCE, PD_OSB>() : tile_fields<TP_FORCE, PD_EP>());
 }
 
 void pd_tile_allow_lds(size_t bytes) {
@@ -431,4 +603,25 @@ void pd_launch_pk_sync(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) 
   if (n <= 0) return;
   hipLaunchKernelGGL(k_pk_sync, dim3((n + PD_WAVES_PER_BLOCK - 1) / PD_WAVES_PER_BLOCK), dim3(256), 0, s, chunks,
                      n, envs);
+}
+
+void pd_launch_tile_theta32(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  if (n <= 0) return;
+  if (lgG == 1) launch_theta32_lg<1>(s, tiles, n, envs, hcap);
+  else if (lgG == 2) launch_theta32_lg<2>(s, tiles, n, envs, hcap);
+  else launch_theta32_lg<3>(s, tiles, n, envs, hcap);
+}
+
+void pd_launch_tile_force32(hipStream_t s, int kind, int lgG, const Tile* tiles, int n, EnvDev* envs, int flags,
+                            int hcap) {
+  if (n <= 0) return;
+  if (kind == PD_BB) {
+    if (lgG == 1) launch_force32_lg<PD_BB, 1>(s, tiles, n, envs, flags, hcap);
+    else if (lgG == 2) launch_force32_lg<PD_BB, 2>(s, tiles, n, envs, flags, hcap);
+    else launch_force32_lg<PD_BB, 3>(s, tiles, n, envs, flags, hcap);
+  } else {
+    if (lgG == 1) launch_force32_lg<PD_OSB, 1>(s, tiles, n, envs, flags, hcap);
+    else if (lgG == 2) launch_force32_lg<PD_OSB, 2>(s, tiles, n, envs, flags, hcap);
+    else launch_force32_lg<PD_OSB, 3>(s, tiles, n, envs, flags, hcap);
+  }
 }

@@ -1,5 +1,5 @@
-"""Tiled FP64 passes (halos staged in LDS) against the untiled FP64 kernels and
-the oracle.
+"""Tiled passes (halos staged in LDS) against the untiled kernels of the same
+form (FP64 and FP32-compensated) and against the oracle.
 
 The tiled kernels evaluate every bond with the untiled kernels' expressions in
 the same per-lane order, so for the same lanes per particle the results are
@@ -48,9 +48,9 @@ def tiles(on: bool):
             os.environ["PERIDYN_TILES"] = old
 
 
-def make(pd, spec, tiled, lanes):
+def make(pd, spec, tiled, lanes, precision="fp64"):
     with tiles(tiled):
-        env = pd.Env.from_spec(spec, precision="fp64", lanes_per_particle=lanes)
+        env = pd.Env.from_spec(spec, precision=precision, lanes_per_particle=lanes)
     lay = env.layout()
     assert lay["tiled"] == tiled, lay
     assert lay["lanes_per_particle"] == lanes
@@ -187,4 +187,39 @@ def test_tiled_and_untiled_batched_equal_alone(pd):
     for a, b in zip(alone, mix):
         same_state(a, b)
     for e in alone + mix:
+        e.close()
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_tiled_fp32c_equals_untiled(pd, kind, lanes):
+    """FP32-compensated form: BB / OSB blocks run the tiled FP32C kernels, EP
+    blocks the tiled FP64 ones; both bit-identical to their untiled kernels."""
+    say("tiled fp32c vs untiled", kind, lanes)
+    spec = ep_tensile() if kind == "ep" else W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.002)
+    a = make(pd, spec, True, lanes, "fp32c")
+    b = make(pd, spec, False, lanes, "fp32c")
+    a.run(40, 1)
+    b.run(40, 1)
+    same_state(a, b)
+    a.close()
+    b.close()
+
+
+def test_tiled_fp32c_breaking_and_batch(pd):
+    say("tiled fp32c breaking + batch")
+    s1 = W.tensile_bb(n=10, dt=1e-3, kind="bb", pull_per_step=0.05)
+    s2 = W.tensile_bb(n=8, dt=1e-4, kind="osb", pull_per_step=0.002)
+    alone = [make(pd, s1, True, 4, "fp32c"), make(pd, s2, True, 2, "fp32c")]
+    ref = [make(pd, s1, False, 4, "fp32c"), make(pd, s2, False, 2, "fp32c")]
+    for e in alone + ref:
+        e.run(50, 1)
+    assert alone[0].damage().max() > 0.0
+    for a, b in zip(alone, ref):
+        same_state(a, b)
+    mix = [make(pd, s1, True, 4, "fp32c"), make(pd, s2, False, 2, "fp32c")]
+    pd.run_batch(mix, 50, 1)
+    for a, b in zip(alone, mix):
+        same_state(a, b)
+    for e in alone + ref + mix:
         e.close()
