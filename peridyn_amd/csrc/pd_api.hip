@@ -445,11 +445,79 @@ constexpr size_t kTileLdsBudget = 80 * 1024;
 // of its particles; pk receives every bond's halo-local index with the intact
 // bit (every bond is intact after the family build).  Leaves e.tiled false
 // (untiled passes) when a halo exceeds 15-bit indices or the LDS budget.
+// LDS record bytes of the env's largest tiled pass
+size_t tile_record_bytes(const Env& e) {
+  size_t rb = 0;
+  for (auto& b : e.blocks) {
+    rb = std::max<size_t>(rb, pd_tile_record_bytes(2, b.d.kind, e.precision));
+    if (b.d.kind != PD_BB) rb = std::max<size_t>(rb, pd_tile_record_bytes(0, b.d.kind, e.precision));
+    if (b.d.kind == PD_EP) rb = std::max<size_t>(rb, pd_tile_record_bytes(1, PD_EP, e.precision));
+  }
+  return rb;
+}
+
+// Device build (k_halo_count / k_halo_fill, one workgroup per tile): tiles
+// whose slots fit a 32768-entry LDS sort buffer.  Returns false (nothing
+// allocated) when a halo exceeds the limits.
+bool build_tiles_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& tr, int ncap) {
+  hipStream_t s = c->stream;
+  const int nt = (int)tr.size();
+  std::vector<Tile> tt(nt);
+  for (int t = 0; t < nt; ++t) {
+    Tile& T = tt[t];
+    std::memset(&T, 0, sizeof(T));
+    T.block = e.chunks[tr[t].first].block;
+    T.nch = tr[t].second;
+    for (int k = 0; k < T.nch; ++k) T.ch[k] = e.chunks[tr[t].first + k];
+  }
+  Tile* dt = nullptr;
+  int32_t* dcnt = nullptr;
+  std::vector<int32_t> cnt(nt);
+  auto release = [&] {
+    if (dt) (void)hipFree(dt);
+    if (dcnt) (void)hipFree(dcnt);
+    dt = nullptr;
+    dcnt = nullptr;
+  };
+  try {
+    PD_HIP(hipMalloc(&dt, sizeof(Tile) * std::max(nt, 1)));
+    PD_HIP(hipMalloc(&dcnt, sizeof(int32_t) * std::max(nt, 1)));
+    PD_HIP(hipMemcpyAsync(dt, tt.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
+    pd_launch_halo_count(s, dt, nt, e.dev.col, dcnt, ncap);
+    PD_HIP(hipGetLastError());
+    PD_HIP(hipMemcpyAsync(cnt.data(), dcnt, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    size_t hmax = 1, total = 0;
+    for (int t = 0; t < nt; ++t) {
+      hmax = std::max<size_t>(hmax, (size_t)cnt[t]);
+      tt[t].hoff = (uint32_t)total;
+      tt[t].hn = cnt[t];
+      total += (size_t)cnt[t];
+    }
+    if (hmax > 0x7fff || tile_record_bytes(e) * hmax > kTileLdsBudget || total * sizeof(int32_t) >= PD_MAX_ARRAY_BYTES) {
+      release();
+      return false;
+    }
+    PD_HIP(hipMemcpyAsync(dt, tt.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
+    e.dev.halo = dalloc<int32_t>(e, std::max<size_t>(total, 1));
+    e.dev.pk = dalloc<uint64_t>(e, (size_t)std::max<int64_t>(e.npk, 1));
+    PD_HIP(hipMemsetAsync(e.dev.pk, 0, sizeof(uint64_t) * (size_t)std::max<int64_t>(e.npk, 1), s));
+    pd_launch_halo_fill(s, dt, nt, e.dev.col, e.dev.halo, e.dev.pk, ncap);
+    PD_HIP(hipGetLastError());
+    PD_HIP(hipStreamSynchronize(s));
+    e.hmax = (int32_t)hmax;
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    release();
+    throw;
+  }
+  release();
+  e.tiles = std::move(tt);
+  return true;
+}
+
 void build_tiles(Ctx* c, Env& e) {
   hipStream_t s = c->stream;
-  std::vector<int32_t> col((size_t)std::max<int64_t>(e.nslots, 1));
-  PD_HIP(hipMemcpyAsync(col.data(), e.dev.col, sizeof(int32_t) * (size_t)e.nslots, hipMemcpyDeviceToHost, s));
-  PD_HIP(hipStreamSynchronize(s));
   // tiles: chunk ranges
   std::vector<std::pair<int, int>> tr;
   for (int k = 0; k < (int)e.chunks.size();) {
@@ -462,6 +530,27 @@ void build_tiles(Ctx* c, Env& e) {
     k += n;
   }
   const int nt = (int)tr.size();
+  int64_t maxent = 1;
+  for (auto& r : tr) {
+    int64_t ent = 0;
+    for (int k = r.first; k < r.first + r.second; ++k) ent += (int64_t)PD_WAVE * e.chunks[k].width;
+    maxent = std::max(maxent, ent);
+  }
+  int ncap = 1;
+  while (ncap < maxent) ncap <<= 1;
+  // PERIDYN_TILE_BUILD=host selects the host build below (it is the path of
+  // tiles beyond the device sort buffer; tests compare the two)
+  const char* hb = std::getenv("PERIDYN_TILE_BUILD");
+  const bool host_build = hb && std::string(hb) == "host";
+  if (ncap <= 32768 && !host_build) {
+    e.tiled = build_tiles_device(c, e, tr, ncap);
+    if (!e.tiled) e.tiles.clear();
+    return;
+  }
+  // host build for tiles beyond the device sort buffer
+  std::vector<int32_t> col((size_t)std::max<int64_t>(e.nslots, 1));
+  PD_HIP(hipMemcpyAsync(col.data(), e.dev.col, sizeof(int32_t) * (size_t)e.nslots, hipMemcpyDeviceToHost, s));
+  PD_HIP(hipStreamSynchronize(s));
   std::vector<std::vector<int32_t>> halos(nt);
   std::vector<uint64_t> pk((size_t)std::max<int64_t>(e.npk, 1), 0ull);
   const int64_t N = e.N;
@@ -504,13 +593,7 @@ void build_tiles(Ctx* c, Env& e) {
     hmax = std::max(hmax, h.size());
     total += h.size();
   }
-  size_t rb = 0;  // LDS bytes per halo record of the env's largest pass
-  for (auto& b : e.blocks) {
-    rb = std::max<size_t>(rb, pd_tile_record_bytes(2, b.d.kind, e.precision));
-    if (b.d.kind != PD_BB) rb = std::max<size_t>(rb, pd_tile_record_bytes(0, b.d.kind, e.precision));
-    if (b.d.kind == PD_EP) rb = std::max<size_t>(rb, pd_tile_record_bytes(1, PD_EP, e.precision));
-  }
-  if (hmax > 0x7fff || rb * hmax > kTileLdsBudget ||
+  if (hmax > 0x7fff || tile_record_bytes(e) * hmax > kTileLdsBudget ||
       total * sizeof(int32_t) >= PD_MAX_ARRAY_BYTES) {
     e.tiles.clear();
     e.tiled = false;

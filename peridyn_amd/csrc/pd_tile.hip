@@ -484,6 +484,127 @@ __global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ t
   finish_particle(E, p, gld4(E.x, p), yp, a1, a2, a3, flags);
 }
 
+// ---------------------------------------------------------------------------
+// Device build of the tiles' halos and packed streams (one workgroup per
+// tile; ncap = power of two >= the largest tile's bond slot count, the LDS
+// sort buffer).  Every bond of a freshly built family is intact.
+
+// Loads the neighbour ids of every bond slot of tile T into buf (padding and
+// empty slots as INT32_MAX) and sorts them ascending (bitonic).
+__device__ void tile_sorted_ids(const Tile& T, const int32_t* __restrict__ col, int32_t* __restrict__ buf, int ncap) {
+  int pre[PD_TILE_CHUNKS + 1];
+  pre[0] = 0;
+  for (int k = 0; k < PD_TILE_CHUNKS; ++k) pre[k + 1] = pre[k] + (k < T.nch ? PD_WAVE * T.ch[k].width : 0);
+  for (int i = threadIdx.x; i < ncap; i += blockDim.x) {
+    int32_t v = INT32_MAX;
+    if (i < pre[T.nch]) {
+      int k = 0;
+      while (i >= pre[k + 1]) ++k;
+      const int32_t j = col[T.ch[k].slot0 + (uint32_t)(i - pre[k])];
+      v = j >= 0 ? j : INT32_MAX;
+    }
+    buf[i] = v;
+  }
+  __syncthreads();
+  for (int k = 2; k <= ncap; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < ncap; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const int32_t a = buf[i], b = buf[ixj];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            buf[i] = b;
+            buf[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+__device__ __forceinline__ bool first_of_run(const int32_t* buf, int i) {
+  return buf[i] != INT32_MAX && (i == 0 || buf[i] != buf[i - 1]);
+}
+
+// per-thread counts of run starts over contiguous segments, exclusive scan in LDS
+__device__ int segment_scan(const int32_t* buf, int ncap, int* part, int& seg0, int& seg1) {
+  const int per = ncap / (int)blockDim.x > 0 ? ncap / (int)blockDim.x : 1;
+  seg0 = min((int)threadIdx.x * per, ncap);
+  seg1 = (threadIdx.x + 1 == blockDim.x) ? ncap : min(seg0 + per, ncap);
+  int cnt = 0;
+  for (int i = seg0; i < seg1; ++i) cnt += first_of_run(buf, i) ? 1 : 0;
+  part[threadIdx.x] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      const int c = part[t];
+      part[t] = acc;
+      acc += c;
+    }
+    part[blockDim.x] = acc;
+  }
+  __syncthreads();
+  return part[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_halo_count(const Tile* __restrict__ tiles, int ntiles,
+                                                    const int32_t* __restrict__ col, int32_t* __restrict__ counts,
+                                                    int ncap) {
+  extern __shared__ int32_t sbuf[];
+  if ((int)blockIdx.x >= ntiles) return;
+  const Tile T = tiles[blockIdx.x];
+  int32_t* buf = sbuf;
+  int* part = sbuf + ncap;
+  tile_sorted_ids(T, col, buf, ncap);
+  int s0, s1;
+  segment_scan(buf, ncap, part, s0, s1);
+  if (threadIdx.x == 0) counts[blockIdx.x] = part[blockDim.x];
+}
+
+__global__ __launch_bounds__(256) void k_halo_fill(const Tile* __restrict__ tiles, int ntiles,
+                                                   const int32_t* __restrict__ col, int32_t* __restrict__ halo,
+                                                   uint64_t* __restrict__ pk, int ncap) {
+  extern __shared__ int32_t sbuf[];
+  if ((int)blockIdx.x >= ntiles) return;
+  const Tile T = tiles[blockIdx.x];
+  int32_t* buf = sbuf;
+  int* part = sbuf + ncap;
+  tile_sorted_ids(T, col, buf, ncap);
+  int s0, s1;
+  int pos = segment_scan(buf, ncap, part, s0, s1);
+  for (int i = s0; i < s1; ++i)
+    if (first_of_run(buf, i)) halo[T.hoff + (uint32_t)(pos++)] = buf[i];
+  __syncthreads();
+  // the compacted (sorted, unique) halo back into LDS for the index lookup
+  const int hn = T.hn;
+  for (int i = threadIdx.x; i < hn; i += blockDim.x) buf[i] = halo[T.hoff + (uint32_t)i];
+  __syncthreads();
+  for (int k = 0; k < T.nch; ++k) {
+    const Chunk& ch = T.ch[k];
+    const int nt4 = (ch.width + 3) >> 2;
+    for (int w = threadIdx.x; w < PD_WAVE * nt4; w += blockDim.x) {
+      const int t4 = w / PD_WAVE, lane = w % PD_WAVE;
+      uint64_t word = 0;
+      for (int u = 0; u < 4; ++u) {
+        const int st = 4 * t4 + u;
+        if (st >= ch.width) break;
+        const int32_t j = col[ch.slot0 + (uint32_t)(PD_WAVE * st + lane)];
+        if (j < 0) continue;
+        int lo = 0, hi = hn;  // lower_bound of j in buf[0, hn)
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (buf[mid] < j) lo = mid + 1;
+          else hi = mid;
+        }
+        word |= (uint64_t)(0x8000u | (uint32_t)lo) << (16 * u);
+      }
+      pk[ch.pk0 + (uint32_t)w] = word;
+    }
+  }
+}
+
 // pk intact bits := state (after bonds were broken by the untiled passes or the
 // state was loaded); one wavefront per chunk
 __global__ __launch_bounds__(256) void k_pk_sync(const Chunk* __restrict__ chunks, int nchunks,
@@ -565,6 +686,8 @@ int pd_tile_record_bytes(int pass, int kind, int prec) {
 }
 
 void pd_tile_allow_lds(size_t bytes) {
+  (void)hipFuncSetAttribute((const void*)k_halo_count, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipFuncSetAttribute((const void*)k_halo_fill, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   allow_lds_lg<1>(bytes);
   allow_lds_lg<2>(bytes);
   allow_lds_lg<3>(bytes);
@@ -624,4 +747,19 @@ void pd_launch_tile_force32(hipStream_t s, int kind, int lgG, const Tile* tiles,
     else if (lgG == 2) launch_force32_lg<PD_OSB, 2>(s, tiles, n, envs, flags, hcap);
     else launch_force32_lg<PD_OSB, 3>(s, tiles, n, envs, flags, hcap);
   }
+}
+
+int pd_tile_build_lds(int ncap) { return (int)sizeof(int32_t) * (ncap + 257); }
+
+void pd_launch_halo_count(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* counts, int ncap) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_halo_count, dim3(n), dim3(256), (size_t)pd_tile_build_lds(ncap), s, tiles, n, col, counts,
+                     ncap);
+}
+
+void pd_launch_halo_fill(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* halo, uint64_t* pk,
+                         int ncap) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_halo_fill, dim3(n), dim3(256), (size_t)pd_tile_build_lds(ncap), s, tiles, n, col, halo, pk,
+                     ncap);
 }

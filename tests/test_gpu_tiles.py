@@ -223,3 +223,27 @@ def test_tiled_fp32c_breaking_and_batch(pd):
         same_state(a, b)
     for e in alone + ref + mix:
         e.close()
+
+
+@pytest.mark.parametrize("kind", ["bb", "ep"])
+def test_device_and_host_tile_builds_agree(pd, kind):
+    """The device halo build (sorted unique neighbour ids per tile, packed
+    local indices) and the host build give the same tiles: identical runs."""
+    say("tile builds", kind)
+    spec = ep_tensile() if kind == "ep" else W.tensile_bb(n=10, dt=1e-3, kind="bb", pull_per_step=0.05)
+    a = make(pd, spec, True, 4)
+    old = os.environ.get("PERIDYN_TILE_BUILD")
+    os.environ["PERIDYN_TILE_BUILD"] = "host"
+    try:
+        b = make(pd, spec, True, 4)
+    finally:
+        if old is None:
+            del os.environ["PERIDYN_TILE_BUILD"]
+        else:
+            os.environ["PERIDYN_TILE_BUILD"] = old
+    assert a.layout() == b.layout()
+    a.run(40, 1)
+    b.run(40, 1)
+    same_state(a, b)
+    a.close()
+    b.close()
