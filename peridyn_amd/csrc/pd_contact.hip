@@ -174,10 +174,11 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Visits every block-2 member within `distance` of p (27-cell stencil), the
-// members of a cell spread over the 64 lanes of the calling wavefront: f(hit, j)
-// is called by all lanes together (wave-uniform trip count) with hit = lane's
-// member j is a contact.
+// Visits every block-2 member within `distance` of p (27-cell stencil) with the
+// 64 lanes of the calling wavefront: lanes 0..26 read the ranges of the 27
+// cells at once, and the cells' members, concatenated in stencil order, are
+// dealt to the lanes 64 at a time.  f(hit, j) is called by all lanes together
+// (wave-uniform trip count) with hit = the lane's member j is a contact.
 template <typename F>
 __device__ __forceinline__ void wave_candidates(const ContactDev& C, const double4* y, const double4& p,
                                                 const double* box, int lane, F&& f) {
@@ -185,32 +186,55 @@ __device__ __forceinline__ void wave_candidates(const ContactDev& C, const doubl
   cell_id(p, box, cc);
   const int nc[3] = {(int)box[8], (int)box[9], (int)box[10]};
   const double d2max = C.distance * C.distance;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int zz = cc[2] + dz;
-    if (zz < 0 || zz >= nc[2]) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int yy = cc[1] + dy;
-      if (yy < 0 || yy >= nc[1]) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int xx = cc[0] + dx;
-        if (xx < 0 || xx >= nc[0]) continue;
-        const int cell = xx + nc[0] * (yy + nc[1] * zz);
-        const int s = C.cell_start[cell], e = s + C.cell_count[cell];
-        for (int t0 = s; t0 < e; t0 += 64) {
-          const int t = t0 + lane;
-          bool hit = false;
-          int j = -1;
-          if (t < e) {
-            j = C.cell_items[t];
-            const double4 q = y[C.idx2[j]];
-            const double a = p.x - q.x, b = p.y - q.y, c = p.z - q.z;
-            hit = (a * a + b * b + c * c < d2max);
-          }
-          f(hit, j);
-        }
-      }
+  int my_s = 0, my_n = 0;
+  if (lane < 27) {
+    const int xx = cc[0] + lane % 3 - 1, yy = cc[1] + (lane / 3) % 3 - 1, zz = cc[2] + lane / 9 - 1;
+    if (xx >= 0 && xx < nc[0] && yy >= 0 && yy < nc[1] && zz >= 0 && zz < nc[2]) {
+      const int cell = xx + nc[0] * (yy + nc[1] * zz);
+      my_s = C.cell_start[cell];
+      my_n = C.cell_count[cell];
     }
   }
+  int incl = my_n;  // inclusive scan of the cell sizes over the lanes
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  const int total = __shfl(incl, 63);
+  const int excl = incl - my_n;
+  for (int t0 = 0; t0 < total; t0 += 64) {
+    const int t = t0 + lane;
+    // the cell holding concatenated index t: the last of lanes 0..26 whose
+    // first index is <= t (empty cells share their successor's first index)
+    int c = 0;
+    for (int b = 16; b > 0; b >>= 1) {
+      const int cand = c + b;
+      const int ex = __shfl(excl, cand < 27 ? cand : 26);
+      if (cand < 27 && ex <= t) c = cand;
+    }
+    const int cs = __shfl(my_s, c), ce = __shfl(excl, c);
+    bool hit = false;
+    int j = -1;
+    if (t < total) {
+      j = C.cell_items[cs + (t - ce)];
+      const double4 q = y[C.idx2[j]];
+      const double a = p.x - q.x, b = p.y - q.y, d = p.z - q.z;
+      hit = (a * a + b * b + d * d < d2max);
+    }
+    f(hit, j);
+  }
+}
+
+// Rank of this lane's value among the n (<= 64) distinct values held by lanes
+// 0..n-1: the number of them that sort before it (descending if DESC).
+template <bool DESC>
+__device__ __forceinline__ int wave_rank(int32_t v, int n) {
+  int r = 0;
+  for (int m = 0; m < n; ++m) {
+    const int32_t w = __shfl(v, m);
+    r += DESC ? (w > v) : (w < v);
+  }
+  return r;
 }
 
 // Descending bitonic sort of buf[0, n) (n <= kCandCap) by one wavefront; the
@@ -267,7 +291,14 @@ __global__ __launch_bounds__(256) void k_csearch(const EnvDev* __restrict__ envs
       }
       total += __popcll(m);
     });
-    if (total <= kCandCap) {
+    if (total <= 64) {
+      // few candidates: descending ranks in registers, the M largest kept
+      wave_sync();  // buf[] written by other lanes above
+      const int32_t v = lane < total ? buf[lane] : -1;
+      const int r = wave_rank<true>(v, total);
+      cnt = min(total, M);
+      if (lane < total && r < M) own[r] = v;
+    } else if (total <= kCandCap) {
       wave_sort_desc(buf, total, lane);
       cnt = min(total, M);
       for (int k = lane; k < cnt; k += 64) own[k] = buf[k];
@@ -334,7 +365,12 @@ __global__ __launch_bounds__(256) void k_crev_sort(const EnvDev* __restrict__ en
   int32_t* r = C.rev + C.rev_off[j];
   const int n = C.rev_cnt[j];
   if (n <= 1) return;
-  if (n <= kCandCap) {
+  if (n <= 64) {  // ascending ranks in registers
+    const int32_t v = lane < n ? r[lane] : INT_MAX;
+    const int rk = wave_rank<false>(v, n);
+    wave_sync();  // every lane has read r[] before any lane overwrites it
+    if (lane < n) r[rk] = v;
+  } else if (n <= kCandCap) {
     int32_t* buf = cand[wv];
     for (int k = lane; k < n; k += 64) buf[k] = r[k];
     wave_sort_desc(buf, n, lane);
