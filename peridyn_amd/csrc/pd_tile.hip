@@ -74,6 +74,30 @@ __device__ __forceinline__ bool tile_wave(const Tile& T, TileWave& w) {
   return true;
 }
 
+// Packed words of a lane's bond groups, the next D groups' words in flight.
+template <int D>
+struct PkStream {
+  const uint64_t* pk;
+  uint32_t wb;
+  int nt4;
+  uint64_t w[D];
+  __device__ __forceinline__ uint64_t word(int g) const {
+    return g < nt4 ? gld(pk, wb + 64u * (uint32_t)g) : 0ull;
+  }
+  __device__ __forceinline__ PkStream(const uint64_t* p, uint32_t b, int n) : pk(p), wb(b), nt4(n) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) w[d] = word(d);
+  }
+  // word of group g (called for g = 0, 1, 2, ... in order)
+  __device__ __forceinline__ uint64_t next(int g) {
+    const uint64_t cw = w[0];
+#pragma unroll
+    for (int d = 0; d + 1 < D; ++d) w[d] = w[d + 1];
+    w[D - 1] = word(g + D);
+    return cw;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // dilatation theta (src/peri.jl:155-172), OSB and EP blocks
 template <int LG>
@@ -97,10 +121,9 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   double acc = 0.0;
-  uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+  PkStream<3> ps(pk, wb, nt4);
   for (int t4 = 0; t4 < nt4; ++t4) {
-    const uint64_t cw = nxt;
-    if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+    const uint64_t cw = ps.next(t4);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
@@ -155,16 +178,25 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
   double acc = 0.0;
-  uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
-  for (int t4 = 0; t4 < nt4; ++t4) {
-    const uint64_t cw = nxt;
-    if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
-    double eo[4];
+  // own plastic extensions of the next group loaded while this one is computed
+  auto load_eo = [&](uint64_t cw, int g, double* eo) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const bool on = (cw >> (16 * u + 15)) & 1ull;
-      eo[u] = (on && yielded) ? gld(edp, base + 64u * (uint32_t)(4 * t4 + u)) : 0.0;
+      eo[u] = (on && yielded) ? gld(edp, base + 64u * (uint32_t)(4 * g + u)) : 0.0;
     }
+  };
+  PkStream<3> ps(pk, wb, nt4);
+  uint64_t cn = ps.next(0);
+  double en[4];
+  load_eo(cn, 0, en);
+  for (int t4 = 0; t4 < nt4; ++t4) {
+    const uint64_t cw = cn;
+    double eo[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) eo[u] = en[u];
+    cn = ps.next(t4 + 1);
+    load_eo(cn, t4 + 1, en);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
@@ -203,6 +235,92 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
 }
 
 // ---------------------------------------------------------------------------
+// Elastoplastic bond loop of the tiled force pass, software-pipelined over
+// groups of 4 bond steps: while group g is computed, the reverse bonds' plastic
+// extensions of group g+1, the own plastic extensions and reverse slots of
+// group g+2 and the packed word of group g+3 are in flight (these kernels'
+// occupancy is bounded by their LDS footprint, not their registers).  Bonds are
+// evaluated in the same order with the same expressions as the untiled kernel.
+struct EpGrp {
+  uint64_t cw;
+  double eo[4], er[4];
+  uint32_t rv[4];
+  uint32_t need;  // bit u: the reverse bond's plastic extension is needed
+};
+
+__device__ __forceinline__ void ep_stage_a(EpGrp& G, uint64_t cw, uint32_t base, int t4, bool yielded_p,
+                                           const double* __restrict__ edp_in, const uint32_t* __restrict__ rev,
+                                           const double* __restrict__ sh, int hcap) {
+  G.cw = cw;
+  G.need = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+    const bool on = (e & 0x8000u) != 0;
+    const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
+    G.eo[u] = (on && yielded_p) ? gld(edp_in, slot) : 0.0;
+    const bool nd = on && __builtin_signbit(sh[9 * hcap + (int)(e & 0x7fffu)]);
+    G.rv[u] = nd ? gld(rev, slot) : 0u;
+    G.need |= (nd ? 1u : 0u) << u;
+  }
+}
+
+__device__ __forceinline__ void ep_stage_b(EpGrp& G, const double* __restrict__ edp_in) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) G.er[u] = ((G.need >> u) & 1u) ? gld(edp_in, G.rv[u]) : 0.0;
+}
+
+__device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, uint32_t wb, const double4& xp,
+                                         const double4& yp, const double4& ap, bool yielded_p, double s0_pp,
+                                         double K0, double G0, double inv_den, const double* __restrict__ sh,
+                                         int hcap, double* __restrict__ edp_out, uint8_t* __restrict__ st,
+                                         uint16_t* __restrict__ pk16, double& a1, double& a2, double& a3) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const uint32_t e = (uint32_t)(G.cw >> (16 * u)) & 0xffffu;
+    if (!(e & 0x8000u)) continue;
+    const int j = (int)(e & 0x7fffu);
+    const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
+    const double X1 = sh[j] - xp.x, X2 = sh[hcap + j] - xp.y, X3 = sh[2 * hcap + j] - xp.z;
+    const double Y1 = sh[4 * hcap + j] - yp.x, Y2 = sh[5 * hcap + j] - yp.y, Y3 = sh[6 * hcap + j] - yp.z;
+    const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+    const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+    const double inv_x = pd_rsqrt(xx2);
+    const double inv_y = pd_rsqrt(yy2);
+    const double xij = xx2 * inv_x;
+    const double yij = yy2 * inv_y;
+    const double ext = yij - xij;
+    if (ext * inv_x < s0_pp) {
+      const double th_j = sh[7 * hcap + j], rm_j = sh[8 * hcap + j];
+      const double sm = ap.y + rm_j;
+      const double twm = ap.x * ap.y + th_j * rm_j;
+      const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
+      const double g15 = 15.0 * G0 * (inv_x * sm);
+      const double Cp = ap.w, Cj = sh[10 * hcap + j];
+      const double ede = ed - G.eo[u];
+      double td_own = g15 * ede;
+      double e_new = G.eo[u];
+      if (!(Cp > 1.0)) {
+        td_own *= Cp;
+        e_new = G.eo[u] + ede * (1.0 - Cp);
+      }
+      if (yielded_p) gst(edp_out, slot, e_new);  // else e_new = 0 = stored
+      double td_rev = g15 * (ed - G.er[u]);
+      if (!(Cj > 1.0)) td_rev *= Cj;
+      const double t = 3.0 * K0 * twm + 0.5 * (td_own + td_rev);
+      const double xx = t * inv_y * (sh[3 * hcap + j] * inv_den);
+      a1 += xx * Y1;
+      a2 += xx * Y2;
+      a3 += xx * Y3;
+    } else {
+      gst(st, slot, (uint8_t)0);
+      gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
+      if (yielded_p) gst(edp_out, slot, G.eo[u]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // fused force density + velocity-Verlet (material part of
 // general_material.jl:296-369 with the plugins' t_ij, then contacts and the
 // step epilogue of pd_wave.h)
@@ -230,6 +348,32 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
     uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
     uint8_t* __restrict__ st = E.state;
     const double inv_den = 1.0 / B.density[0];
+    if (KIND == PD_EP) {
+      const double4 ap = E.aux[p];
+      const double* __restrict__ edp_in = E.edp[E.edp_cur];
+      double* __restrict__ edp_out = E.edp[E.edp_cur ^ 1];
+      const bool yielded_p = pd_has_yielded(ap);
+      const double s0_pp = B.s0[0], K0 = B.K[0], G0 = B.G[0];
+      const int nt4 = (w.ch.width + 3) >> 2;
+      const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
+      const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
+      PkStream<2> ps(pk, wb, nt4);
+      EpGrp g0, g1, g2;
+      ep_stage_a(g0, ps.next(0), base, 0, yielded_p, edp_in, E.rev, sh, hcap);
+      ep_stage_b(g0, edp_in);
+      ep_stage_a(g1, ps.next(1), base, 1, yielded_p, edp_in, E.rev, sh, hcap);
+      for (int t4 = 0; t4 < nt4; ++t4) {
+        ep_stage_b(g1, edp_in);
+        ep_stage_a(g2, ps.next(t4 + 2), base, t4 + 2, yielded_p, edp_in, E.rev, sh, hcap);
+        ep_group(g0, t4, base, wb, xp, yp, ap, yielded_p, s0_pp, K0, G0, inv_den, sh, hcap, edp_out, st, pk16,
+                 a1, a2, a3);
+        g0 = g1;
+        g1 = g2;
+      }
+      a1 = group_sum(a1, C);
+      a2 = group_sum(a2, C);
+      a3 = group_sum(a3, C);
+    } else {
     double4 ap = make_double4(0.0, 0.0, 0.0, 0.0);
     const double* __restrict__ edp_in = nullptr;
     double* __restrict__ edp_out = nullptr;
@@ -246,10 +390,9 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
     const int nt4 = (w.ch.width + 3) >> 2;
     const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
     const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
-    uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+    PkStream<3> ps(pk, wb, nt4);
     for (int t4 = 0; t4 < nt4; ++t4) {
-      const uint64_t cw = nxt;
-      if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+      const uint64_t cw = ps.next(t4);
       double eo[4], er[4];
       if (KIND == PD_EP) {
         // own plastic extensions (0 until p yields) and the reverse bonds' (0
@@ -324,6 +467,7 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
     a1 = group_sum(a1, C);
     a2 = group_sum(a2, C);
     a3 = group_sum(a3, C);
+    }
   }
   if (!w.active || w.g != 0) return;
   add_contacts<false>(E, p, yp, y, a1, a2, a3);
@@ -383,10 +527,9 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   float s = 0.0f, c = 0.0f;
-  uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+  PkStream<3> ps(pk, wb, nt4);
   for (int t4 = 0; t4 < nt4; ++t4) {
-    const uint64_t cw = nxt;
-    if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+    const uint64_t cw = ps.next(t4);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
@@ -443,10 +586,9 @@ __global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ t
     const int nt4 = (w.ch.width + 3) >> 2;
     const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
     const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
-    uint64_t nxt = nt4 > 0 ? gld(pk, wb) : 0ull;
+    PkStream<3> ps(pk, wb, nt4);
     for (int t4 = 0; t4 < nt4; ++t4) {
-      const uint64_t cw = nxt;
-      if (t4 + 1 < nt4) nxt = gld(pk, wb + 64u * (uint32_t)(t4 + 1));
+      const uint64_t cw = ps.next(t4);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
