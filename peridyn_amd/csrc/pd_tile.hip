@@ -31,30 +31,51 @@ struct TileWave {
   bool active;
 };
 
-// Stages the halo of tile T into LDS; every thread of the workgroup takes part.
+// Stages the halo of tile T into LDS; every thread of the workgroup takes part,
+// R records per thread with all their loads in flight together (halo ids,
+// then the records, then the LDS stores).
+constexpr int kStageR = 4;
+
 template <int NF>
 __device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap) {
   const double4* __restrict__ x = E.x;
   const double4* __restrict__ y = E.y[E.cur];
   const int32_t* __restrict__ halo = E.halo;
-  for (int i = threadIdx.x; i < T.hn; i += 256) {
-    const uint32_t id = (uint32_t)gld(halo, T.hoff + (uint32_t)i);
-    const double4 a = gld4(x, id);
-    const double4 b = gld4(y, id);
-    sh[i] = a.x;
-    sh[hcap + i] = a.y;
-    sh[2 * hcap + i] = a.z;
-    sh[3 * hcap + i] = a.w;
-    sh[4 * hcap + i] = b.x;
-    sh[5 * hcap + i] = b.y;
-    sh[6 * hcap + i] = b.z;
-    if (NF > 7) {
-      const double4 c = gld4(E.aux, id);
-      sh[7 * hcap + i] = c.x;
-      sh[8 * hcap + i] = c.y;
-      if (NF > 9) {
-        sh[9 * hcap + i] = c.z;
-        sh[10 * hcap + i] = c.w;
+  for (int i0 = threadIdx.x; i0 < T.hn; i0 += 256 * kStageR) {
+    uint32_t id[kStageR];
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      const int i = i0 + 256 * r;
+      id[r] = i < T.hn ? (uint32_t)gld(halo, T.hoff + (uint32_t)i) : 0u;
+    }
+    double4 a[kStageR], b[kStageR], c[kStageR];
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      if (i0 + 256 * r < T.hn) {
+        a[r] = gld4(x, id[r]);
+        b[r] = gld4(y, id[r]);
+        if (NF > 7) c[r] = gld4(E.aux, id[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      const int i = i0 + 256 * r;
+      if (i < T.hn) {
+        sh[i] = a[r].x;
+        sh[hcap + i] = a[r].y;
+        sh[2 * hcap + i] = a[r].z;
+        sh[3 * hcap + i] = a[r].w;
+        sh[4 * hcap + i] = b[r].x;
+        sh[5 * hcap + i] = b[r].y;
+        sh[6 * hcap + i] = b[r].z;
+        if (NF > 7) {
+          sh[7 * hcap + i] = c[r].x;
+          sh[8 * hcap + i] = c[r].y;
+          if (NF > 9) {
+            sh[9 * hcap + i] = c[r].z;
+            sh[10 * hcap + i] = c[r].w;
+          }
+        }
       }
     }
   }
@@ -335,11 +356,15 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   const bool deformed = B.deformed != 0;  // uniform over the workgroup
+  TileWave w;
+  const bool has = tile_wave<LG>(T, w);
+  const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
+  EpiPre ep;  // epilogue inputs in flight during the staging and the bond loop
+  epi_load_a(E, p, ep);
   if (deformed) stage_halo<tile_fields<TP_FORCE, KIND>()>(E, T, sh, hcap);
   __syncthreads();
-  TileWave w;
-  if (!tile_wave<LG>(T, w)) return;
-  const int p = w.active ? w.p : w.ch.first;
+  if (!has) return;
+  epi_load_b(E, ep);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 xp = gld4(E.x, p), yp = gld4(y, p);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -470,8 +495,8 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
     }
   }
   if (!w.active || w.g != 0) return;
-  add_contacts<false>(E, p, yp, y, a1, a2, a3);
-  finish_particle(E, p, xp, yp, a1, a2, a3, flags);
+  add_contacts_pre<false>(E, p, yp, y, ep, a1, a2, a3);
+  finish_particle_v(E, p, xp, yp, a1, a2, a3, flags, ep.v, ep.bcid);
 }
 
 // ---------------------------------------------------------------------------
@@ -484,21 +509,39 @@ __device__ __forceinline__ void stage_halo32(const EnvDev& E, const Tile& T, dou
   const double4* __restrict__ y = E.y[E.cur];
   const int32_t* __restrict__ halo = E.halo;
   float* __restrict__ fs = (float*)(sh + 3 * hcap);
-  for (int i = threadIdx.x; i < T.hn; i += 256) {
-    const uint32_t id = (uint32_t)gld(halo, T.hoff + (uint32_t)i);
-    const float4 a = gld4(x, id);
-    const double4 b = gld4(y, id);
-    sh[i] = b.x;
-    sh[hcap + i] = b.y;
-    sh[2 * hcap + i] = b.z;
-    fs[i] = a.x;
-    fs[hcap + i] = a.y;
-    fs[2 * hcap + i] = a.z;
-    fs[3 * hcap + i] = a.w;
-    if (NFF > 4) {
-      const float4 c = gld4(E.aux32, id);
-      fs[4 * hcap + i] = c.x;
-      fs[5 * hcap + i] = c.y;
+  for (int i0 = threadIdx.x; i0 < T.hn; i0 += 256 * kStageR) {
+    uint32_t id[kStageR];
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      const int i = i0 + 256 * r;
+      id[r] = i < T.hn ? (uint32_t)gld(halo, T.hoff + (uint32_t)i) : 0u;
+    }
+    float4 a[kStageR], c[kStageR];
+    double4 b[kStageR];
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      if (i0 + 256 * r < T.hn) {
+        a[r] = gld4(x, id[r]);
+        b[r] = gld4(y, id[r]);
+        if (NFF > 4) c[r] = gld4(E.aux32, id[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      const int i = i0 + 256 * r;
+      if (i < T.hn) {
+        sh[i] = b[r].x;
+        sh[hcap + i] = b[r].y;
+        sh[2 * hcap + i] = b[r].z;
+        fs[i] = a[r].x;
+        fs[hcap + i] = a[r].y;
+        fs[2 * hcap + i] = a[r].z;
+        fs[3 * hcap + i] = a[r].w;
+        if (NFF > 4) {
+          fs[4 * hcap + i] = c[r].x;
+          fs[5 * hcap + i] = c[r].y;
+        }
+      }
     }
   }
 }
@@ -562,11 +605,15 @@ __global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ t
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   const bool deformed = B.deformed != 0;  // uniform over the workgroup
+  TileWave w;
+  const bool has = tile_wave<LG>(T, w);
+  const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
+  EpiPre ep;  // epilogue inputs in flight during the staging and the bond loop
+  epi_load_a(E, p, ep);
   if (deformed) stage_halo32<KIND == PD_OSB ? 6 : 4>(E, T, sh, hcap);
   __syncthreads();
-  TileWave w;
-  if (!tile_wave<LG>(T, w)) return;
-  const int p = w.active ? w.p : w.ch.first;
+  if (!has) return;
+  epi_load_b(E, ep);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 yp = gld4(y, p);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -622,8 +669,8 @@ __global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ t
     a3 = comp_total(s3, c3, C);
   }
   if (!w.active || w.g != 0) return;
-  add_contacts<false>(E, p, yp, y, a1, a2, a3);
-  finish_particle(E, p, gld4(E.x, p), yp, a1, a2, a3, flags);
+  add_contacts_pre<false>(E, p, yp, y, ep, a1, a2, a3);
+  finish_particle_v(E, p, gld4(E.x, p), yp, a1, a2, a3, flags, ep.v, ep.bcid);
 }
 
 // ---------------------------------------------------------------------------

@@ -137,17 +137,89 @@ __device__ __forceinline__ void add_contacts(const EnvDev& E, int p, const doubl
   }
 }
 
+// Per-particle inputs of the force kernels' epilogue (the contact terms of
+// model 0 and the Verlet update), loaded early so that their latency overlaps
+// the bond loop: epi_load_a at kernel start, epi_load_b (the loads that depend
+// on the first ones) before the bond loop.
+struct EpiPre {
+  int i1, i2, cnt1, off2, cnt2, bcid;
+  double inv_den;
+  double4 v;
+};
+
+__device__ __forceinline__ void epi_load_a(const EnvDev& E, int p, EpiPre& e) {
+  e.i1 = -1;
+  e.i2 = -1;
+  if (E.ncontacts > 0 && !E.contacts[0].self_contact) {
+    e.i1 = E.contacts[0].inv1[p];
+    e.i2 = E.contacts[0].inv2[p];
+  }
+  e.inv_den = E.inv_den[p];
+  e.v = E.v[p];
+  e.bcid = E.bcid[p];
+}
+
+__device__ __forceinline__ void epi_load_b(const EnvDev& E, EpiPre& e) {
+  e.cnt1 = 0;
+  e.cnt2 = 0;
+  e.off2 = 0;
+  if (e.i1 >= 0) e.cnt1 = E.contacts[0].own_cnt[e.i1];
+  if (e.i2 >= 0) {
+    e.off2 = E.contacts[0].rev_off[e.i2];
+    e.cnt2 = E.contacts[0].rev_cnt[e.i2];
+  }
+}
+
+// add_contacts with model 0's lookups taken from e (same terms, same order)
+template <bool GPOW>
+__device__ __forceinline__ void add_contacts_pre(const EnvDev& E, int p, const double4& yp, const double4* y,
+                                                 const EpiPre& e, double& a1, double& a2, double& a3) {
+  const double4* __restrict__ x = E.x;
+  for (int cm = 0; cm < E.ncontacts; ++cm) {
+    const ContactDev& C = E.contacts[cm];
+    if (C.self_contact) {
+      add_self_contact<GPOW>(E, C, p, yp, y, a1, a2, a3);
+      continue;
+    }
+    const int i1 = cm == 0 ? e.i1 : C.inv1[p];
+    if (i1 >= 0) {
+      const double inv_den_p = e.inv_den;
+      const int cnt = cm == 0 ? e.cnt1 : C.own_cnt[i1];
+      for (int k = 0; k < cnt; ++k) {
+        const int q = C.idx2[C.own[(size_t)i1 * C.max_neighs + k]];
+        const double4 yq = y[q];
+        double F1, F2, F3;
+        contact_fn<GPOW>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+        const double a = x[q].w * inv_den_p;
+        a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
+      }
+    }
+    const int i2 = cm == 0 ? e.i2 : C.inv2[p];
+    if (i2 >= 0) {
+      const double inv_den_p = e.inv_den;
+      const int off = cm == 0 ? e.off2 : C.rev_off[i2], cnt = cm == 0 ? e.cnt2 : C.rev_cnt[i2];
+      for (int k = 0; k < cnt; ++k) {
+        const int q = C.idx1[C.rev[off + k]];
+        const double4 yq = y[q];
+        double F1, F2, F3;
+        contact_fn<GPOW>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+        const double b = x[q].w * inv_den_p;
+        a1 -= F1 * b; a2 -= F2 * b; a3 -= F3 * b;
+      }
+    }
+  }
+}
+
 // Stores f and runs the FP64 velocity-Verlet part of the fused step:
 //   KICK : v += dt/2 f; velocity BCs                  (velocity_verlet.jl:74-79)
 //   DRIFT: v += dt/2 f; y' = y + dt v; position BCs of the next step
 //          (velocity_verlet.jl:63-69) into the other y buffer.
-__device__ __forceinline__ void finish_particle(EnvDev& E, int p, const double4& xp, const double4& yp,
-                                                double a1, double a2, double a3, int flags) {
+__device__ __forceinline__ void finish_particle_v(EnvDev& E, int p, const double4& xp, const double4& yp,
+                                                  double a1, double a2, double a3, int flags, double4 v,
+                                                  const int bci) {
   E.f[p] = make_double4(a1, a2, a3, 0.0);
   if (!(flags & (PD_F_KICK | PD_F_DRIFT))) return;
   const double dt = E.dt, c = dt / 2.0;
-  double4 v = E.v[p];
-  const int bci = E.bcid[p];
   if (flags & PD_F_KICK) {
     v.x = v.x + c * a1; v.y = v.y + c * a2; v.z = v.z + c * a3;
     if (bci >= 0) {
@@ -175,4 +247,9 @@ __device__ __forceinline__ void finish_particle(EnvDev& E, int p, const double4&
     E.y[E.cur ^ 1][p] = yn;
   }
   E.v[p] = v;
+}
+
+__device__ __forceinline__ void finish_particle(EnvDev& E, int p, const double4& xp, const double4& yp,
+                                                double a1, double a2, double a3, int flags) {
+  finish_particle_v(E, p, xp, yp, a1, a2, a3, flags, E.v[p], E.bcid[p]);
 }
