@@ -6,6 +6,7 @@
 // chunk descriptors, so any set of envs is stepped together by one launch per
 // kernel class (the batched / served form) and a single env is a set of one.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -63,6 +64,8 @@ struct Env {
   bool want_tiles = false, tiled = false;
   std::vector<int32_t> brick;
   std::vector<Tile> tiles;
+  Tile* d_tiles = nullptr;                       // device copy (run sets copy from it)
+  std::vector<std::array<int, 3>> tile_blocks;   // (block, first tile, count), tiles in block order
   int32_t hmax = 0;  // largest tile halo
   std::vector<Chunk> chunks;
   Chunk* d_chunks = nullptr;
@@ -85,6 +88,7 @@ struct Ctx {
   // cached run set
   std::vector<int64_t> set_ids;
   bool set_valid = false;
+  bool set_full = false;  // the all-chunk lists per pass (th, td, fk) were built too
   // stepping lists: chunks of untiled envs per pass, tiles of tiled envs per
   // pass and lanes per particle (lgG 1..3)
   int n_theta_u = 0, n_td_u = 0, n_force_u[3] = {0, 0, 0};
@@ -118,6 +122,8 @@ struct Ctx {
   double4* d_latch_part = nullptr;  // per-slice partial sums of the latch test
   size_t latch_part_cap = 0;
   int latch_nparts = 1;
+  int32_t* d_latch_flags = nullptr;  // per latch item, read back after a run
+  size_t latch_flags_cap = 0;
   // per-kernel-class HIP-event timing on the context stream (pd_ctx_profile)
   bool prof = false;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -979,6 +985,16 @@ void commit_env(Ctx* c, Env& e) {
   PD_HIP(hipMemcpyAsync(e.d_self, &e.slot, sizeof(int32_t), hipMemcpyHostToDevice, s));
   e.d_chunks = dalloc<Chunk>(e, e.chunks.size());
   PD_HIP(hipMemcpyAsync(e.d_chunks, e.chunks.data(), sizeof(Chunk) * e.chunks.size(), hipMemcpyHostToDevice, s));
+  e.tile_blocks.clear();
+  if (e.tiled && !e.tiles.empty()) {
+    e.d_tiles = dalloc<Tile>(e, e.tiles.size());
+    PD_HIP(hipMemcpyAsync(e.d_tiles, e.tiles.data(), sizeof(Tile) * e.tiles.size(), hipMemcpyHostToDevice, s));
+    for (int t = 0; t < (int)e.tiles.size(); ++t) {
+      if (e.tile_blocks.empty() || e.tile_blocks.back()[0] != e.tiles[t].block)
+        e.tile_blocks.push_back({e.tiles[t].block, t, 0});
+      ++e.tile_blocks.back()[2];
+    }
+  }
   upload_env_table(c, e);
   PD_HIP(hipStreamSynchronize(s));
   e.committed = true;
@@ -1001,82 +1017,119 @@ bool env_gpow(const Env& e) {
   return false;
 }
 
-void build_set(Ctx* c, const std::vector<int64_t>& ids) {
-  if (c->set_valid && ids == c->set_ids) return;
-  // chunk lists of every env (all passes), chunk lists of the untiled envs for
-  // stepping, tile lists of the tiled envs per pass and lanes per particle
-  std::vector<Chunk> all, th, td, fk[3], thu, tdu, fku[3];
-  std::vector<Tile> tth[2][4], ttd[4], tfk[3][2][4];
+// Run set of the listed envs.  The all-chunk list and the tile lists are
+// gathered on the device from each env's own arrays (no host staging); the
+// untiled envs' stepping lists and, with full (QSDrag's passes), the all-chunk
+// lists per pass are built on the host.
+void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
+  if (c->set_valid && ids == c->set_ids && (c->set_full || !full)) return;
+  std::vector<Chunk> th, td, fk[3], thu, tdu, fku[3];
   std::vector<int32_t> slots;
   std::vector<int2> ranges;
   bool has_ep = false, has_th = false, gpow = false, tiled = false;
-  int hcap = 1;
+  int hcap = 1, n_all = 0, n_th = 0, n_td = 0, n_fk[3] = {0, 0, 0};
+  // tile list sizes: theta [pc][lg], td [lg], force [kind][pc][lg]
+  int s_th[2][4] = {}, s_td[4] = {}, s_fk[3][2][4] = {};
   for (int64_t id : ids) {
     Env& e = get_env(c, id);
     if (!e.committed) throw PdError(PD_E_STATE, "env " + std::to_string(id) + " is not committed");
     slots.push_back(e.slot);
     gpow = gpow || env_gpow(e);
-    ranges.push_back(make_int2((int)all.size(), (int)e.chunks.size()));
-    for (const Chunk& ch : e.chunks) {
-      all.push_back(ch);
-      const int kind = e.blocks[ch.block].d.kind;
-      fk[kind].push_back(ch);
-      if (kind != PD_BB) { th.push_back(ch); has_th = true; }
-      if (kind == PD_EP) { td.push_back(ch); has_ep = true; }
-      if (!e.tiled) {
-        fku[kind].push_back(ch);
-        if (kind != PD_BB) thu.push_back(ch);
-        if (kind == PD_EP) tdu.push_back(ch);
-      }
+    ranges.push_back(make_int2(n_all, (int)e.chunks.size()));
+    n_all += (int)e.chunks.size();
+    for (auto& b : e.blocks) {
+      const int kind = b.d.kind;
+      n_fk[kind] += b.nchunks;
+      if (kind != PD_BB) { n_th += b.nchunks; has_th = true; }
+      if (kind == PD_EP) { n_td += b.nchunks; has_ep = true; }
     }
+    if (full || !e.tiled)
+      for (const Chunk& ch : e.chunks) {
+        const int kind = e.blocks[ch.block].d.kind;
+        if (full) {
+          fk[kind].push_back(ch);
+          if (kind != PD_BB) th.push_back(ch);
+          if (kind == PD_EP) td.push_back(ch);
+        }
+        if (!e.tiled) {
+          fku[kind].push_back(ch);
+          if (kind != PD_BB) thu.push_back(ch);
+          if (kind == PD_EP) tdu.push_back(ch);
+        }
+      }
     if (e.tiled) {
       tiled = true;
       hcap = std::max(hcap, (int)e.hmax);
-      for (const Tile& t : e.tiles) {
-        const int kind = e.blocks[t.block].d.kind;
+      for (auto& tb : e.tile_blocks) {
+        const int kind = e.blocks[tb[0]].d.kind;
         const int pc = (e.precision == PD_PREC_FP32C && kind != PD_EP) ? 1 : 0;  // EP runs FP64
-        tfk[kind][pc][e.lgG].push_back(t);
-        if (kind != PD_BB) tth[pc][e.lgG].push_back(t);
-        if (kind == PD_EP) ttd[e.lgG].push_back(t);
+        s_fk[kind][pc][e.lgG] += tb[2];
+        if (kind != PD_BB) s_th[pc][e.lgG] += tb[2];
+        if (kind == PD_EP) s_td[e.lgG] += tb[2];
       }
     }
   }
-  std::vector<Chunk> cat;
-  cat.insert(cat.end(), all.begin(), all.end());
-  c->off_theta = (int)cat.size(); cat.insert(cat.end(), th.begin(), th.end());
-  c->off_td = (int)cat.size(); cat.insert(cat.end(), td.begin(), td.end());
-  for (int k = 0; k < 3; ++k) { c->off_force[k] = (int)cat.size(); cat.insert(cat.end(), fk[k].begin(), fk[k].end()); }
-  c->off_theta_u = (int)cat.size(); cat.insert(cat.end(), thu.begin(), thu.end());
-  c->off_td_u = (int)cat.size(); cat.insert(cat.end(), tdu.begin(), tdu.end());
+  // chunk set: [all | th td fk (full) | thu tdu fku]
+  std::vector<Chunk> rest;
+  c->off_theta = n_all + (int)rest.size(); rest.insert(rest.end(), th.begin(), th.end());
+  c->off_td = n_all + (int)rest.size(); rest.insert(rest.end(), td.begin(), td.end());
+  for (int k = 0; k < 3; ++k) { c->off_force[k] = n_all + (int)rest.size(); rest.insert(rest.end(), fk[k].begin(), fk[k].end()); }
+  c->off_theta_u = n_all + (int)rest.size(); rest.insert(rest.end(), thu.begin(), thu.end());
+  c->off_td_u = n_all + (int)rest.size(); rest.insert(rest.end(), tdu.begin(), tdu.end());
   for (int k = 0; k < 3; ++k) {
-    c->off_force_u[k] = (int)cat.size();
-    cat.insert(cat.end(), fku[k].begin(), fku[k].end());
+    c->off_force_u[k] = n_all + (int)rest.size();
+    rest.insert(rest.end(), fku[k].begin(), fku[k].end());
   }
-  c->n_all = (int)all.size();
-  c->n_theta = (int)th.size();
-  c->n_td = (int)td.size();
-  for (int k = 0; k < 3; ++k) c->n_force[k] = (int)fk[k].size();
+  c->n_all = n_all;
+  c->n_theta = n_th;
+  c->n_td = n_td;
+  for (int k = 0; k < 3; ++k) c->n_force[k] = n_fk[k];
   c->n_theta_u = (int)thu.size();
   c->n_td_u = (int)tdu.size();
   for (int k = 0; k < 3; ++k) c->n_force_u[k] = (int)fku[k].size();
-  ensure_cap(c->d_set, c->set_cap, cat.size());
-  PD_HIP(hipMemcpyAsync(c->d_set, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
-  std::vector<Tile> tcat;
+  ensure_cap(c->d_set, c->set_cap, (size_t)n_all + rest.size());
+  {
+    int off = 0;
+    for (int64_t id : ids) {
+      Env& e = get_env(c, id);
+      if (!e.chunks.empty())
+        PD_HIP(hipMemcpyAsync(c->d_set + off, e.d_chunks, sizeof(Chunk) * e.chunks.size(), hipMemcpyDeviceToDevice,
+                              c->stream));
+      off += (int)e.chunks.size();
+    }
+  }
+  if (!rest.empty())
+    PD_HIP(hipMemcpyAsync(c->d_set + n_all, rest.data(), sizeof(Chunk) * rest.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  // tile set: per lgG: theta [pc], force [kind][pc], td
+  int tot = 0;
   for (int lg = 1; lg <= 3; ++lg) {
     for (int pc = 0; pc < 2; ++pc) {
-      c->ot_theta[pc][lg] = (int)tcat.size(); c->nt_theta[pc][lg] = (int)tth[pc][lg].size();
-      tcat.insert(tcat.end(), tth[pc][lg].begin(), tth[pc][lg].end());
-      for (int k = 0; k < 3; ++k) {
-        c->ot_force[k][pc][lg] = (int)tcat.size(); c->nt_force[k][pc][lg] = (int)tfk[k][pc][lg].size();
-        tcat.insert(tcat.end(), tfk[k][pc][lg].begin(), tfk[k][pc][lg].end());
+      c->ot_theta[pc][lg] = tot; c->nt_theta[pc][lg] = s_th[pc][lg]; tot += s_th[pc][lg];
+      for (int k = 0; k < 3; ++k) { c->ot_force[k][pc][lg] = tot; c->nt_force[k][pc][lg] = s_fk[k][pc][lg]; tot += s_fk[k][pc][lg]; }
+    }
+    c->ot_td[lg] = tot; c->nt_td[lg] = s_td[lg]; tot += s_td[lg];
+  }
+  if (tot > 0) {
+    ensure_cap(c->d_tset, c->tset_cap, (size_t)tot);
+    int f_th[2][4] = {}, f_td[4] = {}, f_fk[3][2][4] = {};  // fill positions
+    auto put = [&](int dst, const Env& e, const std::array<int, 3>& tb) {
+      PD_HIP(hipMemcpyAsync(c->d_tset + dst, e.d_tiles + tb[1], sizeof(Tile) * tb[2], hipMemcpyDeviceToDevice,
+                            c->stream));
+    };
+    for (int64_t id : ids) {
+      Env& e = get_env(c, id);
+      if (!e.tiled) continue;
+      const int lg = e.lgG;
+      for (auto& tb : e.tile_blocks) {
+        const int kind = e.blocks[tb[0]].d.kind;
+        const int pc = (e.precision == PD_PREC_FP32C && kind != PD_EP) ? 1 : 0;
+        put(c->ot_force[kind][pc][lg] + f_fk[kind][pc][lg], e, tb);
+        f_fk[kind][pc][lg] += tb[2];
+        if (kind != PD_BB) { put(c->ot_theta[pc][lg] + f_th[pc][lg], e, tb); f_th[pc][lg] += tb[2]; }
+        if (kind == PD_EP) { put(c->ot_td[lg] + f_td[lg], e, tb); f_td[lg] += tb[2]; }
       }
     }
-    c->ot_td[lg] = (int)tcat.size(); c->nt_td[lg] = (int)ttd[lg].size();
-    tcat.insert(tcat.end(), ttd[lg].begin(), ttd[lg].end());
-  }
-  if (!tcat.empty()) {
-    ensure_cap(c->d_tset, c->tset_cap, tcat.size());
-    PD_HIP(hipMemcpyAsync(c->d_tset, tcat.data(), sizeof(Tile) * tcat.size(), hipMemcpyHostToDevice, c->stream));
   }
   c->set_hcap = hcap;
   c->set_tiled = tiled;
@@ -1085,13 +1138,14 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids) {
   ensure_cap(c->d_obs_ranges, c->obs_ranges_cap, ranges.size());
   PD_HIP(hipMemcpyAsync(c->d_obs_ranges, ranges.data(), sizeof(int2) * ranges.size(), hipMemcpyHostToDevice,
                         c->stream));
-  ensure_cap(c->d_obs_part, c->obs_part_cap, std::max<size_t>(all.size(), 1));
+  ensure_cap(c->d_obs_part, c->obs_part_cap, std::max<size_t>((size_t)n_all, 1));
   PD_HIP(hipStreamSynchronize(c->stream));
   c->n_slots = (int)slots.size();
   c->set_has_ep = has_ep;
   c->set_has_theta = has_th;
   c->set_gpow = gpow;
   c->set_ids = ids;
+  c->set_full = full;
   c->set_valid = true;
 }
 
@@ -1120,21 +1174,27 @@ int prepare_latch(Ctx* c, const std::vector<int64_t>& ids) {
   if (items.empty()) return 0;
   c->latch_nparts = pd_latch_parts(max_n);
   ensure_cap(c->d_latch, c->latch_cap, items.size());
+  ensure_cap(c->d_latch_flags, c->latch_flags_cap, items.size());
   ensure_cap(c->d_latch_part, c->latch_part_cap, items.size() * (size_t)c->latch_nparts);
   PD_HIP(hipMemcpyAsync(c->d_latch, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
   return (int)items.size();
 }
 
-void refresh_latch(Ctx* c, const std::vector<int64_t>& ids) {
+// After a run with nl latch items (prepare_latch, same ids): the latches of
+// every listed block in one readback.
+void refresh_latch(Ctx* c, const std::vector<int64_t>& ids, int nl) {
+  if (nl <= 0) return;
+  std::vector<int32_t> fl(nl);
+  pd_launch_latch_flags(c->stream, c->d_envs, c->d_latch, nl, c->d_latch_flags);
+  PD_HIP(hipMemcpyAsync(fl.data(), c->d_latch_flags, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->stream));
+  PD_HIP(hipStreamSynchronize(c->stream));
+  int k = 0;
   for (int64_t id : ids) {
     Env& e = get_env(c, id);
     if (!e.maybe_undeformed) continue;
-    std::vector<BlockDev> bd(e.blocks.size());
-    PD_HIP(hipMemcpyAsync(bd.data(), e.dev.blocks, sizeof(BlockDev) * bd.size(), hipMemcpyDeviceToHost, c->stream));
-    PD_HIP(hipStreamSynchronize(c->stream));
     bool any = false;
-    for (auto& B : bd) any = any || !B.deformed;
+    for (size_t b = 0; b < e.blocks.size(); ++b) any = any || !fl[k++];
     e.maybe_undeformed = any;
   }
 }
@@ -1163,10 +1223,11 @@ void force_pass(Ctx* c, Env& e, const std::vector<Chunk>& sub, int flags_extra) 
   for (int k = 0; k < 3; ++k) { o_f[k] = (int)cat.size(); cat.insert(cat.end(), fk[k].begin(), fk[k].end()); }
   Chunk* d = nullptr;
   PD_HIP(hipMalloc(&d, sizeof(Chunk) * std::max<size_t>(cat.size(), 1)));
+  int nl = 0;
   try {
     PD_HIP(hipMemcpyAsync(d, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
     std::vector<int64_t> one{e.id};
-    const int nl = prepare_latch(c, one);
+    nl = prepare_latch(c, one);
     if (nl) pd_launch_latch(c->stream, c->d_envs, c->d_latch, nl, c->latch_nparts, c->d_latch_part);
     pd_launch_theta(c->stream, e.precision, d, (int)th.size(), c->d_envs);
     pd_launch_td(c->stream, e.precision, d + o_td, (int)td.size(), c->d_envs);
@@ -1188,7 +1249,7 @@ void force_pass(Ctx* c, Env& e, const std::vector<Chunk>& sub, int flags_extra) 
     throw;
   }
   PD_HIP(hipFree(d));
-  refresh_latch(c, {e.id});
+  refresh_latch(c, {e.id}, nl);
 }
 
 // 3 x N reference-order host array <-> internal-order double4 device array
@@ -1424,7 +1485,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
     if (e.has_ep) e.edp_cur ^= (int32_t)(nsteps & 1);
     e.lists_current = (nsteps % nuf) == 0;  // rebuilt after the last step
   }
-  if (nl) refresh_latch(c, ids);
+  refresh_latch(c, ids, nl);
 }
 
 // ---- resident-state snapshot ----------------------------------------------
@@ -1491,6 +1552,7 @@ int pd_ctx_destroy(void* ctx) {
   if (c->d_slots) (void)hipFree(c->d_slots);
   if (c->d_latch) (void)hipFree(c->d_latch);
   if (c->d_latch_part) (void)hipFree(c->d_latch_part);
+  if (c->d_latch_flags) (void)hipFree(c->d_latch_flags);
   if (c->d_obs_ranges) (void)hipFree(c->d_obs_ranges);
   if (c->d_obs_part) (void)hipFree(c->d_obs_part);
   if (c->d_obs_rows) (void)hipFree(c->d_obs_rows);
@@ -2040,7 +2102,7 @@ int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64
     }
     const std::vector<int64_t> ids{env};
     const int prec = e.precision;
-    build_set(c, ids);
+    build_set(c, ids, true);
     hipStream_t s = c->stream;
     const int nl = prepare_latch(c, ids);
     EnvDev* T = c->d_envs;
@@ -2092,7 +2154,7 @@ int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64
     PD_HIP(hipFree(buf));
     e.time_step += 1;
     e.lists_current = false;
-    if (nl) refresh_latch(c, ids);
+    refresh_latch(c, ids, nl);
   });
 }
 

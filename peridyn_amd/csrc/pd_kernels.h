@@ -87,3 +87,5 @@ int pd_tile_build_lds(int ncap);
 void pd_launch_halo_count(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* counts, int ncap);
 void pd_launch_halo_fill(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* halo, uint64_t* pk,
                          int ncap);
+// deformed latch of n (env slot, block) items into out[n]
+void pd_launch_latch_flags(hipStream_t s, EnvDev* envs, const int2* items, int n, int32_t* out);

@@ -904,6 +904,18 @@ int pd_latch_parts(int64_t max_n) {
   return (int)std::min<int64_t>(1024, std::max<int64_t>(1, (max_n + per - 1) / per));
 }
 
+// deformed latch of each (env slot, block) item, for one readback
+__global__ void k_latch_flags(const EnvDev* __restrict__ envs, const int2* __restrict__ items, int n,
+                              int32_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = envs[items[i].x].blocks[items[i].y].deformed;
+}
+
+void pd_launch_latch_flags(hipStream_t s, EnvDev* envs, const int2* items, int n, int32_t* out) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_latch_flags, dim3((n + 255) / 256), dim3(256), 0, s, envs, items, n, out);
+}
+
 void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n, int nparts, double4* part) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_latch_part, dim3(n, nparts), dim3(256), 0, s, envs, items, nparts, part);
