@@ -373,23 +373,43 @@ void build_brick_order(Env& e) {
         q[3 * i + d] = qi;
         nb[d] = std::max(nb[d], qi / dims[d] + 1);
       }
-    struct K { int64_t brick, cell; int32_t i; };
-    std::vector<K> key(n);
-    for (int64_t i = 0; i < n; ++i) {
+    // one uint64 key per particle: brick (high 30 bits), cell in the brick (7
+    // bits: at most 128 cells), input index (27 bits: particle arrays stay
+    // below 4 GiB); more bricks fall back to the three-field comparison
+    const int64_t nbr = nb[0] * nb[1] * nb[2];
+    auto brick_of = [&](int64_t i) {
       const int64_t* qq = &q[3 * i];
-      const int64_t br = ((qq[2] / bz) * nb[1] + qq[1] / by) * nb[0] + qq[0] / bx;
-      const int64_t cl = ((qq[2] % bz) * by + qq[1] % by) * bx + qq[0] % bx;
-      key[i] = K{br, cl, (int32_t)i};
+      return ((qq[2] / bz) * nb[1] + qq[1] / by) * nb[0] + qq[0] / bx;
+    };
+    auto cell_of = [&](int64_t i) {
+      const int64_t* qq = &q[3 * i];
+      return ((qq[2] % bz) * by + qq[1] % by) * bx + qq[0] % bx;
+    };
+    std::vector<int32_t> order(n);
+    if (nbr < (1LL << 30) && n < (1LL << 27) && bx * by * bz <= 128) {
+      std::vector<uint64_t> key(n);
+      for (int64_t i = 0; i < n; ++i)
+        key[i] = ((uint64_t)brick_of(i) << 34) | ((uint64_t)cell_of(i) << 27) | (uint64_t)i;
+      std::sort(key.begin(), key.end());
+      for (int64_t k = 0; k < n; ++k) order[k] = (int32_t)(key[k] & ((1ull << 27) - 1));
+    } else {
+      for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+      std::sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
+        const int64_t ba = brick_of(a), bc = brick_of(c);
+        if (ba != bc) return ba < bc;
+        const int64_t ca = cell_of(a), cc = cell_of(c);
+        if (ca != cc) return ca < cc;
+        return a < c;
+      });
     }
-    std::sort(key.begin(), key.end(), [](const K& a, const K& c) {
-      if (a.brick != c.brick) return a.brick < c.brick;
-      if (a.cell != c.cell) return a.cell < c.cell;
-      return a.i < c.i;
-    });
+    int64_t prev = -1;
     for (int64_t k = 0; k < n; ++k) {
-      if (k > 0 && key[k].brick != key[k - 1].brick) ++next_brick;
-      e.perm[b.first + k] = b.first + key[k].i;
-      e.iperm[b.first + key[k].i] = (int32_t)(b.first + k);
+      const int32_t i = order[k];
+      const int64_t br = brick_of(i);
+      if (k > 0 && br != prev) ++next_brick;
+      prev = br;
+      e.perm[b.first + k] = b.first + i;
+      e.iperm[b.first + i] = (int32_t)(b.first + k);
       e.brick[b.first + k] = next_brick;
     }
     ++next_brick;
@@ -462,9 +482,9 @@ size_t tile_record_bytes(const Env& e) {
   return rb;
 }
 
-// Device build (k_halo_count / k_halo_fill, one workgroup per tile): tiles
-// whose slots fit a 32768-entry LDS sort buffer.  Returns false (nothing
-// allocated) when a halo exceeds the limits.
+// Device build (k_halo_count / k_halo_fill, one workgroup per tile): tiles of
+// at most 16384 bond slots (an LDS hash set of twice that).  Returns false
+// (nothing allocated) when a halo exceeds the limits.
 bool build_tiles_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& tr, int ncap) {
   hipStream_t s = c->stream;
   const int nt = (int)tr.size();
@@ -489,7 +509,8 @@ bool build_tiles_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& 
     PD_HIP(hipMalloc(&dt, sizeof(Tile) * std::max(nt, 1)));
     PD_HIP(hipMalloc(&dcnt, sizeof(int32_t) * std::max(nt, 1)));
     PD_HIP(hipMemcpyAsync(dt, tt.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
-    pd_launch_halo_count(s, dt, nt, e.dev.col, dcnt, ncap);
+    const int hc = std::max(2 * ncap, 512);  // LDS hash set, at most half full
+    pd_launch_halo_count(s, dt, nt, e.dev.col, dcnt, hc);
     PD_HIP(hipGetLastError());
     PD_HIP(hipMemcpyAsync(cnt.data(), dcnt, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s));
     PD_HIP(hipStreamSynchronize(s));
@@ -508,7 +529,7 @@ bool build_tiles_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& 
     e.dev.halo = dalloc<int32_t>(e, std::max<size_t>(total, 1));
     e.dev.pk = dalloc<uint64_t>(e, (size_t)std::max<int64_t>(e.npk, 1));
     PD_HIP(hipMemsetAsync(e.dev.pk, 0, sizeof(uint64_t) * (size_t)std::max<int64_t>(e.npk, 1), s));
-    pd_launch_halo_fill(s, dt, nt, e.dev.col, e.dev.halo, e.dev.pk, ncap);
+    pd_launch_halo_fill(s, dt, nt, e.dev.col, e.dev.halo, e.dev.pk, hc);
     PD_HIP(hipGetLastError());
     PD_HIP(hipStreamSynchronize(s));
     e.hmax = (int32_t)hmax;
@@ -548,7 +569,7 @@ void build_tiles(Ctx* c, Env& e) {
   // tiles beyond the device sort buffer; tests compare the two)
   const char* hb = std::getenv("PERIDYN_TILE_BUILD");
   const bool host_build = hb && std::string(hb) == "host";
-  if (ncap <= 32768 && !host_build) {
+  if (ncap <= 16384 && !host_build) {
     e.tiled = build_tiles_device(c, e, tr, ncap);
     if (!e.tiled) e.tiles.clear();
     return;

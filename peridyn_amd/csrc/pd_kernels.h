@@ -81,11 +81,12 @@ void pd_launch_pk_sync(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_tile_theta32(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap);
 void pd_launch_tile_force32(hipStream_t s, int kind, int lgG, const Tile* tiles, int n, EnvDev* envs, int flags,
                             int hcap);
-// device build of the tile halos: LDS bytes for a sort buffer of ncap (power of
-// two) entries; counts[t] = halo size of tile t; fill writes halo[hoff..] and pk
-int pd_tile_build_lds(int ncap);
-void pd_launch_halo_count(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* counts, int ncap);
+// device build of the tile halos: LDS bytes for a hash set of hc (power of two,
+// >= twice a tile's bond slots) entries; counts[t] = halo size of tile t; fill
+// writes the sorted halo[hoff..] and pk
+int pd_tile_build_lds(int hc);
+void pd_launch_halo_count(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* counts, int hc);
 void pd_launch_halo_fill(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* halo, uint64_t* pk,
-                         int ncap);
+                         int hc);
 // deformed latch of n (env slot, block) items into out[n]
 void pd_launch_latch_flags(hipStream_t s, EnvDev* envs, const int2* items, int n, int32_t* out);

@@ -675,29 +675,16 @@ __global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ t
 
 // ---------------------------------------------------------------------------
 // Device build of the tiles' halos and packed streams (one workgroup per
-// tile; ncap = power of two >= the largest tile's bond slot count, the LDS
-// sort buffer).  Every bond of a freshly built family is intact.
+// tile).  The neighbour ids of the tile's bond slots are inserted into an LDS
+// hash set of hc slots (open addressing, hc >= twice the slot count, so every
+// probe sequence ends), the distinct ids are compacted, sorted (bitonic) and
+// each bond's index in the sorted halo is found by bisection.  Every bond of a
+// freshly built family is intact.
 
-// Loads the neighbour ids of every bond slot of tile T into buf (padding and
-// empty slots as INT32_MAX) and sorts them ascending (bitonic).
-__device__ void tile_sorted_ids(const Tile& T, const int32_t* __restrict__ col, int32_t* __restrict__ buf, int ncap) {
-  int pre[PD_TILE_CHUNKS + 1];
-  pre[0] = 0;
-  for (int k = 0; k < PD_TILE_CHUNKS; ++k) pre[k + 1] = pre[k] + (k < T.nch ? PD_WAVE * T.ch[k].width : 0);
-  for (int i = threadIdx.x; i < ncap; i += blockDim.x) {
-    int32_t v = INT32_MAX;
-    if (i < pre[T.nch]) {
-      int k = 0;
-      while (i >= pre[k + 1]) ++k;
-      const int32_t j = col[T.ch[k].slot0 + (uint32_t)(i - pre[k])];
-      v = j >= 0 ? j : INT32_MAX;
-    }
-    buf[i] = v;
-  }
-  __syncthreads();
-  for (int k = 2; k <= ncap; k <<= 1)
+__device__ __forceinline__ void lds_bitonic_sort(int32_t* __restrict__ buf, int n) {
+  for (int k = 2; k <= n; k <<= 1)
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < ncap; i += blockDim.x) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int ixj = i ^ j;
         if (ixj > i) {
           const int32_t a = buf[i], b = buf[ixj];
@@ -712,17 +699,36 @@ __device__ void tile_sorted_ids(const Tile& T, const int32_t* __restrict__ col, 
     }
 }
 
-__device__ __forceinline__ bool first_of_run(const int32_t* buf, int i) {
-  return buf[i] != INT32_MAX && (i == 0 || buf[i] != buf[i - 1]);
+// inserts every neighbour id of tile T's slots into table[hc] (-1 = empty)
+__device__ void tile_hash_ids(const Tile& T, const int32_t* __restrict__ col, int32_t* __restrict__ table, int hc) {
+  for (int i = threadIdx.x; i < hc; i += blockDim.x) table[i] = -1;
+  __syncthreads();
+  const int lg = __builtin_ctz(hc);
+  for (int k = 0; k < T.nch; ++k) {
+    const Chunk& ch = T.ch[k];
+    const int n = PD_WAVE * ch.width;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int32_t j = col[ch.slot0 + (uint32_t)i];
+      if (j < 0) continue;
+      uint32_t h = ((uint32_t)j * 2654435761u) >> (32 - lg);
+      while (true) {
+        const int32_t old = atomicCAS(&table[h], -1, j);
+        if (old == -1 || old == j) break;
+        h = (h + 1) & (uint32_t)(hc - 1);
+      }
+    }
+  }
+  __syncthreads();
 }
 
-// per-thread counts of run starts over contiguous segments, exclusive scan in LDS
-__device__ int segment_scan(const int32_t* buf, int ncap, int* part, int& seg0, int& seg1) {
-  const int per = ncap / (int)blockDim.x > 0 ? ncap / (int)blockDim.x : 1;
-  seg0 = min((int)threadIdx.x * per, ncap);
-  seg1 = (threadIdx.x + 1 == blockDim.x) ? ncap : min(seg0 + per, ncap);
+// exclusive positions of the occupied table slots over per-thread segments;
+// part[blockDim.x] receives the total
+__device__ int occupied_scan(const int32_t* table, int hc, int* part, int& seg0, int& seg1) {
+  const int per = hc / (int)blockDim.x > 0 ? hc / (int)blockDim.x : 1;
+  seg0 = min((int)threadIdx.x * per, hc);
+  seg1 = (threadIdx.x + 1 == blockDim.x) ? hc : min(seg0 + per, hc);
   int cnt = 0;
-  for (int i = seg0; i < seg1; ++i) cnt += first_of_run(buf, i) ? 1 : 0;
+  for (int i = seg0; i < seg1; ++i) cnt += table[i] >= 0 ? 1 : 0;
   part[threadIdx.x] = cnt;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -740,36 +746,40 @@ __device__ int segment_scan(const int32_t* buf, int ncap, int* part, int& seg0, 
 
 __global__ __launch_bounds__(256) void k_halo_count(const Tile* __restrict__ tiles, int ntiles,
                                                     const int32_t* __restrict__ col, int32_t* __restrict__ counts,
-                                                    int ncap) {
+                                                    int hc) {
   extern __shared__ int32_t sbuf[];
   if ((int)blockIdx.x >= ntiles) return;
   const Tile T = tiles[blockIdx.x];
-  int32_t* buf = sbuf;
-  int* part = sbuf + ncap;
-  tile_sorted_ids(T, col, buf, ncap);
+  int32_t* table = sbuf;
+  int* part = sbuf + hc;
+  tile_hash_ids(T, col, table, hc);
   int s0, s1;
-  segment_scan(buf, ncap, part, s0, s1);
+  occupied_scan(table, hc, part, s0, s1);
   if (threadIdx.x == 0) counts[blockIdx.x] = part[blockDim.x];
 }
 
 __global__ __launch_bounds__(256) void k_halo_fill(const Tile* __restrict__ tiles, int ntiles,
                                                    const int32_t* __restrict__ col, int32_t* __restrict__ halo,
-                                                   uint64_t* __restrict__ pk, int ncap) {
+                                                   uint64_t* __restrict__ pk, int hc) {
   extern __shared__ int32_t sbuf[];
   if ((int)blockIdx.x >= ntiles) return;
   const Tile T = tiles[blockIdx.x];
   int32_t* buf = sbuf;
-  int* part = sbuf + ncap;
-  tile_sorted_ids(T, col, buf, ncap);
+  int* part = sbuf + hc;
+  tile_hash_ids(T, col, buf, hc);
   int s0, s1;
-  int pos = segment_scan(buf, ncap, part, s0, s1);
+  int pos = occupied_scan(buf, hc, part, s0, s1);
   for (int i = s0; i < s1; ++i)
-    if (first_of_run(buf, i)) halo[T.hoff + (uint32_t)(pos++)] = buf[i];
+    if (buf[i] >= 0) halo[T.hoff + (uint32_t)(pos++)] = buf[i];
   __syncthreads();
-  // the compacted (sorted, unique) halo back into LDS for the index lookup
+  // the distinct ids back into LDS, sorted
   const int hn = T.hn;
-  for (int i = threadIdx.x; i < hn; i += blockDim.x) buf[i] = halo[T.hoff + (uint32_t)i];
+  int P = 1;
+  while (P < hn) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < hn ? halo[T.hoff + (uint32_t)i] : INT32_MAX;
   __syncthreads();
+  lds_bitonic_sort(buf, P);
+  for (int i = threadIdx.x; i < hn; i += blockDim.x) halo[T.hoff + (uint32_t)i] = buf[i];
   for (int k = 0; k < T.nch; ++k) {
     const Chunk& ch = T.ch[k];
     const int nt4 = (ch.width + 3) >> 2;
@@ -938,17 +948,15 @@ void pd_launch_tile_force32(hipStream_t s, int kind, int lgG, const Tile* tiles,
   }
 }
 
-int pd_tile_build_lds(int ncap) { return (int)sizeof(int32_t) * (ncap + 257); }
+int pd_tile_build_lds(int hc) { return (int)sizeof(int32_t) * (hc + 257); }
 
-void pd_launch_halo_count(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* counts, int ncap) {
+void pd_launch_halo_count(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* counts, int hc) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_halo_count, dim3(n), dim3(256), (size_t)pd_tile_build_lds(ncap), s, tiles, n, col, counts,
-                     ncap);
+  hipLaunchKernelGGL(k_halo_count, dim3(n), dim3(256), (size_t)pd_tile_build_lds(hc), s, tiles, n, col, counts, hc);
 }
 
 void pd_launch_halo_fill(hipStream_t s, const Tile* tiles, int n, const int32_t* col, int32_t* halo, uint64_t* pk,
-                         int ncap) {
+                         int hc) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_halo_fill, dim3(n), dim3(256), (size_t)pd_tile_build_lds(ncap), s, tiles, n, col, halo, pk,
-                     ncap);
+  hipLaunchKernelGGL(k_halo_fill, dim3(n), dim3(256), (size_t)pd_tile_build_lds(hc), s, tiles, n, col, halo, pk, hc);
 }
