@@ -419,7 +419,9 @@ def main():
                     help="batched: specimens timed one at a time in FP64 (0: all)")
     ap.add_argument("--served-requests", type=int, default=0,
                     help="served: number of requests; with other workloads, add a served summary")
-    ap.add_argument("--served-rate", type=float, default=40.0, help="served: Poisson arrivals per second")
+    ap.add_argument("--served-rate", type=float, default=200.0,
+                    help="served: Poisson arrivals per second (the default exceeds the GPU's service rate of about "
+                         "24 sims/s, so requests queue and the GPU stays saturated)")
     ap.add_argument("--served-lanes", type=int, default=4,
                     help="served: lanes per particle of the served envs (0: each env's own choice; "
                          "4 measured best for the default stream)")

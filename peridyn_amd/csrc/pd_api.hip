@@ -1395,7 +1395,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
     if (!e.contacts.empty() && !e.lists_current) contact_update(c, e);
   }
   if (nsteps == 0) return;
-  const int nl = prepare_latch(c, ids);
+  int nl = prepare_latch(c, ids);
   EnvDev* T = c->d_envs;
   Chunk* d = c->d_set;
   timed(c, PD_PROF_PROLOGUE, [&] { pd_launch_prologue(s, d, c->n_all, T); });
@@ -1461,7 +1461,10 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
     bool rebuild = false;
     if (st % nuf == 0)
       for (int64_t id : ids) rebuild = rebuild || !get_env(c, id).contacts.empty();
-    if (use_graph && !last && !rebuild && !logged) {
+    // with latch items pending, step 1 runs eagerly and the latches are read
+    // back after it (below), so that the captured steps carry no latch kernels
+    // once every block has latched (the latch never resets, general_material.jl:240-256)
+    if (use_graph && !last && !rebuild && !logged && !(st == 1 && nl)) {
       if (!graph.exec) {
         graph.capture(s, [&] {
           step_kernels(false, false);
@@ -1479,6 +1482,10 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
         if (!e.contacts.empty()) timed(c, PD_PROF_CONTACT, [&] { contact_update(c, e); });
       }
     book(last);
+    if (st == 1 && nl && !last) {
+      refresh_latch(c, ids, nl);
+      nl = prepare_latch(c, ids);
+    }
   }
   PD_HIP(hipGetLastError());
   PD_HIP(hipStreamSynchronize(s));  // also before the step graph is destroyed
