@@ -124,21 +124,18 @@ __device__ __forceinline__ void gst4(double4* p, uint32_t i, double4 v) {
 }
 
 // ---- fast-form FP64 reciprocals (not used by the reference-order form) -------
-// Hardware estimates refined by two Newton-Raphson steps: ~1 ulp, far inside the
-// 1e-10 agreement of the FP64 form, at a fraction of the cost of the
-// correctly-rounded division and square-root sequences.
+// Hardware estimates (relative error <= 5.3e-8 measured on gfx950) refined by
+// one Newton-Raphson step: <= 4.3e-15 relative (2.2e-15 for the reciprocal),
+// far inside the 1e-10 agreement of the FP64 form, at a fraction of the cost
+// of the correctly-rounded division and square-root sequences.
 __device__ __forceinline__ double pd_rsqrt(double a) {
-  double r = __builtin_amdgcn_rsq(a);
-  double e = __builtin_fma(-a * r, r, 1.0);
-  r = __builtin_fma(0.5 * r, e, r);
-  e = __builtin_fma(-a * r, r, 1.0);
+  const double r = __builtin_amdgcn_rsq(a);
+  const double e = __builtin_fma(-a * r, r, 1.0);
   return __builtin_fma(0.5 * r, e, r);
 }
 __device__ __forceinline__ double pd_rcp(double a) {
-  double r = __builtin_amdgcn_rcp(a);
-  double e = __builtin_fma(-a, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-a, r, 1.0);
+  const double r = __builtin_amdgcn_rcp(a);
+  const double e = __builtin_fma(-a, r, 1.0);
   return __builtin_fma(r, e, r);
 }
 
