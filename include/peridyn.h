@@ -129,7 +129,10 @@ int pd_env_schedule(void* ctx, int64_t env, int32_t* lanes_per_particle);
  * particle, whether the FP64 passes run tiled (halos staged in LDS; the
  * default for FP64 envs of single-type blocks with integer contact exponents
  * and 2..8 lanes per particle whose halos fit; PERIDYN_TILES=0 at commit turns
- * it off) and the largest tile halo.  Any pointer may be NULL. */
+ * it off): *tiled = 0 untiled, 1 tiled, 2 tiled with compact halo records
+ * (every reference position and volume is exactly an FP32 value, so they are
+ * staged as FP32 and read back unchanged) -- and the largest tile halo.  Any
+ * pointer may be NULL. */
 int pd_env_layout(void* ctx, int64_t env, int32_t* lanes_per_particle, int32_t* tiled, int32_t* max_halo);
 /* Uploads the env, builds every block's family on the GPU (cell list -> CSR),
  * the weighted volume m and the per-particle BC map.  PD_E_OVERFLOW where

@@ -322,11 +322,13 @@ class Env:
         return {"particles": n.value, "bonds": b.value, "device_bytes": by.value}
 
     def layout(self) -> dict:
-        """Bond layout: lanes per particle, tiled FP64 passes, largest tile halo."""
+        """Bond layout: lanes per particle, tiled FP64 passes, compact (FP32-exact
+        x and volume) halo records, largest tile halo."""
         g, t, h = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         self.ctx.check(self.ctx.lib.pd_env_layout(self.ctx.h, self.id, ctypes.byref(g), ctypes.byref(t),
                                                   ctypes.byref(h)))
-        return {"lanes_per_particle": g.value, "tiled": bool(t.value), "max_halo": h.value}
+        return {"lanes_per_particle": g.value, "tiled": bool(t.value), "compact_records": t.value == 2,
+                "max_halo": h.value}
 
     @property
     def lanes_per_particle(self) -> int:

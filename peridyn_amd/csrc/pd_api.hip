@@ -61,7 +61,7 @@ struct Env {
   std::vector<int32_t> perm, iperm;  // internal id -> reference index and back (empty: identity)
   // tiled FP64 passes: brick order requested at commit (want_tiles), tiles
   // built (tiled: the halos fit the LDS budget); brick of each internal particle
-  bool want_tiles = false, tiled = false;
+  bool want_tiles = false, tiled = false, xf32 = false;
   std::vector<int32_t> brick;
   std::vector<Tile> tiles;
   Tile* d_tiles = nullptr;                       // device copy (run sets copy from it)
@@ -100,6 +100,7 @@ struct Ctx {
   int ot_theta[2][4] = {}, ot_td[4] = {}, ot_force[3][2][4] = {};
   int set_hcap = 1;
   bool set_tiled = false;
+  int set_xf = 0;  // every tiled env of the set has FP32-exact x and volumes
   Chunk* d_set = nullptr;
   size_t set_cap = 0;
   int n_all = 0, n_theta = 0, n_td = 0, n_force[3] = {0, 0, 0};
@@ -475,9 +476,9 @@ constexpr size_t kTileLdsBudget = 80 * 1024;
 size_t tile_record_bytes(const Env& e) {
   size_t rb = 0;
   for (auto& b : e.blocks) {
-    rb = std::max<size_t>(rb, pd_tile_record_bytes(2, b.d.kind, e.precision));
-    if (b.d.kind != PD_BB) rb = std::max<size_t>(rb, pd_tile_record_bytes(0, b.d.kind, e.precision));
-    if (b.d.kind == PD_EP) rb = std::max<size_t>(rb, pd_tile_record_bytes(1, PD_EP, e.precision));
+    rb = std::max<size_t>(rb, pd_tile_record_bytes(2, b.d.kind, e.precision, 0));
+    if (b.d.kind != PD_BB) rb = std::max<size_t>(rb, pd_tile_record_bytes(0, b.d.kind, e.precision, 0));
+    if (b.d.kind == PD_EP) rb = std::max<size_t>(rb, pd_tile_record_bytes(1, PD_EP, e.precision, 0));
   }
   return rb;
 }
@@ -874,6 +875,14 @@ void commit_env(Ctx* c, Env& e) {
   D.has_ep = e.has_ep ? 1 : 0;
   D.lgG = e.lgG;
   D.tiled = e.tiled ? 1 : 0;
+  // tiled FP64 passes may stage x and volume as FP32 when that is exact
+  e.xf32 = e.tiled;
+  for (auto& b : e.blocks)
+    for (int64_t i = 0; e.xf32 && i < b.d.n; ++i) {
+      for (int d = 0; d < 3; ++d) e.xf32 = e.xf32 && (double)(float)b.x[3 * i + d] == b.x[3 * i + d];
+      e.xf32 = e.xf32 && (double)(float)b.vol[i] == b.vol[i];
+    }
+  D.xf32 = e.xf32 ? 1 : 0;
 
   // ---- blocks
   std::vector<BlockDev> bd(e.blocks.size());
@@ -1047,7 +1056,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   std::vector<Chunk> th, td, fk[3], thu, tdu, fku[3];
   std::vector<int32_t> slots;
   std::vector<int2> ranges;
-  bool has_ep = false, has_th = false, gpow = false, tiled = false;
+  bool has_ep = false, has_th = false, gpow = false, tiled = false, xf = true;
   int hcap = 1, n_all = 0, n_th = 0, n_td = 0, n_fk[3] = {0, 0, 0};
   // tile list sizes: theta [pc][lg], td [lg], force [kind][pc][lg]
   int s_th[2][4] = {}, s_td[4] = {}, s_fk[3][2][4] = {};
@@ -1080,6 +1089,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
       }
     if (e.tiled) {
       tiled = true;
+      xf = xf && e.xf32;
       hcap = std::max(hcap, (int)e.hmax);
       for (auto& tb : e.tile_blocks) {
         const int kind = e.blocks[tb[0]].d.kind;
@@ -1154,6 +1164,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   }
   c->set_hcap = hcap;
   c->set_tiled = tiled;
+  c->set_xf = xf ? 1 : 0;
   ensure_cap(c->d_slots, c->slots_cap, slots.size());
   PD_HIP(hipMemcpyAsync(c->d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, c->stream));
   ensure_cap(c->d_obs_ranges, c->obs_ranges_cap, ranges.size());
@@ -1408,12 +1419,12 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
   auto step_kernels = [&](bool last, bool logged) {
     if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part); });
     Tile* dt = c->d_tset;
-    const int hc = c->set_hcap;
+    const int hc = c->set_hcap, xf = c->set_xf;
     if (c->set_has_theta)
       timed(c, PD_PROF_THETA, [&] {
         if (c->n_theta_u) pd_launch_theta(s, prec, d + c->off_theta_u, c->n_theta_u, T);
         for (int lg = 1; lg <= 3; ++lg) {
-          if (c->nt_theta[0][lg]) pd_launch_tile_theta(s, lg, dt + c->ot_theta[0][lg], c->nt_theta[0][lg], T, hc);
+          if (c->nt_theta[0][lg]) pd_launch_tile_theta(s, lg, xf, dt + c->ot_theta[0][lg], c->nt_theta[0][lg], T, hc);
           if (c->nt_theta[1][lg])
             pd_launch_tile_theta32(s, lg, dt + c->ot_theta[1][lg], c->nt_theta[1][lg], T, hc);
         }
@@ -1422,7 +1433,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
       timed(c, PD_PROF_TD, [&] {
         if (c->n_td_u) pd_launch_td(s, prec, d + c->off_td_u, c->n_td_u, T);
         for (int lg = 1; lg <= 3; ++lg)
-          if (c->nt_td[lg]) pd_launch_tile_td(s, lg, dt + c->ot_td[lg], c->nt_td[lg], T, hc);
+          if (c->nt_td[lg]) pd_launch_tile_td(s, lg, xf, dt + c->ot_td[lg], c->nt_td[lg], T, hc);
       });
     double* rows = logged ? c->d_obs_rows + (size_t)row * nenv * 8 : nullptr;
     if (logged) pd_launch_observables(s, 0, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
@@ -1434,7 +1445,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
             pd_launch_force(s, k, prec, d + c->off_force_u[k], c->n_force_u[k], T, fl, c->set_gpow);
           for (int lg = 1; lg <= 3; ++lg) {
             if (c->nt_force[k][0][lg])
-              pd_launch_tile_force(s, k, lg, dt + c->ot_force[k][0][lg], c->nt_force[k][0][lg], T, fl, hc);
+              pd_launch_tile_force(s, k, lg, xf, dt + c->ot_force[k][0][lg], c->nt_force[k][0][lg], T, fl, hc);
             if (c->nt_force[k][1][lg])
               pd_launch_tile_force32(s, k, lg, dt + c->ot_force[k][1][lg], c->nt_force[k][1][lg], T, fl, hc);
           }
@@ -1723,7 +1734,7 @@ int pd_env_layout(void* ctx, int64_t env, int32_t* lanes_per_particle, int32_t* 
     Env& e = get_env(C(ctx), env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     if (lanes_per_particle) *lanes_per_particle = 1 << e.lgG;
-    if (tiled) *tiled = e.tiled ? 1 : 0;
+    if (tiled) *tiled = e.tiled ? (e.xf32 ? 2 : 1) : 0;
     if (max_halo) *max_halo = e.tiled ? e.hmax : 0;
   });
 }

@@ -90,6 +90,7 @@ struct EnvDev {
   int32_t N, nblocks, ncontacts, nbc;
   int32_t precision, cur, edp_cur, has_ep;
   int32_t lgG, tiled;        // log2(lanes per particle) of the bond layout; tiles built
+  int32_t xf32, pad1;        // every x and volume is exactly an FP32 value (compact tile records)
   double dt;
   int64_t time_step;         // completed steps (env.time_step)
   double4* x;

@@ -68,13 +68,15 @@ void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags, bool gpow);
 
 // tiled passes (pd_tile.hip): LDS bytes per halo record for pass (0 theta,
-// 1 td, 2 force), kind and form; hcap = largest halo of the launched tiles
-int pd_tile_record_bytes(int pass, int kind, int prec);
+// 1 td, 2 force), kind and form; xf = every x and volume of the launched
+// envs is exactly an FP32 value (compact records); hcap = largest halo of the
+// launched tiles
+int pd_tile_record_bytes(int pass, int kind, int prec, int xf);
 void pd_tile_allow_lds(size_t bytes);
-void pd_launch_tile_theta(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap);
-void pd_launch_tile_td(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap);
-void pd_launch_tile_force(hipStream_t s, int kind, int lgG, const Tile* tiles, int n, EnvDev* envs, int flags,
-                          int hcap);
+void pd_launch_tile_theta(hipStream_t s, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs, int hcap);
+void pd_launch_tile_td(hipStream_t s, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs, int hcap);
+void pd_launch_tile_force(hipStream_t s, int kind, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs,
+                          int flags, int hcap);
 // pk intact bits := state for the listed chunks of tiled envs
 void pd_launch_pk_sync(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 // FP32-compensated tiled passes (BB / OSB blocks of PD_PREC_FP32C envs)

@@ -19,11 +19,36 @@ namespace {
 
 enum { TP_THETA = 0, TP_TD = 1, TP_FORCE = 2 };
 
-// LDS fields: 0..2 x, 3 volume, 4..6 y, 7 theta, 8 1/m, 9 signed td_norm2, 10 C
+// LDS halo records of the FP64 passes: one array per field.  Double fields in
+// order: x0 x1 x2 vol y0 y1 y2 | theta 1/m | Cs, where Cs = C with the
+// neighbour's "has yielded" flag as its sign (C >= 0).  With XF (every x and
+// volume of the set's envs is exactly an FP32 value) x0 x1 x2 vol go to a float
+// region after the doubles instead -- the values read back are the same
+// doubles -- which shrinks the records and lets more tiles reside per CU.
 template <int PASS, int KIND>
-__host__ __device__ constexpr int tile_fields() {
-  return PASS == TP_THETA ? 7 : PASS == TP_TD ? 9 : (KIND == PD_BB ? 7 : KIND == PD_OSB ? 9 : 11);
+__host__ __device__ constexpr int tile_extra() {  // double fields beyond x, vol, y
+  return PASS == TP_THETA ? 0 : PASS == TP_TD ? 2 : (KIND == PD_BB ? 0 : KIND == PD_OSB ? 2 : 3);
 }
+template <int PASS, int KIND, bool XF>
+__host__ __device__ constexpr int tile_record_bytes() {
+  return XF ? 8 * (3 + tile_extra<PASS, KIND>()) + 4 * 4 : 8 * (7 + tile_extra<PASS, KIND>());
+}
+
+template <bool XF, int NX>  // NX = extra double fields staged
+struct Rec {
+  const double* sh;
+  int hcap;
+  __device__ __forceinline__ Rec(const double* s, int h) : sh(s), hcap(h) {}
+  __device__ __forceinline__ const float* fs() const { return (const float*)(sh + (3 + NX) * hcap); }
+  __device__ __forceinline__ double x(int k, int j) const {
+    return XF ? (double)fs()[k * hcap + j] : sh[k * hcap + j];
+  }
+  __device__ __forceinline__ double vol(int j) const { return XF ? (double)fs()[3 * hcap + j] : sh[3 * hcap + j]; }
+  __device__ __forceinline__ double y(int k, int j) const { return sh[((XF ? 0 : 4) + k) * hcap + j]; }
+  __device__ __forceinline__ double th(int j) const { return sh[(XF ? 3 : 7) * hcap + j]; }
+  __device__ __forceinline__ double rm(int j) const { return sh[(XF ? 4 : 8) * hcap + j]; }
+  __device__ __forceinline__ double cs(int j) const { return sh[(XF ? 5 : 9) * hcap + j]; }
+};
 
 struct TileWave {
   Chunk ch;
@@ -36,11 +61,13 @@ struct TileWave {
 // then the records, then the LDS stores).
 constexpr int kStageR = 4;
 
-template <int NF>
+template <bool XF, int NX>
 __device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap) {
   const double4* __restrict__ x = E.x;
   const double4* __restrict__ y = E.y[E.cur];
   const int32_t* __restrict__ halo = E.halo;
+  constexpr int Y0 = XF ? 0 : 4, T0 = XF ? 3 : 7;
+  float* __restrict__ fs = (float*)(sh + (3 + NX) * hcap);
   for (int i0 = threadIdx.x; i0 < T.hn; i0 += 256 * kStageR) {
     uint32_t id[kStageR];
 #pragma unroll
@@ -54,27 +81,31 @@ __device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, doubl
       if (i0 + 256 * r < T.hn) {
         a[r] = gld4(x, id[r]);
         b[r] = gld4(y, id[r]);
-        if (NF > 7) c[r] = gld4(E.aux, id[r]);
+        if (NX > 0) c[r] = gld4(E.aux, id[r]);
       }
     }
 #pragma unroll
     for (int r = 0; r < kStageR; ++r) {
       const int i = i0 + 256 * r;
       if (i < T.hn) {
-        sh[i] = a[r].x;
-        sh[hcap + i] = a[r].y;
-        sh[2 * hcap + i] = a[r].z;
-        sh[3 * hcap + i] = a[r].w;
-        sh[4 * hcap + i] = b[r].x;
-        sh[5 * hcap + i] = b[r].y;
-        sh[6 * hcap + i] = b[r].z;
-        if (NF > 7) {
-          sh[7 * hcap + i] = c[r].x;
-          sh[8 * hcap + i] = c[r].y;
-          if (NF > 9) {
-            sh[9 * hcap + i] = c[r].z;
-            sh[10 * hcap + i] = c[r].w;
-          }
+        if (XF) {
+          fs[i] = (float)a[r].x;
+          fs[hcap + i] = (float)a[r].y;
+          fs[2 * hcap + i] = (float)a[r].z;
+          fs[3 * hcap + i] = (float)a[r].w;
+        } else {
+          sh[i] = a[r].x;
+          sh[hcap + i] = a[r].y;
+          sh[2 * hcap + i] = a[r].z;
+          sh[3 * hcap + i] = a[r].w;
+        }
+        sh[Y0 * hcap + i] = b[r].x;
+        sh[(Y0 + 1) * hcap + i] = b[r].y;
+        sh[(Y0 + 2) * hcap + i] = b[r].z;
+        if (NX > 0) {
+          sh[T0 * hcap + i] = c[r].x;
+          sh[(T0 + 1) * hcap + i] = c[r].y;
+          if (NX > 2) sh[(T0 + 2) * hcap + i] = __builtin_copysign(c[r].w, c[r].z);
         }
       }
     }
@@ -119,9 +150,17 @@ struct PkStream {
   }
 };
 
+// bond geometry against the staged neighbour j (expressions of the untiled kernels)
+template <class R>
+__device__ __forceinline__ void tile_geom(const R& rc, int j, const double4& xp, const double4& yp, double& X1,
+                                          double& X2, double& X3, double& Y1, double& Y2, double& Y3) {
+  X1 = rc.x(0, j) - xp.x; X2 = rc.x(1, j) - xp.y; X3 = rc.x(2, j) - xp.z;
+  Y1 = rc.y(0, j) - yp.x; Y2 = rc.y(1, j) - yp.y; Y3 = rc.y(2, j) - yp.z;
+}
+
 // ---------------------------------------------------------------------------
 // dilatation theta (src/peri.jl:155-172), OSB and EP blocks
-template <int LG>
+template <int LG, bool XF>
 __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ tiles, int ntiles,
                                                     EnvDev* __restrict__ envs, int hcap) {
   extern __shared__ double sh[];
@@ -132,10 +171,11 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   if (B.kind == PD_BB || !B.deformed) return;
-  stage_halo<7>(E, T, sh, hcap);
+  stage_halo<XF, 0>(E, T, sh, hcap);
   __syncthreads();
   TileWave w;
   if (!tile_wave<LG>(T, w)) return;
+  const Rec<XF, 0> rc(sh, hcap);
   const uint64_t* __restrict__ pk = E.pk;
   const int p = w.active ? w.p : w.ch.first;
   const double4 xp = gld4(E.x, p), yp = gld4(E.y[E.cur], p);
@@ -150,12 +190,12 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
       const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
       if (!(e & 0x8000u)) continue;
       const int j = (int)(e & 0x7fffu);
-      const double X1 = sh[j] - xp.x, X2 = sh[hcap + j] - xp.y, X3 = sh[2 * hcap + j] - xp.z;
-      const double Y1 = sh[4 * hcap + j] - yp.x, Y2 = sh[5 * hcap + j] - yp.y, Y3 = sh[6 * hcap + j] - yp.z;
+      double X1, X2, X3, Y1, Y2, Y3;
+      tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
       const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
       const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
       const double ext = yy2 * pd_rsqrt(yy2) - xx2 * pd_rsqrt(xx2);
-      acc += ext * sh[3 * hcap + j];
+      acc += ext * rc.vol(j);
     }
   }
   acc = group_sum(acc, C);
@@ -169,7 +209,7 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
 // ---------------------------------------------------------------------------
 // td_norm2 (elasto_plastic.jl:168-172,282-295; src/peri.jl:411-445), EP
 // blocks; breaks bonds with s >= s0 (state and the pk intact bit)
-template <int LG>
+template <int LG, bool XF>
 __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles, int ntiles,
                                                  EnvDev* __restrict__ envs, int hcap) {
   extern __shared__ double sh[];
@@ -180,10 +220,11 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   if (B.kind != PD_EP || !B.deformed) return;
-  stage_halo<9>(E, T, sh, hcap);
+  stage_halo<XF, 2>(E, T, sh, hcap);
   __syncthreads();
   TileWave w;
   if (!tile_wave<LG>(T, w)) return;
+  const Rec<XF, 2> rc(sh, hcap);
   const uint64_t* __restrict__ pk = E.pk;
   uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
   uint8_t* __restrict__ st = E.state;
@@ -224,20 +265,20 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
       if (!(e & 0x8000u)) continue;
       const int j = (int)(e & 0x7fffu);
       const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
-      const double X1 = sh[j] - xp.x, X2 = sh[hcap + j] - xp.y, X3 = sh[2 * hcap + j] - xp.z;
-      const double Y1 = sh[4 * hcap + j] - yp.x, Y2 = sh[5 * hcap + j] - yp.y, Y3 = sh[6 * hcap + j] - yp.z;
+      double X1, X2, X3, Y1, Y2, Y3;
+      tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
       const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
       const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
       const double inv_x = pd_rsqrt(xx2);
       const double xij = xx2 * inv_x;
       const double ext = yy2 * pd_rsqrt(yy2) - xij;
       if (ext * inv_x < s0) {
-        const double th_j = sh[7 * hcap + j], rm_j = sh[8 * hcap + j];
+        const double th_j = rc.th(j), rm_j = rc.rm(j);
         const double sm = ap.y + rm_j;
         const double twm = ap.x * ap.y + th_j * rm_j;
         const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
         const double td = g15 * (ed - eo[u]) * (inv_x * sm);
-        acc += td * td * sh[3 * hcap + j];
+        acc += td * td * rc.vol(j);
       } else {
         gst(st, slot, (uint8_t)0);
         gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
@@ -269,9 +310,10 @@ struct EpGrp {
   uint32_t need;  // bit u: the reverse bond's plastic extension is needed
 };
 
+template <class R>
 __device__ __forceinline__ void ep_stage_a(EpGrp& G, uint64_t cw, uint32_t base, int t4, bool yielded_p,
                                            const double* __restrict__ edp_in, const uint32_t* __restrict__ rev,
-                                           const double* __restrict__ sh, int hcap) {
+                                           const R& rc) {
   G.cw = cw;
   G.need = 0;
 #pragma unroll
@@ -280,7 +322,7 @@ __device__ __forceinline__ void ep_stage_a(EpGrp& G, uint64_t cw, uint32_t base,
     const bool on = (e & 0x8000u) != 0;
     const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
     G.eo[u] = (on && yielded_p) ? gld(edp_in, slot) : 0.0;
-    const bool nd = on && __builtin_signbit(sh[9 * hcap + (int)(e & 0x7fffu)]);
+    const bool nd = on && __builtin_signbit(rc.cs((int)(e & 0x7fffu)));
     G.rv[u] = nd ? gld(rev, slot) : 0u;
     G.need |= (nd ? 1u : 0u) << u;
   }
@@ -291,10 +333,11 @@ __device__ __forceinline__ void ep_stage_b(EpGrp& G, const double* __restrict__ 
   for (int u = 0; u < 4; ++u) G.er[u] = ((G.need >> u) & 1u) ? gld(edp_in, G.rv[u]) : 0.0;
 }
 
+template <class R>
 __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, uint32_t wb, const double4& xp,
                                          const double4& yp, const double4& ap, bool yielded_p, double s0_pp,
-                                         double K0, double G0, double inv_den, const double* __restrict__ sh,
-                                         int hcap, double* __restrict__ edp_out, uint8_t* __restrict__ st,
+                                         double K0, double G0, double inv_den, const R& rc,
+                                         double* __restrict__ edp_out, uint8_t* __restrict__ st,
                                          uint16_t* __restrict__ pk16, double& a1, double& a2, double& a3) {
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -302,8 +345,8 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
     if (!(e & 0x8000u)) continue;
     const int j = (int)(e & 0x7fffu);
     const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
-    const double X1 = sh[j] - xp.x, X2 = sh[hcap + j] - xp.y, X3 = sh[2 * hcap + j] - xp.z;
-    const double Y1 = sh[4 * hcap + j] - yp.x, Y2 = sh[5 * hcap + j] - yp.y, Y3 = sh[6 * hcap + j] - yp.z;
+    double X1, X2, X3, Y1, Y2, Y3;
+    tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
     const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
     const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
     const double inv_x = pd_rsqrt(xx2);
@@ -312,12 +355,12 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
     const double yij = yy2 * inv_y;
     const double ext = yij - xij;
     if (ext * inv_x < s0_pp) {
-      const double th_j = sh[7 * hcap + j], rm_j = sh[8 * hcap + j];
+      const double th_j = rc.th(j), rm_j = rc.rm(j);
       const double sm = ap.y + rm_j;
       const double twm = ap.x * ap.y + th_j * rm_j;
       const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
       const double g15 = 15.0 * G0 * (inv_x * sm);
-      const double Cp = ap.w, Cj = sh[10 * hcap + j];
+      const double Cp = ap.w, Cj = __builtin_fabs(rc.cs(j));
       const double ede = ed - G.eo[u];
       double td_own = g15 * ede;
       double e_new = G.eo[u];
@@ -329,7 +372,7 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
       double td_rev = g15 * (ed - G.er[u]);
       if (!(Cj > 1.0)) td_rev *= Cj;
       const double t = 3.0 * K0 * twm + 0.5 * (td_own + td_rev);
-      const double xx = t * inv_y * (sh[3 * hcap + j] * inv_den);
+      const double xx = t * inv_y * (rc.vol(j) * inv_den);
       a1 += xx * Y1;
       a2 += xx * Y2;
       a3 += xx * Y3;
@@ -345,11 +388,12 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
 // fused force density + velocity-Verlet (material part of
 // general_material.jl:296-369 with the plugins' t_ij, then contacts and the
 // step epilogue of pd_wave.h)
-template <int KIND, int LG>
+template <int KIND, int LG, bool XF>
 __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
                                                     EnvDev* __restrict__ envs, int flags, int hcap) {
   extern __shared__ double sh[];
   constexpr int C = 64 >> LG;
+  constexpr int NX = tile_extra<TP_FORCE, KIND>();
   const int t = pd_xcd_block(gridDim.x);
   if (t >= ntiles) return;
   const Tile& T = tiles[t];
@@ -361,10 +405,11 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
   EpiPre ep;  // epilogue inputs in flight during the staging and the bond loop
   epi_load_a(E, p, ep);
-  if (deformed) stage_halo<tile_fields<TP_FORCE, KIND>()>(E, T, sh, hcap);
+  if (deformed) stage_halo<XF, NX>(E, T, sh, hcap);
   __syncthreads();
   if (!has) return;
   epi_load_b(E, ep);
+  const Rec<XF, NX> rc(sh, hcap);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 xp = gld4(E.x, p), yp = gld4(y, p);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -373,126 +418,75 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
     uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
     uint8_t* __restrict__ st = E.state;
     const double inv_den = 1.0 / B.density[0];
-    if (KIND == PD_EP) {
+    const int nt4 = (w.ch.width + 3) >> 2;
+    const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
+    const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
+    const double s0_pp = B.s0[0], K0 = B.K[0], G0 = B.G[0];
+    if constexpr (KIND == PD_EP) {
       const double4 ap = E.aux[p];
       const double* __restrict__ edp_in = E.edp[E.edp_cur];
       double* __restrict__ edp_out = E.edp[E.edp_cur ^ 1];
       const bool yielded_p = pd_has_yielded(ap);
-      const double s0_pp = B.s0[0], K0 = B.K[0], G0 = B.G[0];
-      const int nt4 = (w.ch.width + 3) >> 2;
-      const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
-      const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
       PkStream<2> ps(pk, wb, nt4);
       EpGrp g0, g1, g2;
-      ep_stage_a(g0, ps.next(0), base, 0, yielded_p, edp_in, E.rev, sh, hcap);
+      ep_stage_a(g0, ps.next(0), base, 0, yielded_p, edp_in, E.rev, rc);
       ep_stage_b(g0, edp_in);
-      ep_stage_a(g1, ps.next(1), base, 1, yielded_p, edp_in, E.rev, sh, hcap);
+      ep_stage_a(g1, ps.next(1), base, 1, yielded_p, edp_in, E.rev, rc);
       for (int t4 = 0; t4 < nt4; ++t4) {
         ep_stage_b(g1, edp_in);
-        ep_stage_a(g2, ps.next(t4 + 2), base, t4 + 2, yielded_p, edp_in, E.rev, sh, hcap);
-        ep_group(g0, t4, base, wb, xp, yp, ap, yielded_p, s0_pp, K0, G0, inv_den, sh, hcap, edp_out, st, pk16,
-                 a1, a2, a3);
+        ep_stage_a(g2, ps.next(t4 + 2), base, t4 + 2, yielded_p, edp_in, E.rev, rc);
+        ep_group(g0, t4, base, wb, xp, yp, ap, yielded_p, s0_pp, K0, G0, inv_den, rc, edp_out, st, pk16, a1, a2,
+                 a3);
         g0 = g1;
         g1 = g2;
       }
-      a1 = group_sum(a1, C);
-      a2 = group_sum(a2, C);
-      a3 = group_sum(a3, C);
     } else {
-    double4 ap = make_double4(0.0, 0.0, 0.0, 0.0);
-    const double* __restrict__ edp_in = nullptr;
-    double* __restrict__ edp_out = nullptr;
-    if (KIND != PD_BB) ap = E.aux[p];
-    bool yielded_p = false;
-    if (KIND == PD_EP) {
-      edp_in = E.edp[E.edp_cur];
-      edp_out = E.edp[E.edp_cur ^ 1];
-      yielded_p = pd_has_yielded(ap);
-    }
-    const double c11 = B.c11;
-    const double s0_pp = B.s0[0];
-    const double K0 = B.K[0], G0 = B.G[0];
-    const int nt4 = (w.ch.width + 3) >> 2;
-    const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
-    const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
-    PkStream<3> ps(pk, wb, nt4);
-    for (int t4 = 0; t4 < nt4; ++t4) {
-      const uint64_t cw = ps.next(t4);
-      double eo[4], er[4];
-      if (KIND == PD_EP) {
-        // own plastic extensions (0 until p yields) and the reverse bonds' (0
-        // until the neighbour yields), all loads of the group issued first
-        uint32_t rv[4];
-        bool need[4];
+      // bond-based and ordinary state-based blocks
+      double4 ap = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (KIND != PD_BB) ap = E.aux[p];
+      const double c11 = B.c11;
+      PkStream<3> ps(pk, wb, nt4);
+      for (int t4 = 0; t4 < nt4; ++t4) {
+        const uint64_t cw = ps.next(t4);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
-          const bool on = (e & 0x8000u) != 0;
+          if (!(e & 0x8000u)) continue;
+          const int j = (int)(e & 0x7fffu);
           const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
-          eo[u] = (on && yielded_p) ? gld(edp_in, slot) : 0.0;
-          need[u] = on && __builtin_signbit(sh[9 * hcap + (int)(e & 0x7fffu)]);
-          rv[u] = need[u] ? gld(E.rev, slot) : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) er[u] = need[u] ? gld(edp_in, rv[u]) : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
-        if (!(e & 0x8000u)) continue;
-        const int j = (int)(e & 0x7fffu);
-        const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
-        const double X1 = sh[j] - xp.x, X2 = sh[hcap + j] - xp.y, X3 = sh[2 * hcap + j] - xp.z;
-        const double Y1 = sh[4 * hcap + j] - yp.x, Y2 = sh[5 * hcap + j] - yp.y, Y3 = sh[6 * hcap + j] - yp.z;
-        const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-        const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
-        const double inv_x = pd_rsqrt(xx2);
-        const double inv_y = pd_rsqrt(yy2);
-        const double xij = xx2 * inv_x;
-        const double yij = yy2 * inv_y;
-        const double ext = yij - xij;
-        if (ext * inv_x < s0_pp) {
-          double t;
-          if (KIND == PD_BB) {
-            t = ext * inv_x * c11;
-          } else {
-            const double th_j = sh[7 * hcap + j], rm_j = sh[8 * hcap + j];
-            const double sm = ap.y + rm_j;
-            const double twm = ap.x * ap.y + th_j * rm_j;
-            if (KIND == PD_OSB) {
-              t = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
+          double X1, X2, X3, Y1, Y2, Y3;
+          tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
+          const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
+          const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+          const double inv_x = pd_rsqrt(xx2);
+          const double inv_y = pd_rsqrt(yy2);
+          const double xij = xx2 * inv_x;
+          const double yij = yy2 * inv_y;
+          const double ext = yij - xij;
+          if (ext * inv_x < s0_pp) {
+            double tt;
+            if (KIND == PD_BB) {
+              tt = ext * inv_x * c11;
             } else {
-              const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
-              const double g15 = 15.0 * G0 * (inv_x * sm);
-              const double Cp = ap.w, Cj = sh[10 * hcap + j];
-              const double ede = ed - eo[u];
-              double td_own = g15 * ede;
-              double e_new = eo[u];
-              if (!(Cp > 1.0)) {
-                td_own *= Cp;
-                e_new = eo[u] + ede * (1.0 - Cp);
-              }
-              if (yielded_p) gst(edp_out, slot, e_new);  // else e_new = 0 = stored
-              double td_rev = g15 * (ed - er[u]);
-              if (!(Cj > 1.0)) td_rev *= Cj;
-              t = 3.0 * K0 * twm + 0.5 * (td_own + td_rev);
+              const double th_j = rc.th(j), rm_j = rc.rm(j);
+              const double sm = ap.y + rm_j;
+              const double twm = ap.x * ap.y + th_j * rm_j;
+              tt = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
             }
+            const double xx = tt * inv_y * (rc.vol(j) * inv_den);
+            a1 += xx * Y1;
+            a2 += xx * Y2;
+            a3 += xx * Y3;
+          } else {
+            gst(st, slot, (uint8_t)0);
+            gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
           }
-          const double xx = t * inv_y * (sh[3 * hcap + j] * inv_den);
-          a1 += xx * Y1;
-          a2 += xx * Y2;
-          a3 += xx * Y3;
-        } else {
-          gst(st, slot, (uint8_t)0);
-          gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
-          if (KIND == PD_EP && yielded_p) gst(edp_out, slot, eo[u]);
         }
       }
     }
     a1 = group_sum(a1, C);
     a2 = group_sum(a2, C);
     a3 = group_sum(a3, C);
-    }
   }
   if (!w.active || w.g != 0) return;
   add_contacts_pre<false>(E, p, yp, y, ep, a1, a2, a3);
@@ -828,20 +822,20 @@ __global__ __launch_bounds__(256) void k_pk_sync(const Chunk* __restrict__ chunk
   }
 }
 
-template <int LG>
+template <int LG, bool XF>
 void launch_theta_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const size_t lds = sizeof(double) * tile_fields<TP_THETA, 0>() * (size_t)hcap;
-  hipLaunchKernelGGL(k_tile_theta<LG>, dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+  const size_t lds = (size_t)tile_record_bytes<TP_THETA, 0, XF>() * (size_t)hcap;
+  hipLaunchKernelGGL((k_tile_theta<LG, XF>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
 }
-template <int LG>
+template <int LG, bool XF>
 void launch_td_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const size_t lds = sizeof(double) * tile_fields<TP_TD, 0>() * (size_t)hcap;
-  hipLaunchKernelGGL(k_tile_td<LG>, dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+  const size_t lds = (size_t)tile_record_bytes<TP_TD, 0, XF>() * (size_t)hcap;
+  hipLaunchKernelGGL((k_tile_td<LG, XF>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
 }
-template <int KIND, int LG>
+template <int KIND, int LG, bool XF>
 void launch_force_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  const size_t lds = sizeof(double) * tile_fields<TP_FORCE, KIND>() * (size_t)hcap;
-  hipLaunchKernelGGL((k_tile_force<KIND, LG>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
+  const size_t lds = (size_t)tile_record_bytes<TP_FORCE, KIND, XF>() * (size_t)hcap;
+  hipLaunchKernelGGL((k_tile_force<KIND, LG, XF>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
 }
 
 template <int LG>
@@ -856,32 +850,41 @@ void launch_force32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, in
 }
 
 // dynamic LDS beyond the default 64 KiB per workgroup must be enabled per kernel
+template <class K>
+void allow_lds(K* k, size_t bytes) {
+  (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+template <int LG, bool XF>
+void allow_lds_fp64(size_t bytes) {
+  allow_lds(k_tile_theta<LG, XF>, bytes);
+  allow_lds(k_tile_td<LG, XF>, bytes);
+  allow_lds(k_tile_force<PD_BB, LG, XF>, bytes);
+  allow_lds(k_tile_force<PD_OSB, LG, XF>, bytes);
+  allow_lds(k_tile_force<PD_EP, LG, XF>, bytes);
+}
 template <int LG>
 void allow_lds_lg(size_t bytes) {
-  (void)hipFuncSetAttribute((const void*)k_tile_theta32<LG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_tile_force32<PD_BB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_tile_force32<PD_OSB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_tile_theta<LG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_tile_td<LG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_tile_force<PD_BB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_tile_force<PD_OSB, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_tile_force<PD_EP, LG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bytes);
+  allow_lds(k_tile_theta32<LG>, bytes);
+  allow_lds(k_tile_force32<PD_BB, LG>, bytes);
+  allow_lds(k_tile_force32<PD_OSB, LG>, bytes);
+  allow_lds_fp64<LG, false>(bytes);
+  allow_lds_fp64<LG, true>(bytes);
+}
+
+template <int PASS, bool XF>
+int fp64_record_bytes(int kind) {
+  return kind == PD_BB ? tile_record_bytes<PASS, PD_BB, XF>()
+                       : kind == PD_OSB ? tile_record_bytes<PASS, PD_OSB, XF>() : tile_record_bytes<PASS, PD_EP, XF>();
 }
 
 }  // namespace
 
-int pd_tile_record_bytes(int pass, int kind, int prec) {
+int pd_tile_record_bytes(int pass, int kind, int prec, int xf) {
   if (prec == PD_PREC_FP32C && kind != PD_EP && pass != 1)
     return (int)(3 * sizeof(double) + ((pass == 2 && kind == PD_OSB) ? 6 : 4) * sizeof(float));
-  if (pass == 0) return 8 * tile_fields<TP_THETA, 0>();
-  if (pass == 1) return 8 * tile_fields<TP_TD, 0>();
-  return 8 * (kind == PD_BB ? tile_fields<TP_FORCE, PD_BB>()
-                            : kind == PD_OSB ? tile_fields<TP_FORCE, PD_OSB>() : tile_fields<TP_FORCE, PD_EP>());
+  if (pass == 0) return xf ? fp64_record_bytes<TP_THETA, true>(kind) : fp64_record_bytes<TP_THETA, false>(kind);
+  if (pass == 1) return xf ? fp64_record_bytes<TP_TD, true>(kind) : fp64_record_bytes<TP_TD, false>(kind);
+  return xf ? fp64_record_bytes<TP_FORCE, true>(kind) : fp64_record_bytes<TP_FORCE, false>(kind);
 }
 
 void pd_tile_allow_lds(size_t bytes) {
@@ -892,34 +895,42 @@ void pd_tile_allow_lds(size_t bytes) {
   allow_lds_lg<3>(bytes);
 }
 
-void pd_launch_tile_theta(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  if (n <= 0) return;
-  if (lgG == 1) launch_theta_lg<1>(s, tiles, n, envs, hcap);
-  else if (lgG == 2) launch_theta_lg<2>(s, tiles, n, envs, hcap);
-  else launch_theta_lg<3>(s, tiles, n, envs, hcap);
-}
-
-void pd_launch_tile_td(hipStream_t s, int lgG, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  if (n <= 0) return;
-  if (lgG == 1) launch_td_lg<1>(s, tiles, n, envs, hcap);
-  else if (lgG == 2) launch_td_lg<2>(s, tiles, n, envs, hcap);
-  else launch_td_lg<3>(s, tiles, n, envs, hcap);
-}
-
-void pd_launch_tile_force(hipStream_t s, int kind, int lgG, const Tile* tiles, int n, EnvDev* envs, int flags,
-                          int hcap) {
-  if (n <= 0) return;
-#define PD_TF(KIND)                                                              \
-  do {                                                                           \
-    if (lgG == 1) launch_force_lg<KIND, 1>(s, tiles, n, envs, flags, hcap);      \
-    else if (lgG == 2) launch_force_lg<KIND, 2>(s, tiles, n, envs, flags, hcap); \
-    else launch_force_lg<KIND, 3>(s, tiles, n, envs, flags, hcap);               \
+#define PD_LG_XF(F, ...)                                      \
+  do {                                                        \
+    if (xf) {                                                 \
+      if (lgG == 1) F<1, true>(__VA_ARGS__);                  \
+      else if (lgG == 2) F<2, true>(__VA_ARGS__);             \
+      else F<3, true>(__VA_ARGS__);                           \
+    } else {                                                  \
+      if (lgG == 1) F<1, false>(__VA_ARGS__);                 \
+      else if (lgG == 2) F<2, false>(__VA_ARGS__);            \
+      else F<3, false>(__VA_ARGS__);                          \
+    }                                                         \
   } while (0)
-  if (kind == PD_BB) PD_TF(PD_BB);
-  else if (kind == PD_OSB) PD_TF(PD_OSB);
-  else PD_TF(PD_EP);
-#undef PD_TF
+
+void pd_launch_tile_theta(hipStream_t s, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  if (n <= 0) return;
+  PD_LG_XF(launch_theta_lg, s, tiles, n, envs, hcap);
 }
+
+void pd_launch_tile_td(hipStream_t s, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  if (n <= 0) return;
+  PD_LG_XF(launch_td_lg, s, tiles, n, envs, hcap);
+}
+
+template <int LG, bool XF>
+void launch_force_any(int kind, hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
+  if (kind == PD_BB) launch_force_lg<PD_BB, LG, XF>(s, tiles, n, envs, flags, hcap);
+  else if (kind == PD_OSB) launch_force_lg<PD_OSB, LG, XF>(s, tiles, n, envs, flags, hcap);
+  else launch_force_lg<PD_EP, LG, XF>(s, tiles, n, envs, flags, hcap);
+}
+
+void pd_launch_tile_force(hipStream_t s, int kind, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs,
+                          int flags, int hcap) {
+  if (n <= 0) return;
+  PD_LG_XF(launch_force_any, kind, s, tiles, n, envs, flags, hcap);
+}
+#undef PD_LG_XF
 
 void pd_launch_pk_sync(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) {
   if (n <= 0) return;

@@ -92,6 +92,51 @@ def test_tiled_equals_untiled(pd, kind, lanes):
     b.close()
 
 
+def jittered(spec, scale=1e-3, seed=5):
+    """Reference positions and volumes that are not FP32 values: the tiled
+    passes stage full FP64 halo records instead of the compact ones."""
+    rng = np.random.default_rng(seed)
+    for b in spec["blocks"]:
+        x = np.asarray(b["x"], dtype=np.float64)
+        b["x"] = np.ascontiguousarray(x + rng.uniform(-scale, scale, size=x.shape) + 1e-9)
+        b["vol"] = np.asarray(b["vol"], dtype=np.float64) * (1.0 + 1e-10)
+    return spec
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_tiled_full_records_equal_untiled(pd, kind, lanes):
+    say("tiled full records vs untiled", kind, lanes)
+    spec = ep_tensile() if kind == "ep" else W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.002)
+    spec = jittered(spec)
+    a = make(pd, spec, True, lanes)
+    b = make(pd, spec, False, lanes)
+    assert not a.layout()["compact_records"]
+    a.run(40, 1)
+    b.run(40, 1)
+    same_state(a, b)
+    a.close()
+    b.close()
+
+
+def test_compact_and_full_records_batched(pd):
+    """One batch with a compact-record env and a full-record env: the set
+    launches the full records for both; each result equals its run alone."""
+    say("compact + full batch")
+    s1 = ep_tensile()
+    s2 = jittered(ep_tensile(n=8))
+    alone = [make(pd, s1, True, 4), make(pd, s2, True, 4)]
+    assert alone[0].layout()["compact_records"] and not alone[1].layout()["compact_records"]
+    for e in alone:
+        e.run(25, 1)
+    mix = [make(pd, s1, True, 4), make(pd, s2, True, 4)]
+    pd.run_batch(mix, 25, 1)
+    for a, b in zip(alone, mix):
+        same_state(a, b)
+    for e in alone + mix:
+        e.close()
+
+
 @pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
 def test_tiled_within_fp64_bar_of_oracle(pd, orc, kind):
     say("tiled vs oracle", kind)
