@@ -108,8 +108,10 @@ int pd_ctx_create(int device, void** ctx);
 int pd_ctx_destroy(void* ctx);
 const char* pd_last_error(void* ctx);
 int pd_sync(void* ctx);
-/* free / total device memory of the context's GPU (hipMemGetInfo); no
- * reference counterpart: the serving loop admits simulations by footprint */
+/* free / total device memory of the context's GPU (hipMemGetInfo; free
+ * includes the memory the context keeps from destroyed envs for its next
+ * ones, at most 8 GiB); no reference counterpart: the serving loop admits
+ * simulations by footprint */
 int pd_ctx_mem_info(void* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 /* ---- environment (Env, src/environment.jl:166-235) ------------------- */

@@ -22,6 +22,68 @@
 
 namespace {
 
+// Device memory of one context's envs.  Released blocks are kept by size class
+// and handed to later allocations of the same class: hipMalloc maps pages and
+// hipFree waits for the whole device, both slow next to admitting a small env.
+// All work of a context is ordered on its one stream, so a block reused by a
+// later allocation is only touched by work enqueued after its last use.
+struct DevMem {
+  static constexpr size_t kMaxCached = 8ull << 30;   // bytes kept at most
+  static constexpr size_t kMaxBlock = 1ull << 30;    // larger blocks are freed at once
+  std::multimap<size_t, void*> spare;                // size class -> block
+  size_t cached = 0;
+  // powers of two up to 1 MiB, then steps of 1/8 of the power of two below
+  static size_t size_class(size_t bytes) {
+    size_t p = 256;
+    while (p < bytes && p < (1u << 20)) p <<= 1;
+    if (p >= bytes) return p;
+    size_t q = 1u << 20;
+    while ((q << 1) <= bytes) q <<= 1;
+    const size_t step = q >> 3;
+    return (bytes + step - 1) / step * step;
+  }
+  void* get(size_t cls) {
+    auto it = spare.find(cls);
+    if (it != spare.end()) {
+      void* p = it->second;
+      spare.erase(it);
+      cached -= cls;
+      return p;
+    }
+    void* p = nullptr;
+    hipError_t r = hipMalloc(&p, cls);
+    if (r == hipErrorOutOfMemory && cached > 0) {
+      (void)hipGetLastError();
+      trim(0);
+      r = hipMalloc(&p, cls);
+    }
+    if (r != hipSuccess) {
+      (void)hipGetLastError();
+      throw PdError(r == hipErrorOutOfMemory ? PD_E_NOMEM : PD_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(r));
+    }
+    return p;
+  }
+  void put(void* p, size_t cls) {
+    if (!p) return;
+    if (cls > kMaxBlock) {
+      (void)hipFree(p);
+      return;
+    }
+    spare.emplace(cls, p);
+    cached += cls;
+    if (cached > kMaxCached) trim(kMaxCached / 2);
+  }
+  // frees spare blocks, largest first, until at most keep bytes are cached
+  void trim(size_t keep) {
+    while (cached > keep && !spare.empty()) {
+      auto it = std::prev(spare.end());
+      (void)hipFree(it->second);
+      cached -= it->first;
+      spare.erase(it);
+    }
+  }
+};
+
 struct HostBlock {
   pd_block_desc d{};
   std::vector<double> x, y, v, vol;
@@ -72,13 +134,15 @@ struct Env {
   int32_t* d_self = nullptr;  // this env's slot, for single-env bookkeeping launches
   EnvDev dev{};
   std::vector<ContactDev> cdev;
-  std::vector<void*> allocs;
+  DevMem* mem = nullptr;  // the context's device memory
+  std::vector<std::pair<void*, size_t>> allocs;  // block, size class
   size_t bytes = 0;
 };
 
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  DevMem mem;  // env arrays and build temporaries
   std::string err;
   std::map<int64_t, Env*> envs;
   int64_t next_id = 1;
@@ -174,21 +238,15 @@ void drain_profile(Ctx* c) {
 
 template <typename T>
 T* dalloc(Env& e, size_t n) {
-  if (n == 0) n = 1;
-  void* p = nullptr;
-  hipError_t r = hipMalloc(&p, n * sizeof(T));
-  if (r != hipSuccess) {
-    (void)hipGetLastError();
-    throw PdError(r == hipErrorOutOfMemory ? PD_E_NOMEM : PD_E_HIP,
-                  std::string("hipMalloc: ") + hipGetErrorString(r));
-  }
-  e.allocs.push_back(p);
-  e.bytes += n * sizeof(T);
+  const size_t cls = DevMem::size_class(std::max<size_t>(n, 1) * sizeof(T));
+  void* p = e.mem->get(cls);
+  e.allocs.emplace_back(p, cls);
+  e.bytes += cls;
   return static_cast<T*>(p);
 }
 
 void free_env_device(Env& e) {
-  for (void* p : e.allocs) (void)hipFree(p);
+  for (auto& a : e.allocs) e.mem->put(a.first, a.second);
   e.allocs.clear();
   e.bytes = 0;
 }
@@ -223,23 +281,20 @@ std::vector<double> copy_mat(const double* p, int nt, const char* name, bool req
 struct FamTmp {
   FamilyGrid g{};
   int32_t *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cursor = nullptr, *items = nullptr;
-  std::vector<void*> mem;
+  DevMem* dm = nullptr;
+  std::vector<std::pair<void*, size_t>> mem;
 };
 
 void fam_free(FamTmp& t) {
-  for (void* p : t.mem) (void)hipFree(p);
+  for (auto& a : t.mem) t.dm->put(a.first, a.second);
   t.mem.clear();
 }
 
 template <typename T>
 T* talloc(FamTmp& t, size_t n) {
-  void* p = nullptr;
-  hipError_t r = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
-  if (r != hipSuccess) {
-    (void)hipGetLastError();
-    throw PdError(PD_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(r));
-  }
-  t.mem.push_back(p);
+  const size_t cls = DevMem::size_class(std::max<size_t>(n, 1) * sizeof(T));
+  void* p = t.dm->get(cls);
+  t.mem.emplace_back(p, cls);
   return static_cast<T*>(p);
 }
 
@@ -500,15 +555,17 @@ bool build_tiles_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& 
   Tile* dt = nullptr;
   int32_t* dcnt = nullptr;
   std::vector<int32_t> cnt(nt);
+  const size_t cls_t = DevMem::size_class(sizeof(Tile) * std::max(nt, 1));
+  const size_t cls_c = DevMem::size_class(sizeof(int32_t) * std::max(nt, 1));
   auto release = [&] {
-    if (dt) (void)hipFree(dt);
-    if (dcnt) (void)hipFree(dcnt);
+    c->mem.put(dt, cls_t);
+    c->mem.put(dcnt, cls_c);
     dt = nullptr;
     dcnt = nullptr;
   };
   try {
-    PD_HIP(hipMalloc(&dt, sizeof(Tile) * std::max(nt, 1)));
-    PD_HIP(hipMalloc(&dcnt, sizeof(int32_t) * std::max(nt, 1)));
+    dt = static_cast<Tile*>(c->mem.get(cls_t));
+    dcnt = static_cast<int32_t*>(c->mem.get(cls_c));
     PD_HIP(hipMemcpyAsync(dt, tt.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
     const int hc = std::max(2 * ncap, 512);  // LDS hash set, at most half full
     pd_launch_halo_count(s, dt, nt, e.dev.col, dcnt, hc);
@@ -745,6 +802,7 @@ void commit_env(Ctx* c, Env& e) {
 
   // ---- family: degrees
   std::vector<FamTmp> tmp(e.blocks.size());
+  for (auto& t : tmp) t.dm = &c->mem;
   std::vector<int32_t> deg(N);
   try {
     for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
@@ -827,7 +885,7 @@ void commit_env(Ctx* c, Env& e) {
       HostBlock& b = e.blocks[bi];
       FamTmp& t = tmp[bi];
       pd_launch_fill(s, D.x, b.first, (int)b.d.n, t.g, b.d.horizon, t.cell_of, t.cell_start, t.cell_count,
-                     t.items, D.pslot0, e.lgG, d_ref, D.col, D.state);
+                     t.items, D.pslot0, e.lgG, d_ref, D.col, D.state, b.maxdeg);
       pd_launch_wvol(s, D.x, b.first, (int)b.d.n, D.pslot0, e.lgG, D.deg, D.col, D.m);
     }
     D.aux = dalloc<double4>(e, N);
@@ -1585,6 +1643,7 @@ int pd_ctx_destroy(void* ctx) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->envs) { free_env_device(*kv.second); delete kv.second; }
+  c->mem.trim(0);
   if (c->d_envs) (void)hipFree(c->d_envs);
   if (c->d_set) (void)hipFree(c->d_set);
   if (c->d_tset) (void)hipFree(c->d_tset);
@@ -1613,6 +1672,7 @@ int pd_ctx_mem_info(void* ctx, int64_t* free_bytes, int64_t* total_bytes) {
     PD_HIP(hipSetDevice(C(ctx)->device));
     size_t fr = 0, tot = 0;
     PD_HIP(hipMemGetInfo(&fr, &tot));
+    fr += C(ctx)->mem.cached;  // kept for this context's next envs
     if (free_bytes) *free_bytes = (int64_t)fr;
     if (total_bytes) *total_bytes = (int64_t)tot;
   });
@@ -1626,6 +1686,7 @@ int pd_env_create(void* ctx, double dt, int32_t precision, int64_t* env) {
       throw PdError(PD_E_ARG, "unsupported precision form " + std::to_string(precision));
     Ctx* c = C(ctx);
     Env* e = new Env();
+    e->mem = &c->mem;
     e->id = c->next_id++;
     e->dt = dt;
     e->precision = precision;

@@ -149,14 +149,6 @@ __global__ void k_cfill(const EnvDev* __restrict__ envs, int slot, int cidx, int
   C.cell_items[C.cell_start[c] + atomicAdd(&cursor[c], 1)] = j;
 }
 
-// Wavefront-level LDS ordering: every lane's earlier LDS writes are visible to
-// every lane of the wavefront after this point.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ int wave_add(int v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -168,10 +160,6 @@ __device__ __forceinline__ int wave_min(int v) {
 __device__ __forceinline__ int wave_max(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
   return v;
-}
-// number of set bits of m below this lane
-__device__ __forceinline__ int lanes_below(uint64_t m) {
-  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Visits every block-2 member within `distance` of p (27-cell stencil) with the

@@ -68,6 +68,19 @@ __device__ __forceinline__ int pd_wave_id() {
 }
 __device__ __forceinline__ int pd_lane() { return threadIdx.x & 63; }
 
+// Wavefront-level LDS ordering: every lane's earlier LDS writes are visible to
+// every lane of the wavefront after this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// number of set bits of m below this lane
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // XCD-aware block id: consecutive logical blocks (spatial neighbours sharing
 // neighbour data) land on the same XCD / L2.  Hardware dispatches workgroup b
 // to XCD b % 8.

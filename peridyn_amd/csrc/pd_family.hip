@@ -102,6 +102,100 @@ __global__ void k_pairs(const double4* __restrict__ x, int first, int n, FamilyG
   for (int a = 0; a < k; ++a) state[pd_slot(base, a, lgG)] = 1;
 }
 
+// The same pairs with one wavefront per particle (4 per workgroup): lanes 0..26
+// read the ranges of the 27 cells around the particle's cell, and the cells'
+// items, concatenated in stencil order, are dealt to the lanes 64 at a time
+// with k_pairs' predicate.  The degree pass counts the hits; the fill pass
+// gathers them into LDS and writes each at its rank by reference index
+// (ascending; the indices are distinct), the row k_pairs' insertion sort gives.
+// Rows of more than kFamRowCap bonds are left to k_pairs.
+constexpr int kFamRowCap = 1024;
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_pairs_wave(
+    const double4* __restrict__ x, int first, int n, FamilyGrid g, double horizon,
+    const int32_t* __restrict__ cell_of, const int32_t* __restrict__ cell_start,
+    const int32_t* __restrict__ cell_count, const int32_t* __restrict__ items, const uint32_t* __restrict__ pslot0,
+    int lgG, const int32_t* __restrict__ ref, int32_t* __restrict__ deg, int32_t* __restrict__ col,
+    uint8_t* __restrict__ state) {
+  __shared__ int32_t s_q[FILL ? 4 * kFamRowCap : 1];
+  __shared__ int32_t s_r[FILL ? 4 * kFamRowCap : 1];
+  const int wv = pd_wave_id(), lane = pd_lane();
+  const int i = (int)blockIdx.x * 4 + wv;
+  if (i >= n) return;
+  const int p = first + i;
+  const double4 xp = x[p];
+  const double hh = horizon * horizon;
+  const int c = cell_of[i];
+  const int cx = c % g.nc[0];
+  const int cy = (c / g.nc[0]) % g.nc[1];
+  const int cz = c / (g.nc[0] * g.nc[1]);
+  int my_s = 0, my_n = 0;
+  if (lane < 27) {
+    const int xx = cx + lane % 3 - 1, yy = cy + (lane / 3) % 3 - 1, zz = cz + lane / 9 - 1;
+    if (xx >= 0 && xx < g.nc[0] && yy >= 0 && yy < g.nc[1] && zz >= 0 && zz < g.nc[2]) {
+      const int cc = xx + g.nc[0] * (yy + g.nc[1] * zz);
+      my_s = cell_start[cc];
+      my_n = cell_count[cc];
+    }
+  }
+  int incl = my_n;  // inclusive scan of the cell sizes over the lanes
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  const int total = __shfl(incl, 63);
+  const int excl = incl - my_n;
+  int32_t* qb = s_q + (FILL ? wv * kFamRowCap : 0);
+  int32_t* rb = s_r + (FILL ? wv * kFamRowCap : 0);
+  int k = 0;
+  for (int t0 = 0; t0 < total; t0 += 64) {
+    const int t = t0 + lane;
+    // the cell holding concatenated index t: the last of lanes 0..26 whose
+    // first index is <= t (empty cells share their successor's first index)
+    int cl = 0;
+    for (int b = 16; b > 0; b >>= 1) {
+      const int cand = cl + b;
+      const int ex = __shfl(excl, cand < 27 ? cand : 26);
+      if (cand < 27 && ex <= t) cl = cand;
+    }
+    const int cs = __shfl(my_s, cl), ce = __shfl(excl, cl);
+    bool hit = false;
+    int q = -1;
+    if (t < total) {
+      q = items[cs + (t - ce)];
+      if (q != p) {
+        const double4 xq = x[q];
+        const double a = xp.x - xq.x, b = xp.y - xq.y, cz2 = xp.z - xq.z;
+        hit = (a * a + b * b + cz2 * cz2) < hh;
+      }
+    }
+    const uint64_t m = __ballot(hit);
+    if (FILL && hit) {
+      const int pos = k + lanes_below(m);
+      if (pos < kFamRowCap) {
+        qb[pos] = q;
+        rb[pos] = ref[q];
+      }
+    }
+    k += __popcll(m);
+  }
+  if (!FILL) {
+    if (lane == 0) deg[p] = k;
+    return;
+  }
+  if (k > kFamRowCap) return;  // the host sends such blocks to k_pairs
+  wave_sync();
+  const uint32_t base = pslot0[p];
+  for (int a = lane; a < k; a += 64) {
+    const int32_t ra = rb[a];
+    int r = 0;
+    for (int b = 0; b < k; ++b) r += rb[b] < ra ? 1 : 0;
+    col[pd_slot(base, r, lgG)] = qb[a];
+    state[pd_slot(base, r, lgG)] = 1;
+  }
+}
+
 // weighted_volume (src/peri.jl:342-354), summed in the family's descending order
 __global__ void k_wvol(const double4* __restrict__ x, int first, int n, const uint32_t* __restrict__ pslot0,
                        int lgG, const int32_t* __restrict__ deg, const int32_t* __restrict__ col,
@@ -169,7 +263,7 @@ void pd_launch_degree(hipStream_t s, const double4* x, int first, int n, FamilyG
                       const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
                       const int32_t* cell_items, int32_t* deg) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_pairs<false>, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
+  hipLaunchKernelGGL(k_pairs_wave<false>, dim3((n + 3) / 4), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
                      cell_start, cell_count, cell_items, (const uint32_t*)nullptr, 0, (const int32_t*)nullptr, deg,
                      (int32_t*)nullptr, (uint8_t*)nullptr);
 }
@@ -177,10 +271,14 @@ void pd_launch_degree(hipStream_t s, const double4* x, int first, int n, FamilyG
 void pd_launch_fill(hipStream_t s, const double4* x, int first, int n, FamilyGrid g, double horizon,
                     const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
                     const int32_t* cell_items, const uint32_t* pslot0, int lgG, const int32_t* ref, int32_t* col,
-                    uint8_t* state) {
+                    uint8_t* state, int maxdeg) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_pairs<true>, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
-                     cell_start, cell_count, cell_items, pslot0, lgG, ref, (int32_t*)nullptr, col, state);
+  if (maxdeg <= kFamRowCap)
+    hipLaunchKernelGGL(k_pairs_wave<true>, dim3((n + 3) / 4), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
+                       cell_start, cell_count, cell_items, pslot0, lgG, ref, (int32_t*)nullptr, col, state);
+  else
+    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks_for(n)), dim3(256), 0, s, x, first, n, g, horizon, cell_of,
+                       cell_start, cell_count, cell_items, pslot0, lgG, ref, (int32_t*)nullptr, col, state);
 }
 
 void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0, int lgG,

@@ -37,7 +37,7 @@ void pd_launch_degree(hipStream_t s, const double4* x, int first, int n, FamilyG
 void pd_launch_fill(hipStream_t s, const double4* x, int first, int n, FamilyGrid g, double horizon,
                     const int32_t* cell_of, const int32_t* cell_start, const int32_t* cell_count,
                     const int32_t* cell_items, const uint32_t* pslot0, int lgG, const int32_t* ref, int32_t* col,
-                    uint8_t* state);
+                    uint8_t* state, int maxdeg);
 void pd_launch_wvol(hipStream_t s, const double4* x, int first, int n, const uint32_t* pslot0, int lgG,
                     const int32_t* deg, const int32_t* col, double* m);
 void pd_launch_rev(hipStream_t s, int first, int n, const uint32_t* pslot0, int lgG, const int32_t* ref,
