@@ -165,6 +165,13 @@ struct Ctx {
   int set_hcap = 1;
   bool set_tiled = false;
   int set_xf = 0;  // every tiled env of the set has FP32-exact x and volumes
+  // pair contacts of the set, rebuilt together (pd_launch_contact_update_batch)
+  int2* d_citems = nullptr;
+  size_t citems_cap = 0;
+  int32_t* d_cstarts = nullptr;
+  size_t cstarts_cap = 0;
+  int n_citems = 0;
+  int cgrid[PD_CB_NCLS] = {};
   Chunk* d_set = nullptr;
   size_t set_cap = 0;
   int n_all = 0, n_theta = 0, n_td = 0, n_force[3] = {0, 0, 0};
@@ -1223,6 +1230,40 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   c->set_hcap = hcap;
   c->set_tiled = tiled;
   c->set_xf = xf ? 1 : 0;
+  // pair-contact items: (slot, contact index) and per-class workgroup offsets
+  {
+    std::vector<int2> items;
+    std::vector<std::array<int, 3>> sz;  // n1, n2, ncell_cap
+    for (int64_t id : ids) {
+      Env& e = get_env(c, id);
+      for (size_t k = 0; k < e.cdev.size(); ++k)
+        if (!e.cdev[k].self_contact) {
+          items.push_back(make_int2(e.slot, (int)k));
+          sz.push_back({e.cdev[k].n1, e.cdev[k].n2, e.cdev[k].ncell_cap});
+        }
+    }
+    const int ni = (int)items.size();
+    c->n_citems = ni;
+    if (ni > 0) {
+      std::vector<int32_t> st((size_t)PD_CB_NCLS * (ni + 1));
+      for (int cls = 0; cls < PD_CB_NCLS; ++cls) {
+        int64_t run = 0;
+        for (int k = 0; k < ni; ++k) {
+          st[(size_t)cls * (ni + 1) + k] = (int32_t)run;
+          run += pd_contact_batch_blocks(cls, sz[k][0], sz[k][1], sz[k][2]);
+        }
+        if (run >= (1LL << 31)) throw PdError(PD_E_ARG, "contact batch too large");
+        st[(size_t)cls * (ni + 1) + ni] = (int32_t)run;
+        c->cgrid[cls] = (int)run;
+      }
+      ensure_cap(c->d_citems, c->citems_cap, (size_t)ni);
+      ensure_cap(c->d_cstarts, c->cstarts_cap, st.size());
+      PD_HIP(hipMemcpyAsync(c->d_citems, items.data(), sizeof(int2) * ni, hipMemcpyHostToDevice, c->stream));
+      PD_HIP(hipMemcpyAsync(c->d_cstarts, st.data(), sizeof(int32_t) * st.size(), hipMemcpyHostToDevice,
+                            c->stream));
+      PD_HIP(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+    }
+  }
   ensure_cap(c->d_slots, c->slots_cap, slots.size());
   PD_HIP(hipMemcpyAsync(c->d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, c->stream));
   ensure_cap(c->d_obs_ranges, c->obs_ranges_cap, ranges.size());
@@ -1294,6 +1335,18 @@ void contact_update(Ctx* c, Env& e) {
     if (e.cdev[k].self_contact) pd_launch_self_contact_update(c->stream, c->d_envs, e.slot, e.cdev[k], (int)k);
     else pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k);
   e.lists_current = true;
+}
+
+// contact_update of every env of the current run set: the pair contacts of all
+// of them in one batched sequence, self contacts per env
+void contact_update_set(Ctx* c, const std::vector<int64_t>& ids) {
+  pd_launch_contact_update_batch(c->stream, c->d_envs, c->d_citems, c->d_cstarts, c->n_citems, c->cgrid);
+  for (int64_t id : ids) {
+    Env& e = get_env(c, id);
+    for (size_t k = 0; k < e.cdev.size(); ++k)
+      if (e.cdev[k].self_contact) pd_launch_self_contact_update(c->stream, c->d_envs, e.slot, e.cdev[k], (int)k);
+    if (!e.contacts.empty()) e.lists_current = true;
+  }
 }
 
 // one force evaluation (update_acc!) for a chunk subset of one env
@@ -1516,11 +1569,12 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
                      PD_B_STEP | (last ? 0 : PD_B_FLIP_Y) | (c->set_has_ep ? PD_B_FLIP_EDP : 0));
     });
   };
-  // Steps that are neither the last, nor logged, nor followed by a contact
-  // rebuild are identical launch sequences (all step-dependent state lives on
-  // the device), so they replay one captured HIP graph.  Profiled runs launch
-  // eagerly so that every kernel class keeps its own events.
-  StepGraph graph;
+  // Steps that are neither the last nor logged are identical launch sequences
+  // (all step-dependent state lives on the device), so they replay captured
+  // HIP graphs: one for plain steps, one for steps followed by the contact
+  // rebuild (whose kernels read the positions on the device; no host sync).
+  // Profiled runs launch eagerly so that every kernel class keeps its own events.
+  StepGraph graph, rgraph;
   const bool use_graph = graphs_enabled() && !c->prof && nsteps >= 3;
   for (int64_t st = 1; st <= nsteps; ++st) {
     const bool last = (st == nsteps);
@@ -1533,23 +1587,21 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
     // with latch items pending, step 1 runs eagerly and the latches are read
     // back after it (below), so that the captured steps carry no latch kernels
     // once every block has latched (the latch never resets, general_material.jl:240-256)
-    if (use_graph && !last && !rebuild && !logged && !(st == 1 && nl)) {
-      if (!graph.exec) {
-        graph.capture(s, [&] {
+    if (use_graph && !last && !logged && !(st == 1 && nl)) {
+      StepGraph& g = rebuild ? rgraph : graph;
+      if (!g.exec) {
+        g.capture(s, [&] {
           step_kernels(false, false);
+          if (rebuild) contact_update_set(c, ids);
           book(false);
         });
       }
-      PD_HIP(hipGraphLaunch(graph.exec, s));
+      PD_HIP(hipGraphLaunch(g.exec, s));
       continue;
     }
     step_kernels(last, logged);
     if (logged) ++row;
-    if (rebuild)
-      for (int64_t id : ids) {
-        Env& e = get_env(c, id);
-        if (!e.contacts.empty()) timed(c, PD_PROF_CONTACT, [&] { contact_update(c, e); });
-      }
+    if (rebuild) timed(c, PD_PROF_CONTACT, [&] { contact_update_set(c, ids); });
     book(last);
     if (st == 1 && nl && !last) {
       refresh_latch(c, ids, nl);
@@ -1654,6 +1706,8 @@ int pd_ctx_destroy(void* ctx) {
   if (c->d_obs_ranges) (void)hipFree(c->d_obs_ranges);
   if (c->d_obs_part) (void)hipFree(c->d_obs_part);
   if (c->d_obs_rows) (void)hipFree(c->d_obs_rows);
+  if (c->d_citems) (void)hipFree(c->d_citems);
+  if (c->d_cstarts) (void)hipFree(c->d_cstarts);
   for (auto& r : c->pending) { c->pool.push_back(r.a); c->pool.push_back(r.b); }
   for (hipEvent_t ev : c->pool) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);

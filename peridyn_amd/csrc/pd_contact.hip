@@ -26,23 +26,20 @@ __device__ __forceinline__ double red_op(int d, double a, double b) {
   return ((d / 3) % 2 == 0) ? fmin(a, b) : fmax(a, b);
 }
 
-// stage 1: per-block partial min/max over both member sets
-__global__ __launch_bounds__(256) void k_cbox1(const EnvDev* __restrict__ envs, int slot, int cidx,
-                                               double* __restrict__ part) {
-  const EnvDev& E = envs[slot];
-  const ContactDev& C = E.contacts[cidx];
+// stage 1: per-block partial min/max over both member sets (block b of nb)
+__device__ void cbox1_body(const EnvDev& E, const ContactDev& C, double* __restrict__ part, int b, int nb) {
   const double4* y = cur_y(E);
   double v[12];
   for (int d = 0; d < 3; ++d) {
     v[d] = INFINITY; v[3 + d] = -INFINITY; v[6 + d] = INFINITY; v[9 + d] = -INFINITY;
   }
-  const int stride = gridDim.x * blockDim.x;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < C.n1; i += stride) {
+  const int stride = nb * blockDim.x;
+  for (int i = b * blockDim.x + threadIdx.x; i < C.n1; i += stride) {
     const double4 p = y[C.idx1[i]];
     v[0] = fmin(v[0], p.x); v[1] = fmin(v[1], p.y); v[2] = fmin(v[2], p.z);
     v[3] = fmax(v[3], p.x); v[4] = fmax(v[4], p.y); v[5] = fmax(v[5], p.z);
   }
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < C.n2; i += stride) {
+  for (int i = b * blockDim.x + threadIdx.x; i < C.n2; i += stride) {
     const double4 p = y[C.idx2[i]];
     v[6] = fmin(v[6], p.x); v[7] = fmin(v[7], p.y); v[8] = fmin(v[8], p.z);
     v[9] = fmax(v[9], p.x); v[10] = fmax(v[10], p.y); v[11] = fmax(v[11], p.z);
@@ -58,16 +55,19 @@ __global__ __launch_bounds__(256) void k_cbox1(const EnvDev* __restrict__ envs, 
     for (int d = 0; d < 12; ++d) {
       double r = sh[d][0];
       for (int q = 1; q < 4; ++q) r = red_op(d, r, sh[d][q]);
-      part[blockIdx.x * 12 + d] = r;
+      part[b * 12 + d] = r;
     }
   }
 }
 
-// stage 2 (one wavefront): box, overlap flag and cell grid (<= ncell_cap cells)
-__global__ __launch_bounds__(64) void k_cbox2(const EnvDev* __restrict__ envs, int slot, int cidx,
-                                              const double* __restrict__ part, int nparts) {
+__global__ __launch_bounds__(256) void k_cbox1(const EnvDev* __restrict__ envs, int slot, int cidx,
+                                               double* __restrict__ part) {
   const EnvDev& E = envs[slot];
-  const ContactDev& C = E.contacts[cidx];
+  cbox1_body(E, E.contacts[cidx], part, blockIdx.x, gridDim.x);
+}
+
+// stage 2 (one wavefront): box, overlap flag and cell grid (<= ncell_cap cells)
+__device__ void cbox2_body(const ContactDev& C, const double* __restrict__ part, int nparts) {
   double v[12];
   for (int d = 0; d < 12; ++d) v[d] = ((d / 3) % 2 == 0) ? INFINITY : -INFINITY;
   for (int b = threadIdx.x; b < nparts; b += 64)
@@ -103,6 +103,11 @@ __global__ __launch_bounds__(64) void k_cbox2(const EnvDev* __restrict__ envs, i
   box[8] = nc[0]; box[9] = nc[1]; box[10] = nc[2];
 }
 
+__global__ __launch_bounds__(64) void k_cbox2(const EnvDev* __restrict__ envs, int slot, int cidx,
+                                              const double* __restrict__ part, int nparts) {
+  cbox2_body(envs[slot].contacts[cidx], part, nparts);
+}
+
 __global__ void k_izero(int32_t* __restrict__ a, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0;
 }
@@ -123,10 +128,7 @@ __device__ __forceinline__ int cell_id(const double4& p, const double* box, int*
 }
 
 // bin in-box block-2 members
-__global__ void k_cbin(const EnvDev* __restrict__ envs, int slot, int cidx) {
-  const EnvDev& E = envs[slot];
-  const ContactDev& C = E.contacts[cidx];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void cbin_body(const EnvDev& E, const ContactDev& C, int j) {
   if (j >= C.n2) return;
   const double* box = C.box;
   const double4 p = cur_y(E)[C.idx2[j]];
@@ -139,14 +141,22 @@ __global__ void k_cbin(const EnvDev* __restrict__ envs, int slot, int cidx) {
   C.cell_of[j] = c;
 }
 
-__global__ void k_cfill(const EnvDev* __restrict__ envs, int slot, int cidx, int32_t* __restrict__ cursor) {
+__global__ void k_cbin(const EnvDev* __restrict__ envs, int slot, int cidx) {
   const EnvDev& E = envs[slot];
-  const ContactDev& C = E.contacts[cidx];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  cbin_body(E, E.contacts[cidx], blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// cursor = the second half of cell_start (ncell_cap entries)
+__device__ void cfill_body(const ContactDev& C, int j) {
   if (j >= C.n2) return;
   const int c = C.cell_of[j];
   if (c < 0) return;
+  int32_t* cursor = C.cell_start + C.ncell_cap;
   C.cell_items[C.cell_start[c] + atomicAdd(&cursor[c], 1)] = j;
+}
+
+__global__ void k_cfill(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  cfill_body(envs[slot].contacts[cidx], blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __device__ __forceinline__ int wave_add(int v) {
@@ -253,14 +263,8 @@ __device__ void wave_sort_desc(int32_t* buf, int n, int lane) {
 // are compacted in scan order into the wavefront's LDS buffer and sorted there;
 // with more than kCandCap candidates the max_neighs-th largest id is found by
 // bisection over the candidates' id range first.
-__global__ __launch_bounds__(256) void k_csearch(const EnvDev* __restrict__ envs, int slot, int cidx) {
-  __shared__ int32_t cand[4][kCandCap];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const EnvDev& E = envs[slot];
-  const ContactDev& C = E.contacts[cidx];
-  const int i = blockIdx.x * 4 + wv;
-  if (i >= C.n1) return;  // whole wavefront; the kernel uses no workgroup barrier
-  int32_t* buf = cand[wv];
+__device__ void csearch_body(const EnvDev& E, const ContactDev& C, int i, int lane, int32_t* buf) {
+  if (i >= C.n1) return;  // whole wavefront; the kernels use no workgroup barrier
   const double* box = C.box;
   const double4* y = cur_y(E);
   const double4 p = y[C.idx1[i]];
@@ -327,12 +331,16 @@ __global__ __launch_bounds__(256) void k_csearch(const EnvDev* __restrict__ envs
   for (int k = lane; k < cnt; k += 64) atomicAdd(&C.rev_cnt[own[k]], 1);
 }
 
+__global__ __launch_bounds__(256) void k_csearch(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  __shared__ int32_t cand[4][kCandCap];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const EnvDev& E = envs[slot];
+  csearch_body(E, E.contacts[cidx], blockIdx.x * 4 + wv, lane, cand[wv]);
+}
+
 // One wavefront per block-1 member: append i to the transposed list of each of
 // its contacts (order fixed afterwards by k_crev_sort).
-__global__ __launch_bounds__(256) void k_crev_fill(const EnvDev* __restrict__ envs, int slot, int cidx) {
-  const EnvDev& E = envs[slot];
-  const ContactDev& C = E.contacts[cidx];
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+__device__ void crev_fill_body(const ContactDev& C, int i, int lane) {
   if (i >= C.n1) return;
   const int cnt = C.own_cnt[i];
   const int32_t* own = C.own + (size_t)i * C.max_neighs;
@@ -342,13 +350,12 @@ __global__ __launch_bounds__(256) void k_crev_fill(const EnvDev* __restrict__ en
   }
 }
 
+__global__ __launch_bounds__(256) void k_crev_fill(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  crev_fill_body(envs[slot].contacts[cidx], blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
 // One wavefront per block-2 member: its transposed list sorted ascending.
-__global__ __launch_bounds__(256) void k_crev_sort(const EnvDev* __restrict__ envs, int slot, int cidx) {
-  __shared__ int32_t cand[4][kCandCap];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const EnvDev& E = envs[slot];
-  const ContactDev& C = E.contacts[cidx];
-  const int j = blockIdx.x * 4 + wv;
+__device__ void crev_sort_body(const ContactDev& C, int j, int lane, int32_t* buf) {
   if (j >= C.n2) return;  // whole wavefront; no workgroup barrier below
   int32_t* r = C.rev + C.rev_off[j];
   const int n = C.rev_cnt[j];
@@ -359,7 +366,6 @@ __global__ __launch_bounds__(256) void k_crev_sort(const EnvDev* __restrict__ en
     wave_sync();  // every lane has read r[] before any lane overwrites it
     if (lane < n) r[rk] = v;
   } else if (n <= kCandCap) {
-    int32_t* buf = cand[wv];
     for (int k = lane; k < n; k += 64) buf[k] = r[k];
     wave_sort_desc(buf, n, lane);
     for (int k = lane; k < n; k += 64) r[k] = buf[n - 1 - k];
@@ -371,6 +377,12 @@ __global__ __launch_bounds__(256) void k_crev_sort(const EnvDev* __restrict__ en
       r[b + 1] = v;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_crev_sort(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  __shared__ int32_t cand[4][kCandCap];
+  const int wv = threadIdx.x >> 6;
+  crev_sort_body(envs[slot].contacts[cidx], blockIdx.x * 4 + wv, threadIdx.x & 63, cand[wv]);
 }
 
 // Self contact (update_contact_neighs! for ContactModel11, contacts.jl:539-591).
@@ -429,18 +441,22 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* sh, int32
   return wofs + x - v;
 }
 
-__global__ __launch_bounds__(256) void k_scan_tiles(const int32_t* __restrict__ in, int n,
-                                                    int32_t* __restrict__ tile_sum) {
-  const int base = blockIdx.x * kTile + threadIdx.x * 4;
+__device__ void scan_tiles_body(const int32_t* __restrict__ in, int n, int32_t* __restrict__ tile_sum, int tile) {
+  const int base = tile * kTile + threadIdx.x * 4;
   int32_t s = 0;
   for (int k = 0; k < 4; ++k) s += (base + k < n) ? in[base + k] : 0;
   __shared__ int32_t sh[4];
   int32_t tot;
   block_excl_scan(s, sh, tot);
-  if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+  if (threadIdx.x == 0) tile_sum[tile] = tot;
 }
 
-__global__ __launch_bounds__(256) void k_scan_sums(int32_t* __restrict__ tile_sum, int ntiles) {
+__global__ __launch_bounds__(256) void k_scan_tiles(const int32_t* __restrict__ in, int n,
+                                                    int32_t* __restrict__ tile_sum) {
+  scan_tiles_body(in, n, tile_sum, blockIdx.x);
+}
+
+__device__ void scan_sums_body(int32_t* __restrict__ tile_sum, int ntiles) {
   __shared__ int32_t sh[4];
   int32_t carry = 0;
   for (int b = 0; b < ntiles; b += 256) {
@@ -453,19 +469,149 @@ __global__ __launch_bounds__(256) void k_scan_sums(int32_t* __restrict__ tile_su
   }
 }
 
-__global__ __launch_bounds__(256) void k_scan_apply(const int32_t* __restrict__ in, int32_t* __restrict__ out,
-                                                    int n, const int32_t* __restrict__ tile_off) {
-  const int base = blockIdx.x * kTile + threadIdx.x * 4;
+__global__ __launch_bounds__(256) void k_scan_sums(int32_t* __restrict__ tile_sum, int ntiles) {
+  scan_sums_body(tile_sum, ntiles);
+}
+
+__device__ void scan_apply_body(const int32_t* __restrict__ in, int32_t* __restrict__ out, int n,
+                                const int32_t* __restrict__ tile_off, int tile) {
+  const int base = tile * kTile + threadIdx.x * 4;
   int32_t v[4];
   int32_t s = 0;
   for (int k = 0; k < 4; ++k) { v[k] = (base + k < n) ? in[base + k] : 0; s += v[k]; }
   __shared__ int32_t sh[4];
   int32_t tot;
-  int32_t run = block_excl_scan(s, sh, tot) + tile_off[blockIdx.x];
+  int32_t run = block_excl_scan(s, sh, tot) + tile_off[tile];
   for (int k = 0; k < 4; ++k) {
     if (base + k < n) out[base + k] = run;
     run += v[k];
   }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                    int n, const int32_t* __restrict__ tile_off) {
+  scan_apply_body(in, out, n, tile_off, blockIdx.x);
+}
+
+// ---- contact updates of many envs at once ----------------------------------
+// items[k] = (env slot, contact index); starts[cls * (nitems + 1) + k] = first
+// workgroup of item k in the launches of block class cls (no workgroup spans
+// two items), starts[cls * (nitems + 1) + nitems] = that class's grid.  Every
+// item runs the per-env kernels' bodies, so the lists equal per-env updates.
+__device__ __forceinline__ int item_of(const int32_t* __restrict__ st, int nitems, int b) {
+  int lo = 0, hi = nitems - 1;  // last k with st[k] <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (st[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct BItem {
+  const EnvDev* E;
+  const ContactDev* C;
+  int local;  // workgroup index inside the item
+};
+
+__device__ __forceinline__ BItem bitem(const EnvDev* envs, const int2* items, const int32_t* starts, int nitems,
+                                       int cls) {
+  const int32_t* st = starts + cls * (nitems + 1);
+  const int k = item_of(st, nitems, blockIdx.x);
+  const int2 it = items[k];
+  const EnvDev* E = &envs[it.x];
+  return BItem{E, &E->contacts[it.y], (int)blockIdx.x - st[k]};
+}
+
+__device__ __forceinline__ int32_t* scan_scratch(const ContactDev& C) {
+  return reinterpret_cast<int32_t*>(C.box + 16 + 12 * kRedBlocks);
+}
+
+__global__ __launch_bounds__(256) void kb_box1(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                               int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_BOX);
+  cbox1_body(*b.E, *b.C, b.C->box + 16, b.local, kRedBlocks);
+}
+
+__global__ __launch_bounds__(64) void kb_box2(const EnvDev* envs, const int2* items, int nitems) {
+  const int2 it = items[blockIdx.x];
+  const ContactDev& C = envs[it.x].contacts[it.y];
+  cbox2_body(C, C.box + 16, kRedBlocks);
+}
+
+// zeroes cell_count, the cell cursor, rev_cursor and rev_cnt
+__global__ __launch_bounds__(256) void kb_zero(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                               int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_ZERO);
+  const ContactDev& C = *b.C;
+  int l = b.local * 256 + threadIdx.x;
+  if (l < C.ncell_cap) { C.cell_count[l] = 0; return; }
+  l -= C.ncell_cap;
+  if (l < C.ncell_cap) { C.cell_start[C.ncell_cap + l] = 0; return; }
+  l -= C.ncell_cap;
+  if (l < C.n2) { C.rev_cursor[l] = 0; return; }
+  l -= C.n2;
+  if (l < C.n2) C.rev_cnt[l] = 0;
+}
+
+__global__ __launch_bounds__(256) void kb_bin(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                              int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N2T);
+  cbin_body(*b.E, *b.C, b.local * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void kb_fill(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                               int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N2T);
+  cfill_body(*b.C, b.local * 256 + threadIdx.x);
+}
+
+// REV: scan rev_cnt -> rev_off (n2) instead of cell_count -> cell_start (ncell_cap)
+template <bool REV>
+__global__ __launch_bounds__(256) void kb_scan_tiles(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                                     int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, REV ? PD_CB_N2TILE : PD_CB_CTILE);
+  const ContactDev& C = *b.C;
+  scan_tiles_body(REV ? C.rev_cnt : C.cell_count, REV ? C.n2 : C.ncell_cap, scan_scratch(C), b.local);
+}
+
+template <bool REV>
+__global__ __launch_bounds__(256) void kb_scan_sums(const EnvDev* envs, const int2* items, int nitems) {
+  const int2 it = items[blockIdx.x];
+  const ContactDev& C = envs[it.x].contacts[it.y];
+  const int n = REV ? C.n2 : C.ncell_cap;
+  if (n <= 0) return;
+  scan_sums_body(scan_scratch(C), (n + kTile - 1) / kTile);
+}
+
+template <bool REV>
+__global__ __launch_bounds__(256) void kb_scan_apply(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                                     int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, REV ? PD_CB_N2TILE : PD_CB_CTILE);
+  const ContactDev& C = *b.C;
+  scan_apply_body(REV ? C.rev_cnt : C.cell_count, REV ? C.rev_off : C.cell_start, REV ? C.n2 : C.ncell_cap,
+                  scan_scratch(C), b.local);
+}
+
+__global__ __launch_bounds__(256) void kb_search(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                                 int nitems) {
+  __shared__ int32_t cand[4][kCandCap];
+  const int wv = threadIdx.x >> 6;
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N1W);
+  csearch_body(*b.E, *b.C, b.local * 4 + wv, threadIdx.x & 63, cand[wv]);
+}
+
+__global__ __launch_bounds__(256) void kb_rev_fill(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                                   int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N1W);
+  crev_fill_body(*b.C, b.local * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+__global__ __launch_bounds__(256) void kb_rev_sort(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                                   int nitems) {
+  __shared__ int32_t cand[4][kCandCap];
+  const int wv = threadIdx.x >> 6;
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N2W);
+  crev_sort_body(*b.C, b.local * 4 + wv, threadIdx.x & 63, cand[wv]);
 }
 
 inline int nb(int n) { return n > 0 ? (n + 255) / 256 : 1; }
@@ -499,12 +645,47 @@ void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDe
   hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cnt, c.n2);
   hipLaunchKernelGGL(k_cbin, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
   pd_scan_exclusive(s, c.cell_count, c.cell_start, c.ncell_cap, scan_tmp);
-  hipLaunchKernelGGL(k_cfill, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx, cell_cursor);
+  hipLaunchKernelGGL(k_cfill, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
   const int wgrid = c.n1 > 0 ? (c.n1 + 3) / 4 : 1;  // one wavefront per block-1 member
   hipLaunchKernelGGL(k_csearch, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
   pd_scan_exclusive(s, c.rev_cnt, c.rev_off, c.n2, scan_tmp);
   hipLaunchKernelGGL(k_crev_fill, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
   hipLaunchKernelGGL(k_crev_sort, dim3(c.n2 > 0 ? (c.n2 + 3) / 4 : 1), dim3(256), 0, s, envs, slot, cidx);
+}
+
+int pd_contact_batch_blocks(int cls, int n1, int n2, int ncell_cap) {
+  auto cdiv = [](int a, int b) { return a > 0 ? (a + b - 1) / b : 0; };
+  switch (cls) {
+    case PD_CB_BOX: return kRedBlocks;
+    case PD_CB_ZERO: return cdiv(2 * ncell_cap + 2 * n2, 256);
+    case PD_CB_N2T: return cdiv(n2, 256);
+    case PD_CB_CTILE: return cdiv(ncell_cap, kTile);
+    case PD_CB_N2TILE: return cdiv(n2, kTile);
+    case PD_CB_N1W: return cdiv(n1, 4);
+    default: return cdiv(n2, 4);  // PD_CB_N2W
+  }
+}
+
+void pd_launch_contact_update_batch(hipStream_t s, const EnvDev* envs, const int2* items, const int32_t* starts,
+                                    int nitems, const int* grid) {
+  if (nitems <= 0) return;
+  auto go = [&](auto kern, int cls) {
+    if (grid[cls] > 0) hipLaunchKernelGGL(kern, dim3(grid[cls]), dim3(256), 0, s, envs, items, starts, nitems);
+  };
+  go(kb_box1, PD_CB_BOX);
+  hipLaunchKernelGGL(kb_box2, dim3(nitems), dim3(64), 0, s, envs, items, nitems);
+  go(kb_zero, PD_CB_ZERO);
+  go(kb_bin, PD_CB_N2T);
+  go(kb_scan_tiles<false>, PD_CB_CTILE);
+  hipLaunchKernelGGL(kb_scan_sums<false>, dim3(nitems), dim3(256), 0, s, envs, items, nitems);
+  go(kb_scan_apply<false>, PD_CB_CTILE);
+  go(kb_fill, PD_CB_N2T);
+  go(kb_search, PD_CB_N1W);
+  go(kb_scan_tiles<true>, PD_CB_N2TILE);
+  hipLaunchKernelGGL(kb_scan_sums<true>, dim3(nitems), dim3(256), 0, s, envs, items, nitems);
+  go(kb_scan_apply<true>, PD_CB_N2TILE);
+  go(kb_rev_fill, PD_CB_N1W);
+  go(kb_rev_sort, PD_CB_N2W);
 }
 
 void pd_launch_self_contact_update(hipStream_t s, EnvDev* envs, int slot, const ContactDev& c, int cidx) {

@@ -325,10 +325,11 @@ def test_fp32c_contact_plates(pd, orc):
 
 
 def test_graph_replay_equals_eager(pd, orc, monkeypatch):
-    """Steps replayed from the captured HIP graph equal eagerly launched steps
+    """Steps replayed from the captured HIP graphs equal eagerly launched steps
     bit for bit, over several pd_run calls (each capturing and replaying its own
-    graph) with eager contact rebuilds between replays; and both equal the
-    oracle in the reference-order form."""
+    graphs: plain steps, and steps followed by the contact rebuild, replayed at
+    every rebuild after the first); and both equal the oracle in the
+    reference-order form."""
     spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=2)
     for prec in ("exact", "fp64"):
         runs = {}
