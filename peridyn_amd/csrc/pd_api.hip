@@ -1466,6 +1466,10 @@ thread_local std::string g_err;
 // pd_run call that made it.
 struct StepGraph {
   hipGraphExec_t exec = nullptr;
+  void reset() {  // the stream must be idle
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
   template <typename F>
   void capture(hipStream_t s, F&& enqueue) {
     hipGraph_t g = nullptr;
@@ -1489,6 +1493,8 @@ struct StepGraph {
     if (exec) (void)hipGraphExecDestroy(exec);
   }
 };
+
+constexpr int64_t kLatchPoll = 16;
 
 // PERIDYN_GRAPH=0 launches every step eagerly (for comparison)
 bool graphs_enabled() {
@@ -1597,15 +1603,27 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
         });
       }
       PD_HIP(hipGraphLaunch(g.exec, s));
-      continue;
+    } else {
+      step_kernels(last, logged);
+      if (logged) ++row;
+      if (rebuild) timed(c, PD_PROF_CONTACT, [&] { contact_update_set(c, ids); });
+      book(last);
+      if (st == 1 && nl && !last) {
+        refresh_latch(c, ids, nl);
+        nl = prepare_latch(c, ids);
+      }
     }
-    step_kernels(last, logged);
-    if (logged) ++row;
-    if (rebuild) timed(c, PD_PROF_CONTACT, [&] { contact_update_set(c, ids); });
-    book(last);
-    if (st == 1 && nl && !last) {
-      refresh_latch(c, ids, nl);
-      nl = prepare_latch(c, ids);
+    // while latches are pending, they are read back every kLatchPoll steps;
+    // when envs have latched, the graphs are captured again without their
+    // latch kernels (a no-op once latched)
+    if (nl && use_graph && !last && st % kLatchPoll == 0) {
+      refresh_latch(c, ids, nl);  // synchronizes the stream
+      const int nl2 = prepare_latch(c, ids);
+      if (nl2 != nl) {
+        graph.reset();
+        rgraph.reset();
+      }
+      nl = nl2;
     }
   }
   PD_HIP(hipGetLastError());
