@@ -84,6 +84,45 @@ struct DevMem {
   }
 };
 
+// Pinned host staging of one context, reused by every commit: fresh pageable
+// vectors fault their pages in on first touch, and copies from pageable memory
+// go through the runtime's own staging buffer.
+struct HostStage {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  template <typename T>
+  struct Span {
+    T* p;
+    T& operator[](size_t i) const { return p[i]; }
+    T* data() const { return p; }
+  };
+  // starts a commit's staging (every commit ends with the stream idle, so no
+  // copy from an earlier one is in flight); grows by half again when short
+  void begin(size_t bytes, hipStream_t s) {
+    used = 0;
+    if (bytes <= cap) return;
+    PD_HIP(hipStreamSynchronize(s));
+    if (base) (void)hipHostFree(base);
+    base = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 2, (size_t)1 << 20);
+    PD_HIP(hipHostMalloc((void**)&base, want, hipHostMallocDefault));
+    cap = want;
+  }
+  template <typename T>
+  Span<T> take(size_t n) {
+    const size_t off = (used + 255) & ~(size_t)255;
+    const size_t end = off + std::max<size_t>(n, 1) * sizeof(T);
+    if (end > cap) throw PdError(PD_E_STATE, "host staging overflow");
+    used = end;
+    return Span<T>{reinterpret_cast<T*>(base + off)};
+  }
+};
+// staged bytes per particle in a commit (x, y, v, block, BC, type, 1/density,
+// degree, row start, reference index, FP32C x32, aux32 and m), and the padding
+constexpr size_t kStagePerParticle = 3 * 32 + 4 + 4 + 1 + 8 + 4 + 4 + 4 + 16 + 16 + 8;
+constexpr size_t kStagePad = 16 * 256;
+
 struct HostBlock {
   pd_block_desc d{};
   std::vector<double> x, y, v, vol;
@@ -143,6 +182,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   DevMem mem;  // env arrays and build temporaries
+  HostStage stage;  // commit staging
   std::string err;
   std::map<int64_t, Env*> envs;
   int64_t next_id = 1;
@@ -436,39 +476,44 @@ void build_brick_order(Env& e) {
         q[3 * i + d] = qi;
         nb[d] = std::max(nb[d], qi / dims[d] + 1);
       }
-    // one uint64 key per particle: brick (high 30 bits), cell in the brick (7
-    // bits: at most 128 cells), input index (27 bits: particle arrays stay
-    // below 4 GiB); more bricks fall back to the three-field comparison
+    // brick and cell of every particle (brick dimensions are powers of two),
+    // then a stable sort by (brick, cell): ties keep the input order
+    const int lx = __builtin_ctz(bx), ly = __builtin_ctz(by), lz = __builtin_ctz(bz);
     const int64_t nbr = nb[0] * nb[1] * nb[2];
-    auto brick_of = [&](int64_t i) {
+    std::vector<int64_t> brk(n);
+    std::vector<uint8_t> cel(n);
+    for (int64_t i = 0; i < n; ++i) {
       const int64_t* qq = &q[3 * i];
-      return ((qq[2] / bz) * nb[1] + qq[1] / by) * nb[0] + qq[0] / bx;
-    };
-    auto cell_of = [&](int64_t i) {
-      const int64_t* qq = &q[3 * i];
-      return ((qq[2] % bz) * by + qq[1] % by) * bx + qq[0] % bx;
-    };
+      brk[i] = ((qq[2] >> lz) * nb[1] + (qq[1] >> ly)) * nb[0] + (qq[0] >> lx);
+      cel[i] = (uint8_t)((((qq[2] & (bz - 1)) << ly | (qq[1] & (by - 1))) << lx) | (qq[0] & (bx - 1)));
+    }
     std::vector<int32_t> order(n);
-    if (nbr < (1LL << 30) && n < (1LL << 27) && bx * by * bz <= 128) {
-      std::vector<uint64_t> key(n);
-      for (int64_t i = 0; i < n; ++i)
-        key[i] = ((uint64_t)brick_of(i) << 34) | ((uint64_t)cell_of(i) << 27) | (uint64_t)i;
-      std::sort(key.begin(), key.end());
-      for (int64_t k = 0; k < n; ++k) order[k] = (int32_t)(key[k] & ((1ull << 27) - 1));
+    if (nbr < (1LL << 30) && bx * by * bz <= 128) {
+      // LSD radix sort of the 37-bit keys brick << 7 | cell, 16 bits per pass
+      std::vector<int32_t> tmp(n);
+      for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+      auto key = [&](int32_t i) { return ((uint64_t)brk[i] << 7) | cel[i]; };
+      std::vector<int64_t> cnt(1 << 16);
+      for (int shift = 0; shift < 37; shift += 16) {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int64_t k = 0; k < n; ++k) ++cnt[(key(order[k]) >> shift) & 0xffff];
+        int64_t run = 0;
+        for (auto& c : cnt) { const int64_t v = c; c = run; run += v; }
+        for (int64_t k = 0; k < n; ++k) tmp[cnt[(key(order[k]) >> shift) & 0xffff]++] = order[k];
+        order.swap(tmp);
+      }
     } else {
       for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
       std::sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
-        const int64_t ba = brick_of(a), bc = brick_of(c);
-        if (ba != bc) return ba < bc;
-        const int64_t ca = cell_of(a), cc = cell_of(c);
-        if (ca != cc) return ca < cc;
+        if (brk[a] != brk[c]) return brk[a] < brk[c];
+        if (cel[a] != cel[c]) return cel[a] < cel[c];
         return a < c;
       });
     }
     int64_t prev = -1;
     for (int64_t k = 0; k < n; ++k) {
       const int32_t i = order[k];
-      const int64_t br = brick_of(i);
+      const int64_t br = brk[i];
       if (k > 0 && br != prev) ++next_brick;
       prev = br;
       e.perm[b.first + k] = b.first + i;
@@ -746,10 +791,13 @@ void commit_env(Ctx* c, Env& e) {
   e.tiled = false;
   build_order(e);
   // ---- particle arrays (host staging, internal order)
-  std::vector<double4> x4(N), y4(N), v4(N);
-  std::vector<int32_t> pblock(N), bcid(N, -1);
-  std::vector<uint8_t> ptype(N);
-  std::vector<double> inv_den(N);
+  HostStage& hs = c->stage;
+  hs.begin((size_t)N * kStagePerParticle + kStagePad, c->stream);
+  auto x4 = hs.take<double4>(N), y4 = hs.take<double4>(N), v4 = hs.take<double4>(N);
+  auto pblock = hs.take<int32_t>(N), bcid = hs.take<int32_t>(N);
+  auto ptype = hs.take<uint8_t>(N);
+  auto inv_den = hs.take<double>(N);
+  std::fill(bcid.p, bcid.p + N, -1);
   for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
     HostBlock& b = e.blocks[bi];
     for (int64_t i = 0; i < b.d.n; ++i) {
@@ -810,7 +858,7 @@ void commit_env(Ctx* c, Env& e) {
   // ---- family: degrees
   std::vector<FamTmp> tmp(e.blocks.size());
   for (auto& t : tmp) t.dm = &c->mem;
-  std::vector<int32_t> deg(N);
+  auto deg = hs.take<int32_t>(N);
   try {
     for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
       HostBlock& b = e.blocks[bi];
@@ -822,7 +870,7 @@ void commit_env(Ctx* c, Env& e) {
     PD_HIP(hipMemcpyAsync(deg.data(), D.deg, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
     PD_HIP(hipStreamSynchronize(s));
     // ---- SELL-64 layout
-    std::vector<uint32_t> pslot0(N);
+    auto pslot0 = hs.take<uint32_t>(N);
     uint64_t slots = 0;
     e.chunks.clear();
     e.npk = 0;
@@ -882,7 +930,7 @@ void commit_env(Ctx* c, Env& e) {
     D.col = dalloc<int32_t>(e, slots);
     D.state = dalloc<uint8_t>(e, slots);
     // reference index of every internal particle: rows are ordered by it
-    std::vector<int32_t> refh(N);
+    auto refh = hs.take<int32_t>(N);
     for (int64_t p = 0; p < N; ++p) refh[p] = (int32_t)reference_of(e, p);
     int32_t* d_ref = dalloc<int32_t>(e, N);
     PD_HIP(hipMemcpyAsync(d_ref, refh.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
@@ -909,8 +957,8 @@ void commit_env(Ctx* c, Env& e) {
     if (e.want_tiles) build_tiles(c, e);
     if (e.precision == PD_PREC_FP32C) {
       // x32 relative to each block's minimum corner (bonds never leave a block)
-      std::vector<float4> x32(N), a32(N);
-      std::vector<double> mh(N);
+      auto x32 = hs.take<float4>(N), a32 = hs.take<float4>(N);
+      auto mh = hs.take<double>(N);
       PD_HIP(hipMemcpyAsync(mh.data(), D.m, sizeof(double) * N, hipMemcpyDeviceToHost, s));
       PD_HIP(hipStreamSynchronize(s));
       for (auto& b : e.blocks) {
@@ -1714,6 +1762,7 @@ int pd_ctx_destroy(void* ctx) {
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->envs) { free_env_device(*kv.second); delete kv.second; }
   c->mem.trim(0);
+  if (c->stage.base) (void)hipHostFree(c->stage.base);
   if (c->d_envs) (void)hipFree(c->d_envs);
   if (c->d_set) (void)hipFree(c->d_set);
   if (c->d_tset) (void)hipFree(c->d_tset);
