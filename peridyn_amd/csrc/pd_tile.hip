@@ -388,8 +388,13 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
 // fused force density + velocity-Verlet (material part of
 // general_material.jl:296-369 with the plugins' t_ij, then contacts and the
 // step epilogue of pd_wave.h)
+// Waves per SIMD the register allocation aims for (measured best with the
+// compact LDS records, which leave LDS for as many tiles per CU): BB 6 (80
+// VGPRs), OSB 5 (96), EP 4 (126); the BB and OSB targets spill 20-28 B per lane
+// outside the bond loop.
 template <int KIND, int LG, bool XF>
-__global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == PD_BB ? 6 : KIND == PD_OSB ? 5 : 4)))
+void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
                                                     EnvDev* __restrict__ envs, int flags, int hcap) {
   extern __shared__ double sh[];
   constexpr int C = 64 >> LG;
@@ -403,12 +408,12 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
   TileWave w;
   const bool has = tile_wave<LG>(T, w);
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
-  EpiPre ep;  // epilogue inputs in flight during the staging and the bond loop
-  epi_load_a(E, p, ep);
+  // the epilogue's inputs are loaded after the bond loop: the loop needs their
+  // registers more (waves per SIMD as above)
+  EpiPre ep;
   if (deformed) stage_halo<XF, NX>(E, T, sh, hcap);
   __syncthreads();
   if (!has) return;
-  epi_load_b(E, ep);
   const Rec<XF, NX> rc(sh, hcap);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 xp = gld4(E.x, p), yp = gld4(y, p);
@@ -489,6 +494,8 @@ __global__ __launch_bounds__(256) void k_tile_force(const Tile* __restrict__ til
     a3 = group_sum(a3, C);
   }
   if (!w.active || w.g != 0) return;
+  epi_load_a(E, p, ep);
+  epi_load_b(E, ep);
   add_contacts_pre<false>(E, p, yp, y, ep, a1, a2, a3);
   finish_particle_v(E, p, xp, yp, a1, a2, a3, flags, ep.v, ep.bcid);
 }
