@@ -596,7 +596,8 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
 
 // fused force density (BB / OSB) + FP64 velocity-Verlet
 template <int KIND, int LG>
-__global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == PD_BB ? 6 : 5)))
+void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
                                                       EnvDev* __restrict__ envs, int flags, int hcap) {
   extern __shared__ double sh[];
   constexpr int C = 64 >> LG;
@@ -609,12 +610,10 @@ __global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ t
   TileWave w;
   const bool has = tile_wave<LG>(T, w);
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
-  EpiPre ep;  // epilogue inputs in flight during the staging and the bond loop
-  epi_load_a(E, p, ep);
+  EpiPre ep;  // the epilogue's inputs, loaded after the bond loop (its registers)
   if (deformed) stage_halo32<KIND == PD_OSB ? 6 : 4>(E, T, sh, hcap);
   __syncthreads();
   if (!has) return;
-  epi_load_b(E, ep);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 yp = gld4(y, p);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -670,6 +669,8 @@ __global__ __launch_bounds__(256) void k_tile_force32(const Tile* __restrict__ t
     a3 = comp_total(s3, c3, C);
   }
   if (!w.active || w.g != 0) return;
+  epi_load_a(E, p, ep);
+  epi_load_b(E, ep);
   add_contacts_pre<false>(E, p, yp, y, ep, a1, a2, a3);
   finish_particle_v(E, p, gld4(E.x, p), yp, a1, a2, a3, flags, ep.v, ep.bcid);
 }
