@@ -1103,6 +1103,8 @@ void commit_env(Ctx* c, Env& e) {
     cd.cell_count = dalloc<int32_t>(e, cd.ncell_cap);
     cd.cell_items = dalloc<int32_t>(e, cd.n2);
     cd.cell_of = dalloc<int32_t>(e, cd.n2);
+    cd.wlist = dalloc<int32_t>(e, std::max(cd.n1, cd.n2));
+    cd.wcount = dalloc<int32_t>(e, 2);
     // reduction partials + scan scratch (pd_contact.hip)
     cd.box = dalloc<double>(e, 16 + 128 * 12 + pd_scan_scratch_ints(std::max(cd.ncell_cap, cd.n2)) / 2 + 8);
     PD_HIP(hipMemcpyAsync(didx1, idx1.data(), sizeof(int32_t) * idx1.size(), hipMemcpyHostToDevice, s));

@@ -331,15 +331,9 @@ __device__ void csearch_body(const EnvDev& E, const ContactDev& C, int i, int la
   for (int k = lane; k < cnt; k += 64) atomicAdd(&C.rev_cnt[own[k]], 1);
 }
 
-__global__ __launch_bounds__(256) void k_csearch(const EnvDev* __restrict__ envs, int slot, int cidx) {
-  __shared__ int32_t cand[4][kCandCap];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const EnvDev& E = envs[slot];
-  csearch_body(E, E.contacts[cidx], blockIdx.x * 4 + wv, lane, cand[wv]);
-}
 
 // One wavefront per block-1 member: append i to the transposed list of each of
-// its contacts (order fixed afterwards by k_crev_sort).
+// its contacts (order fixed afterwards by the sort below).
 __device__ void crev_fill_body(const ContactDev& C, int i, int lane) {
   if (i >= C.n1) return;
   const int cnt = C.own_cnt[i];
@@ -379,10 +373,73 @@ __device__ void crev_sort_body(const ContactDev& C, int j, int lane, int32_t* bu
   }
 }
 
-__global__ __launch_bounds__(256) void k_crev_sort(const EnvDev* __restrict__ envs, int slot, int cidx) {
+// Work lists: most members have nothing to search (outside the box) or to
+// sort (fewer than two partners), so one lane per member appends those that
+// do to wlist (one atomic per wavefront), and a grid sized from the member
+// count walks the list, one member per wavefront at a time (search: a
+// workgroup per 16 block-1 members, at most 4096; sort: per 64 block-2
+// members, at most 1024, as few members have two partners).  Each member is
+// processed exactly as before; the order of the list does not enter any result.
+
+__device__ __forceinline__ void wlist_append(const ContactDev& C, int which, bool need, int m) {
+  const uint64_t b = __ballot(need);
+  if (b == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_ctzll(b);
+  int base = 0;
+  if (lane == leader) base = atomicAdd(&C.wcount[which], __popcll(b));
+  base = __shfl(base, leader);
+  if (need) C.wlist[base + lanes_below(b)] = m;
+}
+
+// block-1 member i: an empty list at once, or queued for the search
+__device__ void csearch_mark(const EnvDev& E, const ContactDev& C, int i) {
+  bool need = false;
+  if (i < C.n1) {
+    need = C.max_neighs > 0 && C.box[6] != 0.0 && in_box(cur_y(E)[C.idx1[i]], C.box);
+    if (!need) C.own_cnt[i] = 0;
+  }
+  wlist_append(C, 0, need, i);
+}
+
+// block-2 member j with two or more partners: queued for the sort
+__device__ void crev_mark(const ContactDev& C, int j) {
+  wlist_append(C, 1, j < C.n2 && C.rev_cnt[j] > 1, j);
+}
+
+// wavefront gw of nw walks list `which`
+template <bool SORT>
+__device__ void walk_list(const EnvDev& E, const ContactDev& C, int gw, int nw, int lane, int32_t* buf) {
+  const int n = C.wcount[SORT ? 1 : 0];
+  for (int k = gw; k < n; k += nw) {
+    const int m = C.wlist[k];
+    if (SORT) crev_sort_body(C, m, lane, buf);
+    else csearch_body(E, C, m, lane, buf);
+  }
+}
+
+int list_blocks(int n, bool sort) {
+  if (n <= 0) return 0;
+  const int per = sort ? 64 : 16, cap = sort ? 1024 : 4096;
+  const int b = (n + per - 1) / per;
+  return b < cap ? b : cap;
+}
+
+__global__ __launch_bounds__(256) void k_csearch_mark(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  const EnvDev& E = envs[slot];
+  csearch_mark(E, E.contacts[cidx], blockIdx.x * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_crev_mark(const EnvDev* __restrict__ envs, int slot, int cidx) {
+  crev_mark(envs[slot].contacts[cidx], blockIdx.x * 256 + threadIdx.x);
+}
+
+template <bool SORT>
+__global__ __launch_bounds__(256) void k_walk(const EnvDev* __restrict__ envs, int slot, int cidx) {
   __shared__ int32_t cand[4][kCandCap];
   const int wv = threadIdx.x >> 6;
-  crev_sort_body(envs[slot].contacts[cidx], blockIdx.x * 4 + wv, threadIdx.x & 63, cand[wv]);
+  const EnvDev& E = envs[slot];
+  walk_list<SORT>(E, E.contacts[cidx], blockIdx.x * 4 + wv, gridDim.x * 4, threadIdx.x & 63, cand[wv]);
 }
 
 // Self contact (update_contact_neighs! for ContactModel11, contacts.jl:539-591).
@@ -538,7 +595,7 @@ __global__ __launch_bounds__(64) void kb_box2(const EnvDev* envs, const int2* it
   cbox2_body(C, C.box + 16, kRedBlocks);
 }
 
-// zeroes cell_count, the cell cursor, rev_cursor and rev_cnt
+// zeroes cell_count, the cell cursor, rev_cursor, rev_cnt and the list lengths
 __global__ __launch_bounds__(256) void kb_zero(const EnvDev* envs, const int2* items, const int32_t* starts,
                                                int nitems) {
   const BItem b = bitem(envs, items, starts, nitems, PD_CB_ZERO);
@@ -550,7 +607,9 @@ __global__ __launch_bounds__(256) void kb_zero(const EnvDev* envs, const int2* i
   l -= C.ncell_cap;
   if (l < C.n2) { C.rev_cursor[l] = 0; return; }
   l -= C.n2;
-  if (l < C.n2) C.rev_cnt[l] = 0;
+  if (l < C.n2) { C.rev_cnt[l] = 0; return; }
+  l -= C.n2;
+  if (l < 2) C.wcount[l] = 0;
 }
 
 __global__ __launch_bounds__(256) void kb_bin(const EnvDev* envs, const int2* items, const int32_t* starts,
@@ -592,12 +651,29 @@ __global__ __launch_bounds__(256) void kb_scan_apply(const EnvDev* envs, const i
                   scan_scratch(C), b.local);
 }
 
-__global__ __launch_bounds__(256) void kb_search(const EnvDev* envs, const int2* items, const int32_t* starts,
-                                                 int nitems) {
+__global__ __launch_bounds__(256) void kb_search_mark(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                                      int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N1T);
+  csearch_mark(*b.E, *b.C, b.local * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void kb_sort_mark(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                                    int nitems) {
+  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N2T);
+  crev_mark(*b.C, b.local * 256 + threadIdx.x);
+}
+
+template <bool SORT>
+__global__ __launch_bounds__(256) void kb_walk(const EnvDev* envs, const int2* items, const int32_t* starts,
+                                               int nitems) {
   __shared__ int32_t cand[4][kCandCap];
   const int wv = threadIdx.x >> 6;
-  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N1W);
-  csearch_body(*b.E, *b.C, b.local * 4 + wv, threadIdx.x & 63, cand[wv]);
+  const int cls = SORT ? PD_CB_SORT : PD_CB_SRCH;
+  const BItem b = bitem(envs, items, starts, nitems, cls);
+  const int32_t* st = starts + cls * (nitems + 1);
+  const int k = item_of(st, nitems, blockIdx.x);
+  const int nb = st[k + 1] - st[k];
+  walk_list<SORT>(*b.E, *b.C, b.local * 4 + wv, nb * 4, threadIdx.x & 63, cand[wv]);
 }
 
 __global__ __launch_bounds__(256) void kb_rev_fill(const EnvDev* envs, const int2* items, const int32_t* starts,
@@ -606,13 +682,6 @@ __global__ __launch_bounds__(256) void kb_rev_fill(const EnvDev* envs, const int
   crev_fill_body(*b.C, b.local * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-__global__ __launch_bounds__(256) void kb_rev_sort(const EnvDev* envs, const int2* items, const int32_t* starts,
-                                                   int nitems) {
-  __shared__ int32_t cand[4][kCandCap];
-  const int wv = threadIdx.x >> 6;
-  const BItem b = bitem(envs, items, starts, nitems, PD_CB_N2W);
-  crev_sort_body(*b.C, b.local * 4 + wv, threadIdx.x & 63, cand[wv]);
-}
 
 inline int nb(int n) { return n > 0 ? (n + 255) / 256 : 1; }
 
@@ -643,26 +712,31 @@ void pd_launch_contact_update(hipStream_t s, EnvDev* envs, int slot, const EnvDe
   hipLaunchKernelGGL(k_izero, dim3(nb(c.ncell_cap)), dim3(256), 0, s, cell_cursor, c.ncell_cap);
   hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cursor, c.n2);
   hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cnt, c.n2);
+  hipLaunchKernelGGL(k_izero, dim3(1), dim3(256), 0, s, c.wcount, 2);
   hipLaunchKernelGGL(k_cbin, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
   pd_scan_exclusive(s, c.cell_count, c.cell_start, c.ncell_cap, scan_tmp);
   hipLaunchKernelGGL(k_cfill, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
   const int wgrid = c.n1 > 0 ? (c.n1 + 3) / 4 : 1;  // one wavefront per block-1 member
-  hipLaunchKernelGGL(k_csearch, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
+  hipLaunchKernelGGL(k_csearch_mark, dim3(nb(c.n1)), dim3(256), 0, s, envs, slot, cidx);
+  if (c.n1 > 0) hipLaunchKernelGGL(k_walk<false>, dim3(list_blocks(c.n1, false)), dim3(256), 0, s, envs, slot, cidx);
   pd_scan_exclusive(s, c.rev_cnt, c.rev_off, c.n2, scan_tmp);
   hipLaunchKernelGGL(k_crev_fill, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
-  hipLaunchKernelGGL(k_crev_sort, dim3(c.n2 > 0 ? (c.n2 + 3) / 4 : 1), dim3(256), 0, s, envs, slot, cidx);
+  hipLaunchKernelGGL(k_crev_mark, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
+  if (c.n2 > 0) hipLaunchKernelGGL(k_walk<true>, dim3(list_blocks(c.n2, true)), dim3(256), 0, s, envs, slot, cidx);
 }
 
 int pd_contact_batch_blocks(int cls, int n1, int n2, int ncell_cap) {
   auto cdiv = [](int a, int b) { return a > 0 ? (a + b - 1) / b : 0; };
   switch (cls) {
     case PD_CB_BOX: return kRedBlocks;
-    case PD_CB_ZERO: return cdiv(2 * ncell_cap + 2 * n2, 256);
+    case PD_CB_ZERO: return cdiv(2 * ncell_cap + 2 * n2 + 2, 256);
     case PD_CB_N2T: return cdiv(n2, 256);
     case PD_CB_CTILE: return cdiv(ncell_cap, kTile);
     case PD_CB_N2TILE: return cdiv(n2, kTile);
     case PD_CB_N1W: return cdiv(n1, 4);
-    default: return cdiv(n2, 4);  // PD_CB_N2W
+    case PD_CB_N1T: return cdiv(n1, 256);
+    case PD_CB_SRCH: return list_blocks(n1, false);
+    default: return list_blocks(n2, true);  // PD_CB_SORT
   }
 }
 
@@ -680,21 +754,25 @@ void pd_launch_contact_update_batch(hipStream_t s, const EnvDev* envs, const int
   hipLaunchKernelGGL(kb_scan_sums<false>, dim3(nitems), dim3(256), 0, s, envs, items, nitems);
   go(kb_scan_apply<false>, PD_CB_CTILE);
   go(kb_fill, PD_CB_N2T);
-  go(kb_search, PD_CB_N1W);
+  go(kb_search_mark, PD_CB_N1T);
+  go(kb_walk<false>, PD_CB_SRCH);
   go(kb_scan_tiles<true>, PD_CB_N2TILE);
   hipLaunchKernelGGL(kb_scan_sums<true>, dim3(nitems), dim3(256), 0, s, envs, items, nitems);
   go(kb_scan_apply<true>, PD_CB_N2TILE);
   go(kb_rev_fill, PD_CB_N1W);
-  go(kb_rev_sort, PD_CB_N2W);
+  go(kb_sort_mark, PD_CB_N2T);
+  go(kb_walk<true>, PD_CB_SORT);
 }
 
 void pd_launch_self_contact_update(hipStream_t s, EnvDev* envs, int slot, const ContactDev& c, int cidx) {
   int32_t* scan_tmp = reinterpret_cast<int32_t*>(c.box + 16 + 12 * kRedBlocks);
   hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cursor, c.n2);
   hipLaunchKernelGGL(k_izero, dim3(nb(c.n2)), dim3(256), 0, s, c.rev_cnt, c.n2);
+  hipLaunchKernelGGL(k_izero, dim3(1), dim3(256), 0, s, c.wcount, 2);
   hipLaunchKernelGGL(k_cself, dim3(nb(c.n1)), dim3(256), 0, s, envs, slot, cidx);
   pd_scan_exclusive(s, c.rev_cnt, c.rev_off, c.n2, scan_tmp);
   const int wgrid = c.n1 > 0 ? (c.n1 + 3) / 4 : 1;
   hipLaunchKernelGGL(k_crev_fill, dim3(wgrid), dim3(256), 0, s, envs, slot, cidx);
-  hipLaunchKernelGGL(k_crev_sort, dim3(c.n2 > 0 ? (c.n2 + 3) / 4 : 1), dim3(256), 0, s, envs, slot, cidx);
+  hipLaunchKernelGGL(k_crev_mark, dim3(nb(c.n2)), dim3(256), 0, s, envs, slot, cidx);
+  if (c.n2 > 0) hipLaunchKernelGGL(k_walk<true>, dim3(list_blocks(c.n2, true)), dim3(256), 0, s, envs, slot, cidx);
 }

@@ -82,6 +82,8 @@ struct ContactDev {
   int32_t* cell_items;       // n2
   int32_t* cell_of;          // n2 (cell id or -1 outside the box)
   double* box;               // [0..2] min, [3..5] max, [6] overlap flag (1/0), [7..] grid
+  int32_t* wlist;            // max(n1, n2): members with work in a rebuild (search, then sort)
+  int32_t* wcount;           // 2: lengths of the two lists
   int32_t ncell_cap;
   int32_t rev_cap;
 };

@@ -66,7 +66,8 @@ void pd_launch_self_contact_update(hipStream_t s, EnvDev* envs, int slot, const 
 // per kernel of the sequence (pair contacts; self contacts use the per-env
 // form): blocks of item k in class cls = pd_contact_batch_blocks(cls, ...),
 // starts[cls * (nitems + 1) + k] their prefix sums, grid[cls] the totals
-enum { PD_CB_BOX = 0, PD_CB_ZERO, PD_CB_N2T, PD_CB_CTILE, PD_CB_N2TILE, PD_CB_N1W, PD_CB_N2W, PD_CB_NCLS };
+enum { PD_CB_BOX = 0, PD_CB_ZERO, PD_CB_N2T, PD_CB_CTILE, PD_CB_N2TILE, PD_CB_N1W, PD_CB_N1T, PD_CB_SRCH, PD_CB_SORT,
+       PD_CB_NCLS };
 int pd_contact_batch_blocks(int cls, int n1, int n2, int ncell_cap);
 void pd_launch_contact_update_batch(hipStream_t s, const EnvDev* envs, const int2* items, const int32_t* starts,
                                     int nitems, const int* grid);
