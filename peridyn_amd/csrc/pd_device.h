@@ -68,6 +68,14 @@ __device__ __forceinline__ int pd_wave_id() {
 }
 __device__ __forceinline__ int pd_lane() { return threadIdx.x & 63; }
 
+// FP64 form: a squared length with fused multiply-adds, fma(c, c, fma(b, b,
+// a * a)), one rounding per term (the reference-order form keeps the
+// reference's separate products and sums).  The tiled and untiled FP64
+// kernels use the same fused expressions, so they stay bit-identical.
+__device__ __forceinline__ double pd_norm2(double a, double b, double c) {
+  return __builtin_fma(c, c, __builtin_fma(b, b, a * a));
+}
+
 // Wavefront-level LDS ordering: every lane's earlier LDS writes are visible to
 // every lane of the wavefront after this point.
 __device__ __forceinline__ void wave_sync() {

@@ -133,10 +133,10 @@ __global__ __launch_bounds__(256) void k_theta(const Chunk* __restrict__ chunks,
         if (!sv[u]) continue;
         const double X1 = xj[u].x - xp.x, X2 = xj[u].y - xp.y, X3 = xj[u].z - xp.z;
         const double Y1 = yj[u].x - yp.x, Y2 = yj[u].y - yp.y, Y3 = yj[u].z - yp.z;
-        const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-        const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+        const double xx2 = pd_norm2(X1, X2, X3);
+        const double yy2 = pd_norm2(Y1, Y2, Y3);
         const double ext = yy2 * pd_rsqrt(yy2) - xx2 * pd_rsqrt(xx2);
-        acc += ext * xj[u].w;
+        acc = __builtin_fma(ext, xj[u].w, acc);
       }
     }
     acc = group_sum(acc, w.C);
@@ -245,8 +245,8 @@ __global__ __launch_bounds__(256) void k_td(const Chunk* __restrict__ chunks, in
         const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
         const double X1 = xj[u].x - xp.x, X2 = xj[u].y - xp.y, X3 = xj[u].z - xp.z;
         const double Y1 = yj[u].x - yp.x, Y2 = yj[u].y - yp.y, Y3 = yj[u].z - yp.z;
-        const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-        const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+        const double xx2 = pd_norm2(X1, X2, X3);
+        const double yy2 = pd_norm2(Y1, Y2, Y3);
         const double inv_x = pd_rsqrt(xx2);
         const double xij = xx2 * inv_x;
         const double ext = yy2 * pd_rsqrt(yy2) - xij;
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_td(const Chunk* __restrict__ chunks, in
           const double twm = ap.x * ap.y + aj[u].x * aj[u].y;
           const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
           const double td = (15.0 * B.G[tp + nt * tj]) * (ed - eo[u]) * (inv_x * sm);
-          acc += td * td * xj[u].w;
+          acc = __builtin_fma(td * td, xj[u].w, acc);
         } else {
           gst(st, slot, (uint8_t)0);
           if (yielded) gst(edp_o, slot, eo[u]);  // else 0 in both buffers already
@@ -469,8 +469,8 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
       const int j = jv[u];
       const double X1 = xj[u].x - xp.x, X2 = xj[u].y - xp.y, X3 = xj[u].z - xp.z;
       const double Y1 = yj[u].x - yp.x, Y2 = yj[u].y - yp.y, Y3 = yj[u].z - yp.z;
-      const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-      const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+      const double xx2 = pd_norm2(X1, X2, X3);
+      const double yy2 = pd_norm2(Y1, Y2, Y3);
       const double inv_x = pd_rsqrt(xx2);
       const double inv_y = pd_rsqrt(yy2);
       const double xij = xx2 * inv_x;
@@ -509,9 +509,9 @@ __device__ __forceinline__ void material_fast(const EnvDev& E, const BlockDev& B
           }
         }
         const double xx = t * inv_y * (xj[u].w * inv_den);
-        a1 += xx * Y1;
-        a2 += xx * Y2;
-        a3 += xx * Y3;
+        a1 = __builtin_fma(xx, Y1, a1);
+        a2 = __builtin_fma(xx, Y2, a2);
+        a3 = __builtin_fma(xx, Y3, a3);
       } else {
         gst(st, slot, (uint8_t)0);
         if (KIND == PD_EP && yielded_p) gst(edp_out, slot, eo[u]);

@@ -192,10 +192,10 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
       const int j = (int)(e & 0x7fffu);
       double X1, X2, X3, Y1, Y2, Y3;
       tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-      const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-      const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+      const double xx2 = pd_norm2(X1, X2, X3);
+      const double yy2 = pd_norm2(Y1, Y2, Y3);
       const double ext = yy2 * pd_rsqrt(yy2) - xx2 * pd_rsqrt(xx2);
-      acc += ext * rc.vol(j);
+      acc = __builtin_fma(ext, rc.vol(j), acc);
     }
   }
   acc = group_sum(acc, C);
@@ -267,8 +267,8 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
       const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
       double X1, X2, X3, Y1, Y2, Y3;
       tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-      const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-      const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+      const double xx2 = pd_norm2(X1, X2, X3);
+      const double yy2 = pd_norm2(Y1, Y2, Y3);
       const double inv_x = pd_rsqrt(xx2);
       const double xij = xx2 * inv_x;
       const double ext = yy2 * pd_rsqrt(yy2) - xij;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
         const double twm = ap.x * ap.y + th_j * rm_j;
         const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
         const double td = g15 * (ed - eo[u]) * (inv_x * sm);
-        acc += td * td * rc.vol(j);
+        acc = __builtin_fma(td * td, rc.vol(j), acc);
       } else {
         gst(st, slot, (uint8_t)0);
         gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
@@ -347,8 +347,8 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
     const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
     double X1, X2, X3, Y1, Y2, Y3;
     tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-    const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-    const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+    const double xx2 = pd_norm2(X1, X2, X3);
+    const double yy2 = pd_norm2(Y1, Y2, Y3);
     const double inv_x = pd_rsqrt(xx2);
     const double inv_y = pd_rsqrt(yy2);
     const double xij = xx2 * inv_x;
@@ -373,9 +373,9 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
       if (!(Cj > 1.0)) td_rev *= Cj;
       const double t = 3.0 * K0 * twm + 0.5 * (td_own + td_rev);
       const double xx = t * inv_y * (rc.vol(j) * inv_den);
-      a1 += xx * Y1;
-      a2 += xx * Y2;
-      a3 += xx * Y3;
+      a1 = __builtin_fma(xx, Y1, a1);
+      a2 = __builtin_fma(xx, Y2, a2);
+      a3 = __builtin_fma(xx, Y3, a3);
     } else {
       gst(st, slot, (uint8_t)0);
       gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
@@ -461,8 +461,8 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
           const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
           double X1, X2, X3, Y1, Y2, Y3;
           tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-          const double xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-          const double yy2 = Y1 * Y1 + Y2 * Y2 + Y3 * Y3;
+          const double xx2 = pd_norm2(X1, X2, X3);
+          const double yy2 = pd_norm2(Y1, Y2, Y3);
           const double inv_x = pd_rsqrt(xx2);
           const double inv_y = pd_rsqrt(yy2);
           const double xij = xx2 * inv_x;
@@ -479,9 +479,9 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
               tt = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
             }
             const double xx = tt * inv_y * (rc.vol(j) * inv_den);
-            a1 += xx * Y1;
-            a2 += xx * Y2;
-            a3 += xx * Y3;
+            a1 = __builtin_fma(xx, Y1, a1);
+            a2 = __builtin_fma(xx, Y2, a2);
+            a3 = __builtin_fma(xx, Y3, a3);
           } else {
             gst(st, slot, (uint8_t)0);
             gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
