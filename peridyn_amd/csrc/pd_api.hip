@@ -212,6 +212,8 @@ struct Ctx {
   size_t cstarts_cap = 0;
   int n_citems = 0;
   int cgrid[PD_CB_NCLS] = {};
+  PdCopy* d_copies = nullptr;  // build_set's copy descriptors
+  size_t copies_cap = 0;
   Chunk* d_set = nullptr;
   size_t set_cap = 0;
   int n_all = 0, n_theta = 0, n_td = 0, n_force[3] = {0, 0, 0};
@@ -1168,6 +1170,10 @@ bool env_gpow(const Env& e) {
 // lists per pass are built on the host.
 void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   if (c->set_valid && ids == c->set_ids && (c->set_full || !full)) return;
+  static_assert(sizeof(Chunk) % 4 == 0 && sizeof(Tile) % 4 == 0, "copied as 4-byte words");
+  std::vector<PdCopy> copies;  // per-env chunk and tile lists into the set's arrays
+  std::vector<int2> citems;    // pair-contact items
+  std::vector<int32_t> cst;    // their per-class workgroup offsets
   std::vector<Chunk> th, td, fk[3], thu, tdu, fku[3];
   std::vector<int32_t> slots;
   std::vector<int2> ranges;
@@ -1239,8 +1245,9 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
     for (int64_t id : ids) {
       Env& e = get_env(c, id);
       if (!e.chunks.empty())
-        PD_HIP(hipMemcpyAsync(c->d_set + off, e.d_chunks, sizeof(Chunk) * e.chunks.size(), hipMemcpyDeviceToDevice,
-                              c->stream));
+        copies.push_back(PdCopy{reinterpret_cast<const uint32_t*>(e.d_chunks),
+                                reinterpret_cast<uint32_t*>(c->d_set + off),
+                                (uint32_t)(sizeof(Chunk) * e.chunks.size() / 4), 0u});
       off += (int)e.chunks.size();
     }
   }
@@ -1260,8 +1267,9 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
     ensure_cap(c->d_tset, c->tset_cap, (size_t)tot);
     int f_th[2][4] = {}, f_td[4] = {}, f_fk[3][2][4] = {};  // fill positions
     auto put = [&](int dst, const Env& e, const std::array<int, 3>& tb) {
-      PD_HIP(hipMemcpyAsync(c->d_tset + dst, e.d_tiles + tb[1], sizeof(Tile) * tb[2], hipMemcpyDeviceToDevice,
-                            c->stream));
+      copies.push_back(PdCopy{reinterpret_cast<const uint32_t*>(e.d_tiles + tb[1]),
+                              reinterpret_cast<uint32_t*>(c->d_tset + dst), (uint32_t)(sizeof(Tile) * tb[2] / 4),
+                              0u});
     };
     for (int64_t id : ids) {
       Env& e = get_env(c, id);
@@ -1282,7 +1290,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   c->set_xf = xf ? 1 : 0;
   // pair-contact items: (slot, contact index) and per-class workgroup offsets
   {
-    std::vector<int2> items;
+    std::vector<int2>& items = citems;
     std::vector<std::array<int, 3>> sz;  // n1, n2, ncell_cap
     for (int64_t id : ids) {
       Env& e = get_env(c, id);
@@ -1295,7 +1303,8 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
     const int ni = (int)items.size();
     c->n_citems = ni;
     if (ni > 0) {
-      std::vector<int32_t> st((size_t)PD_CB_NCLS * (ni + 1));
+      std::vector<int32_t>& st = cst;
+      st.assign((size_t)PD_CB_NCLS * (ni + 1), 0);
       for (int cls = 0; cls < PD_CB_NCLS; ++cls) {
         int64_t run = 0;
         for (int k = 0; k < ni; ++k) {
@@ -1311,7 +1320,6 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
       PD_HIP(hipMemcpyAsync(c->d_citems, items.data(), sizeof(int2) * ni, hipMemcpyHostToDevice, c->stream));
       PD_HIP(hipMemcpyAsync(c->d_cstarts, st.data(), sizeof(int32_t) * st.size(), hipMemcpyHostToDevice,
                             c->stream));
-      PD_HIP(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
     }
   }
   ensure_cap(c->d_slots, c->slots_cap, slots.size());
@@ -1320,7 +1328,14 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   PD_HIP(hipMemcpyAsync(c->d_obs_ranges, ranges.data(), sizeof(int2) * ranges.size(), hipMemcpyHostToDevice,
                         c->stream));
   ensure_cap(c->d_obs_part, c->obs_part_cap, std::max<size_t>((size_t)n_all, 1));
-  PD_HIP(hipStreamSynchronize(c->stream));
+  if (!copies.empty()) {
+    ensure_cap(c->d_copies, c->copies_cap, copies.size());
+    PD_HIP(hipMemcpyAsync(c->d_copies, copies.data(), sizeof(PdCopy) * copies.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    pd_launch_copy_words(c->stream, c->d_copies, (int)copies.size());
+    PD_HIP(hipGetLastError());
+  }
+  PD_HIP(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
   c->n_slots = (int)slots.size();
   c->set_has_ep = has_ep;
   c->set_has_theta = has_th;
@@ -1776,6 +1791,7 @@ int pd_ctx_destroy(void* ctx) {
   if (c->d_obs_part) (void)hipFree(c->d_obs_part);
   if (c->d_obs_rows) (void)hipFree(c->d_obs_rows);
   if (c->d_citems) (void)hipFree(c->d_citems);
+  if (c->d_copies) (void)hipFree(c->d_copies);
   if (c->d_cstarts) (void)hipFree(c->d_cstarts);
   for (auto& r : c->pending) { c->pool.push_back(r.a); c->pool.push_back(r.b); }
   for (hipEvent_t ev : c->pool) (void)hipEventDestroy(ev);

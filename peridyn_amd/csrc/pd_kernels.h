@@ -101,3 +101,10 @@ void pd_launch_halo_fill(hipStream_t s, const Tile* tiles, int n, const int32_t*
                          int hc);
 // deformed latch of n (env slot, block) items into out[n]
 void pd_launch_latch_flags(hipStream_t s, EnvDev* envs, const int2* items, int n, int32_t* out);
+// n device-to-device copies of 4-byte words in one launch
+struct PdCopy {
+  const uint32_t* src;
+  uint32_t* dst;
+  uint32_t words, pad;
+};
+void pd_launch_copy_words(hipStream_t s, const PdCopy* d, int n);

@@ -62,6 +62,13 @@ __device__ __forceinline__ double ep_t(double C, double K, double G, double xij,
   return tn + td;
 }
 
+// many small device-to-device copies in one launch: one workgroup per
+// descriptor (the run set's per-env chunk and tile lists)
+__global__ __launch_bounds__(256) void k_copy_words(const PdCopy* __restrict__ d, int n) {
+  const PdCopy c = d[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < c.words; i += 256) c.dst[i] = c.src[i];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -909,6 +916,11 @@ __global__ void k_latch_flags(const EnvDev* __restrict__ envs, const int2* __res
                               int32_t* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = envs[items[i].x].blocks[items[i].y].deformed;
+}
+
+void pd_launch_copy_words(hipStream_t s, const PdCopy* d, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy_words, dim3(n), dim3(256), 0, s, d, n);
 }
 
 void pd_launch_latch_flags(hipStream_t s, EnvDev* envs, const int2* items, int n, int32_t* out) {
