@@ -264,7 +264,7 @@ class GpuBackend:
     """The HIP library as a serving backend: one env per request, all envs of a
     chunk stepped by one pd_run."""
 
-    def __init__(self, ctx, precision: str = "fp32c", keep_state: bool = True, lanes_per_particle: int = 0):
+    def __init__(self, ctx, precision: str = "fp64", keep_state: bool = True, lanes_per_particle: int = 0):
         self.ctx = ctx
         self.precision = precision
         self.keep_state = keep_state
