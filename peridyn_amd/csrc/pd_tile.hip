@@ -300,8 +300,8 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
 // Elastoplastic bond loop of the tiled force pass, software-pipelined over
 // groups of 4 bond steps: while group g is computed, the reverse bonds' plastic
 // extensions of group g+1, the own plastic extensions and reverse slots of
-// group g+2 and the packed word of group g+3 are in flight (these kernels'
-// occupancy is bounded by their LDS footprint, not their registers).  Bonds are
+// group g+2 and the packed words of groups g+3 and g+4 are in flight (126
+// VGPRs: 4 waves per SIMD, as many as the compact LDS records allow).  Bonds are
 // evaluated in the same order with the same expressions as the untiled kernel.
 struct EpGrp {
   uint64_t cw;

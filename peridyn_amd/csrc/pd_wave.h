@@ -137,10 +137,11 @@ __device__ __forceinline__ void add_contacts(const EnvDev& E, int p, const doubl
   }
 }
 
-// Per-particle inputs of the force kernels' epilogue (the contact terms of
-// model 0 and the Verlet update), loaded early so that their latency overlaps
-// the bond loop: epi_load_a at kernel start, epi_load_b (the loads that depend
-// on the first ones) before the bond loop.
+// Per-particle inputs of the tiled force kernels' epilogue (the contact terms
+// of model 0 and the Verlet update), loaded in two batches whose loads are in
+// flight together: epi_load_a, then epi_load_b (the loads that depend on the
+// first ones).  The kernels load them after the bond loop: the loop's
+// registers are worth more than hiding this latency behind it.
 struct EpiPre {
   int i1, i2, cnt1, off2, cnt2, bcid;
   double inv_den;
