@@ -160,9 +160,10 @@ __device__ __forceinline__ void tile_geom(const R& rc, int j, const double4& xp,
 
 // ---------------------------------------------------------------------------
 // dilatation theta (src/peri.jl:155-172), OSB and EP blocks
-template <int LG, bool XF>
+template <int LG, bool XF, int S>
 __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ tiles, int ntiles,
                                                     EnvDev* __restrict__ envs, int hcap) {
+  const int hs = S > 0 ? S : hcap;  // LDS stride of the record fields
   extern __shared__ double sh[];
   constexpr int C = 64 >> LG;
   const int t = pd_xcd_block(gridDim.x);
@@ -171,11 +172,11 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   if (B.kind == PD_BB || !B.deformed) return;
-  stage_halo<XF, 0>(E, T, sh, hcap);
+  stage_halo<XF, 0>(E, T, sh, hs);
   __syncthreads();
   TileWave w;
   if (!tile_wave<LG>(T, w)) return;
-  const Rec<XF, 0> rc(sh, hcap);
+  const Rec<XF, 0> rc(sh, hs);
   const uint64_t* __restrict__ pk = E.pk;
   const int p = w.active ? w.p : w.ch.first;
   const double4 xp = gld4(E.x, p), yp = gld4(E.y[E.cur], p);
@@ -209,9 +210,10 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
 // ---------------------------------------------------------------------------
 // td_norm2 (elasto_plastic.jl:168-172,282-295; src/peri.jl:411-445), EP
 // blocks; breaks bonds with s >= s0 (state and the pk intact bit)
-template <int LG, bool XF>
+template <int LG, bool XF, int S>
 __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles, int ntiles,
                                                  EnvDev* __restrict__ envs, int hcap) {
+  const int hs = S > 0 ? S : hcap;
   extern __shared__ double sh[];
   constexpr int C = 64 >> LG;
   const int t = pd_xcd_block(gridDim.x);
@@ -220,11 +222,11 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   if (B.kind != PD_EP || !B.deformed) return;
-  stage_halo<XF, 2>(E, T, sh, hcap);
+  stage_halo<XF, 2>(E, T, sh, hs);
   __syncthreads();
   TileWave w;
   if (!tile_wave<LG>(T, w)) return;
-  const Rec<XF, 2> rc(sh, hcap);
+  const Rec<XF, 2> rc(sh, hs);
   const uint64_t* __restrict__ pk = E.pk;
   uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
   uint8_t* __restrict__ st = E.state;
@@ -392,10 +394,11 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
 // compact LDS records, which leave LDS for as many tiles per CU): BB 6 (80
 // VGPRs), OSB 5 (96), EP 4 (126); the BB and OSB targets spill 20-28 B per lane
 // outside the bond loop.
-template <int KIND, int LG, bool XF>
+template <int KIND, int LG, bool XF, int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == PD_BB ? 6 : KIND == PD_OSB ? 5 : 4)))
 void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
                                                     EnvDev* __restrict__ envs, int flags, int hcap) {
+  const int hs = S > 0 ? S : hcap;
   extern __shared__ double sh[];
   constexpr int C = 64 >> LG;
   constexpr int NX = tile_extra<TP_FORCE, KIND>();
@@ -411,10 +414,10 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
   // the epilogue's inputs are loaded after the bond loop: the loop needs their
   // registers more (waves per SIMD as above)
   EpiPre ep;
-  if (deformed) stage_halo<XF, NX>(E, T, sh, hcap);
+  if (deformed) stage_halo<XF, NX>(E, T, sh, hs);
   __syncthreads();
   if (!has) return;
-  const Rec<XF, NX> rc(sh, hcap);
+  const Rec<XF, NX> rc(sh, hs);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 xp = gld4(E.x, p), yp = gld4(y, p);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -830,20 +833,51 @@ __global__ __launch_bounds__(256) void k_pk_sync(const Chunk* __restrict__ chunk
   }
 }
 
+// LDS stride of the FP64 record fields: the smaller of the compile-time
+// strides kStrideA / kStrideB that the set's largest halo fits (the field
+// offsets then fold into the LDS instructions' immediate offsets), else the
+// halo capacity at run time.  608 covers 4x4x4-brick halos at delta = 3 dx, 896
+// at 4 dx; neither costs tiles per CU against the run-time capacity.
+constexpr int kStrideA = 608, kStrideB = 896;
+
+inline int tile_stride(int hcap) { return hcap <= kStrideA ? kStrideA : hcap <= kStrideB ? kStrideB : 0; }
+
+template <int LG, bool XF, int S>
+void launch_theta_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  const size_t lds = (size_t)tile_record_bytes<TP_THETA, 0, XF>() * (size_t)(S > 0 ? S : hcap);
+  hipLaunchKernelGGL((k_tile_theta<LG, XF, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+}
+template <int LG, bool XF, int S>
+void launch_td_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  const size_t lds = (size_t)tile_record_bytes<TP_TD, 0, XF>() * (size_t)(S > 0 ? S : hcap);
+  hipLaunchKernelGGL((k_tile_td<LG, XF, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+}
+template <int KIND, int LG, bool XF, int S>
+void launch_force_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
+  const size_t lds = (size_t)tile_record_bytes<TP_FORCE, KIND, XF>() * (size_t)(S > 0 ? S : hcap);
+  hipLaunchKernelGGL((k_tile_force<KIND, LG, XF, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
+}
+
 template <int LG, bool XF>
 void launch_theta_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const size_t lds = (size_t)tile_record_bytes<TP_THETA, 0, XF>() * (size_t)hcap;
-  hipLaunchKernelGGL((k_tile_theta<LG, XF>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+  const int st = tile_stride(hcap);
+  if (st == kStrideA) launch_theta_s<LG, XF, kStrideA>(s, tiles, n, envs, hcap);
+  else if (st == kStrideB) launch_theta_s<LG, XF, kStrideB>(s, tiles, n, envs, hcap);
+  else launch_theta_s<LG, XF, 0>(s, tiles, n, envs, hcap);
 }
 template <int LG, bool XF>
 void launch_td_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const size_t lds = (size_t)tile_record_bytes<TP_TD, 0, XF>() * (size_t)hcap;
-  hipLaunchKernelGGL((k_tile_td<LG, XF>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+  const int st = tile_stride(hcap);
+  if (st == kStrideA) launch_td_s<LG, XF, kStrideA>(s, tiles, n, envs, hcap);
+  else if (st == kStrideB) launch_td_s<LG, XF, kStrideB>(s, tiles, n, envs, hcap);
+  else launch_td_s<LG, XF, 0>(s, tiles, n, envs, hcap);
 }
 template <int KIND, int LG, bool XF>
 void launch_force_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  const size_t lds = (size_t)tile_record_bytes<TP_FORCE, KIND, XF>() * (size_t)hcap;
-  hipLaunchKernelGGL((k_tile_force<KIND, LG, XF>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
+  const int st = tile_stride(hcap);
+  if (st == kStrideA) launch_force_s<KIND, LG, XF, kStrideA>(s, tiles, n, envs, flags, hcap);
+  else if (st == kStrideB) launch_force_s<KIND, LG, XF, kStrideB>(s, tiles, n, envs, flags, hcap);
+  else launch_force_s<KIND, LG, XF, 0>(s, tiles, n, envs, flags, hcap);
 }
 
 template <int LG>
@@ -862,13 +896,19 @@ template <class K>
 void allow_lds(K* k, size_t bytes) {
   (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
+template <int LG, bool XF, int S>
+void allow_lds_fp64_s(size_t bytes) {
+  allow_lds(k_tile_theta<LG, XF, S>, bytes);
+  allow_lds(k_tile_td<LG, XF, S>, bytes);
+  allow_lds(k_tile_force<PD_BB, LG, XF, S>, bytes);
+  allow_lds(k_tile_force<PD_OSB, LG, XF, S>, bytes);
+  allow_lds(k_tile_force<PD_EP, LG, XF, S>, bytes);
+}
 template <int LG, bool XF>
 void allow_lds_fp64(size_t bytes) {
-  allow_lds(k_tile_theta<LG, XF>, bytes);
-  allow_lds(k_tile_td<LG, XF>, bytes);
-  allow_lds(k_tile_force<PD_BB, LG, XF>, bytes);
-  allow_lds(k_tile_force<PD_OSB, LG, XF>, bytes);
-  allow_lds(k_tile_force<PD_EP, LG, XF>, bytes);
+  allow_lds_fp64_s<LG, XF, kStrideA>(bytes);
+  allow_lds_fp64_s<LG, XF, kStrideB>(bytes);
+  allow_lds_fp64_s<LG, XF, 0>(bytes);
 }
 template <int LG>
 void allow_lds_lg(size_t bytes) {
