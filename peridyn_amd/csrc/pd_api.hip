@@ -86,9 +86,14 @@ struct DevMem {
 
 // Pinned host staging of one context, reused by every commit: fresh pageable
 // vectors fault their pages in on first touch, and copies from pageable memory
-// go through the runtime's own staging buffer.
+// go through the runtime's own staging buffer.  Commits needing more than
+// kMaxPinned stage in pageable memory instead (no gigabytes held pinned).
 struct HostStage {
-  char* base = nullptr;
+  static constexpr size_t kMaxPinned = 1ull << 30;
+  char* base = nullptr;      // the buffer in use (pinned or big.data())
+  char* pinned = nullptr;
+  size_t pinned_cap = 0;
+  std::vector<char> big;     // pageable staging of the largest commits
   size_t cap = 0, used = 0;
   template <typename T>
   struct Span {
@@ -100,14 +105,26 @@ struct HostStage {
   // copy from an earlier one is in flight); grows by half again when short
   void begin(size_t bytes, hipStream_t s) {
     used = 0;
-    if (bytes <= cap) return;
-    PD_HIP(hipStreamSynchronize(s));
-    if (base) (void)hipHostFree(base);
-    base = nullptr;
-    cap = 0;
-    const size_t want = std::max<size_t>(bytes + bytes / 2, (size_t)1 << 20);
-    PD_HIP(hipHostMalloc((void**)&base, want, hipHostMallocDefault));
-    cap = want;
+    if (bytes > kMaxPinned) {
+      PD_HIP(hipStreamSynchronize(s));
+      std::vector<char>().swap(big);
+      big.resize(bytes);
+      base = big.data();
+      cap = bytes;
+      return;
+    }
+    std::vector<char>().swap(big);
+    if (bytes > pinned_cap) {
+      PD_HIP(hipStreamSynchronize(s));
+      if (pinned) (void)hipHostFree(pinned);
+      pinned = nullptr;
+      pinned_cap = 0;
+      const size_t want = std::min(std::max<size_t>(bytes + bytes / 2, (size_t)1 << 20), kMaxPinned);
+      PD_HIP(hipHostMalloc((void**)&pinned, want, hipHostMallocDefault));
+      pinned_cap = want;
+    }
+    base = pinned;
+    cap = pinned_cap;
   }
   template <typename T>
   Span<T> take(size_t n) {
@@ -491,17 +508,20 @@ void build_brick_order(Env& e) {
     }
     std::vector<int32_t> order(n);
     if (nbr < (1LL << 30) && bx * by * bz <= 128) {
-      // LSD radix sort of the 37-bit keys brick << 7 | cell, 16 bits per pass
+      // LSD radix sort of the keys brick << 7 | cell, 11 bits per pass, as many
+      // passes as the largest key has digits
       std::vector<int32_t> tmp(n);
       for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
       auto key = [&](int32_t i) { return ((uint64_t)brk[i] << 7) | cel[i]; };
-      std::vector<int64_t> cnt(1 << 16);
-      for (int shift = 0; shift < 37; shift += 16) {
+      uint64_t kmax = 0;
+      for (int64_t i = 0; i < n; ++i) kmax = std::max(kmax, key((int32_t)i));
+      std::vector<int64_t> cnt(1 << 11);
+      for (int shift = 0; shift < 64 && (kmax >> shift) != 0; shift += 11) {
         std::fill(cnt.begin(), cnt.end(), 0);
-        for (int64_t k = 0; k < n; ++k) ++cnt[(key(order[k]) >> shift) & 0xffff];
+        for (int64_t k = 0; k < n; ++k) ++cnt[(key(order[k]) >> shift) & 0x7ff];
         int64_t run = 0;
         for (auto& c : cnt) { const int64_t v = c; c = run; run += v; }
-        for (int64_t k = 0; k < n; ++k) tmp[cnt[(key(order[k]) >> shift) & 0xffff]++] = order[k];
+        for (int64_t k = 0; k < n; ++k) tmp[cnt[(key(order[k]) >> shift) & 0x7ff]++] = order[k];
         order.swap(tmp);
       }
     } else {
@@ -1779,7 +1799,7 @@ int pd_ctx_destroy(void* ctx) {
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->envs) { free_env_device(*kv.second); delete kv.second; }
   c->mem.trim(0);
-  if (c->stage.base) (void)hipHostFree(c->stage.base);
+  if (c->stage.pinned) (void)hipHostFree(c->stage.pinned);
   if (c->d_envs) (void)hipFree(c->d_envs);
   if (c->d_set) (void)hipFree(c->d_set);
   if (c->d_tset) (void)hipFree(c->d_tset);
