@@ -550,10 +550,12 @@ __device__ __forceinline__ void stage_halo32(const EnvDev& E, const Tile& T, dou
   }
 }
 
-// dilatation theta (src/peri.jl:155-172) of OSB blocks
-template <int LG>
+// dilatation theta (src/peri.jl:155-172) of OSB blocks; S as in the FP64
+// kernels (compile-time LDS stride, 0: the run-time capacity)
+template <int LG, int S>
 __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ tiles, int ntiles,
-                                                      EnvDev* __restrict__ envs, int hcap) {
+                                                      EnvDev* __restrict__ envs, int hcap_rt) {
+  const int hcap = S > 0 ? S : hcap_rt;
   extern __shared__ double sh[];
   constexpr int C = 64 >> LG;
   const int t = pd_xcd_block(gridDim.x);
@@ -598,10 +600,11 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
 }
 
 // fused force density (BB / OSB) + FP64 velocity-Verlet
-template <int KIND, int LG>
+template <int KIND, int LG, int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == PD_BB ? 6 : 5)))
 void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
-                                                      EnvDev* __restrict__ envs, int flags, int hcap) {
+                                                      EnvDev* __restrict__ envs, int flags, int hcap_rt) {
+  const int hcap = S > 0 ? S : hcap_rt;
   extern __shared__ double sh[];
   constexpr int C = 64 >> LG;
   const int t = pd_xcd_block(gridDim.x);
@@ -880,21 +883,41 @@ void launch_force_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int 
   else launch_force_s<KIND, LG, XF, 0>(s, tiles, n, envs, flags, hcap);
 }
 
+template <int LG, int S>
+void launch_theta32_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
+  const size_t lds = (3 * sizeof(double) + 4 * sizeof(float)) * (size_t)(S > 0 ? S : hcap);
+  hipLaunchKernelGGL((k_tile_theta32<LG, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+}
 template <int LG>
 void launch_theta32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const size_t lds = (3 * sizeof(double) + 4 * sizeof(float)) * (size_t)hcap;
-  hipLaunchKernelGGL(k_tile_theta32<LG>, dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
+  const int st = tile_stride(hcap);
+  if (st == kStrideA) launch_theta32_s<LG, kStrideA>(s, tiles, n, envs, hcap);
+  else if (st == kStrideB) launch_theta32_s<LG, kStrideB>(s, tiles, n, envs, hcap);
+  else launch_theta32_s<LG, 0>(s, tiles, n, envs, hcap);
+}
+template <int KIND, int LG, int S>
+void launch_force32_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
+  const size_t lds = (3 * sizeof(double) + (KIND == PD_OSB ? 6 : 4) * sizeof(float)) * (size_t)(S > 0 ? S : hcap);
+  hipLaunchKernelGGL((k_tile_force32<KIND, LG, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
 }
 template <int KIND, int LG>
 void launch_force32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  const size_t lds = (3 * sizeof(double) + (KIND == PD_OSB ? 6 : 4) * sizeof(float)) * (size_t)hcap;
-  hipLaunchKernelGGL((k_tile_force32<KIND, LG>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
+  const int st = tile_stride(hcap);
+  if (st == kStrideA) launch_force32_s<KIND, LG, kStrideA>(s, tiles, n, envs, flags, hcap);
+  else if (st == kStrideB) launch_force32_s<KIND, LG, kStrideB>(s, tiles, n, envs, flags, hcap);
+  else launch_force32_s<KIND, LG, 0>(s, tiles, n, envs, flags, hcap);
 }
 
 // dynamic LDS beyond the default 64 KiB per workgroup must be enabled per kernel
 template <class K>
 void allow_lds(K* k, size_t bytes) {
   (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+template <int LG, int S>
+void allow_lds_fp32c(size_t bytes) {
+  allow_lds(k_tile_theta32<LG, S>, bytes);
+  allow_lds(k_tile_force32<PD_BB, LG, S>, bytes);
+  allow_lds(k_tile_force32<PD_OSB, LG, S>, bytes);
 }
 template <int LG, bool XF, int S>
 void allow_lds_fp64_s(size_t bytes) {
@@ -912,9 +935,9 @@ void allow_lds_fp64(size_t bytes) {
 }
 template <int LG>
 void allow_lds_lg(size_t bytes) {
-  allow_lds(k_tile_theta32<LG>, bytes);
-  allow_lds(k_tile_force32<PD_BB, LG>, bytes);
-  allow_lds(k_tile_force32<PD_OSB, LG>, bytes);
+  allow_lds_fp32c<LG, kStrideA>(bytes);
+  allow_lds_fp32c<LG, kStrideB>(bytes);
+  allow_lds_fp32c<LG, 0>(bytes);
   allow_lds_fp64<LG, false>(bytes);
   allow_lds_fp64<LG, true>(bytes);
 }
