@@ -292,3 +292,23 @@ def test_device_and_host_tile_builds_agree(pd, kind):
     same_state(a, b)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("kind,horizon,stride", [("bb", 2.6, "608"), ("bb", 3.8, "896"), ("bb", 4.6, "run time"),
+                                                 ("osb", 2.6, "608"), ("osb", 3.8, "896")])
+def test_halo_stride_classes_equal_untiled(pd, kind, horizon, stride):
+    """The tiled passes use a compile-time LDS stride when the set's largest halo
+    fits 608 or 896 records, else the halo capacity at run time; each class
+    gives the untiled kernels' bits.  (An OSB block with halos beyond 896 runs
+    untiled: its records exceed the LDS budget.)"""
+    say("stride class", kind, horizon, stride)
+    spec = W.tensile(3000, horizon=horizon, kind=kind, pull_per_step=0.002)
+    a = make(pd, spec, True, 4)
+    hmax = a.layout()["max_halo"]
+    assert (hmax <= 608) if stride == "608" else (608 < hmax <= 896) if stride == "896" else (hmax > 896), hmax
+    b = make(pd, spec, False, 4)
+    a.run(30, 1)
+    b.run(30, 1)
+    same_state(a, b)
+    for e in (a, b):
+        e.close()
