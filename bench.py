@@ -78,14 +78,17 @@ def workload(name: str):
 
 
 def cpu_baseline(spec, nuf: int, budget_s: float, nbonds: int):
-    """The oracle (C++ restatement, single thread) on the first steps of the
-    same workload until ~budget_s of stepping has elapsed.  The deformed latch
+    """The oracle (C++ restatement) on the first steps of the same workload
+    until ~budget_s of stepping has elapsed, with the loops the reference runs
+    under Threads.@threads (force, dilatation, td_norm2, contact search) on
+    OpenMP threads (OMP_NUM_THREADS, else every host core).  The deformed latch
     of every block is set first so that the sample times steady-state steps
     (bond loops running), as in almost all of the GPU's timed steps; the first
     steps of the workload, with rigidly moving blocks, would skip them."""
     from oracle import oracle as O
     O.build()
     env = O.OracleEnv(spec)
+    env.threaded = True
     env.update_neighs()
     env.apply_bc_at0()
     for b in env.blocks:
@@ -100,10 +103,11 @@ def cpu_baseline(spec, nuf: int, budget_s: float, nbonds: int):
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 500:
             break
-    return {"value": nbonds * steps / el, "unit": "bond-updates/s", "cores": 1, "kind": "port",
+    cores = env.threads_used
+    return {"value": nbonds * steps / el, "unit": "bond-updates/s", "cores": cores, "kind": "port",
             "sample": f"first {steps} steps of the same workload with every block's deformed latch "
-                      f"set ({el:.1f} s), single-threaded C++ restatement of the reference loops (oracle/), "
-                      "not Julia"}
+                      f"set ({el:.1f} s), C++ restatement of the reference loops (oracle/) with its threaded "
+                      f"loops on {cores} OpenMP threads, not Julia"}
 
 
 def traffic_for(kernel: str, workload_name: str):

@@ -50,7 +50,7 @@ _lib = None
 
 def build(force: bool = False) -> str:
     """Compile the C++ restatement (g++ -O2 -ffp-contract=off, OpenMP for the
-    timing-only multithreaded BB loop)."""
+    timing-only threaded loops)."""
     src = os.path.join(_HERE, "peridyn_oracle.cpp")
     if force or not os.path.exists(_SO) or os.path.getmtime(_SO) < os.path.getmtime(src):
         subprocess.check_call(
@@ -96,9 +96,10 @@ def lib():
         L.orc_dilatation.argtypes = [I64, I64, P, P, P, P, P, P, P]
         L.orc_td_norm2.argtypes = [ctypes.POINTER(_OrcBlock), P]
         L.orc_force_density.argtypes = [ctypes.c_int, ctypes.POINTER(_OrcBlock), P, P]
-        L.orc_force_density_bb_omp.argtypes = [ctypes.POINTER(_OrcBlock), P, P]
-        L.orc_force_density_bb_omp.restype = ctypes.c_int
+        L.orc_force_density_omp.argtypes = [ctypes.c_int, ctypes.POINTER(_OrcBlock), P, P]
+        L.orc_force_density_omp.restype = ctypes.c_int
         L.orc_contact_neighs.argtypes = [I64, P, I64, P, D, I64, P]
+        L.orc_contact_neighs_omp.argtypes = [I64, P, I64, P, D, I64, P]
         L.orc_contact_force.argtypes = [I64, P, P, P, P, I64, P, P, P, P, I64, P, D, D, D, ctypes.c_int]
         _lib = L
     return _lib
@@ -355,8 +356,14 @@ class OracleBlock:
             setattr(s, name, _p(getattr(self, name)).value)
         return s
 
-    def force_density(self, y_blk: np.ndarray, f_blk: np.ndarray):
+    def force_density(self, y_blk: np.ndarray, f_blk: np.ndarray, threaded: bool = False) -> int:
+        """Adds the block's force density to f_blk; returns the threads used.
+        threaded: OpenMP loops, for timing only (f then differs from the
+        single-threaded sums by rounding)."""
+        if threaded:
+            return lib().orc_force_density_omp(self.kind, ctypes.byref(self._s), _p(y_blk), _p(f_blk))
         lib().orc_force_density(self.kind, ctypes.byref(self._s), _p(y_blk), _p(f_blk))
+        return 1
 
 
 class OracleEnv:
@@ -400,6 +407,10 @@ class OracleEnv:
                 d.setdefault("applyafter", 0)
             self.bcs.append(d)
         self.time_step = 0
+        # threaded: the reference's threaded loops on OpenMP threads, for the
+        # CPU baseline's timing only; threads_used is what the last force pass used
+        self.threaded = False
+        self.threads_used = 1
 
     # -- contacts ----------------------------------------------------------
     def _make_contact(self, c: dict) -> dict:
@@ -448,8 +459,8 @@ class OracleEnv:
                 continue
             y1 = np.ascontiguousarray(self.y[c["mask1"]])
             y2 = np.ascontiguousarray(self.y[c["mask2"]])
-            lib().orc_contact_neighs(y1.shape[0], _p(y1), y2.shape[0], _p(y2), c["distance"],
-                                     c["max_neighs"], _p(c["neighs"]))
+            search = lib().orc_contact_neighs_omp if self.threaded else lib().orc_contact_neighs
+            search(y1.shape[0], _p(y1), y2.shape[0], _p(y2), c["distance"], c["max_neighs"], _p(c["neighs"]))
 
     def _contact_force(self, c: dict):
         if "self_block" in c:
@@ -488,7 +499,7 @@ class OracleEnv:
                 if self._undeformed(b, lo, hi):
                     continue
                 b.deformed = True
-            b.force_density(self.y[lo:hi], self.f[lo:hi])
+            self.threads_used = b.force_density(self.y[lo:hi], self.f[lo:hi], self.threaded)
         for c in self.contacts:
             self._contact_force(c)
         # update_misc! (solvers.jl:289-291): momentum from the current velocities

@@ -2,7 +2,8 @@
 //
 // A single-threaded C++ restatement of the reference's explicit-dynamics loops,
 // used by tests/, bench.py's cpu_baseline leg and __graft_entry__.smoke() as the
-// checker. The product (peridyn_amd/) never links, loads or calls this file.
+// checker.  The cpu_baseline leg times it with the reference's threaded loops
+// on OpenMP threads (the *_omp entry points and the `par` loops). The product (peridyn_amd/) never links, loads or calls this file.
 //
 // Arithmetic rules (the reference is FP64 Julia with no contraction):
 //   * compiled with -O2 -ffp-contract=off (no FMA fusion, no fast-math);
@@ -209,12 +210,19 @@ void orc_weighted_volume(int64_t n, int64_t Mp, const int64_t* fam, const double
   }
 }
 
+// The per-particle loops below take `par`: true runs them on OpenMP threads, for
+// the CPU baseline's timing only (the reference runs them under Threads.@threads,
+// src/peri.jl:158,420).  An iteration writes only particle i's entries, so the
+// threaded loops give exactly the single-threaded results.
+
 // dilatation! cpu (src/peri.jl:155-172):
 // theta_i += c_i * w * xij * extension * vol_j * horizon_correction, c_i = 3/m_i
-void orc_dilatation(int64_t n, int64_t Mp, const int64_t* fam, const uint8_t* intact,
-                    const double* x, const double* y, const double* vol, const double* m,
-                    double* theta) {
+static void dilatation_loop(int64_t n, int64_t Mp, const int64_t* fam, const uint8_t* intact,
+                            const double* x, const double* y, const double* vol, const double* m,
+                            double* theta, bool par) {
+  (void)par;
   for (int64_t i = 0; i < n; ++i) theta[i] = 0.0;
+#pragma omp parallel for schedule(static) if (par)
   for (int64_t i = 0; i < n; ++i) {
     const double c_i = 3.0 / m[i];
     for (int64_t k = 0; k < Mp; ++k) {
@@ -230,6 +238,12 @@ void orc_dilatation(int64_t n, int64_t Mp, const int64_t* fam, const uint8_t* in
       }
     }
   }
+}
+
+void orc_dilatation(int64_t n, int64_t Mp, const int64_t* fam, const uint8_t* intact,
+                    const double* x, const double* y, const double* vol, const double* m,
+                    double* theta) {
+  dilatation_loop(n, Mp, fam, intact, x, y, vol, m, theta, false);
 }
 
 // Material kinds.
@@ -257,9 +271,11 @@ struct OrcBlock {
 // EP td_norm2 pass: out_of_loop! (src/materials/elasto_plastic.jl:168-172) ->
 // loop_over_neighs! (src/peri.jl:411-445) with td_norm2_ij! (elasto_plastic.jl:282-295).
 // Breaks bonds with s >= s0 (peri.jl:439).
-void orc_td_norm2(const OrcBlock* b, const double* y) {
+static void td_norm2_loop(const OrcBlock* b, const double* y, bool par) {
+  (void)par;
   const int64_t n = b->n, Mp = b->Mp, nt = b->ntype;
   for (int64_t i = 0; i < n; ++i) b->td_norm2[i] = 0.0;
+#pragma omp parallel for schedule(static) if (par)
   for (int64_t i = 0; i < n; ++i) {
     for (int64_t k = 0; k < Mp; ++k) {
       const size_t s_ik = (size_t)i * Mp + k;
@@ -292,6 +308,8 @@ void orc_td_norm2(const OrcBlock* b, const double* y) {
     }
   }
 }
+
+void orc_td_norm2(const OrcBlock* b, const double* y) { td_norm2_loop(b, y, false); }
 
 // force_density_t_ij plug-ins.
 // BB  src/materials/bond_based.jl:225-232 (items[1] = c[1,1], bond_force = s*c :195-197)
@@ -375,26 +393,35 @@ void orc_force_density(int kind, const OrcBlock* b, const double* y, double* f) 
   }
 }
 
-// Multithreaded CPU baseline for timing only (never a parity source): the same
-// BB loop nest with OpenMP over particles and thread-private force arrays, the
-// race-free equivalent of the reference's threaded scatter.  Returns threads used.
-int orc_force_density_bb_omp(const OrcBlock* b, const double* y, double* f) {
+// force_density for the CPU baseline's timing (never a parity source): the
+// loops of orc_force_density with OpenMP over particles.  The pre-passes equal
+// the single-threaded ones exactly; the force loop's scatter to j goes to one
+// array per thread, summed afterwards (the race-free form of the reference's
+// threaded scatter, general_material.jl:330), so f differs from the
+// single-threaded sums by rounding.  Returns the threads used.
+int orc_force_density_omp(int kind, const OrcBlock* b, const double* y, double* f) {
   const int64_t n = b->n, Mp = b->Mp, nt = b->ntype;
+  if (kind == ORC_OSB || kind == ORC_EP)
+    dilatation_loop(n, Mp, b->fam, b->intact, b->x, y, b->vol, b->m, b->theta, true);
+  if (kind == ORC_EP) td_norm2_loop(b, y, true);
   const double* x = b->x;
-  int used = 1;
+  int nth = 1;
 #ifdef _OPENMP
-#pragma omp parallel
+  nth = omp_get_max_threads();
+#endif
+  static std::vector<double> fl;  // per-thread force arrays, kept between calls
+  const size_t stride = (size_t)3 * n;
+  if (fl.size() < stride * nth) fl.resize(stride * nth);
+#pragma omp parallel num_threads(nth)
   {
-#pragma omp single
-    used = omp_get_num_threads();
-    std::vector<double> fl((size_t)3 * n, 0.0);
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* ft = fl.data() + stride * tid;
+    std::fill(ft, ft + stride, 0.0);
 #pragma omp for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
-#else
-  {
-    std::vector<double> fl((size_t)3 * n, 0.0);
-    for (int64_t i = 0; i < n; ++i) {
-#endif
       const int64_t t1 = b->type[i] - (b->type_start - 1);
       const double den_i = b->density[t1 - 1];
       for (int64_t k = 0; k < Mp; ++k) {
@@ -407,26 +434,32 @@ int orc_force_density_bb_omp(const OrcBlock* b, const double* y, double* f) {
         const double Y1 = y[3 * j] - y[3 * i], Y2 = y[3 * j + 1] - y[3 * i + 1], Y3 = y[3 * j + 2] - y[3 * i + 2];
         const double yij = mag3(Y1, Y2, Y3);
         const double xij = mag3(X1, X2, X3);
-        const double s = (yij - xij) / xij;
+        const double extension = yij - xij;
+        const double s = extension / xij;
         if (s < mat_at(b->s0, nt, t1, t2)) {
-          const double xx = 0.5 * (s * b->c[0]) / yij;
-          fl[3 * i + 0] += xx * Y1 * b->vol[j] / den_i;
-          fl[3 * i + 1] += xx * Y2 * b->vol[j] / den_i;
-          fl[3 * i + 2] += xx * Y3 * b->vol[j] / den_i;
-          fl[3 * j + 0] -= xx * Y1 * b->vol[i] / den_j;
-          fl[3 * j + 1] -= xx * Y2 * b->vol[i] / den_j;
-          fl[3 * j + 2] -= xx * Y3 * b->vol[i] / den_j;
+          const double wij = 1.0 / mag3(X1, X2, X3);
+          const double wji = 1.0 / mag3(-X1, -X2, -X3);
+          const double tij = orc_t_ij(kind, b, i, j, s_ik, t1, t2, xij, yij, extension, s, wij, wji);
+          const double xx = 0.5 * tij / yij;
+          ft[3 * i + 0] += xx * Y1 * b->vol[j] / den_i;
+          ft[3 * i + 1] += xx * Y2 * b->vol[j] / den_i;
+          ft[3 * i + 2] += xx * Y3 * b->vol[j] / den_i;
+          ft[3 * j + 0] -= xx * Y1 * b->vol[i] / den_j;
+          ft[3 * j + 1] -= xx * Y2 * b->vol[i] / den_j;
+          ft[3 * j + 2] -= xx * Y3 * b->vol[i] / den_j;
         } else {
           b->intact[s_ik] = 0;
         }
       }
     }
-#ifdef _OPENMP
-#pragma omp critical
-#endif
-    for (int64_t q = 0; q < 3 * n; ++q) f[q] += fl[q];
   }
-  return used;
+#pragma omp parallel for schedule(static) num_threads(nth)
+  for (int64_t q = 0; q < (int64_t)stride; ++q) {
+    double acc = 0.0;
+    for (int t = 0; t < nth; ++t) acc += fl[stride * t + q];
+    f[q] += acc;
+  }
+  return nth;
 }
 
 // ---------------------------------------------------------------------------
@@ -436,8 +469,9 @@ int orc_force_density_bb_omp(const OrcBlock* b, const double* y, double* f) {
 // block-1 particle, the `max_neighs` largest block-2 indices (1-based) within
 // `distance`, descending, zero padded.
 // ---------------------------------------------------------------------------
-void orc_contact_neighs(int64_t n1, const double* y1, int64_t n2, const double* y2,
-                        double distance, int64_t max_neighs, int64_t* neighs_out) {
+static void contact_neighs_loop(int64_t n1, const double* y1, int64_t n2, const double* y2,
+                                double distance, int64_t max_neighs, int64_t* neighs_out, bool par) {
+  (void)par;
   std::memset(neighs_out, 0, sizeof(int64_t) * (size_t)max_neighs * n1);
   if (n1 <= 0 || n2 <= 0) return;
   double mn1[3], mx1[3], mn2[3], mx2[3], bmin[3], bmax[3];
@@ -469,8 +503,13 @@ void orc_contact_neighs(int64_t n1, const double* y1, int64_t n2, const double* 
     if (ok) in2.push_back(i);
   }
   const double dist2 = distance * distance;
+  const int64_t nin1 = (int64_t)in1.size();
+#pragma omp parallel if (par)
+  {
   std::vector<int64_t> cand;
-  for (int64_t i : in1) {
+#pragma omp for schedule(dynamic, 64)
+  for (int64_t q1 = 0; q1 < nin1; ++q1) {
+    const int64_t i = in1[q1];
     cand.clear();
     const double a1 = y1[3 * i], b1 = y1[3 * i + 1], c1 = y1[3 * i + 2];
     for (int64_t j : in2) {
@@ -484,6 +523,19 @@ void orc_contact_neighs(int64_t n1, const double* y1, int64_t n2, const double* 
     for (int64_t q = (int64_t)cand.size() - 1; q >= 0 && r < max_neighs; --q, ++r)
       neighs_out[(size_t)i * max_neighs + r] = cand[q];
   }
+  }
+}
+
+void orc_contact_neighs(int64_t n1, const double* y1, int64_t n2, const double* y2,
+                        double distance, int64_t max_neighs, int64_t* neighs_out) {
+  contact_neighs_loop(n1, y1, n2, y2, distance, max_neighs, neighs_out, false);
+}
+
+// The same with the block-1 loop on OpenMP threads (contacts.jl:459 threads it);
+// rows are independent, so the lists equal the single-threaded ones.
+void orc_contact_neighs_omp(int64_t n1, const double* y1, int64_t n2, const double* y2,
+                            double distance, int64_t max_neighs, int64_t* neighs_out) {
+  contact_neighs_loop(n1, y1, n2, y2, distance, max_neighs, neighs_out, true);
 }
 
 // short_range_contact! RM12 cpu (src/contacts/contacts.jl:210-247) with the

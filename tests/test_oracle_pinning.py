@@ -495,3 +495,41 @@ def test_rm11_force_two_particle_closed_form(orc):
     F = (K * strain * strain) / (0.6 + 1e-10) * (y[0] - y[1])
     assert np.allclose(f[0], F * (vol[1] / den[0]), rtol=1e-15, atol=0)
     assert np.allclose(f[1], -F * (vol[1] / den[1]), rtol=1e-15, atol=0)  # vol_j on both sides
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+def test_threaded_timing_loops_match_single_thread(orc, kind):
+    """The CPU baseline times the oracle with the reference's threaded loops on
+    OpenMP threads.  Its pre-passes and bond states must equal the
+    single-threaded oracle exactly, and its forces must equal them up to the
+    rounding of the per-thread scatter sums, so the timed work is the same."""
+    spec = W.tensile_bb(n=10, kind=kind)
+    b = spec["blocks"][0]
+    rng = np.random.default_rng(5)
+    b["y"] = b["x"] * 1.002 + rng.normal(scale=4e-3, size=b["x"].shape)
+    if kind == "ep":
+        b["yield_stress"] = 1e-3  # yields: the EP update writes edp
+    a, t = orc.OracleEnv(spec), orc.OracleEnv(spec)
+    t.threaded = True
+    for _ in range(3):  # edp carries over between passes
+        a.update_acc()
+        t.update_acc()
+        ba, bt = a.blocks[0], t.blocks[0]
+        assert np.array_equal(ba.intact, bt.intact)
+        assert np.array_equal(ba.theta, bt.theta) and np.array_equal(ba.td_norm2, bt.td_norm2)
+        assert np.array_equal(ba.edp, bt.edp)
+        assert np.abs(a.f - t.f).max() <= 1e-12 * np.abs(a.f).max()
+    assert (ba.intact == 0).sum() > (ba.fam == 0).sum()  # some bonds broke
+    if kind == "ep":
+        assert np.abs(ba.edp).max() > 0
+
+
+def test_threaded_contact_search_matches_single_thread(orc):
+    spec = W.plates_ep(nx=10, ny=10, nz=4, gap=0.5)
+    a, t = orc.OracleEnv(spec), orc.OracleEnv(spec)
+    t.threaded = True
+    for e in (a, t):
+        e.y[e.limits[1][0]:] -= [0.0, 0.0, 1.2]  # interpenetrate: many candidates
+        e.update_neighs()
+    assert all(np.array_equal(ca["neighs"], ct["neighs"]) for ca, ct in zip(a.contacts, t.contacts))
+    assert sum(int((c["neighs"] > 0).sum()) for c in a.contacts) > 100
