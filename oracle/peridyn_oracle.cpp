@@ -409,14 +409,17 @@ int orc_force_density_omp(int kind, const OrcBlock* b, const double* y, double* 
 #ifdef _OPENMP
   nth = omp_get_max_threads();
 #endif
-  static std::vector<double> fl;  // per-thread force arrays, kept between calls
+  static std::vector<double> fl;  // per-thread force arrays, kept between calls (one caller at a time)
   const size_t stride = (size_t)3 * n;
   if (fl.size() < stride * nth) fl.resize(stride * nth);
+  int used = 1;  // the team actually started; only its arrays are zeroed and summed
 #pragma omp parallel num_threads(nth)
   {
     int tid = 0;
 #ifdef _OPENMP
     tid = omp_get_thread_num();
+#pragma omp single
+    used = omp_get_num_threads();
 #endif
     double* ft = fl.data() + stride * tid;
     std::fill(ft, ft + stride, 0.0);
@@ -456,10 +459,10 @@ int orc_force_density_omp(int kind, const OrcBlock* b, const double* y, double* 
 #pragma omp parallel for schedule(static) num_threads(nth)
   for (int64_t q = 0; q < (int64_t)stride; ++q) {
     double acc = 0.0;
-    for (int t = 0; t < nth; ++t) acc += fl[stride * t + q];
+    for (int t = 0; t < used; ++t) acc += fl[stride * t + q];
     f[q] += acc;
   }
-  return nth;
+  return used;
 }
 
 // ---------------------------------------------------------------------------
