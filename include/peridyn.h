@@ -4,7 +4,7 @@
  * The reference is Julia; its GPU seam is multiple dispatch on a device tag
  * (DEVICE[], src/misc/const_macros.jl:10) reaching per-device methods such as
  * force_density_T!(f, y, limits, mat, Val{device}) (src/materials/general_material.jl:274-276).
- * A Julia shim binds these entry points with ccall.  Every
+ * A Julia shim (julia/PeriDynHIP.jl) binds these entry points with ccall.  Every
  * signature uses plain pointers and sizes; arrays use the reference's layouts:
  *   positions / velocities / forces  3 x n column-major Float64 (x[3*i + d])
  *   family / intact                  M' x n column-major, 1-based ids, 0 padded
