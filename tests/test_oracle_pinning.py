@@ -533,3 +533,16 @@ def test_threaded_contact_search_matches_single_thread(orc):
         e.update_neighs()
     assert all(np.array_equal(ca["neighs"], ct["neighs"]) for ca, ct in zip(a.contacts, t.contacts))
     assert sum(int((c["neighs"] > 0).sum()) for c in a.contacts) > 100
+
+
+def test_cpu_baseline_leg_runs_threaded(orc):
+    """bench.py's cpu_baseline leg on a small specimen: it steps the threaded
+    oracle and reports the threads it used."""
+    import bench
+    spec = W.tensile_bb(n=8, kind="osb")
+    env = orc.OracleEnv(spec)
+    nb = int(sum((b.intact > 0).sum() for b in env.blocks))
+    cb = bench.cpu_baseline(spec, 10, 0.2, nb)
+    assert cb["unit"] == "bond-updates/s" and cb["kind"] == "port"
+    assert cb["value"] > 0 and cb["cores"] >= 1
+    assert f"{cb['cores']} OpenMP threads" in cb["sample"]
