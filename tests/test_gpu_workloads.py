@@ -1,7 +1,8 @@
 """The benchmark workloads at full size on the GPU: family sizes against the
 committed lattice fixture, the tensile block of the bond-based config
-(20^3, dt = 1e-3, 200 steps) against the oracle, and conservation of total
-momentum by the full-size contact workloads.
+(20^3, dt = 1e-3, 200 steps) against the oracle, conservation of total
+momentum by the full-size contact workloads, and the FP32-compensated form
+against FP64 on the largest batched specimens.
 
 Bars: bond counts exact; reference-order form bit-identical to the oracle over
 the whole literal run; FP64 form: the literal run (dt = 1e-3 is close to the
@@ -107,3 +108,35 @@ def test_full_size_momentum_is_conserved(core, name):
     assert drift <= 1e-9
     assert not np.allclose(v, v0)  # the specimens did collide
     g.close()
+
+
+def test_batched_fp32c_matches_fp64_at_full_size(core):
+    """The FP32-compensated form on the largest specimens of the 256-specimen
+    batch (BB and OSB, up to 60k particles), batched together for the bench's
+    500 steps, against the same specimens batched in plain FP64.  Bars: relative
+    L2 displacement <= 1e-5 per specimen and identical bond masks."""
+    specs = W.batched_specimens()
+    size = [len(s["blocks"][0]["x"]) for s in specs]
+    idx = sorted(range(len(specs)), key=lambda i: -size[i])[:8]
+    e32 = [core.Env.from_spec(specs[i], precision="fp32c") for i in idx]
+    e64 = [core.Env.from_spec(specs[i], precision="fp64") for i in idx]
+    say("fp32c vs fp64, specimens", [(specs[i]["blocks"][0]["kind"], size[i]) for i in idx])
+    core.run_batch(e32, 500)
+    core.run_batch(e64, 500)
+    worst = 0.0
+    broken = 0
+    for a, b, i in zip(e32, e64, idx):
+        x = specs[i]["blocks"][0]["x"]
+        ya, _, _ = a.download()
+        yb, _, _ = b.download()
+        ed = rel_l2(ya - x, yb - x)
+        worst = max(worst, ed)
+        _, ia = a.family(0)
+        _, ib = b.family(0)
+        assert np.array_equal(ia, ib), f"specimen {i}: bond masks differ"
+        fam, _ = b.family(0)
+        broken += int(((fam > 0) & (ib == 0)).sum())
+        assert ed <= 1e-5, f"specimen {i}: displacement rel L2 {ed:.2e}"
+    say(f"worst displacement rel L2 {worst:.2e}, {broken} bonds broken in all")
+    for e in e32 + e64:
+        e.close()
