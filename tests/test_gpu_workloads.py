@@ -140,3 +140,32 @@ def test_batched_fp32c_matches_fp64_at_full_size(core):
     say(f"worst displacement rel L2 {worst:.2e}, {broken} bonds broken in all")
     for e in e32 + e64:
         e.close()
+
+
+def test_plates_fp64_matches_reference_order_at_full_size(core):
+    """The full-size EP plates (100k particles, nonlinear contact): the tiled FP64
+    form against the reference-order form (bit-identical to the oracle on every
+    smaller case) for 50 steps, through the collision and five contact
+    rebuilds.  Bars: relative L2 of displacement and force <= 1e-10, identical
+    bond masks.  (Measured: 1.3e-14 and 4.2e-11.  By step 60 the two forms
+    differ at the contact layers only, after the rebuild at step 50: the lists
+    are a discrete function of the positions, the max_neighs largest ids within
+    the search distance, so a pair at that distance can enter one form's list
+    and not the other's.)"""
+    spec = W.plates_ep()
+    ge = core.Env.from_spec(spec, precision="exact")
+    gf = core.Env.from_spec(spec, precision="fp64")
+    core.run_batch([ge], 50, 10)
+    core.run_batch([gf], 50, 10)
+    ye, ve, fe = ge.download()
+    yf, vf, ff = gf.download()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    ed, ef = rel_l2(yf - x, ye - x), rel_l2(ff, fe)
+    say(f"plates fp64 vs reference order: displacement rel L2 {ed:.2e}, force rel L2 {ef:.2e}")
+    for k in range(2):
+        _, ie = ge.family(k)
+        _, i_f = gf.family(k)
+        assert np.array_equal(ie, i_f), f"block {k}: bond masks differ"
+    assert ed <= 1e-10 and ef <= 1e-10
+    ge.close()
+    gf.close()
