@@ -1,8 +1,9 @@
 """The benchmark workloads at full size on the GPU: family sizes against the
 committed lattice fixture, the tensile block of the bond-based config
 (20^3, dt = 1e-3, 200 steps) against the oracle, conservation of total
-momentum by the full-size contact workloads, and the FP32-compensated form
-against FP64 on the largest batched specimens.
+momentum by the full-size contact workloads, the FP32-compensated form
+against FP64 on the largest batched specimens, and the tiled FP64 form
+against the reference-order form on the full-size plates and disk.
 
 Bars: bond counts exact; reference-order form bit-identical to the oracle over
 the whole literal run; FP64 form: the literal run (dt = 1e-3 is close to the
@@ -162,6 +163,30 @@ def test_plates_fp64_matches_reference_order_at_full_size(core):
     x = np.concatenate([b["x"] for b in spec["blocks"]])
     ed, ef = rel_l2(yf - x, ye - x), rel_l2(ff, fe)
     say(f"plates fp64 vs reference order: displacement rel L2 {ed:.2e}, force rel L2 {ef:.2e}")
+    for k in range(2):
+        _, ie = ge.family(k)
+        _, i_f = gf.family(k)
+        assert np.array_equal(ie, i_f), f"block {k}: bond masks differ"
+    assert ed <= 1e-10 and ef <= 1e-10
+    ge.close()
+    gf.close()
+
+
+def test_disk_fp64_matches_reference_order_at_full_size(core):
+    """The full-size BB disk impact (721k particles, linear contact): the tiled
+    FP64 form against the reference-order form for 50 steps with contact
+    rebuilds every 10.  Bars: relative L2 of displacement and force <= 1e-10,
+    identical bond masks."""
+    spec = W.disk_impact()
+    ge = core.Env.from_spec(spec, precision="exact")
+    gf = core.Env.from_spec(spec, precision="fp64")
+    core.run_batch([ge], 50, 10)
+    core.run_batch([gf], 50, 10)
+    ye, ve, fe = ge.download()
+    yf, vf, ff = gf.download()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    ed, ef = rel_l2(yf - x, ye - x), rel_l2(ff, fe)
+    say(f"disk fp64 vs reference order: displacement rel L2 {ed:.2e}, force rel L2 {ef:.2e}")
     for k in range(2):
         _, ie = ge.family(k)
         _, i_f = gf.family(k)
