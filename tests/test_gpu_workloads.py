@@ -3,7 +3,8 @@ committed lattice fixture, the tensile block of the bond-based config
 (20^3, dt = 1e-3, 200 steps) against the oracle, conservation of total
 momentum by the full-size contact workloads, the FP32-compensated form
 against FP64 on the largest batched specimens, and the tiled FP64 form
-against the reference-order form on the full-size plates and disk.
+against the reference-order form on the full-size plates and disk, and
+specimens of the full batch against the same specimens run alone.
 
 Bars: bond counts exact; reference-order form bit-identical to the oracle over
 the whole literal run; FP64 form: the literal run (dt = 1e-3 is close to the
@@ -194,3 +195,27 @@ def test_disk_fp64_matches_reference_order_at_full_size(core):
     assert ed <= 1e-10 and ef <= 1e-10
     ge.close()
     gf.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_full_batch_results_equal_alone(core, prec):
+    """All 256 specimens of the batched workload (4.3M particles) stepped together
+    for 100 steps: four of them, spread over the size range, equal the same
+    specimens run alone bit for bit (specimens share no data; the lane schedule
+    is the same)."""
+    specs = W.batched_specimens()
+    envs = [core.Env.from_spec(s, precision=prec) for s in specs]
+    core.run_batch(envs, 100)
+    size = [len(s["blocks"][0]["x"]) for s in specs]
+    order = sorted(range(len(specs)), key=lambda i: size[i])
+    picks = [order[0], order[85], order[170], order[-1]]
+    say(f"{prec}: batch of {len(envs)}, checking specimens of {[size[i] for i in picks]} particles")
+    for i in picks:
+        alone = core.Env.from_spec(specs[i], precision=prec)
+        core.run_batch([alone], 100)
+        ya, va, fa = alone.download()
+        yb, vb, fb = envs[i].download()
+        assert np.array_equal(ya, yb) and np.array_equal(va, vb) and np.array_equal(fa, fb), f"specimen {i}"
+        alone.close()
+    for e in envs:
+        e.close()
