@@ -633,6 +633,18 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
   }
 }
 
+// A block's slices follow from its own size only (kLatchSlice particles each,
+// at most kLatchMaxSlices), so its sums, and the step it latches on, do not
+// depend on the other entries of the launch: a specimen latches on the same
+// step alone and batched.  The grid's y extent (nparts) is the largest entry's
+// slice count; other entries skip the slices beyond their own.
+constexpr int kLatchSlice = 256 * 16;
+constexpr int kLatchMaxSlices = 1024;
+__host__ __device__ __forceinline__ int latch_slices(int64_t n) {
+  const int64_t s = (n + kLatchSlice - 1) / kLatchSlice;
+  return (int)(s < 1 ? 1 : (s > kLatchMaxSlices ? kLatchMaxSlices : s));
+}
+
 __global__ __launch_bounds__(256) void k_latch_part(EnvDev* __restrict__ envs, const int2* __restrict__ items,
                                                     int nparts, double4* __restrict__ part) {
   const int item = blockIdx.x;  // entries on x (no 65535 limit), slices on y
@@ -640,10 +652,12 @@ __global__ __launch_bounds__(256) void k_latch_part(EnvDev* __restrict__ envs, c
   const EnvDev& E = envs[it.x];
   const BlockDev& B = E.blocks[it.y];
   if (B.deformed) return;  // uniform per workgroup
+  const int ns = latch_slices(B.n);
+  if ((int)blockIdx.y >= ns) return;  // uniform per workgroup
   const double4* __restrict__ y = E.y[E.cur];
   const double4* __restrict__ x = E.x;
   const double4 y1 = y[B.first], x1 = x[B.first];
-  const int per = (B.n + nparts - 1) / nparts;
+  const int per = (B.n + ns - 1) / ns;
   const int lo = blockIdx.y * per;
   const int hi = min(B.n, lo + per);
   double sd = 0.0, sa = 0.0, sx = 0.0;
@@ -669,8 +683,9 @@ __global__ __launch_bounds__(64) void k_latch_fin(EnvDev* __restrict__ envs, con
   EnvDev& E = envs[it.x];
   BlockDev& B = E.blocks[it.y];
   if (B.deformed) return;
+  const int ns = latch_slices(B.n);
   double D = 0.0, A = 0.0, X = 0.0;
-  for (int q = threadIdx.x; q < nparts; q += 64) {
+  for (int q = threadIdx.x; q < ns; q += 64) {
     const double4 v = part[(size_t)blockIdx.x * nparts + q];
     D += v.x; A += v.y; X += v.z;
   }
@@ -906,10 +921,7 @@ void pd_launch_book(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, in
   hipLaunchKernelGGL(k_book, dim3((n + 255) / 256), dim3(256), 0, s, envs, slots, n, flags);
 }
 
-int pd_latch_parts(int64_t max_n) {
-  const int64_t per = 256 * 16;  // particles per slice
-  return (int)std::min<int64_t>(1024, std::max<int64_t>(1, (max_n + per - 1) / per));
-}
+int pd_latch_parts(int64_t max_n) { return latch_slices(max_n); }
 
 // deformed latch of each (env slot, block) item, for one readback
 __global__ void k_latch_flags(const EnvDev* __restrict__ envs, const int2* __restrict__ items, int n,

@@ -101,3 +101,39 @@ def test_batched_specimens_equal_alone(core):
             assert np.array_equal(u, w)
         a.close()
         e.close()
+
+
+@pytest.mark.xfail(strict=False, reason=(
+    "open defect: request 37 (indentation, OSB, 10,445 particles) differs from its alone run by up to "
+    "6e-8 in y after its first served chunk (a set of 44 resident envs mid-run); batching the same "
+    "request with every other request from step 0, in one call or in pairs, is bit-identical, so the "
+    "difference comes from joining a set whose envs are mid-run; cause not yet found"))
+def test_full_served_stream_equals_alone(core):
+    """The served benchmark's stream at full size (64 Poisson requests,
+    1k-200k particles, 200-2000 steps, tensile and indentation, BB and OSB),
+    served as the bench serves it (FP64, 4 lanes per particle, 50-step chunks):
+    six requests spread over the size range and both shapes equal the same
+    request run alone in one call, bit for bit."""
+    reqs = W.served_requests(count=64, seed=20261204, rate=200.0)
+    for r in reqs:
+        r["spec"] = W.make_spec(r)
+    ctx = core.Context(0)
+    srv = serve.Server(serve.GpuBackend(ctx, "fp64", keep_state=True, lanes_per_particle=4),
+                       int(0.8 * ctx.mem_info()[0]), chunk=50)
+    say(f"serve the full stream of {len(reqs)} requests")
+    res, st = srv.serve(reqs)
+    say(f"  done: {st.completed} sims, max resident {st.max_active}, {st.chunks} chunks, {st.makespan:.2f} s")
+    assert st.completed == len(reqs) and st.max_active > 32
+    order = sorted(reqs, key=lambda r: r["n"])
+    picks = [order[k] for k in (0, 13, 26, 39, 52, 63)]
+    picks += [r for r in order[::-1] if r["shape"] not in {p["shape"] for p in picks}][:1]
+    for r in picks:
+        e = core.Env.from_spec(r["spec"], precision="fp64", ctx=ctx, lanes_per_particle=4)
+        e.run(r["steps"], r["nuf"])
+        y, v, f = e.download()
+        d = e.damage()
+        e.close()
+        got = res[r["id"]]
+        say(f"  request {r['id']}: {r['shape']} {r['kind']} n={r['n']} steps={r['steps']}")
+        assert np.array_equal(got.y, y) and np.array_equal(got.v, v), r["id"]
+        assert np.array_equal(got.f, f) and np.array_equal(got.damage, d), r["id"]
