@@ -107,10 +107,19 @@ def tensile_bb(n: int = 20, dt: float = 1e-3, pull_per_step: float = 0.01, kind:
 
 
 def plates_ep(nx: int = 50, ny: int = 50, nz: int = 20, gap: float = 1.0, dt: float = 5e-5,
-              approach_per_step: float = 0.05, exponent: int = 2) -> dict:
-    """Two elastoplastic plates approaching each other (state-based collision)."""
+              approach_per_step: float = 0.05, exponent: int = 2, shift: float = 0.0) -> dict:
+    """Two elastoplastic plates approaching each other (state-based collision).
+
+    shift: lateral offset (dx units, in x and y) of the second plate.  With 0
+    (the workload as specified) both plates share their x-y columns, and once
+    the weak contact lets the facing layers interpenetrate (from about step 47)
+    particles meet head-on at distances down to 1e-6 dx, where the contact
+    force's direction dr/(|dr| + 1e-10) amplifies any rounding difference about
+    5x per step.  shift = 0.5 puts the second plate's columns between the
+    first's, so facing particles never come closer than 0.5*sqrt(2) dx
+    laterally: the well-conditioned twin used for long FP64 comparisons."""
     xa = lattice_box(nx, ny, nz)
-    xb = lattice_box(nx, ny, nz, origin=(0.0, 0.0, float(nz - 1) + 1.0 + gap))
+    xb = lattice_box(nx, ny, nz, origin=(float(shift), float(shift), float(nz - 1) + 1.0 + gap))
     vz = 0.5 * approach_per_step / dt
     p = dict(bulk_modulus=50e3, shear_modulus=25e3, critical_stretch=0.01, density=RHO0, yield_stress=100.0)
     ba = block("ep", xa, DELTA, type_id=1, v=(0.0, 0.0, vz), **p)

@@ -546,3 +546,29 @@ def test_cpu_baseline_leg_runs_threaded(orc):
     assert cb["unit"] == "bond-updates/s" and cb["kind"] == "port"
     assert cb["value"] > 0 and cb["cores"] >= 1
     assert f"{cb['cores']} OpenMP threads" in cb["sample"]
+
+
+def test_cpu_baseline_batched_leg(orc):
+    """bench.py's batched cpu_baseline leg: the threaded oracle on a sample of
+    small specimens, with the sample stated."""
+    import bench
+    specs = [W.tensile(600, kind=k) for k in ("bb", "osb", "bb")]
+    cb = bench.cpu_baseline_batched(specs, 3, 0.0)
+    assert cb["unit"] == "bond-updates/s" and cb["kind"] == "port"
+    assert cb["value"] > 0 and cb["cores"] >= 1
+    assert cb["sample"].startswith("1 of the 3 specimens")
+
+
+def test_bench_rejects_impossible_roofline():
+    """A bench line whose dominant kernel claims more than the HBM peak, or
+    more time per step than the step took, is rejected instead of printed."""
+    import bench
+    ok = {"roofline": {"frac": 0.4, "avg_launch_ms": 0.5, "launches": 20},
+          "eager_profiled_run": {"ms_per_step": 1.0}}
+    bench.check_line(ok, 20)
+    for bad in ({"roofline": {"frac": 2.7, "avg_launch_ms": 0.01, "launches": 20},
+                 "eager_profiled_run": {"ms_per_step": 0.1}},
+                {"roofline": {"frac": 0.4, "avg_launch_ms": 0.5, "launches": 40},
+                 "eager_profiled_run": {"ms_per_step": 0.6}}):
+        with pytest.raises(SystemExit):
+            bench.check_line(bad, 20)
