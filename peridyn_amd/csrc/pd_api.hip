@@ -32,6 +32,11 @@ struct DevMem {
   static constexpr size_t kMaxBlock = 1ull << 30;    // larger blocks are freed at once
   std::multimap<size_t, void*> spare;                // size class -> block
   size_t cached = 0;
+  // poison: every block handed out is first filled with 0xFF bytes on `stream`
+  // (NaN doubles, -1 integers), so that a kernel reading memory it has not
+  // written gives results that differ from a run without it
+  hipStream_t stream = nullptr;
+  bool poison = false;
   // powers of two up to 1 MiB, then steps of 1/8 of the power of two below
   static size_t size_class(size_t bytes) {
     size_t p = 256;
@@ -43,6 +48,11 @@ struct DevMem {
     return (bytes + step - 1) / step * step;
   }
   void* get(size_t cls) {
+    void* p = take(cls);
+    if (poison) PD_HIP(hipMemsetAsync(p, 0xff, cls, stream));
+    return p;
+  }
+  void* take(size_t cls) {
     auto it = spare.find(cls);
     if (it != spare.end()) {
       void* p = it->second;
@@ -1170,12 +1180,13 @@ void commit_env(Ctx* c, Env& e) {
 
 // ---- run sets ----------------------------------------------------------------
 template <typename T>
-void ensure_cap(T*& p, size_t& cap, size_t n) {
+void ensure_cap(T*& p, size_t& cap, size_t n, const DevMem& dm) {
   if (n <= cap && p) return;
   if (p) PD_HIP(hipFree(p));
   p = nullptr;
   cap = std::max<size_t>(n, 16) * 2;
   PD_HIP(hipMalloc(&p, sizeof(T) * cap));
+  if (dm.poison) PD_HIP(hipMemsetAsync(p, 0xff, sizeof(T) * cap, dm.stream));
 }
 
 bool env_gpow(const Env& e) {
@@ -1259,7 +1270,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   c->n_theta_u = (int)thu.size();
   c->n_td_u = (int)tdu.size();
   for (int k = 0; k < 3; ++k) c->n_force_u[k] = (int)fku[k].size();
-  ensure_cap(c->d_set, c->set_cap, (size_t)n_all + rest.size());
+  ensure_cap(c->d_set, c->set_cap, (size_t)n_all + rest.size(), c->mem);
   {
     int off = 0;
     for (int64_t id : ids) {
@@ -1284,7 +1295,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
     c->ot_td[lg] = tot; c->nt_td[lg] = s_td[lg]; tot += s_td[lg];
   }
   if (tot > 0) {
-    ensure_cap(c->d_tset, c->tset_cap, (size_t)tot);
+    ensure_cap(c->d_tset, c->tset_cap, (size_t)tot, c->mem);
     int f_th[2][4] = {}, f_td[4] = {}, f_fk[3][2][4] = {};  // fill positions
     auto put = [&](int dst, const Env& e, const std::array<int, 3>& tb) {
       copies.push_back(PdCopy{reinterpret_cast<const uint32_t*>(e.d_tiles + tb[1]),
@@ -1335,21 +1346,21 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
         st[(size_t)cls * (ni + 1) + ni] = (int32_t)run;
         c->cgrid[cls] = (int)run;
       }
-      ensure_cap(c->d_citems, c->citems_cap, (size_t)ni);
-      ensure_cap(c->d_cstarts, c->cstarts_cap, st.size());
+      ensure_cap(c->d_citems, c->citems_cap, (size_t)ni, c->mem);
+      ensure_cap(c->d_cstarts, c->cstarts_cap, st.size(), c->mem);
       PD_HIP(hipMemcpyAsync(c->d_citems, items.data(), sizeof(int2) * ni, hipMemcpyHostToDevice, c->stream));
       PD_HIP(hipMemcpyAsync(c->d_cstarts, st.data(), sizeof(int32_t) * st.size(), hipMemcpyHostToDevice,
                             c->stream));
     }
   }
-  ensure_cap(c->d_slots, c->slots_cap, slots.size());
+  ensure_cap(c->d_slots, c->slots_cap, slots.size(), c->mem);
   PD_HIP(hipMemcpyAsync(c->d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, c->stream));
-  ensure_cap(c->d_obs_ranges, c->obs_ranges_cap, ranges.size());
+  ensure_cap(c->d_obs_ranges, c->obs_ranges_cap, ranges.size(), c->mem);
   PD_HIP(hipMemcpyAsync(c->d_obs_ranges, ranges.data(), sizeof(int2) * ranges.size(), hipMemcpyHostToDevice,
                         c->stream));
-  ensure_cap(c->d_obs_part, c->obs_part_cap, std::max<size_t>((size_t)n_all, 1));
+  ensure_cap(c->d_obs_part, c->obs_part_cap, std::max<size_t>((size_t)n_all, 1), c->mem);
   if (!copies.empty()) {
-    ensure_cap(c->d_copies, c->copies_cap, copies.size());
+    ensure_cap(c->d_copies, c->copies_cap, copies.size(), c->mem);
     PD_HIP(hipMemcpyAsync(c->d_copies, copies.data(), sizeof(PdCopy) * copies.size(), hipMemcpyHostToDevice,
                           c->stream));
     pd_launch_copy_words(c->stream, c->d_copies, (int)copies.size());
@@ -1389,9 +1400,9 @@ int prepare_latch(Ctx* c, const std::vector<int64_t>& ids) {
   }
   if (items.empty()) return 0;
   c->latch_nparts = pd_latch_parts(max_n);
-  ensure_cap(c->d_latch, c->latch_cap, items.size());
-  ensure_cap(c->d_latch_flags, c->latch_flags_cap, items.size());
-  ensure_cap(c->d_latch_part, c->latch_part_cap, items.size() * (size_t)c->latch_nparts);
+  ensure_cap(c->d_latch, c->latch_cap, items.size(), c->mem);
+  ensure_cap(c->d_latch_flags, c->latch_flags_cap, items.size(), c->mem);
+  ensure_cap(c->d_latch_part, c->latch_part_cap, items.size() * (size_t)c->latch_nparts, c->mem);
   PD_HIP(hipMemcpyAsync(c->d_latch, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
   return (int)items.size();
@@ -1410,7 +1421,12 @@ void refresh_latch(Ctx* c, const std::vector<int64_t>& ids, int nl) {
     Env& e = get_env(c, id);
     if (!e.maybe_undeformed) continue;
     bool any = false;
-    for (size_t b = 0; b < e.blocks.size(); ++b) any = any || !fl[k++];
+    // every block's flag is consumed, also after an unlatched one was found:
+    // the flags of the envs that follow start right after this env's blocks
+    for (size_t b = 0; b < e.blocks.size(); ++b) {
+      const bool latched = fl[k++] != 0;
+      any = any || !latched;
+    }
     e.maybe_undeformed = any;
   }
 }
@@ -1598,7 +1614,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
   hipStream_t s = c->stream;
   const int nenv = (int)ids.size();
   const int64_t nrows = every > 0 ? pd_log_rows(nsteps, every) : 0;
-  if (nrows > 0) ensure_cap(c->d_obs_rows, c->obs_rows_cap, (size_t)(nrows * nenv * 8));
+  if (nrows > 0) ensure_cap(c->d_obs_rows, c->obs_rows_cap, (size_t)(nrows * nenv * 8), c->mem);
   // run! start: update_neighs! (solvers.jl:114); apply_bc_at0 records
   // BC.last = y, which the resident state already holds.  Contact lists are
   // a deterministic function of the positions (sorted), so lists built from
@@ -1787,6 +1803,10 @@ int pd_ctx_create(int device, void** out) {
     c->device = device;
     hipError_t r = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (r != hipSuccess) { delete c; PD_HIP(r); }
+    // PERIDYN_POISON=1: fill device memory with 0xFF when handed out (tests)
+    const char* pz = std::getenv("PERIDYN_POISON");
+    c->mem.stream = c->stream;
+    c->mem.poison = pz && std::string(pz) == "1";
     pd_tile_allow_lds(kTileLdsBudget);
     *out = c;
   });
@@ -2112,7 +2132,7 @@ int pd_env_observables(void* ctx, int64_t env, double* out) {
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     build_set(c, {env});
-    ensure_cap(c->d_obs_rows, c->obs_rows_cap, 8);
+    ensure_cap(c->d_obs_rows, c->obs_rows_cap, 8, c->mem);
     hipStream_t s = c->stream;
     pd_launch_observables(s, 0, c->d_set, c->n_all, c->d_envs, c->d_obs_part, c->d_obs_ranges, 1, c->d_obs_rows);
     pd_launch_observables(s, 1, c->d_set, c->n_all, c->d_envs, c->d_obs_part, c->d_obs_ranges, 1, c->d_obs_rows);

@@ -17,9 +17,7 @@ each request rebuilds contacts at the same steps as when it runs alone in one
 run! call (run! counts steps from 1 in each call and rebuilds at
 i % neigh_update_freq == 0, src/solvers/solvers.jl:133,177).  Specimens share
 no data arrays.  Tests compare served results with the same request run alone:
-they are bit-identical in the small-request streams.  In the full benchmark
-stream, one indentation request still differs (by up to 6e-8 in y) after
-joining a set of envs that are mid-run.
+they are bit-identical, for every request of the full benchmark stream.
 
 The scheduling logic takes a backend object, so it runs unchanged over the HIP
 library (GpuBackend) or a host-side stand-in in tests.

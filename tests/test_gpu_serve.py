@@ -66,6 +66,30 @@ def test_served_mix_equals_alone(core, prec):
         assert np.array_equal(got.damage, d), r["id"]
 
 
+def test_served_mix_equals_alone_with_poisoned_memory(core, monkeypatch):
+    """The served mix in a context whose device memory is filled with 0xFF
+    bytes whenever the library hands a block out (PERIDYN_POISON=1: NaN doubles,
+    -1 integers), against the same requests alone in a context without it:
+    bit-identical, so no kernel reads device memory it has not written."""
+    reqs = requests(count=8, seed=13)
+    monkeypatch.setenv("PERIDYN_POISON", "1")
+    pz = core.Context(0)
+    monkeypatch.delenv("PERIDYN_POISON")
+    plain = core.Context(0)
+    srv = serve.Server(serve.GpuBackend(pz, precision="fp64"), budget_bytes=4 << 30, chunk=20)
+    res, st = srv.serve(reqs)
+    assert st.completed == len(reqs) and st.max_active > 1
+    for r in reqs:
+        e = core.Env.from_spec(r["spec"], precision="fp64", ctx=plain)
+        e.run(r["steps"], r["nuf"])
+        y, v, f = e.download()
+        e.close()
+        got = res[r["id"]]
+        assert np.array_equal(got.y, y) and np.array_equal(got.v, v) and np.array_equal(got.f, f), r["id"]
+    pz.close()
+    plain.close()
+
+
 def test_served_fp32c_within_bar_of_fp64(core):
     reqs = requests(count=6, seed=12)
     ctx = core.Context(0)
@@ -103,17 +127,11 @@ def test_batched_specimens_equal_alone(core):
         e.close()
 
 
-@pytest.mark.xfail(strict=False, reason=(
-    "open defect: request 37 (indentation, OSB, 10,445 particles) differs from its alone run by up to "
-    "6e-8 in y after its first served chunk (a set of 44 resident envs mid-run); batching the same "
-    "request with every other request from step 0, in one call or in pairs, is bit-identical, so the "
-    "difference comes from joining a set whose envs are mid-run; cause not yet found"))
 def test_full_served_stream_equals_alone(core):
     """The served benchmark's stream at full size (64 Poisson requests,
     1k-200k particles, 200-2000 steps, tensile and indentation, BB and OSB),
     served as the bench serves it (FP64, 4 lanes per particle, 50-step chunks):
-    six requests spread over the size range and both shapes equal the same
-    request run alone in one call, bit for bit."""
+    every request equals the same request run alone in one call, bit for bit."""
     reqs = W.served_requests(count=64, seed=20261204, rate=200.0)
     for r in reqs:
         r["spec"] = W.make_spec(r)
@@ -124,10 +142,7 @@ def test_full_served_stream_equals_alone(core):
     res, st = srv.serve(reqs)
     say(f"  done: {st.completed} sims, max resident {st.max_active}, {st.chunks} chunks, {st.makespan:.2f} s")
     assert st.completed == len(reqs) and st.max_active > 32
-    order = sorted(reqs, key=lambda r: r["n"])
-    picks = [order[k] for k in (0, 13, 26, 39, 52, 63)]
-    picks += [r for r in order[::-1] if r["shape"] not in {p["shape"] for p in picks}][:1]
-    for r in picks:
+    for r in reqs:
         e = core.Env.from_spec(r["spec"], precision="fp64", ctx=ctx, lanes_per_particle=4)
         e.run(r["steps"], r["nuf"])
         y, v, f = e.download()
@@ -137,3 +152,32 @@ def test_full_served_stream_equals_alone(core):
         say(f"  request {r['id']}: {r['shape']} {r['kind']} n={r['n']} steps={r['steps']}")
         assert np.array_equal(got.y, y) and np.array_equal(got.v, v), r["id"]
         assert np.array_equal(got.f, f) and np.array_equal(got.damage, d), r["id"]
+
+
+def test_latch_readback_after_a_two_block_env_with_a_pending_block(core):
+    """Regression: the deformed-latch readback of a run set walks the envs in
+    order and takes one flag per block.  A two-block env whose first block is
+    still undeformed must consume its second block's flag too; otherwise every
+    later env reads its neighbour's flags, and an env can be marked as fully
+    latched while one of its blocks is not, so that block's bond forces stay
+    skipped after it deforms.  Set: an indentation (both blocks rigid until the
+    indenter touches), a tensile box (latches on step 1) and two plates whose
+    first block is deformed from the start and whose second deforms only on
+    contact (step ~23).  The plates must equal the plates run alone."""
+    ctx = core.Context(0)
+    a = W.indentation(2000, steps=200)
+    t = W.tensile(1000)
+    b = W.plates_ep(nx=8, ny=8, nz=4)
+    x0 = b["blocks"][0]["x"]
+    b["blocks"][0]["y"] = x0 + 1e-6 * np.sin(np.arange(x0.size)).reshape(x0.shape)
+    envs = [core.Env.from_spec(s, precision="fp64", ctx=ctx) for s in (a, t, b)]
+    core.run_batch(envs, 60, 10)
+    alone = core.Env.from_spec(b, precision="fp64", ctx=ctx)
+    alone.run(60, 10)
+    ref = alone.download()
+    for u, w in zip(envs[2].download(), ref):
+        assert np.array_equal(u, w)
+    vz2 = ref[1][x0.shape[0]:, 2]
+    assert np.ptp(vz2) > 0.0  # the second plate deformed: its particles no longer move together
+    for e in envs + [alone]:
+        e.close()

@@ -2,14 +2,18 @@
 committed lattice fixture, the tensile block of the bond-based config
 (20^3, dt = 1e-3, 200 steps) against the oracle, conservation of total
 momentum by the full-size contact workloads, the FP32-compensated form
-against FP64 on the largest batched specimens, and the tiled FP64 form
-against the reference-order form on the full-size plates and disk, and
-specimens of the full batch against the same specimens run alone.
+against FP64 on the largest batched specimens, the plates and the disk at full
+size against the oracle, and specimens of the full batch against the same
+specimens run alone.
 
-Bars: bond counts exact; reference-order form bit-identical to the oracle over
-the whole literal run; FP64 form: the literal run (dt = 1e-3 is close to the
-stability limit, so rounding differences grow) is reported as the first step
-past 1e-10 relative L2, and the stable twin (dt = 1e-4) stays within 1e-10."""
+Bars: bond counts exact; reference-order form bit-identical to the oracle; FP64
+form within 1e-10 relative L2 of the oracle, identical bond masks.  Two configs
+are sensitive enough that rounding alone leaves 1e-10, and for those the bar is
+held on a stated twin: the literal tensile run (dt = 1e-3 is close to the
+stability limit) is reported as the first step past 1e-10 and its twin has
+dt = 1e-4; the literal plates meet head-on from step 47 and are compared
+through step 45, their twin (second plate shifted by half a spacing) for all
+500 steps."""
 import json
 import os
 import sys
@@ -144,33 +148,140 @@ def test_batched_fp32c_matches_fp64_at_full_size(core):
         e.close()
 
 
-def test_plates_fp64_matches_reference_order_at_full_size(core):
-    """The full-size EP plates (100k particles, nonlinear contact): the tiled FP64
-    form against the reference-order form (bit-identical to the oracle on every
-    smaller case) for 50 steps, through the collision and five contact
-    rebuilds.  Bars: relative L2 of displacement and force <= 1e-10, identical
-    bond masks.  (Measured: 1.3e-14 and 4.2e-11.  By step 60 the two forms
-    differ at the contact layers only, after the rebuild at step 50: the lists
-    are a discrete function of the positions, the max_neighs largest ids within
-    the search distance, so a pair at that distance can enter one form's list
-    and not the other's.)"""
+def _masks_equal(g, o, nblocks):
+    for k in range(nblocks):
+        _, ig = g.family(k)
+        io = o.blocks[k].intact
+        assert np.array_equal(ig, io), f"block {k}: bond masks differ ({int((ig != io).sum())} bonds)"
+
+
+def test_plates_literal_exact_matches_oracle_at_full_size(core, orc):
+    """Config 1 as specified (100k EP particles, nonlinear contact), 30 steps
+    through first contact (step 23) and the first contact rebuilds: the
+    reference-order form against the single-threaded oracle, bit for bit."""
     spec = W.plates_ep()
-    ge = core.Env.from_spec(spec, precision="exact")
-    gf = core.Env.from_spec(spec, precision="fp64")
-    core.run_batch([ge], 50, 10)
-    core.run_batch([gf], 50, 10)
-    ye, ve, fe = ge.download()
-    yf, vf, ff = gf.download()
-    x = np.concatenate([b["x"] for b in spec["blocks"]])
-    ed, ef = rel_l2(yf - x, ye - x), rel_l2(ff, fe)
-    say(f"plates fp64 vs reference order: displacement rel L2 {ed:.2e}, force rel L2 {ef:.2e}")
+    say("oracle: literal plates, 30 steps, single-threaded")
+    o = orc.OracleEnv(spec)
+    o.run(30, 10)
+    g = core.Env.from_spec(spec, precision="exact")
+    core.run_batch([g], 30, 10)
+    y, v, f = g.download()
+    assert np.array_equal(y, o.y) and np.array_equal(v, o.v) and np.array_equal(f, o.f)
+    _masks_equal(g, o, 2)
     for k in range(2):
-        _, ie = ge.family(k)
-        _, i_f = gf.family(k)
-        assert np.array_equal(ie, i_f), f"block {k}: bond masks differ"
+        assert np.array_equal(g.edp(k), o.blocks[k].edp)
+    g.close()
+
+
+def test_plates_literal_fp64_matches_oracle_until_head_on_contact(core, orc):
+    """Config 1 as specified, FP64 form (tiled) against the oracle through step
+    45, the last step before the facing layers meet head-on.  Both plates share
+    their x-y lattice columns and the contact (0.9 dx spring, K = 18*50e3/(pi
+    delta^4)) does not stop them: from step 47 particles of the two contact
+    layers pass each other at distances down to 1e-6 dx, where the contact
+    force's direction dr/(|dr| + 1e-10) turns any rounding difference into an
+    O(1) change.  The oracle itself is that sensitive: y perturbed by one ulp
+    leaves 1e-10 in force at step 51 and reaches 1.9e-5 at step 60 and 0.11 at
+    step 63 (test_oracle_pinning.py::test_plates_literal_config_is_ill_conditioned).
+    Past step 47 the 1e-10 bar is held on the twin (next test).  Bars: relative
+    L2 of displacement and force <= 1e-10, identical bond masks."""
+    spec = W.plates_ep()
+    o = orc.OracleEnv(spec)
+    o.threaded = True
+    o.run(45, 10)
+    g = core.Env.from_spec(spec, precision="fp64")
+    core.run_batch([g], 45, 10)
+    y, v, f = g.download()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    ed, ef = rel_l2(y - x, o.y - x), rel_l2(f, o.f)
+    say(f"literal plates, 45 steps, fp64 vs oracle: displacement {ed:.2e}, force {ef:.2e}")
+    _masks_equal(g, o, 2)
     assert ed <= 1e-10 and ef <= 1e-10
-    ge.close()
-    gf.close()
+    g.close()
+
+
+@pytest.mark.timeout(400)
+def test_plates_twin_fp64_matches_oracle_500_steps(core, orc):
+    """Config 1's well-conditioned twin for its full 500 steps: the second plate
+    shifted by half a lattice spacing in x and y (workloads.plates_ep(shift=0.5)),
+    so facing particles never meet head-on (closest contact pair >= 0.66 dx over
+    the run; in the oracle a one-ulp perturbation of y stays <= 2.6e-11 in force
+    and 6e-16 in y over all 500 steps).  Collision, contact every step, lists
+    rebuilt every 10 steps, yielding.  FP64 form (tiled) against the oracle
+    with its threaded loops (their force sums differ from the single-threaded
+    order by rounding only).  Bars: relative L2 of displacement, force and
+    plastic extensions <= 1e-10; identical bond masks."""
+    spec = W.plates_ep(shift=0.5)
+    g = core.Env.from_spec(spec, precision="fp64")
+    core.run_batch([g], 500, 10)
+    say("oracle: plates twin, 500 steps, threaded")
+    o = orc.OracleEnv(spec)
+    o.threaded = True
+    o.run(500, 10)
+    y, v, f = g.download()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    ed, ef = rel_l2(y - x, o.y - x), rel_l2(f, o.f)
+    ep = max(rel_l2(g.edp(k), o.blocks[k].edp) for k in range(2))
+    say(f"plates twin, 500 steps, fp64 vs oracle: displacement {ed:.2e}, force {ef:.2e}, edp {ep:.2e}, "
+        f"yielded bonds {sum(int((o.blocks[k].edp != 0).sum()) for k in range(2))}")
+    _masks_equal(g, o, 2)
+    assert ed <= 1e-10 and ef <= 1e-10 and ep <= 1e-10
+    g.close()
+
+
+def _with_family(core, spec):
+    """spec with every block's family and intact taken from the GPU build (the
+    oracle's own cell binning, with the reference's 30-cell clamp, takes 1.6e11
+    pair tests on the disk); the GPU family equals the oracle's on every smaller
+    case (test_gpu_parity.py)."""
+    g = core.Env.from_spec(spec, precision="fp64")
+    out = dict(spec)
+    out["blocks"] = []
+    for k, b in enumerate(spec["blocks"]):
+        fam, intact = g.family(k)
+        b2 = dict(b)
+        b2["family"], b2["intact"] = fam, intact
+        out["blocks"].append(b2)
+    g.close()
+    return out
+
+
+def test_disk_exact_matches_oracle_at_full_size(core, orc):
+    """Config 2 at full size (721k particles): the reference-order form against
+    the single-threaded oracle for 12 steps (first contact at step 6), bit for
+    bit."""
+    spec = _with_family(core, W.disk_impact())
+    say("oracle: disk, 12 steps, single-threaded")
+    o = orc.OracleEnv(spec)
+    o.run(12, 10)
+    g = core.Env.from_spec(spec, precision="exact")
+    core.run_batch([g], 12, 10)
+    y, v, f = g.download()
+    assert np.array_equal(y, o.y) and np.array_equal(v, o.v) and np.array_equal(f, o.f)
+    _masks_equal(g, o, 2)
+    g.close()
+
+
+def test_disk_fp64_matches_oracle_at_full_size(core, orc):
+    """Config 2 at full size, FP64 form (tiled) against the oracle with its
+    threaded loops for 100 steps: impact, contact lists rebuilt every 10 steps,
+    bond breaking.  Bars: relative L2 of displacement and force <= 1e-10,
+    identical bond masks."""
+    spec = _with_family(core, W.disk_impact())
+    g = core.Env.from_spec(spec, precision="fp64")
+    core.run_batch([g], 100, 10)
+    say("oracle: disk, 100 steps, threaded")
+    o = orc.OracleEnv(spec)
+    o.threaded = True
+    o.run(100, 10)
+    y, v, f = g.download()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    ed, ef = rel_l2(y - x, o.y - x), rel_l2(f, o.f)
+    broken = sum(int(((b.fam > 0) & (b.intact == 0)).sum()) for b in o.blocks)
+    say(f"disk, 100 steps, fp64 vs oracle: displacement {ed:.2e}, force {ef:.2e}, {broken} bonds broken")
+    _masks_equal(g, o, 2)
+    assert ed <= 1e-10 and ef <= 1e-10
+    g.close()
 
 
 def test_disk_fp64_matches_reference_order_at_full_size(core):

@@ -572,3 +572,41 @@ def test_bench_rejects_impossible_roofline():
                  "eager_profiled_run": {"ms_per_step": 0.6}}):
         with pytest.raises(SystemExit):
             bench.check_line(bad, 20)
+
+
+def test_plates_literal_config_is_ill_conditioned(orc):
+    """Config 1 as specified (two 50x50x20 EP plates on the same x-y lattice
+    columns): once the weak contact lets the facing layers interpenetrate,
+    particles meet head-on and the contact force direction dr/(|dr| + 1e-10)
+    amplifies rounding.  Two oracle runs that differ by one ulp in the initial
+    positions agree to 1e-10 in force through step 45 and differ by more than
+    1e-3 at step 63, where the closest contact pair is under 1e-4 dx apart.
+    The twin with the second plate shifted by half a spacing stays within 1e-10
+    under the same perturbation (measured: <= 2.6e-11 in force over 500 steps,
+    too long for this suite)."""
+    spec = W.plates_ep()
+    runs = []
+    for pert in (False, True):
+        e = orc.OracleEnv(spec)
+        e.threaded = True
+        if pert:
+            e.y[:] = np.nextafter(e.y, np.inf)
+        e.update_neighs()
+        e.apply_bc_at0()
+        snap = {}
+        for i in range(1, 64):
+            e.step()
+            if i in (45, 63):
+                snap[i] = e.f.copy()
+            if i % 10 == 0:
+                e.update_neighs()
+        c = e.contacts[0]
+        y1, y2, nb = e.y[c["mask1"]], e.y[c["mask2"]], c["neighs"]
+        r, k = np.nonzero(nb > 0)
+        closest = float(np.sqrt(((y1[r] - y2[nb[r, k] - 1]) ** 2).sum(axis=1)).min())
+        runs.append((snap, closest))
+    (a, ca), (b, _) = runs
+    rel = {i: float(np.linalg.norm(b[i] - a[i]) / np.linalg.norm(a[i])) for i in (45, 63)}
+    assert rel[45] <= 1e-10, rel
+    assert rel[63] > 1e-3, rel
+    assert ca < 1e-4, ca
