@@ -170,6 +170,17 @@ int64_t pd_log_rows(int64_t nsteps, int64_t every);
 /* The same 8 columns for the env's current state, momentum from the current
  * velocities (as after update_acc!, which calls update_misc!). */
 int pd_env_observables(void* ctx, int64_t env, double* out);
+/* The 8 columns of pd_env_observables written into out_dev, 8 doubles of
+ * device memory, without allocating: no hipMalloc, hipFree or other allocating
+ * call, so it may be used under HIP graph capture or where a serving loop
+ * forbids allocation.  No counterpart in the reference; it is for callers that
+ * cannot allocate.  The partial sums go to a fixed 8 MiB scratch array per GPU
+ * (one double4 per chunk of the env): envs of more than 262,144 chunks are
+ * rejected with PD_E_ARG.  Asynchronous: it enqueues on the context's stream
+ * and returns; pd_sync (or a wait on that stream) precedes reading out_dev.
+ * Every call on one GPU shares that GPU's scratch, so calls that overlap on one
+ * GPU (two contexts, or two host threads) race. */
+int pd_env_observables_noalloc(void* ctx, int64_t env, double* out_dev);
 /* update_acc! (solvers.jl:320-336) at the current positions; y may be NULL
  * (resident state) or 3 x N positions to upload first.  f receives 3 x N. */
 int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f);

@@ -25,7 +25,7 @@ EXPORTS = [
     "pd_env_time_step", "pd_env_check_finite", "pd_ctx_profile", "pd_ctx_profile_read",
     "pd_env_set_schedule", "pd_env_schedule", "pd_env_layout", "pd_ctx_mem_info", "pd_run_logged", "pd_log_rows",
     "pd_env_observables", "pd_env_state_size", "pd_env_state_save", "pd_env_state_load",
-    "pd_env_minimize",
+    "pd_env_minimize", "pd_env_observables_noalloc",
 ]
 PROF_CLASSES = {"prologue": 0, "latch": 1, "theta": 2, "td": 3, "force_bb": 4, "force_osb": 5,
                 "force_ep": 6, "contact": 7, "book": 8}
@@ -86,6 +86,7 @@ def load() -> ctypes.CDLL:
     L.pd_log_rows.argtypes = [I64, I64]
     L.pd_log_rows.restype = I64
     L.pd_env_observables.argtypes = [P, I64, P]
+    L.pd_env_observables_noalloc.argtypes = [P, I64, P]
     L.pd_env_state_size.argtypes = [P, I64, ctypes.POINTER(I64)]
     L.pd_env_state_save.argtypes = [P, I64, P, I64]
     L.pd_env_state_load.argtypes = [P, I64, P, I64]

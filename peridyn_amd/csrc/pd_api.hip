@@ -20,6 +20,14 @@
 #include "pd_internal.h"
 #include "pd_kernels.h"
 
+// Scratch of pd_env_observables_noalloc: one double4 of partial sums per chunk
+// of the env (8 MiB, held by every GPU the library runs on until the process
+// ends).  Host code never passes the array's name to a kernel: each context
+// resolves this GPU's copy with hipGetSymbolAddress at creation and uses that
+// address (Ctx::scratch).
+constexpr int kObsScratchChunks = 1 << 18;
+static __device__ __align__(256) double4 scratch[kObsScratchChunks];
+
 namespace {
 
 // Device memory of one context's envs.  Released blocks are kept by size class
@@ -208,6 +216,7 @@ struct Env {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  double4* scratch = nullptr;  // this GPU's copy of the file-scope scratch array
   DevMem mem;  // env arrays and build temporaries
   HostStage stage;  // commit staging
   std::string err;
@@ -1803,6 +1812,16 @@ int pd_ctx_create(int device, void** out) {
     c->device = device;
     hipError_t r = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (r != hipSuccess) { delete c; PD_HIP(r); }
+    // the address of this GPU's copy of `scratch` (device is current)
+    void* sp = nullptr;
+    r = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (r != hipSuccess || !sp) {
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      if (r == hipSuccess) throw PdError(PD_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+      PD_HIP(r);
+    }
+    c->scratch = static_cast<double4*>(sp);
     // PERIDYN_POISON=1: fill device memory with 0xFF when handed out (tests)
     const char* pz = std::getenv("PERIDYN_POISON");
     c->mem.stream = c->stream;
@@ -2141,6 +2160,30 @@ int pd_env_observables(void* ctx, int64_t env, double* out) {
     PD_HIP(hipStreamSynchronize(s));
     out[0] = (double)e.time_step * e.dt;
     out[7] = out[7] / (double)e.N;
+  });
+}
+
+int pd_env_observables_noalloc(void* ctx, int64_t env, double* out_dev) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    if (!out_dev) throw PdError(PD_E_ARG, "null output");
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    // out_dev must be device memory a kernel of this GPU may write
+    hipPointerAttribute_t pa{};
+    const hipError_t ar = hipPointerGetAttributes(&pa, out_dev);
+    if (ar != hipSuccess) (void)hipGetLastError();
+    if (ar != hipSuccess || (pa.type != hipMemoryTypeDevice && pa.type != hipMemoryTypeManaged) ||
+        pa.device != c->device)
+      throw PdError(PD_E_ARG, "pd_env_observables_noalloc: out_dev must be device memory of the context's GPU");
+    const size_t n = e.chunks.size();
+    if (n > (size_t)kObsScratchChunks)
+      throw PdError(PD_E_ARG, "pd_env_observables_noalloc: the env has " + std::to_string(n) +
+                                  " chunks, more than the " + std::to_string(kObsScratchChunks) +
+                                  " its fixed scratch holds; use pd_env_observables");
+    pd_launch_observables_env(c->stream, e.d_chunks, (int)n, c->d_envs, e.slot, c->scratch, out_dev);
+    PD_HIP(hipGetLastError());
   });
 }
 

@@ -51,6 +51,10 @@ void pd_launch_qs_update(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs
 // printdata columns: mode 0 momentum into rows[8 e + 1..3], mode 1 force and
 // damage sums into rows[8 e + 4..7] for the nenv envs whose chunks (in `chunks`,
 // n in all) are the ranges (first, count); part: n double4 of scratch
+// printdata columns of one env into row[0..7] (device), partial sums in part
+// (one double4 per chunk): the kernels of pd_env_observables_noalloc
+void pd_launch_observables_env(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs, int slot, double4* part,
+                               double* row);
 void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, EnvDev* envs, double4* part,
                            const int2* ranges, int nenv, double* rows);
 // device-wide exclusive scan of n int32 (scratch: pd_scan_scratch_ints(n) ints)

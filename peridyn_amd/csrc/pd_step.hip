@@ -769,6 +769,31 @@ __global__ __launch_bounds__(64) void k_obs_fin(const double4* __restrict__ part
   }
 }
 
+// One wavefront, one env: its chunk partials [0, nchunks) in order, as
+// k_obs_fin sums them; MODE 1 also writes the time column (time_step * dt)
+// and divides the damage sum by the particle count, the two columns the host
+// finishes for pd_env_observables.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_obs_fin_env(const double4* __restrict__ part, int nchunks,
+                                                    const EnvDev* __restrict__ envs, int slot,
+                                                    double* __restrict__ row) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int i = threadIdx.x; i < nchunks; i += 64) {
+    const double4 v = part[i];
+    s0 += v.x; s1 += v.y; s2 += v.z; s3 += v.w;
+  }
+  s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
+  if (threadIdx.x != 0) return;
+  if (MODE == 0) {
+    row[1] = s0; row[2] = s1; row[3] = s2;
+  } else {
+    const EnvDev& E = envs[slot];
+    row[0] = (double)E.time_step * E.dt;
+    row[4] = s0; row[5] = s1; row[6] = s2;
+    row[7] = s3 / (double)E.N;
+  }
+}
+
 // ---- quasi-static minimiser (QSDrag minimize!, src/solvers/minimize.jl:105-231) --
 // apply_bc!(:position) and (:velocity) of every BC once, as minimize! does
 // before iterating (the last BC listed for a particle wins, as in list order).
@@ -956,6 +981,15 @@ void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, 
     hipLaunchKernelGGL(k_obs_part<1>, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, part);
     hipLaunchKernelGGL(k_obs_fin<1>, dim3(nenv), dim3(64), 0, s, (const double4*)part, ranges, rows);
   }
+}
+
+void pd_launch_observables_env(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs, int slot, double4* part,
+                               double* row) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_obs_part<0>, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, part);
+  hipLaunchKernelGGL(k_obs_fin_env<0>, dim3(1), dim3(64), 0, s, (const double4*)part, n, envs, slot, row);
+  hipLaunchKernelGGL(k_obs_part<1>, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs, part);
+  hipLaunchKernelGGL(k_obs_fin_env<1>, dim3(1), dim3(64), 0, s, (const double4*)part, n, envs, slot, row);
 }
 
 void pd_launch_qs_bc(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) {
