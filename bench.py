@@ -51,7 +51,9 @@ PROF_CLASSES = ("prologue", "latch", "theta", "td", "force_bb", "force_osb", "fo
 # and form: int32 column + uint8 state per bond, neighbour gathers counted as
 # cache hits, edp a per-bond stream; the Verlet update (read v, f, y; write v,
 # y: 120 B/particle) is fused into the force kernel.  FP32C reads a 16 B x32
-# record instead of the 32 B FP64 one (and 8 B of FP32 theta, 1/m for OSB).
+# record instead of the 32 B FP64 one (and 8 B of FP32 theta, 1/m for OSB); its
+# elastoplastic passes read the FP32 records and keep the FP64 plastic
+# extensions (8 B read + 8 B write per bond in the force pass, 8 B read in td).
 BYTES = {
     ("force_bb", "fp64"): (5, 204),
     ("force_osb", "fp64"): (5, 220),
@@ -60,9 +62,9 @@ BYTES = {
     ("td", "fp64"): (13, 84),
     ("force_bb", "fp32c"): (5, 188),
     ("force_osb", "fp32c"): (5, 196),
-    ("force_ep", "fp32c"): (21, 228),
+    ("force_ep", "fp32c"): (21, 212),
     ("theta", "fp32c"): (5, 56),
-    ("td", "fp32c"): (13, 84),
+    ("td", "fp32c"): (13, 68),
 }
 
 # untimed steps before the timed region: the rigid approach of the colliding

@@ -819,6 +819,9 @@ void commit_env(Ctx* c, Env& e) {
   // plates, equal to 2 on the BB disk)
   bool tileable = (e.precision == PD_PREC_FP64 || e.precision == PD_PREC_FP32C) && tiles_enabled();
   for (auto& b : e.blocks) tileable = tileable && b.d.ntype == 1;
+  // the FP32C elastoplastic passes are untiled kernels (k_td32, k_force32<EP>)
+  if (e.precision == PD_PREC_FP32C)
+    for (auto& b : e.blocks) tileable = tileable && b.d.kind != PD_EP;
   for (auto& hc : e.contacts) tileable = tileable && hc.d.exponent_is_int;
   if (e.precision == PD_PREC_EXACT) {
     e.lgG = 0;
@@ -1254,7 +1257,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
       hcap = std::max(hcap, (int)e.hmax);
       for (auto& tb : e.tile_blocks) {
         const int kind = e.blocks[tb[0]].d.kind;
-        const int pc = (e.precision == PD_PREC_FP32C && kind != PD_EP) ? 1 : 0;  // EP runs FP64
+        const int pc = (e.precision == PD_PREC_FP32C) ? 1 : 0;  // FP32C envs tile BB / OSB blocks only
         s_fk[kind][pc][e.lgG] += tb[2];
         if (kind != PD_BB) s_th[pc][e.lgG] += tb[2];
         if (kind == PD_EP) s_td[e.lgG] += tb[2];
@@ -1317,7 +1320,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
       const int lg = e.lgG;
       for (auto& tb : e.tile_blocks) {
         const int kind = e.blocks[tb[0]].d.kind;
-        const int pc = (e.precision == PD_PREC_FP32C && kind != PD_EP) ? 1 : 0;
+        const int pc = (e.precision == PD_PREC_FP32C) ? 1 : 0;
         put(c->ot_force[kind][pc][lg] + f_fk[kind][pc][lg], e, tb);
         f_fk[kind][pc][lg] += tb[2];
         if (kind != PD_BB) { put(c->ot_theta[pc][lg] + f_th[pc][lg], e, tb); f_th[pc][lg] += tb[2]; }

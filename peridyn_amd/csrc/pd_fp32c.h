@@ -11,7 +11,7 @@ __device__ __forceinline__ void kahan(float& s, float& c, float v) {
 }
 
 struct G32 {
-  float Y1, Y2, Y3, inv_x, inv_y, ext, s;
+  float Y1, Y2, Y3, inv_x, inv_y, xij, ext, s;
 };
 
 __device__ __forceinline__ void geom32(const float4& xp, const double4& yp, const float4& xj, const double4& yj,
@@ -26,6 +26,7 @@ __device__ __forceinline__ void geom32(const float4& xp, const double4& yp, cons
   const float yy2 = xx2 + xu;
   g.inv_x = __builtin_amdgcn_rsqf(xx2);
   const float xij = xx2 * g.inv_x;
+  g.xij = xij;
   g.inv_y = __builtin_amdgcn_rsqf(yy2);
   const float yij = yy2 * g.inv_y;
   g.ext = xu * __builtin_amdgcn_rcpf(yij + xij);
@@ -36,3 +37,7 @@ __device__ __forceinline__ double comp_total(float s, float c, int C) {
   return group_sum((double)s - (double)c, C);
 }
 
+
+// sticky "has yielded" flag of an elastoplastic particle in the sign of its
+// FP32 record's td_norm2 (aux32.z), as pd_has_yielded reads the FP64 record
+__device__ __forceinline__ bool pd_has_yielded32(const float4& a) { return __builtin_signbit(a.z); }

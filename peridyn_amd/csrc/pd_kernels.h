@@ -78,6 +78,7 @@ void pd_launch_contact_update_batch(hipStream_t s, const EnvDev* envs, const int
 
 // FP32-compensated form (pd_step32.hip)
 void pd_launch_theta32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
+void pd_launch_td32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags, bool gpow);
 
 // tiled passes (pd_tile.hip): LDS bytes per halo record for pass (0 theta,

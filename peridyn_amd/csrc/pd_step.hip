@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_theta(const Chunk* __restrict__ chunks,
   const EnvDev& E = envs[w.ch.env];
   const BlockDev& B = E.blocks[w.ch.block];
   if (B.kind == PD_BB || !B.deformed) return;
-  if (E.precision == PD_PREC_FP32C && B.kind != PD_EP) return;  // done by k_theta32
+  if (E.precision == PD_PREC_FP32C) return;  // done by k_theta32
   const double4* __restrict__ x = E.x;
   const double4* __restrict__ y = E.y[E.cur];
   const int32_t* __restrict__ col = E.col;
@@ -890,9 +890,8 @@ static inline int grid_for(int nchunks) {
 void pd_launch_theta(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* envs) {
   if (n <= 0) return;
   if (prec == PD_PREC_FP32C) {
-    // OSB blocks in FP32; elastoplastic blocks of the same list in FP64
     pd_launch_theta32(s, chunks, n, envs);
-    prec = PD_PREC_FP64;
+    return;
   }
   if (prec == PD_PREC_EXACT)
     hipLaunchKernelGGL(k_theta<PD_PREC_EXACT>, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs);
@@ -902,7 +901,10 @@ void pd_launch_theta(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev
 
 void pd_launch_td(hipStream_t s, int prec, const Chunk* chunks, int n, EnvDev* envs) {
   if (n <= 0) return;
-  if (prec == PD_PREC_FP32C) prec = PD_PREC_FP64;  // elastoplastic blocks run FP64
+  if (prec == PD_PREC_FP32C) {
+    pd_launch_td32(s, chunks, n, envs);
+    return;
+  }
   if (prec == PD_PREC_EXACT)
     hipLaunchKernelGGL(k_td<PD_PREC_EXACT>, dim3(grid_for(n)), dim3(256), 0, s, chunks, n, envs);
   else
@@ -925,11 +927,8 @@ void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int
                      bool gpow) {
   if (n <= 0) return;
   if (prec == PD_PREC_FP32C) {
-    if (kind != PD_EP) {
-      pd_launch_force32(s, kind, chunks, n, envs, flags, gpow);
-      return;
-    }
-    prec = PD_PREC_FP64;
+    pd_launch_force32(s, kind, chunks, n, envs, flags, gpow);
+    return;
   }
   if (kind == PD_BB) launch_force_kind<PD_BB>(s, prec, chunks, n, envs, flags, gpow);
   else if (kind == PD_OSB) launch_force_kind<PD_OSB>(s, prec, chunks, n, envs, flags, gpow);

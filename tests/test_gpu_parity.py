@@ -250,16 +250,20 @@ def test_contact_plates_fp64_within_1e10(pd, orc):
 @pytest.mark.parametrize("prec", ["fp64", "fp32c"])
 def test_contact_noninteger_exponent(pd, orc, prec):
     """Nonlinear spring with exponent 1.5 (the kernels carrying the general
-    power): within the FP64 bar of the oracle (the oracle's pow is the C
-    library's, the device's the ROCm math library's; both round to ~1 ulp)."""
+    power): FP64 within the FP64 bar of the oracle (the oracle's pow is the C
+    library's, the device's the ROCm math library's; both round to ~1 ulp);
+    FP32C (elastoplastic plates in the FP32C form) within its bars, 1e-5 in
+    displacement and 1e-4 in force."""
     spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=2)
     spec["contacts"][0]["exponent"] = 1.5
     o, g = run_both(pd, orc, spec, 30, nuf=5, prec=prec)
     y, v, f = g.download()
     x = np.concatenate([b["x"] for b in spec["blocks"]])
     assert np.abs(o.f).max() > 0
-    assert rel_l2(f, o.f) <= 1e-10
-    assert rel_l2(y - x, o.y - x) <= 1e-10
+    bar_y, bar_f = (1e-10, 1e-10) if prec == "fp64" else (1e-5, 1e-4)
+    say(f"exponent 1.5, {prec}: force {rel_l2(f, o.f):.2e}, displacement {rel_l2(y - x, o.y - x):.2e}")
+    assert rel_l2(f, o.f) <= bar_f
+    assert rel_l2(y - x, o.y - x) <= bar_y
 
 
 def test_contact_truncation_keeps_largest_ids(pd, orc):
@@ -297,11 +301,19 @@ def test_contact_search_large_candidate_sets(pd, orc, max_neighs):
 @pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
 def test_fp32c_error_vs_fp64_oracle(pd, orc, kind):
     """FP32-compensated form against the FP64 oracle after 60 steps: relative
-    L2 displacement error <= 1e-5, force error <= 1e-4, identical bond masks
-    (tolerances stated here; measured values are printed)."""
+    L2 displacement error <= 1e-5, force error <= 1e-4, identical bond masks;
+    for the elastoplastic block (yielding inside the run) also the plastic
+    extensions within 1e-4 relative L2 (tolerances stated here; measured values
+    are printed)."""
     spec = W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.003)
     if kind == "ep":
+        # yielding throughout (max plastic extension 0.7) without fracture: with
+        # s0 = 0.005 this block tears (26k of 60k bonds break within 60 steps)
+        # and the oracle itself answers a 1e-7 perturbation of y with 1.4e-3,
+        # every break decision near s0 being a discrete branch; without breaks
+        # it answers linearly (1e-9 -> 4.3e-9)
         spec["blocks"][0]["yield_stress"] = 20.0
+        spec["blocks"][0]["critical_stretch"] = 1.0
     o, g = run_both(pd, orc, spec, 60, prec="fp32c")
     y, v, f = g.download()
     x = spec["blocks"][0]["x"]
@@ -312,6 +324,10 @@ def test_fp32c_error_vs_fp64_oracle(pd, orc, kind):
     assert ef <= 1e-4
     _, intact = g.family(0)
     assert np.array_equal(intact, o.blocks[0].intact)
+    if kind == "ep":
+        ee = rel_l2(g.edp(0), o.blocks[0].edp)
+        say(f"fp32c ep: plastic extension rel L2 {ee:.2e}, {int((o.blocks[0].edp != 0).sum())} yielded bonds")
+        assert np.count_nonzero(o.blocks[0].edp) > 100 and ee <= 1e-4
 
 
 def test_fp32c_contact_plates(pd, orc):
@@ -372,3 +388,33 @@ def test_graph_replay_equals_eager(pd, orc, monkeypatch):
             o.run(11, 5, start=True)
             assert np.array_equal(runs["1"][-1][0], o.y)
             assert np.array_equal(runs["1"][-1][2], o.f)
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb"])
+@pytest.mark.parametrize("prec", ["exact", "fp64", "fp32c"])
+def test_indentation_fixbc_against_oracle(pd, orc, kind, prec):
+    """FixBC (FixBC.jl:46-65: y = last, v = 0 on the block's three bottom
+    layers) together with a ToFroBC indenter that reverses after 80% of the
+    steps (ToFroBC.jl:51-111) and a linear contact, in the shape of
+    examples/indentation/block_indentor.jl: 1,600 steps, through contact and
+    the reversal.  Bars: reference-order form bit-identical to the oracle; FP64
+    form within 1e-10 relative L2 in displacement and force; FP32C within
+    1e-5 in displacement; the fixed particles never move in any form; identical
+    bond masks."""
+    spec = W.indentation(2000, kind=kind, steps=1600)
+    o, g = run_both(pd, orc, spec, 1600, 10, prec)
+    y, v, f = g.download()
+    fix = spec["bcs"][0]["mask"]
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    assert np.array_equal(y[fix], x[fix]) and not np.any(v[fix])
+    ed, ef = rel_l2(y - x, o.y - x), rel_l2(f, o.f)
+    say(f"indentation {kind} {prec}: displacement {ed:.2e}, force {ef:.2e}, "
+        f"indenter travel {float((o.y[~fix][:, 2] - x[~fix][:, 2]).min()):.3f}")
+    if prec == "exact":
+        assert np.array_equal(y, o.y) and np.array_equal(v, o.v) and np.array_equal(f, o.f)
+    elif prec == "fp64":
+        assert ed <= 1e-10 and ef <= 1e-10
+    else:
+        assert ed <= 1e-5
+    assert np.array_equal(g.damage(), o.damage())
+    g.close()

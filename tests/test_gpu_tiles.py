@@ -235,13 +235,25 @@ def test_tiled_and_untiled_batched_equal_alone(pd):
         e.close()
 
 
-@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+def test_fp32c_envs_with_ep_blocks_run_untiled(pd):
+    """The FP32C elastoplastic passes are untiled kernels: an FP32C env holding
+    an EP block is laid out untiled even where its halos would fit, while the
+    same env in FP64 runs tiled."""
+    spec = ep_tensile()
+    e32 = pd.Env.from_spec(spec, precision="fp32c", lanes_per_particle=4)
+    e64 = pd.Env.from_spec(spec, precision="fp64", lanes_per_particle=4)
+    assert not e32.layout()["tiled"] and e64.layout()["tiled"]
+    e32.close()
+    e64.close()
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb"])
 @pytest.mark.parametrize("lanes", [2, 4, 8])
 def test_tiled_fp32c_equals_untiled(pd, kind, lanes):
-    """FP32-compensated form: BB / OSB blocks run the tiled FP32C kernels, EP
-    blocks the tiled FP64 ones; both bit-identical to their untiled kernels."""
+    """FP32-compensated form: BB / OSB blocks run the tiled FP32C kernels,
+    bit-identical to their untiled kernels."""
     say("tiled fp32c vs untiled", kind, lanes)
-    spec = ep_tensile() if kind == "ep" else W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.002)
+    spec = W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.002)
     a = make(pd, spec, True, lanes, "fp32c")
     b = make(pd, spec, False, lanes, "fp32c")
     a.run(40, 1)

