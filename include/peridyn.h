@@ -189,6 +189,23 @@ int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f);
 int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, double* f, int64_t lo, int64_t hi);
 /* update_contact_neighs! for every contact model of the env at the resident positions */
 int pd_env_update_neighs(void* ctx, int64_t env);
+/* update_contact_neighs!(y, type, RM) for contact model cidx of the env (its
+ * index in pd_env_add_contact order): RM12 (contacts.jl:434-476) or RM11
+ * (contacts.jl:539-591).  y may be NULL (resident positions) or 3 x N env
+ * positions, uploaded into the resident state first. */
+int pd_contact_update(void* ctx, int64_t env, int64_t cidx, const double* y);
+/* RM.neighs as update_contact_neighs! leaves it: max_neighs x n1 column-major
+ * (n1 = members of pair[1], in type-mask order), for each member the largest
+ * pair[2] member indices within `distance` (1-based, descending, 0-padded);
+ * for RM11, n1 = the block's particles and indices into the block.  max_neighs
+ * must equal the model's min(max_neighs, n2). */
+int pd_contact_neighs(void* ctx, int64_t env, int64_t cidx, int64_t* neighs, int64_t max_neighs);
+/* short_range_contact!(y, f, type, bid, vol, den, RM) for contact model cidx
+ * (RM12 contacts.jl:189-247, RM11 :357-380): adds its terms into f (3 x N, in
+ * and out) one by one in the single-threaded reference's order, from the lists
+ * of the last update; y may be NULL (resident positions) or 3 x N positions to
+ * upload first.  vol and den are the env's own (environment.jl:173-214). */
+int pd_contact_force(void* ctx, int64_t env, int64_t cidx, const double* y, double* f);
 
 /* ---- state transfer ----------------------------------------------------- */
 int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f);

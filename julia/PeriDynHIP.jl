@@ -2,17 +2,26 @@
     PeriDynHIP
 
 Binds PeriDyn's explicit-dynamics hot path to libperidyn_hip.so (MI355X, gfx950)
-through `ccall`.  It adds `Val{:hip}` methods at the reference's own dispatch
-seams:
+through `ccall`.  After `set_device!()` (PeriDyn.DEVICE[] = :hip) it adds
+methods at the reference's own dispatch seams:
 
-  * force_density_T!(f, y, limits, mat, ::Type{Val{:hip}})   (general_material.jl:274-296)
-  * short_range_contact!(y, f, type, bid, vol, den, RM, ::Type{Val{:hip}}) (contacts.jl:210)
-  * update_contact_neighs!(y, type, RM::ContactModel12; device=:hip)       (contacts.jl:434)
-  * run_resident!(env, N; neigh_update_freq, average_prop_freq) — the resident,
+  * force_density_T!(f, y, limits, mat, ::Type{Val{:hip}})   (general_material.jl:274-296);
+    bond state is copied back into mat.general.intact after every call, so
+    cal_damage and printdata (peri.jl:587-594, environment.jl:308-320) read it
+  * short_range_contact!(y, f, type, bid, vol, den, RM) for
+    NonLinearSpringContactModel12 / 11 (contacts.jl:189-247, 357-380); these
+    replace the 7-argument method, which sets device = :cpu itself (contacts.jl:191)
+  * update_contact_neighs!(y, type, RM; max_part) for the same two types
+    (contacts.jl:434-476, 539-591); RM.neighs is filled from the device lists
+  * run_resident!(env, N; neigh_update_freq, average_prop_freq) -- the resident,
     fused run loop (run!'s physics cadence, solvers.jl:97-187, with its printdata
     rows but without file output)
   * minimize_resident!(env, solver::QSDrag)                 (minimize.jl:105-231)
-  * checkpoint(id) / restore!(id, bytes) — resident restart state (io.jl:170-187)
+  * checkpoint(id) / restore!(id, bytes) -- resident restart state (io.jl:170-187)
+
+The contact seams act on a device env holding the whole Env: call
+`register!(env)` once after building it.  The material seam uses that env too
+when the block is registered, else a device env of the block alone.
 
 Julia is not installed in the build container, so this file is written against
 the C ABI in include/peridyn.h and the Python binding (peridyn_amd/_lib.py,
@@ -76,6 +85,12 @@ kind_of(m::PeriDyn.BondBasedMaterial) = PD_BB
 kind_of(m::PeriDyn.OrdinaryStateBasedMaterial) = PD_OSB
 kind_of(m::PeriDyn.ElastoPlasticSolidMaterial) = PD_EP
 
+# max_neigh of a block for the device family build: the reference's family has
+# maxdeg + 1 rows after GeneralMaterial's truncation (general_material.jl:88-90),
+# so the device build, which raises PD_E_OVERFLOW only when a degree reaches
+# max_neigh, rebuilds exactly the reference's sets
+family_rows(gen) = size(gen.family, 1)
+
 # env id per material block, built on first use (family rebuilt on the GPU; the
 # sets equal cal_family's, the layout is exported back on demand)
 const ENVS = IdDict{Any,Int64}()
@@ -98,7 +113,7 @@ function env_for(mat, precision=PD_PREC_EXACT)
         GC.@preserve x y v vol typ dens s0 c K G psi begin
             d = BlockDesc(kind_of(mat), Int32(nt), size(x, 2), first(mat.type),
                           pointer(x), pointer(y), pointer(v), pointer(vol), pointer(typ),
-                          gen.horizon, gen.particle_size, size(gen.family.x, 1) * 2 + 1,
+                          gen.horizon, gen.particle_size, family_rows(gen),
                           pointer(dens), pointer(s0), pointer(c), pointer(K), pointer(G), pointer(psi),
                           Int32(all(gen.deformed)), Int32(0))
             check(ccall((:pd_env_add_block, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{BlockDesc}, Ptr{Int64}),
@@ -109,24 +124,127 @@ function env_for(mat, precision=PD_PREC_EXACT)
     end
 end
 
+# ---- seam registration: one device env per PeriDyn Env ----------------------
+const SEAM_BLOCK = IdDict{Any,Tuple{Int64,Int64}}()    # material block -> (env id, block index)
+const SEAM_CONTACT = IdDict{Any,Tuple{Int64,Int64}}()  # contact model -> (env id, model index)
+
+"""
+    register!(env; precision=PD_PREC_EXACT) -> Int64
+
+Uploads `env` (blocks, contact models, BCs; `upload_env`) and routes the seam
+methods of its material blocks and contact models to that device env: block k
+of env.material_blocks is device block k-1, contact model k of
+env.short_range_contact is device model k-1 (update_contact_acc! and
+update_neighs! loop over them in this order, solvers.jl:257-269,351-361).
+"""
+function register!(env; precision=PD_PREC_EXACT)
+    id = upload_env(env; precision=precision)
+    for (k, mat) in enumerate(env.material_blocks)
+        SEAM_BLOCK[mat] = (id, Int64(k - 1))
+    end
+    for (k, RM) in enumerate(env.short_range_contact)
+        SEAM_CONTACT[RM] = (id, Int64(k - 1))
+    end
+    id
+end
+
+"Copies the device bond state of block `blk` of env `id` into mat.general.intact."
+function copy_intact!(mat, id::Int64, blk::Int64)
+    ctx = context()
+    intact = mat.general.intact
+    rows = size(intact, 1)
+    buf = zeros(UInt8, rows, size(intact, 2))
+    check(ccall((:pd_block_family_export, LIB), Cint,
+                (Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Ptr{UInt8}, Int64), ctx.h, id, blk, C_NULL, buf, rows), ctx)
+    intact .= buf .!= 0
+    nothing
+end
+
 """
     force_density_T!(f, y, limits, mat, ::Type{Val{:hip}})
 
 Counterpart of the CPU method (general_material.jl:296-369): out_of_loop!
-passes, the bond loop with breaking, accumulated into f[:, limits[1]:limits[2]].
-Bond state, theta and edp stay resident on the device.
+passes, the bond loop with breaking, accumulated into f[:, limits[1]:limits[2]];
+then the bond state is copied back into mat.general.intact.
 """
 function PeriDyn.force_density_T!(f, y, limits, mat::PeriDyn.PeridynamicsMaterial, ::Type{Val{:hip}}; particles=nothing)
     ctx = context()
-    id = env_for(mat)
-    yb = Array{Float64}(y[:, limits[1]:limits[2]])
-    fb = zeros(Float64, size(yb))
-    GC.@preserve yb fb begin
-        check(ccall((:pd_force_density, LIB), Cint,
-                    (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Int64, Int64),
-                    ctx.h, id, 0, yb, fb, 1, size(yb, 2)), ctx)
+    if haskey(SEAM_BLOCK, mat)
+        id, blk = SEAM_BLOCK[mat]
+        yh = Array{Float64}(y)
+        fh = Array{Float64}(f)
+        GC.@preserve yh fh begin
+            check(ccall((:pd_force_density, LIB), Cint,
+                        (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Int64, Int64),
+                        ctx.h, id, blk, yh, fh, limits[1], limits[2]), ctx)
+        end
+        f[:, limits[1]:limits[2]] .= fh[:, limits[1]:limits[2]]
+    else
+        id, blk = env_for(mat), Int64(0)
+        yb = Array{Float64}(y[:, limits[1]:limits[2]])
+        fb = zeros(Float64, size(yb))
+        GC.@preserve yb fb begin
+            check(ccall((:pd_force_density, LIB), Cint,
+                        (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Int64, Int64),
+                        ctx.h, id, 0, yb, fb, 1, size(yb, 2)), ctx)
+        end
+        f[:, limits[1]:limits[2]] .+= fb
     end
-    f[:, limits[1]:limits[2]] .+= fb
+    copy_intact!(mat, id, blk)
+    nothing
+end
+
+const SeamContact = Union{PeriDyn.NonLinearSpringContactModel12, PeriDyn.NonLinearSpringContactModel11}
+
+seam_of(RM) = haskey(SEAM_CONTACT, RM) ? SEAM_CONTACT[RM] :
+    error("PeriDynHIP: contact model $(RM.name) is not registered; call PeriDynHIP.register!(env) first")
+
+"""
+    short_range_contact!(y, f, type, bid, vol, den, RM)
+
+For DEVICE[] == :hip: the contact accelerations of RM (RM12 contacts.jl:189-247,
+RM11 :357-380) added into f, term by term in the single-threaded reference's
+order (pd_contact_force), from the lists of the last update_contact_neighs!.
+Other devices take the reference's own method.
+"""
+function PeriDyn.short_range_contact!(y, f, type, bid, vol, den, RM::SeamContact)
+    if PeriDyn.DEVICE[] != :hip
+        return invoke(PeriDyn.short_range_contact!,
+                      Tuple{Any,Any,Any,Any,Any,Any,supertype(typeof(RM))}, y, f, type, bid, vol, den, RM)
+    end
+    ctx = context()
+    id, k = seam_of(RM)
+    yh = Array{Float64}(y)
+    fh = Array{Float64}(f)
+    GC.@preserve yh fh check(ccall((:pd_contact_force, LIB), Cint,
+                                   (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}),
+                                   ctx.h, id, k, yh, fh), ctx)
+    f .= fh
+    nothing
+end
+
+"""
+    update_contact_neighs!(y, type, RM; max_part=nothing)
+
+For DEVICE[] == :hip: RM's contact lists rebuilt on the device from y
+(pd_contact_update) and copied into RM.neighs (pd_contact_neighs), as the
+reference's method leaves them (contacts.jl:434-476 / 539-591).
+"""
+function PeriDyn.update_contact_neighs!(y, type, RM::SeamContact; max_part=nothing)
+    if PeriDyn.DEVICE[] != :hip
+        return invoke(PeriDyn.update_contact_neighs!, Tuple{Any,Any,supertype(typeof(RM))}, y, type, RM;
+                      max_part=max_part)
+    end
+    ctx = context()
+    id, k = seam_of(RM)
+    yh = Array{Float64}(y)
+    GC.@preserve yh check(ccall((:pd_contact_update, LIB), Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}),
+                                ctx.h, id, k, yh), ctx)
+    M = size(RM.neighs, 1)
+    nb = zeros(Int64, M, size(RM.neighs, 2))
+    check(ccall((:pd_contact_neighs, LIB), Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Int64),
+                ctx.h, id, k, nb, M), ctx)
+    RM.neighs .= nb
     nothing
 end
 
@@ -163,7 +281,7 @@ function upload_env(env; precision=PD_PREC_FP64)
         x, y, v, vol, typ, dens, s0, c, K, G, psi = arrs
         d = BlockDesc(kind_of(mat), Int32(nt), size(x, 2), first(mat.type),
                       pointer(x), pointer(y), pointer(v), pointer(vol), pointer(typ),
-                      gen.horizon, gen.particle_size, 1 << 20,
+                      gen.horizon, gen.particle_size, family_rows(gen),
                       pointer(dens), pointer(s0), pointer(c), pointer(K), pointer(G), pointer(psi),
                       Int32(all(gen.deformed)), Int32(0))
         GC.@preserve arrs check(ccall((:pd_env_add_block, LIB), Cint,

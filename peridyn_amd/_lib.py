@@ -25,7 +25,8 @@ EXPORTS = [
     "pd_env_time_step", "pd_env_check_finite", "pd_ctx_profile", "pd_ctx_profile_read",
     "pd_env_set_schedule", "pd_env_schedule", "pd_env_layout", "pd_ctx_mem_info", "pd_run_logged", "pd_log_rows",
     "pd_env_observables", "pd_env_state_size", "pd_env_state_save", "pd_env_state_load",
-    "pd_env_minimize", "pd_env_observables_noalloc",
+    "pd_env_minimize", "pd_env_observables_noalloc", "pd_contact_update", "pd_contact_neighs",
+    "pd_contact_force",
 ]
 PROF_CLASSES = {"prologue": 0, "latch": 1, "theta": 2, "td": 3, "force_bb": 4, "force_osb": 5,
                 "force_ep": 6, "contact": 7, "book": 8}
@@ -108,6 +109,9 @@ def load() -> ctypes.CDLL:
     L.pd_env_update_acc.argtypes = [P, I64, P, P]
     L.pd_force_density.argtypes = [P, I64, I64, P, P, I64, I64]
     L.pd_env_update_neighs.argtypes = [P, I64]
+    L.pd_contact_update.argtypes = [P, I64, I64, P]
+    L.pd_contact_neighs.argtypes = [P, I64, I64, P, I64]
+    L.pd_contact_force.argtypes = [P, I64, I64, P, P]
     L.pd_env_download.argtypes = [P, I64, P, P, P]
     L.pd_env_upload.argtypes = [P, I64, P, P, P]
     L.pd_env_damage.argtypes = [P, I64, P]

@@ -270,6 +270,34 @@ class Env:
         hi = self.block_first[block] + self.block_n[block]
         self.ctx.check(self.ctx.lib.pd_force_density(self.ctx.h, self.id, block, _ptr(y), _ptr(f), lo, hi))
 
+    def contact_update(self, cidx: int = 0, y: Optional[np.ndarray] = None):
+        """update_contact_neighs! of contact model cidx (pd_contact_update)."""
+        yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.pd_contact_update(self.ctx.h, self.id, int(cidx), _ptr(yy)))
+
+    def contact_neighs(self, cidx: int = 0, max_neighs: int = 0) -> np.ndarray:
+        """RM.neighs of contact model cidx as (n1, max_neighs) int64 (= Julia
+        max_neighs x n1), 1-based, descending, 0-padded."""
+        c = self.spec["contacts"][cidx]
+        types1 = c["pair"][0]
+        if len(c["pair"]) > 1 and len(c["pair"][1]) > 0:
+            types2 = c["pair"][1]
+            n1 = int(sum(int(np.isin(np.asarray(b["type"]), types1).sum()) for b in self.spec["blocks"]))
+            n2 = int(sum(int(np.isin(np.asarray(b["type"]), types2).sum()) for b in self.spec["blocks"]))
+        else:
+            n1 = n2 = int(sum(int(np.isin(np.asarray(b["type"]), types1).sum()) for b in self.spec["blocks"]))
+        M = max_neighs or max(1, min(int(c.get("max_neighs", 50)), n2))
+        out = np.zeros((n1, M), dtype=np.int64)
+        self.ctx.check(self.ctx.lib.pd_contact_neighs(self.ctx.h, self.id, int(cidx), _ptr(out), M))
+        return out
+
+    def contact_force(self, f: np.ndarray, cidx: int = 0, y: Optional[np.ndarray] = None) -> np.ndarray:
+        """short_range_contact! of contact model cidx: f + its terms (pd_contact_force)."""
+        out = np.ascontiguousarray(f, dtype=np.float64).copy()
+        yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.pd_contact_force(self.ctx.h, self.id, int(cidx), _ptr(yy), _ptr(out)))
+        return out
+
     def update_neighs(self):
         self.ctx.check(self.ctx.lib.pd_env_update_neighs(self.ctx.h, self.id))
 

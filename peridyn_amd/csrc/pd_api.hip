@@ -2249,6 +2249,77 @@ int pd_env_update_neighs(void* ctx, int64_t env) {
   });
 }
 
+static ContactDev& get_contact(Env& e, int64_t cidx) {
+  if (cidx < 0 || cidx >= (int64_t)e.cdev.size()) throw PdError(PD_E_ARG, "contact model index out of range");
+  return e.cdev[cidx];
+}
+
+int pd_contact_update(void* ctx, int64_t env, int64_t cidx, const double* y) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    ContactDev& cd = get_contact(e, cidx);
+    if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
+    if (cd.self_contact) pd_launch_self_contact_update(c->stream, c->d_envs, e.slot, cd, (int)cidx);
+    else pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, cd, (int)cidx);
+    PD_HIP(hipGetLastError());
+    PD_HIP(hipStreamSynchronize(c->stream));
+    // the lists of the other models were built from earlier positions
+    e.lists_current = (e.cdev.size() == 1) && (y != nullptr || e.lists_current);
+  });
+}
+
+int pd_contact_neighs(void* ctx, int64_t env, int64_t cidx, int64_t* neighs, int64_t max_neighs) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    const ContactDev& cd = get_contact(e, cidx);
+    const int64_t M = cd.max_neighs, n1 = cd.n1;
+    if (!neighs || max_neighs != M)
+      throw PdError(PD_E_ARG, "neighs must hold max_neighs x n1 entries with max_neighs = " + std::to_string(M));
+    std::vector<int32_t> cnt((size_t)std::max<int64_t>(n1, 1)), own((size_t)std::max<int64_t>(n1 * M, 1));
+    if (n1 > 0) {
+      PD_HIP(hipMemcpyAsync(cnt.data(), cd.own_cnt, sizeof(int32_t) * n1, hipMemcpyDeviceToHost, c->stream));
+      PD_HIP(hipMemcpyAsync(own.data(), cd.own, sizeof(int32_t) * n1 * M, hipMemcpyDeviceToHost, c->stream));
+    }
+    PD_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n1; ++i)
+      for (int64_t k = 0; k < M; ++k) neighs[i * M + k] = (k < cnt[i]) ? (int64_t)own[i * M + k] + 1 : 0;
+  });
+}
+
+int pd_contact_force(void* ctx, int64_t env, int64_t cidx, const double* y, double* f) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    PD_HIP(hipSetDevice(c->device));
+    Env& e = get_env(c, env);
+    if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
+    const ContactDev& cd = get_contact(e, cidx);
+    if (!f) throw PdError(PD_E_ARG, "null f");
+    if (y) {
+      upload_xyz(c, e, e.dev.y[e.cur], y, false);
+      e.lists_current = false;
+    }
+    const size_t cls = DevMem::size_class(sizeof(double4) * (size_t)std::max<int64_t>(e.N, 1));
+    double4* acc = static_cast<double4*>(c->mem.get(cls));
+    try {
+      upload_xyz(c, e, acc, f, false);
+      pd_launch_contact_model(c->stream, c->d_envs, e.slot, (int)cidx, !cd.expo_is_int, acc, (int)e.N);
+      PD_HIP(hipGetLastError());
+      download_xyz(c, e, acc, f);
+    } catch (...) {
+      (void)hipStreamSynchronize(c->stream);
+      c->mem.put(acc, cls);
+      throw;
+    }
+    c->mem.put(acc, cls);
+  });
+}
+
 int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f) {
   return run_guard(ctx, [&] {
     Ctx* c = C(ctx);

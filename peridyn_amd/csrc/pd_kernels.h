@@ -53,6 +53,8 @@ void pd_launch_qs_update(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs
 // n in all) are the ranges (first, count); part: n double4 of scratch
 // printdata columns of one env into row[0..7] (device), partial sums in part
 // (one double4 per chunk): the kernels of pd_env_observables_noalloc
+// one contact model's terms added into acc (n = env particles, internal order)
+void pd_launch_contact_model(hipStream_t s, const EnvDev* envs, int slot, int cidx, bool gpow, double4* acc, int n);
 void pd_launch_observables_env(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs, int slot, double4* part,
                                double* row);
 void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, EnvDev* envs, double4* part,

@@ -570,6 +570,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
   force_body<KIND, PD_PREC_FP64, false>(chunks, nchunks, envs, flags);
 }
 
+// short_range_contact! of one contact model at the resident positions (RM12,
+// contacts.jl:189-247; RM11, contacts.jl:357-380): its terms added one by one to
+// acc, the caller's f in internal order, in the reference's single-threaded
+// order.  One thread per particle.
+template <bool GPOW>
+__global__ __launch_bounds__(256) void k_contact_model(const EnvDev* __restrict__ envs, int slot, int cidx,
+                                                       double4* __restrict__ acc) {
+  const EnvDev& E = envs[slot];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E.N) return;
+  const double4* __restrict__ y = E.y[E.cur];
+  const double4 a = acc[p];
+  double a1 = a.x, a2 = a.y, a3 = a.z;
+  add_contact_model<GPOW>(E, E.contacts[cidx], p, y[p], y, a1, a2, a3);
+  acc[p] = make_double4(a1, a2, a3, 0.0);
+}
+
 // Start of a run call: kick + drift + position BCs of step time_step + 1 from
 // the stored force (velocity_verlet.jl:63-69).
 __global__ __launch_bounds__(256) void k_prologue(const Chunk* __restrict__ chunks, int nchunks,
@@ -933,6 +950,13 @@ void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int
   if (kind == PD_BB) launch_force_kind<PD_BB>(s, prec, chunks, n, envs, flags, gpow);
   else if (kind == PD_OSB) launch_force_kind<PD_OSB>(s, prec, chunks, n, envs, flags, gpow);
   else launch_force_kind<PD_EP>(s, prec, chunks, n, envs, flags, gpow);
+}
+
+void pd_launch_contact_model(hipStream_t s, const EnvDev* envs, int slot, int cidx, bool gpow, double4* acc, int n) {
+  if (n <= 0) return;
+  const dim3 g((n + 255) / 256), b(256);
+  if (gpow) hipLaunchKernelGGL(k_contact_model<true>, g, b, 0, s, envs, slot, cidx, acc);
+  else hipLaunchKernelGGL(k_contact_model<false>, g, b, 0, s, envs, slot, cidx, acc);
 }
 
 void pd_launch_prologue(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs) {

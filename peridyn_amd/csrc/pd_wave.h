@@ -95,46 +95,52 @@ __device__ __forceinline__ void add_self_contact(const EnvDev& E, const ContactD
   for (; k < cnt; ++k) rev_term(k);
 }
 
+// Terms of one contact model for particle p (short_range_contact! RM12,
+// contacts.jl:230-245; RM11, contacts.jl:357-380), added to (a1, a2, a3) in the
+// order the reference's loop over block-1 members adds them to f1 / f2.
+template <bool GPOW>
+__device__ __forceinline__ void add_contact_model(const EnvDev& E, const ContactDev& C, int p, const double4& yp,
+                                                  const double4* y, double& a1, double& a2, double& a3) {
+  if (C.self_contact) {
+    add_self_contact<GPOW>(E, C, p, yp, y, a1, a2, a3);
+    return;
+  }
+  const double4* __restrict__ x = E.x;
+  const int i1 = C.inv1[p];
+  if (i1 >= 0) {
+    const double inv_den_p = E.inv_den[p];
+    const int cnt = C.own_cnt[i1];
+    for (int k = 0; k < cnt; ++k) {
+      const int q = C.idx2[C.own[(size_t)i1 * C.max_neighs + k]];
+      const double4 yq = y[q];
+      double F1, F2, F3;
+      contact_fn<GPOW>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+      const double a = x[q].w * inv_den_p;
+      a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
+    }
+  }
+  const int i2 = C.inv2[p];
+  if (i2 >= 0) {
+    const double inv_den_p = E.inv_den[p];
+    const int off = C.rev_off[i2], cnt = C.rev_cnt[i2];
+    for (int k = 0; k < cnt; ++k) {
+      const int q = C.idx1[C.rev[off + k]];
+      const double4 yq = y[q];
+      double F1, F2, F3;
+      contact_fn<GPOW>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+      const double b = x[q].w * inv_den_p;
+      a1 -= F1 * b; a2 -= F2 * b; a3 -= F3 * b;
+    }
+  }
+}
+
 // Contact terms of particle p, models in order (update_contact_acc!,
-// solvers.jl:257-269; short_range_contact! RM12, contacts.jl:230-245), added to
-// the material force as the reference's f1/f2 updates are.
+// solvers.jl:257-269), added to the material force as the reference's f1/f2
+// updates are.
 template <bool GPOW>
 __device__ __forceinline__ void add_contacts(const EnvDev& E, int p, const double4& yp, const double4* y,
                                              double& a1, double& a2, double& a3) {
-  const double4* __restrict__ x = E.x;
-  for (int cm = 0; cm < E.ncontacts; ++cm) {
-    const ContactDev& C = E.contacts[cm];
-    if (C.self_contact) {
-      add_self_contact<GPOW>(E, C, p, yp, y, a1, a2, a3);
-      continue;
-    }
-    const int i1 = C.inv1[p];
-    if (i1 >= 0) {
-      const double inv_den_p = E.inv_den[p];
-      const int cnt = C.own_cnt[i1];
-      for (int k = 0; k < cnt; ++k) {
-        const int q = C.idx2[C.own[(size_t)i1 * C.max_neighs + k]];
-        const double4 yq = y[q];
-        double F1, F2, F3;
-        contact_fn<GPOW>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
-        const double a = x[q].w * inv_den_p;
-        a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
-      }
-    }
-    const int i2 = C.inv2[p];
-    if (i2 >= 0) {
-      const double inv_den_p = E.inv_den[p];
-      const int off = C.rev_off[i2], cnt = C.rev_cnt[i2];
-      for (int k = 0; k < cnt; ++k) {
-        const int q = C.idx1[C.rev[off + k]];
-        const double4 yq = y[q];
-        double F1, F2, F3;
-        contact_fn<GPOW>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
-        const double b = x[q].w * inv_den_p;
-        a1 -= F1 * b; a2 -= F2 * b; a3 -= F3 * b;
-      }
-    }
-  }
+  for (int cm = 0; cm < E.ncontacts; ++cm) add_contact_model<GPOW>(E, E.contacts[cm], p, yp, y, a1, a2, a3);
 }
 
 // Per-particle inputs of the tiled force kernels' epilogue (the contact terms

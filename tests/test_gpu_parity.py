@@ -418,3 +418,41 @@ def test_indentation_fixbc_against_oracle(pd, orc, kind, prec):
         assert ed <= 1e-5
     assert np.array_equal(g.damage(), o.damage())
     g.close()
+
+
+def test_contact_seam_per_model_against_oracle(pd, orc):
+    """The per-model contact seam with the caller's positions:
+    update_contact_neighs!(y, type, RM) (contacts.jl:434-476) through
+    pd_contact_update + pd_contact_neighs, and short_range_contact!(y, f, ...)
+    (contacts.jl:189-247) through pd_contact_force, each for one model of an env
+    with two: the plates' own model and a second RM12 model on the reversed pair
+    with a wider search and max_neighs 5, so its lists are truncated to the
+    largest ids.  Positions: the plates pushed into each other and jittered.
+    Bars: RM.neighs identical to the oracle's; f bit-identical to the oracle's
+    f after the same model's terms are added to the same random f."""
+    spec = W.plates_ep(nx=8, ny=8, nz=4, gap=0.5, dt=1e-4)
+    spec["contacts"].append({"pair": ([2], [1]), "exponent": 1, "stiffness": 1234.5, "distanceD": 1.0,
+                             "distanceX": 2.5, "max_neighs": 5})
+    g = pd.Env.from_spec(spec, precision="exact")
+    o = orc.OracleEnv(spec)
+    rng = np.random.default_rng(7)
+    n0 = len(spec["blocks"][0]["x"])
+    y = o.y.copy()
+    y[n0:, 2] -= 1.3
+    y += rng.uniform(-0.05, 0.05, size=y.shape)
+    for cidx in (0, 1):
+        o.y = y.copy()
+        o.update_neighs()
+        g.contact_update(cidx, y)
+        ref = o.contacts[cidx]["neighs"]
+        got = g.contact_neighs(cidx)
+        assert got.shape == ref.shape and np.array_equal(got, ref), cidx
+        assert int((ref > 0).sum()) > 50
+        f0 = rng.normal(size=y.shape)
+        o.f = f0.copy()
+        o._contact_force(o.contacts[cidx])
+        out = g.contact_force(f0, cidx, y)
+        assert np.array_equal(out, o.f), cidx
+        assert not np.array_equal(out, f0)
+    assert (o.contacts[1]["neighs"] > 0).sum(axis=1).max() == 5  # truncated
+    g.close()
