@@ -1059,6 +1059,15 @@ void commit_env(Ctx* c, Env& e) {
     if (!B.deformed) undeformed = true;
     B.c11 = (b.d.kind == PD_BB) ? b.c[0] : 0.0;
     B.horizon = b.d.horizon;
+    // FP32C: origin of x32 (the block's minimum corner) and s0 rounded up to FP32
+    for (int d = 0; d < 3; ++d) B.corner[d] = INFINITY;
+    for (int64_t i = 0; i < b.d.n; ++i)
+      for (int d = 0; d < 3; ++d) B.corner[d] = std::min(B.corner[d], b.x[3 * i + d]);
+    {
+      float f = (float)b.s0[0];
+      if ((double)f < b.s0[0]) f = std::nextafter(f, INFINITY);
+      B.s0f_up = f;
+    }
     for (int t = 0; t < nt; ++t) B.density[t] = b.density[t];
     for (int q = 0; q < nt * nt; ++q) {
       B.s0[q] = b.s0[q];

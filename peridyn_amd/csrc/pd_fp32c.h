@@ -14,15 +14,30 @@ struct G32 {
   float Y1, Y2, Y3, inv_x, inv_y, xij, ext, s;
 };
 
-__device__ __forceinline__ void geom32(const float4& xp, const double4& yp, const float4& xj, const double4& yj,
+// Displacement of a particle from its FP32-rounded reference position, in FP64:
+// w = y - (corner + x32), with x32 the FP32 offset from the block's corner (exact
+// for lattice positions).  Formed once per particle (per halo record in the
+// tiled kernels); a bond's displacement difference is then U = float(w_j - w_i).
+__device__ __forceinline__ double3 disp32(const BlockDev& B, const float4& x32, const double4& y) {
+  return make_double3(y.x - (B.corner[0] + (double)x32.x), y.y - (B.corner[1] + (double)x32.y),
+                      y.z - (B.corner[2] + (double)x32.z));
+}
+
+// Bond geometry from the FP32 reference offsets and the FP64 displacements:
+//   X = x32_j - x32_i,  U = float(w_j - w_i),  Y = X + U,
+//   |Y|^2 = |X|^2 + (2 X.U + U.U),  e = (2 X.U + U.U) / (|Y| + |X|),  s = e / |X|
+// (no |Y| - |X| cancellation); dot products as fused multiply-adds.
+__device__ __forceinline__ void geom32(const float4& xp, const double3& wp, const float4& xj, const double3& wj,
                                        G32& g) {
   const float X1 = xj.x - xp.x, X2 = xj.y - xp.y, X3 = xj.z - xp.z;
-  const float U1 = (float)((yj.x - yp.x) - (double)X1);
-  const float U2 = (float)((yj.y - yp.y) - (double)X2);
-  const float U3 = (float)((yj.z - yp.z) - (double)X3);
+  const float U1 = (float)(wj.x - wp.x);
+  const float U2 = (float)(wj.y - wp.y);
+  const float U3 = (float)(wj.z - wp.z);
   g.Y1 = X1 + U1; g.Y2 = X2 + U2; g.Y3 = X3 + U3;
-  const float xx2 = X1 * X1 + X2 * X2 + X3 * X3;
-  const float xu = 2.0f * (X1 * U1 + X2 * U2 + X3 * U3) + (U1 * U1 + U2 * U2 + U3 * U3);
+  const float xx2 = __builtin_fmaf(X1, X1, __builtin_fmaf(X2, X2, X3 * X3));
+  const float xdu = __builtin_fmaf(X1, U1, __builtin_fmaf(X2, U2, X3 * U3));
+  const float udu = __builtin_fmaf(U1, U1, __builtin_fmaf(U2, U2, U3 * U3));
+  const float xu = __builtin_fmaf(2.0f, xdu, udu);
   const float yy2 = xx2 + xu;
   g.inv_x = __builtin_amdgcn_rsqf(xx2);
   const float xij = xx2 * g.inv_x;
@@ -31,6 +46,16 @@ __device__ __forceinline__ void geom32(const float4& xp, const double4& yp, cons
   const float yij = yy2 * g.inv_y;
   g.ext = xu * __builtin_amdgcn_rcpf(yij + xij);
   g.s = g.ext * g.inv_x;
+}
+
+// ordinary state-based t (ordinary_state_based.jl:176-187) in FP32 from the
+// owner's (theta, 1/m) record ap, its product apxy = theta/m, and the
+// neighbour's (theta_j, 1/m_j): A = 3K - 5G, B15 = 15G
+__device__ __forceinline__ float osb_t32(const float4& ap, float apxy, float thj, float rmj, float A, float B15,
+                                         const G32& g) {
+  const float sm = ap.y + rmj;
+  const float twm = __builtin_fmaf(thj, rmj, apxy);
+  return __builtin_fmaf(A, twm, (B15 * g.ext) * (g.inv_x * sm));
 }
 
 __device__ __forceinline__ double comp_total(float s, float c, int C) {

@@ -50,6 +50,12 @@ struct BlockDev {
   double K[PD_MAX_TYPES * PD_MAX_TYPES];
   double G[PD_MAX_TYPES * PD_MAX_TYPES];
   double psi[PD_MAX_TYPES * PD_MAX_TYPES];
+  // FP32C form: origin of the FP32 reference offsets x32 (the block's minimum
+  // corner), and the smallest FP32 value >= s0[0]: for an FP32 stretch s,
+  // s < s0f_up exactly when (double)s < s0[0]
+  double corner[3];
+  float s0f_up;
+  int32_t pad2;
 };
 
 struct BcDev {

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
   const uint8_t* __restrict__ st = E.state;
   const int p = w.active ? w.p : w.ch.first;
   const float4 xp = x[p];
-  const double4 yp = y[p];
+  const double3 wp = disp32(B, xp, y[p]);
   const int d = w.active ? E.deg[p] : 0;
   const uint32_t base = w.ch.slot0 + w.lane;
   float s = 0.0f, c = 0.0f;
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
     for (int u = 0; u < U; ++u) {
       if (!sv[u]) continue;
       G32 g;
-      geom32(xp, yp, xj[u], yj[u], g);
+      geom32(xp, wp, xj[u], disp32(B, xj[u], yj[u]), g);
       kahan(s, c, g.ext * xj[u].w);
     }
   }
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_td32(const Chunk* __restrict__ chunks, 
   double* __restrict__ edp_o = E.edp[E.edp_cur ^ 1];
   const int p = w.active ? w.p : w.ch.first;
   const float4 xp = x[p];
-  const double4 yp = y[p];
+  const double3 wp = disp32(B, xp, y[p]);
   const float4 ap = E.aux32[p];  // (theta, 1/m, +-td_norm2, C)
   const bool yielded = pd_has_yielded(E.aux[p]);  // p's plastic extensions are 0 until it yields
   const int d = w.active ? E.deg[p] : 0;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_td32(const Chunk* __restrict__ chunks, 
       if (!sv[u]) continue;
       const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
       G32 g;
-      geom32(xp, yp, xj[u], yj[u], g);
+      geom32(xp, wp, xj[u], disp32(B, xj[u], yj[u]), g);
       const int tj = (nt > 1) ? E.ptype[jv[u]] : 0;
       const int ij = tp + nt * tj;
       if ((double)g.s < B.s0[ij]) {
@@ -196,6 +196,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
     const int32_t* __restrict__ col = E.col;
     uint8_t* __restrict__ st = E.state;
     const float4 xp = x[p];
+    const double3 wp = disp32(B, xp, yp);
     const int d = w.active ? E.deg[p] : 0;
     const int nt = B.ntype;
     const int tp = E.ptype[p];
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
         if (!sv[u]) continue;
         const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
         G32 g;
-        geom32(xp, yp, xj[u], yj[u], g);
+        geom32(xp, wp, xj[u], disp32(B, xj[u], yj[u]), g);
         const int tj = (nt > 1) ? E.ptype[jv[u]] : 0;
         const int ij = tp + nt * tj;
         if ((double)g.s < B.s0[ij]) {
@@ -266,10 +267,8 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
           if (KIND == PD_BB) {
             t = g.s * c11;
           } else if (KIND == PD_OSB) {
-            const float sm = ap.y + aj[u].y;
-            const float twm = ap.x * ap.y + aj[u].x * aj[u].y;
             const float Kf = (float)B.K[ij], Gf = (float)B.G[ij];
-            t = (3.0f * Kf - 5.0f * Gf) * twm + 15.0f * Gf * g.ext * (g.inv_x * sm);
+            t = osb_t32(ap, ap.x * ap.y, aj[u].x, aj[u].y, 3.0f * Kf - 5.0f * Gf, 15.0f * Gf, g);
           } else {
             // elasto_plastic.jl:228-255 for both halves of the pair, plastic
             // state in FP64; the own bond's updated extension goes to the

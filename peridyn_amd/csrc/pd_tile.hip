@@ -505,10 +505,12 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
 
 // ---------------------------------------------------------------------------
 // FP32-compensated form (BB / OSB blocks of PD_PREC_FP32C envs; the bond
-// arithmetic of pd_step32.hip).  LDS: y (3 doubles) first, then x32 (4 floats:
+// arithmetic of pd_step32.hip).  LDS: the FP64 displacement w = y - (corner +
+// x32) (3 doubles, formed here once per halo record) first, then x32 (4 floats:
 // offsets from the block corner, volume) and for OSB aux32 (theta, 1/m).
 template <int NFF>
-__device__ __forceinline__ void stage_halo32(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap) {
+__device__ __forceinline__ void stage_halo32(const EnvDev& E, const BlockDev& B, const Tile& T,
+                                             double* __restrict__ sh, int hcap) {
   const float4* __restrict__ x = E.x32;
   const double4* __restrict__ y = E.y[E.cur];
   const int32_t* __restrict__ halo = E.halo;
@@ -534,9 +536,10 @@ __device__ __forceinline__ void stage_halo32(const EnvDev& E, const Tile& T, dou
     for (int r = 0; r < kStageR; ++r) {
       const int i = i0 + 256 * r;
       if (i < T.hn) {
-        sh[i] = b[r].x;
-        sh[hcap + i] = b[r].y;
-        sh[2 * hcap + i] = b[r].z;
+        const double3 wd = disp32(B, a[r], b[r]);
+        sh[i] = wd.x;
+        sh[hcap + i] = wd.y;
+        sh[2 * hcap + i] = wd.z;
         fs[i] = a[r].x;
         fs[hcap + i] = a[r].y;
         fs[2 * hcap + i] = a[r].z;
@@ -564,7 +567,7 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   if (B.kind != PD_OSB || !B.deformed) return;
-  stage_halo32<4>(E, T, sh, hcap);
+  stage_halo32<4>(E, B, T, sh, hcap);
   __syncthreads();
   TileWave w;
   if (!tile_wave<LG>(T, w)) return;
@@ -572,7 +575,7 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
   const uint64_t* __restrict__ pk = E.pk;
   const int p = w.active ? w.p : w.ch.first;
   const float4 xp = gld4(E.x32, p);
-  const double4 yp = gld4(E.y[E.cur], p);
+  const double3 wp = disp32(B, xp, gld4(E.y[E.cur], p));
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   float s = 0.0f, c = 0.0f;
@@ -585,9 +588,9 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
       if (!(e & 0x8000u)) continue;
       const int j = (int)(e & 0x7fffu);
       const float4 xj = make_float4(fs[j], fs[hcap + j], fs[2 * hcap + j], fs[3 * hcap + j]);
-      const double4 yj = make_double4(sh[j], sh[hcap + j], sh[2 * hcap + j], 0.0);
+      const double3 wj = make_double3(sh[j], sh[hcap + j], sh[2 * hcap + j]);
       G32 g;
-      geom32(xp, yp, xj, yj, g);
+      geom32(xp, wp, xj, wj, g);
       kahan(s, c, g.ext * xj.w);
     }
   }
@@ -617,7 +620,7 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
   const bool has = tile_wave<LG>(T, w);
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
   EpiPre ep;  // the epilogue's inputs, loaded after the bond loop (its registers)
-  if (deformed) stage_halo32<KIND == PD_OSB ? 6 : 4>(E, T, sh, hcap);
+  if (deformed) stage_halo32<KIND == PD_OSB ? 6 : 4>(E, B, T, sh, hcap);
   __syncthreads();
   if (!has) return;
   const double4* __restrict__ y = E.y[E.cur];
@@ -629,12 +632,15 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
     uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
     uint8_t* __restrict__ st = E.state;
     const float4 xp = gld4(E.x32, p);
+    const double3 wp = disp32(B, xp, yp);
     const float inv_den = (float)(1.0 / B.density[0]);
     float4 ap = make_float4(0.f, 0.f, 0.f, 0.f);
     if (KIND == PD_OSB) ap = E.aux32[p];
+    const float apxy = ap.x * ap.y;
     const float c11 = (float)B.c11;
-    const double s0 = B.s0[0];
+    const float s0f = B.s0f_up;  // (double)s < s0 exactly when s < s0f
     const float Kf = (float)B.K[0], Gf = (float)B.G[0];
+    const float A = 3.0f * Kf - 5.0f * Gf, B15 = 15.0f * Gf;
     float s1 = 0.f, s2 = 0.f, s3 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
     const int nt4 = (w.ch.width + 3) >> 2;
     const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
@@ -648,17 +654,15 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
         if (!(e & 0x8000u)) continue;
         const int j = (int)(e & 0x7fffu);
         const float4 xj = make_float4(fs[j], fs[hcap + j], fs[2 * hcap + j], fs[3 * hcap + j]);
-        const double4 yj = make_double4(sh[j], sh[hcap + j], sh[2 * hcap + j], 0.0);
+        const double3 wj = make_double3(sh[j], sh[hcap + j], sh[2 * hcap + j]);
         G32 g;
-        geom32(xp, yp, xj, yj, g);
-        if ((double)g.s < s0) {
+        geom32(xp, wp, xj, wj, g);
+        if (g.s < s0f) {
           float tt;
           if (KIND == PD_BB) {
             tt = g.s * c11;
           } else {
-            const float sm = ap.y + fs[5 * hcap + j];
-            const float twm = ap.x * ap.y + fs[4 * hcap + j] * fs[5 * hcap + j];
-            tt = (3.0f * Kf - 5.0f * Gf) * twm + 15.0f * Gf * g.ext * (g.inv_x * sm);
+            tt = osb_t32(ap, apxy, fs[4 * hcap + j], fs[5 * hcap + j], A, B15, g);
           }
           const float xx = tt * g.inv_y * (xj.w * inv_den);
           kahan(s1, c1, xx * g.Y1);
