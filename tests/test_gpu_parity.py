@@ -365,8 +365,8 @@ def test_graph_replay_equals_eager(pd, orc, monkeypatch):
     """Steps replayed from the captured HIP graphs equal eagerly launched steps
     bit for bit, over several pd_run calls (each capturing and replaying its own
     graphs: plain steps, and steps followed by the contact rebuild, replayed at
-    every rebuild after the first); and both equal the oracle in the
-    reference-order form."""
+    every rebuild after the first; calls of at least 32 steps capture them);
+    and both equal the oracle in the reference-order form."""
     spec = W.plates_ep(nx=8, ny=8, nz=5, gap=0.5, dt=1e-4, approach_per_step=0.05, exponent=2)
     for prec in ("exact", "fp64"):
         runs = {}
@@ -374,7 +374,7 @@ def test_graph_replay_equals_eager(pd, orc, monkeypatch):
             monkeypatch.setenv("PERIDYN_GRAPH", mode)
             g = pd.Env.from_spec(spec, precision=prec)
             snaps = []
-            for n in (7, 12, 11):
+            for n in (37, 42, 33):
                 g.run(n, 5)
                 snaps.append(g.download())
             runs[mode] = snaps
@@ -383,9 +383,9 @@ def test_graph_replay_equals_eager(pd, orc, monkeypatch):
                 assert np.array_equal(u, w)
         if prec == "exact":
             o = orc.OracleEnv(spec)
-            o.run(7, 5)
-            o.run(12, 5, start=True)
-            o.run(11, 5, start=True)
+            o.run(37, 5)
+            o.run(42, 5, start=True)
+            o.run(33, 5, start=True)
             assert np.array_equal(runs["1"][-1][0], o.y)
             assert np.array_equal(runs["1"][-1][2], o.f)
 
