@@ -50,13 +50,20 @@ __device__ __forceinline__ void geom32(const float4& xp, const double3& wp, cons
 
 // ordinary state-based t (ordinary_state_based.jl:176-187) in FP32 from the
 // owner's (theta, 1/m) record ap, its product apxy = theta/m, and the
-// neighbour's (theta_j, 1/m_j): A = 3K - 5G, B15 = 15G
+// neighbour's (theta_j, 1/m_j): A = 3K - 5G, B15 = 15G;
+// t = A (theta_i/m_i + theta_j/m_j) + 15G s (1/m_i + 1/m_j)
 __device__ __forceinline__ float osb_t32(const float4& ap, float apxy, float thj, float rmj, float A, float B15,
                                          const G32& g) {
   const float sm = ap.y + rmj;
   const float twm = __builtin_fmaf(thj, rmj, apxy);
-  return __builtin_fmaf(A, twm, (B15 * g.ext) * (g.inv_x * sm));
+  return __builtin_fmaf(A, twm, (B15 * g.s) * sm);
 }
+
+// Per-neighbour factor of the force term xx = (t / |Y|) vol_j / density: for
+// bond-based blocks t = c11 s, so the factor c11 vol_j / density is taken with
+// it; formed once per halo record in the tiled kernels, per bond in the others
+__device__ __forceinline__ float vd32(float vol, float inv_den) { return vol * inv_den; }
+__device__ __forceinline__ float cvd32(float c11, float vol, float inv_den) { return c11 * (vol * inv_den); }
 
 __device__ __forceinline__ double comp_total(float s, float c, int C) {
   return group_sum((double)s - (double)c, C);

@@ -262,13 +262,15 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
         geom32(xp, wp, xj[u], disp32(B, xj[u], yj[u]), g);
         const int tj = (nt > 1) ? E.ptype[jv[u]] : 0;
         const int ij = tp + nt * tj;
+        float xx = 0.0f;  // (t / |Y|) vol_j / density; 0 for a bond broken here
         if ((double)g.s < B.s0[ij]) {
-          float t;
           if (KIND == PD_BB) {
-            t = g.s * c11;
+            const float tt = g.s * g.inv_y;
+            xx = tt * cvd32(c11, xj[u].w, inv_den);
           } else if (KIND == PD_OSB) {
             const float Kf = (float)B.K[ij], Gf = (float)B.G[ij];
-            t = osb_t32(ap, ap.x * ap.y, aj[u].x, aj[u].y, 3.0f * Kf - 5.0f * Gf, 15.0f * Gf, g);
+            const float tt = osb_t32(ap, ap.x * ap.y, aj[u].x, aj[u].y, 3.0f * Kf - 5.0f * Gf, 15.0f * Gf, g) * g.inv_y;
+            xx = tt * vd32(xj[u].w, inv_den);
           } else {
             // elasto_plastic.jl:228-255 for both halves of the pair, plastic
             // state in FP64; the own bond's updated extension goes to the
@@ -292,16 +294,16 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
             if (yielded_p) gst(edp_out, slot, e_new);  // else e_new = 0 = stored
             double td_rev = g15 * (ed - er[u]);
             if (!(Cj > 1.0)) td_rev *= Cj;
-            t = (float)(3.0 * B.K[ij] * twm + 0.5 * (td_own + td_rev));
+            const float t = (float)(3.0 * B.K[ij] * twm + 0.5 * (td_own + td_rev));
+            xx = (t * g.inv_y) * vd32(xj[u].w, inv_den);
           }
-          const float xx = t * g.inv_y * (xj[u].w * inv_den);
-          kahan(s1, c1, xx * g.Y1);
-          kahan(s2, c2, xx * g.Y2);
-          kahan(s3, c3, xx * g.Y3);
         } else {
           gst(st, slot, (uint8_t)0);
           if (KIND == PD_EP && yielded_p) gst(edp_out, slot, eo[u]);
         }
+        kahan(s1, c1, xx * g.Y1);
+        kahan(s2, c2, xx * g.Y2);
+        kahan(s3, c3, xx * g.Y3);
       }
     }
     a1 = comp_total(s1, c1, w.C);

@@ -150,6 +150,47 @@ struct PkStream {
   }
 };
 
+// Runs body(t4, word) over a lane's bond groups t4 = 0 .. nt4-1 in order, the
+// next group's packed word in flight while a group is computed.  The words
+// alternate between two variables, each reloaded only after its group ran, and
+// every iteration issues both loads (beyond the last group they re-read it and
+// the value goes unused), so the wait before a group's first use leaves exactly
+// one load in flight.  body must issue no global memory instruction: loads and
+// stores share one counter, and any would make that wait drain everything.
+template <class F>
+__device__ __forceinline__ void pk_groups(const uint64_t* __restrict__ pk, uint32_t wb, int nt4, F&& body) {
+  if (nt4 <= 0) return;
+  const int last = nt4 - 1;
+  auto word = [&](int g) { return gld(pk, wb + 64u * (uint32_t)(g < last ? g : last)); };
+  uint64_t w0 = word(0), w1 = word(1);
+  for (int t4 = 0; t4 < nt4; t4 += 2) {
+    body(t4, w0);
+    w0 = word(t4 + 2);
+    if (t4 + 1 < nt4) body(t4 + 1, w1);
+    w1 = word(t4 + 3);
+  }
+}
+
+// Records the bonds a lane broke in the bond loop just run: every intact bond
+// for which broken(j) holds (the loop's own test, recomputed from the same
+// staged records) loses its state byte and its pk intact bit.  Called after
+// the loop by the lanes that saw a break, so that the loop stores nothing.
+template <class F>
+__device__ __forceinline__ void pk_record_breaks(const uint64_t* __restrict__ pk, uint16_t* __restrict__ pk16,
+                                                 uint8_t* __restrict__ st, uint32_t wb, uint32_t base, int nt4,
+                                                 F&& broken) {
+  for (int t4 = 0; t4 < nt4; ++t4) {
+    const uint64_t cw = gld(pk, wb + 64u * (uint32_t)t4);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+      if (!(e & 0x8000u) || !broken((int)(e & 0x7fffu))) continue;
+      gst(st, base + 64u * (uint32_t)(4 * t4 + u), (uint8_t)0);
+      gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
+    }
+  }
+}
+
 // bond geometry against the staged neighbour j (expressions of the untiled kernels)
 template <class R>
 __device__ __forceinline__ void tile_geom(const R& rc, int j, const double4& xp, const double4& yp, double& X1,
@@ -183,9 +224,7 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   double acc = 0.0;
-  PkStream<3> ps(pk, wb, nt4);
-  for (int t4 = 0; t4 < nt4; ++t4) {
-    const uint64_t cw = ps.next(t4);
+  pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
@@ -198,7 +237,7 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
       const double ext = yy2 * pd_rsqrt(yy2) - xx2 * pd_rsqrt(xx2);
       acc = __builtin_fma(ext, rc.vol(j), acc);
     }
-  }
+  });
   acc = group_sum(acc, C);
   if (w.active && w.g == 0) {
     const double th = (3.0 * E.aux[p].y) * acc;
@@ -453,15 +492,13 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
       double4 ap = make_double4(0.0, 0.0, 0.0, 0.0);
       if (KIND != PD_BB) ap = E.aux[p];
       const double c11 = B.c11;
-      PkStream<3> ps(pk, wb, nt4);
-      for (int t4 = 0; t4 < nt4; ++t4) {
-        const uint64_t cw = ps.next(t4);
+      bool brk = false;  // this lane broke a bond
+      pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
           if (!(e & 0x8000u)) continue;
           const int j = (int)(e & 0x7fffu);
-          const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
           double X1, X2, X3, Y1, Y2, Y3;
           tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
           const double xx2 = pd_norm2(X1, X2, X3);
@@ -486,11 +523,23 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
             a2 = __builtin_fma(xx, Y2, a2);
             a3 = __builtin_fma(xx, Y3, a3);
           } else {
-            gst(st, slot, (uint8_t)0);
-            gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
+            brk = true;
           }
         }
-      }
+      });
+      if (brk)
+        pk_record_breaks(pk, pk16, st, wb, base, nt4, [&](int j) {
+          double X1, X2, X3, Y1, Y2, Y3;
+          tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
+          const double xx2 = pd_norm2(X1, X2, X3);
+          const double yy2 = pd_norm2(Y1, Y2, Y3);
+          const double inv_x = pd_rsqrt(xx2);
+          const double inv_y = pd_rsqrt(yy2);
+          const double xij = xx2 * inv_x;
+          const double yij = yy2 * inv_y;
+          const double ext = yij - xij;
+          return !(ext * inv_x < s0_pp);
+        });
     }
     a1 = group_sum(a1, C);
     a2 = group_sum(a2, C);
@@ -505,16 +554,23 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
 
 // ---------------------------------------------------------------------------
 // FP32-compensated form (BB / OSB blocks of PD_PREC_FP32C envs; the bond
-// arithmetic of pd_step32.hip).  LDS: the FP64 displacement w = y - (corner +
-// x32) (3 doubles, formed here once per halo record) first, then x32 (4 floats:
-// offsets from the block corner, volume) and for OSB aux32 (theta, 1/m).
-template <int NFF>
+// arithmetic of pd_step32.hip).  LDS record, every field 8 bytes wide so that one
+// address j * 8 reaches them all: the FP64 displacement w = y - (corner + x32)
+// (3 doubles, formed here once per halo record), then float pairs (x32.x,
+// x32.y), (x32.z, v) and for OSB forces (theta, 1/m).  v is the volume for
+// theta, vol / density for OSB forces and c11 vol / density for BB forces (the
+// neighbour factors the untiled kernels form per bond, with the same roundings).
+enum { ST_THETA = 0, ST_BB = 1, ST_OSB = 2 };
+
+template <int MODE>
 __device__ __forceinline__ void stage_halo32(const EnvDev& E, const BlockDev& B, const Tile& T,
                                              double* __restrict__ sh, int hcap) {
   const float4* __restrict__ x = E.x32;
   const double4* __restrict__ y = E.y[E.cur];
   const int32_t* __restrict__ halo = E.halo;
-  float* __restrict__ fs = (float*)(sh + 3 * hcap);
+  float2* __restrict__ f2 = (float2*)(sh + 3 * hcap);
+  const float inv_den = (float)(1.0 / B.density[0]);
+  const float c11 = (float)B.c11;
   for (int i0 = threadIdx.x; i0 < T.hn; i0 += 256 * kStageR) {
     uint32_t id[kStageR];
 #pragma unroll
@@ -529,7 +585,7 @@ __device__ __forceinline__ void stage_halo32(const EnvDev& E, const BlockDev& B,
       if (i0 + 256 * r < T.hn) {
         a[r] = gld4(x, id[r]);
         b[r] = gld4(y, id[r]);
-        if (NFF > 4) c[r] = gld4(E.aux32, id[r]);
+        if (MODE == ST_OSB) c[r] = gld4(E.aux32, id[r]);
       }
     }
 #pragma unroll
@@ -540,17 +596,27 @@ __device__ __forceinline__ void stage_halo32(const EnvDev& E, const BlockDev& B,
         sh[i] = wd.x;
         sh[hcap + i] = wd.y;
         sh[2 * hcap + i] = wd.z;
-        fs[i] = a[r].x;
-        fs[hcap + i] = a[r].y;
-        fs[2 * hcap + i] = a[r].z;
-        fs[3 * hcap + i] = a[r].w;
-        if (NFF > 4) {
-          fs[4 * hcap + i] = c[r].x;
-          fs[5 * hcap + i] = c[r].y;
-        }
+        const float v = MODE == ST_THETA ? a[r].w : MODE == ST_BB ? cvd32(c11, a[r].w, inv_den) : vd32(a[r].w, inv_den);
+        f2[i] = make_float2(a[r].x, a[r].y);
+        f2[hcap + i] = make_float2(a[r].z, v);
+        if (MODE == ST_OSB) f2[2 * hcap + i] = make_float2(c[r].x, c[r].y);
       }
     }
   }
+}
+
+// staged record j: reference offsets (xyz) with v in w, and the displacement
+struct Rec32 {
+  float4 x;
+  double3 w;
+};
+__device__ __forceinline__ Rec32 rec32(const double* __restrict__ sh, const float2* __restrict__ f2, int hcap,
+                                       int j) {
+  const float2 a = f2[j], b = f2[hcap + j];
+  Rec32 r;
+  r.x = make_float4(a.x, a.y, b.x, b.y);
+  r.w = make_double3(sh[j], sh[hcap + j], sh[2 * hcap + j]);
+  return r;
 }
 
 // dilatation theta (src/peri.jl:155-172) of OSB blocks; S as in the FP64
@@ -567,11 +633,11 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   if (B.kind != PD_OSB || !B.deformed) return;
-  stage_halo32<4>(E, B, T, sh, hcap);
+  stage_halo32<ST_THETA>(E, B, T, sh, hcap);
   __syncthreads();
   TileWave w;
   if (!tile_wave<LG>(T, w)) return;
-  const float* __restrict__ fs = (const float*)(sh + 3 * hcap);
+  const float2* __restrict__ f2 = (const float2*)(sh + 3 * hcap);
   const uint64_t* __restrict__ pk = E.pk;
   const int p = w.active ? w.p : w.ch.first;
   const float4 xp = gld4(E.x32, p);
@@ -579,21 +645,18 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   float s = 0.0f, c = 0.0f;
-  PkStream<3> ps(pk, wb, nt4);
-  for (int t4 = 0; t4 < nt4; ++t4) {
-    const uint64_t cw = ps.next(t4);
+  pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
       if (!(e & 0x8000u)) continue;
       const int j = (int)(e & 0x7fffu);
-      const float4 xj = make_float4(fs[j], fs[hcap + j], fs[2 * hcap + j], fs[3 * hcap + j]);
-      const double3 wj = make_double3(sh[j], sh[hcap + j], sh[2 * hcap + j]);
+      const Rec32 r = rec32(sh, f2, hcap, j);
       G32 g;
-      geom32(xp, wp, xj, wj, g);
-      kahan(s, c, g.ext * xj.w);
+      geom32(xp, wp, r.x, r.w, g);
+      kahan(s, c, g.ext * r.x.w);
     }
-  }
+  });
   const double tot = comp_total(s, c, C);
   if (w.active && w.g == 0) {
     const float th = (float)((3.0 * (double)E.aux32[p].y) * tot);
@@ -620,24 +683,22 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
   const bool has = tile_wave<LG>(T, w);
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
   EpiPre ep;  // the epilogue's inputs, loaded after the bond loop (its registers)
-  if (deformed) stage_halo32<KIND == PD_OSB ? 6 : 4>(E, B, T, sh, hcap);
+  if (deformed) stage_halo32<KIND == PD_OSB ? ST_OSB : ST_BB>(E, B, T, sh, hcap);
   __syncthreads();
   if (!has) return;
   const double4* __restrict__ y = E.y[E.cur];
   const double4 yp = gld4(y, p);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
   if (deformed) {
-    const float* __restrict__ fs = (const float*)(sh + 3 * hcap);
+    const float2* __restrict__ f2 = (const float2*)(sh + 3 * hcap);
     const uint64_t* __restrict__ pk = E.pk;
     uint16_t* __restrict__ pk16 = (uint16_t*)E.pk;
     uint8_t* __restrict__ st = E.state;
     const float4 xp = gld4(E.x32, p);
     const double3 wp = disp32(B, xp, yp);
-    const float inv_den = (float)(1.0 / B.density[0]);
     float4 ap = make_float4(0.f, 0.f, 0.f, 0.f);
     if (KIND == PD_OSB) ap = E.aux32[p];
     const float apxy = ap.x * ap.y;
-    const float c11 = (float)B.c11;
     const float s0f = B.s0f_up;  // (double)s < s0 exactly when s < s0f
     const float Kf = (float)B.K[0], Gf = (float)B.G[0];
     const float A = 3.0f * Kf - 5.0f * Gf, B15 = 15.0f * Gf;
@@ -645,35 +706,35 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
     const int nt4 = (w.ch.width + 3) >> 2;
     const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
     const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
-    PkStream<3> ps(pk, wb, nt4);
-    for (int t4 = 0; t4 < nt4; ++t4) {
-      const uint64_t cw = ps.next(t4);
+    bool brk = false;  // this lane broke a bond
+    pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
         if (!(e & 0x8000u)) continue;
         const int j = (int)(e & 0x7fffu);
-        const float4 xj = make_float4(fs[j], fs[hcap + j], fs[2 * hcap + j], fs[3 * hcap + j]);
-        const double3 wj = make_double3(sh[j], sh[hcap + j], sh[2 * hcap + j]);
+        const Rec32 r = rec32(sh, f2, hcap, j);
+        float2 am = make_float2(0.f, 0.f);
+        if (KIND == PD_OSB) am = f2[2 * hcap + j];
         G32 g;
-        geom32(xp, wp, xj, wj, g);
-        if (g.s < s0f) {
-          float tt;
-          if (KIND == PD_BB) {
-            tt = g.s * c11;
-          } else {
-            tt = osb_t32(ap, apxy, fs[4 * hcap + j], fs[5 * hcap + j], A, B15, g);
-          }
-          const float xx = tt * g.inv_y * (xj.w * inv_den);
-          kahan(s1, c1, xx * g.Y1);
-          kahan(s2, c2, xx * g.Y2);
-          kahan(s3, c3, xx * g.Y3);
-        } else {
-          gst(st, base + 64u * (uint32_t)(4 * t4 + u), (uint8_t)0);
-          gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
-        }
+        geom32(xp, wp, r.x, r.w, g);
+        // (t / |Y|) vol_j / density, formed for every bond; 0 for one broken here
+        const float tt = (KIND == PD_BB ? g.s : osb_t32(ap, apxy, am.x, am.y, A, B15, g)) * g.inv_y;
+        const bool intact = g.s < s0f;
+        brk = brk || !intact;
+        const float xx = intact ? tt * r.x.w : 0.0f;
+        kahan(s1, c1, xx * g.Y1);
+        kahan(s2, c2, xx * g.Y2);
+        kahan(s3, c3, xx * g.Y3);
       }
-    }
+    });
+    if (brk)
+      pk_record_breaks(pk, pk16, st, wb, base, nt4, [&](int j) {
+        const Rec32 r = rec32(sh, f2, hcap, j);
+        G32 g;
+        geom32(xp, wp, r.x, r.w, g);
+        return !(g.s < s0f);
+      });
     a1 = comp_total(s1, c1, C);
     a2 = comp_total(s2, c2, C);
     a3 = comp_total(s3, c3, C);
