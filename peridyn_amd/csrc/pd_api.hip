@@ -235,10 +235,12 @@ struct Ctx {
   int off_theta_u = 0, off_td_u = 0, off_force_u[3] = {0, 0, 0};
   Tile* d_tset = nullptr;
   size_t tset_cap = 0;
-  // [form class: 0 FP64 kernels, 1 FP32C kernels][lgG]
-  int nt_theta[2][4] = {}, nt_td[4] = {}, nt_force[3][2][4] = {};
-  int ot_theta[2][4] = {}, ot_td[4] = {}, ot_force[3][2][4] = {};
-  int set_hcap = 1;
+  // tile lists [form class: 0 FP64 kernels, 1 FP32C kernels][lgG][LDS class]
+  struct TileList {
+    int off = 0, n = 0;
+    int hcap = 1;  // largest halo of the listed envs
+  };
+  TileList tl_theta[2][4][PD_TILE_CLASSES], tl_td[4][PD_TILE_CLASSES], tl_force[3][2][4][PD_TILE_CLASSES];
   bool set_tiled = false;
   int set_xf = 0;  // every tiled env of the set has FP32-exact x and volumes
   // pair contacts of the set, rebuilt together (pd_launch_contact_update_batch)
@@ -614,6 +616,9 @@ void build_order(Env& e) {
 
 // LDS a tile may use for its largest pass (2 workgroups fit a CU's 160 KiB)
 constexpr size_t kTileLdsBudget = 80 * 1024;
+// tiles of one launch that keep every CU busy for about two rounds (5
+// workgroups per CU on 256 CUs)
+constexpr int kMergeTiles = 2 * 5 * 256;
 
 // Tiles of a brick-ordered env: consecutive chunks of one brick, up to
 // PD_TILE_CHUNKS per tile.  Each tile's halo is the sorted set of neighbour ids
@@ -800,6 +805,34 @@ void build_tiles(Ctx* c, Env& e) {
   e.tiled = true;
 }
 
+// Lanes per particle (log2) of a tiled env without an explicit schedule: 4
+// (tiles of 4x4x4 particles) unless the horizon makes their halos outgrow the
+// smaller compiled LDS stride, then 8 (4x2x4 tiles, ~25% smaller halos, so more
+// workgroups fit a CU).  The halo of a brick spanning Lx x Ly x Lz particle
+// spacings is estimated by the volume of the brick dilated by the horizon r
+// (Steiner's formula, in particle volumes), which tracks lattice counts within
+// ~10% for r = 2.5 .. 5 spacings.
+int tiled_lanes_log2(const Env& e) {
+  double r = 0.0;
+  for (auto& b : e.blocks) {
+    const double ps = b.d.particle_size > 0.0 ? b.d.particle_size : b.d.horizon / 3.0;
+    r = std::max(r, b.d.horizon / ps);
+  }
+  const double pi = 3.141592653589793;
+  auto halo = [&](double lx, double ly, double lz) {
+    return (int)(lx * ly * lz + 2.0 * r * (lx * ly + ly * lz + lz * lx) + pi * r * r * (lx + ly + lz) +
+                 (4.0 / 3.0) * pi * r * r * r);
+  };
+  const int h4 = halo(3.0, 3.0, 3.0), h8 = halo(3.0, 1.0, 3.0);
+  if (h4 <= 608) return 2;  // 4x4x4 halos this small take the densest LDS classes anyway
+  int c4 = 0, c8 = 0;  // LDS classes of the force pass
+  for (auto& b : e.blocks) {
+    c4 = std::max(c4, pd_tile_class(2, b.d.kind, e.precision, 1, h4));
+    c8 = std::max(c8, pd_tile_class(2, b.d.kind, e.precision, 1, h8));
+  }
+  return c8 < c4 ? 3 : 2;
+}
+
 void commit_env(Ctx* c, Env& e) {
   if (e.committed) throw PdError(PD_E_STATE, "env already committed");
   if (e.blocks.empty()) throw PdError(PD_E_ARG, "env has no material blocks");
@@ -816,7 +849,8 @@ void commit_env(Ctx* c, Env& e) {
   // tiled passes: FP64 form, 2..8 lanes per particle, single-type blocks and
   // integer contact exponents (the tiled force kernel carries no general pow);
   // 4 lanes per particle unless a schedule was set (measured best on the EP
-  // plates, equal to 2 on the BB disk)
+  // plates, equal to 2 on the BB disk) or the horizon calls for 8
+  // (tiled_lanes_log2)
   bool tileable = (e.precision == PD_PREC_FP64 || e.precision == PD_PREC_FP32C) && tiles_enabled();
   for (auto& b : e.blocks) tileable = tileable && b.d.ntype == 1;
   // the FP32C elastoplastic passes are untiled kernels (k_td32, k_force32<EP>)
@@ -826,7 +860,7 @@ void commit_env(Ctx* c, Env& e) {
   if (e.precision == PD_PREC_EXACT) {
     e.lgG = 0;
   } else if (e.lgG < 0 && tileable) {
-    e.lgG = 2;
+    e.lgG = tiled_lanes_log2(e);
   } else if (e.lgG < 0) {
     e.lgG = 0;
     while (e.lgG < 4 && (N << e.lgG) < 786432) ++e.lgG;
@@ -1230,9 +1264,18 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   std::vector<int32_t> slots;
   std::vector<int2> ranges;
   bool has_ep = false, has_th = false, gpow = false, tiled = false, xf = true;
-  int hcap = 1, n_all = 0, n_th = 0, n_td = 0, n_fk[3] = {0, 0, 0};
-  // tile list sizes: theta [pc][lg], td [lg], force [kind][pc][lg]
-  int s_th[2][4] = {}, s_td[4] = {}, s_fk[3][2][4] = {};
+  int n_all = 0, n_th = 0, n_td = 0, n_fk[3] = {0, 0, 0};
+  // tile lists: theta [pc][lg][cls], td [lg][cls], force [kind][pc][lg][cls]
+  using TileList = Ctx::TileList;
+  TileList l_th[2][4][PD_TILE_CLASSES], l_td[4][PD_TILE_CLASSES], l_fk[3][2][4][PD_TILE_CLASSES];
+  for (int64_t id : ids) {
+    const Env& e = get_env(c, id);
+    if (e.tiled) xf = xf && e.xf32;  // the FP64 records' size, hence the LDS classes
+  }
+  auto add = [](TileList& l, int n, int hmax) {
+    l.n += n;
+    l.hcap = std::max(l.hcap, hmax);
+  };
   for (int64_t id : ids) {
     Env& e = get_env(c, id);
     if (!e.committed) throw PdError(PD_E_STATE, "env " + std::to_string(id) + " is not committed");
@@ -1262,14 +1305,13 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
       }
     if (e.tiled) {
       tiled = true;
-      xf = xf && e.xf32;
-      hcap = std::max(hcap, (int)e.hmax);
+      const int hm = (int)e.hmax, lg = e.lgG, pr = e.precision;
+      const int pc = (pr == PD_PREC_FP32C) ? 1 : 0;  // FP32C envs tile BB / OSB blocks only
       for (auto& tb : e.tile_blocks) {
         const int kind = e.blocks[tb[0]].d.kind;
-        const int pc = (e.precision == PD_PREC_FP32C) ? 1 : 0;  // FP32C envs tile BB / OSB blocks only
-        s_fk[kind][pc][e.lgG] += tb[2];
-        if (kind != PD_BB) s_th[pc][e.lgG] += tb[2];
-        if (kind == PD_EP) s_td[e.lgG] += tb[2];
+        add(l_fk[kind][pc][lg][pd_tile_class(2, kind, pr, xf, hm)], tb[2], hm);
+        if (kind != PD_BB) add(l_th[pc][lg][pd_tile_class(0, kind, pr, xf, hm)], tb[2], hm);
+        if (kind == PD_EP) add(l_td[lg][pd_tile_class(1, kind, pr, xf, hm)], tb[2], hm);
       }
     }
   }
@@ -1306,18 +1348,26 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   if (!rest.empty())
     PD_HIP(hipMemcpyAsync(c->d_set + n_all, rest.data(), sizeof(Chunk) * rest.size(), hipMemcpyHostToDevice,
                           c->stream));
-  // tile set: per lgG: theta [pc], force [kind][pc], td
+  // tile set: per lgG, form and pass the lists of the LDS classes in class
+  // order, so that neighbouring classes can share a launch
   int tot = 0;
+  auto place = [&](TileList* l) {
+    for (int h = 0; h < PD_TILE_CLASSES; ++h) {
+      l[h].off = tot;
+      tot += l[h].n;
+    }
+  };
   for (int lg = 1; lg <= 3; ++lg) {
     for (int pc = 0; pc < 2; ++pc) {
-      c->ot_theta[pc][lg] = tot; c->nt_theta[pc][lg] = s_th[pc][lg]; tot += s_th[pc][lg];
-      for (int k = 0; k < 3; ++k) { c->ot_force[k][pc][lg] = tot; c->nt_force[k][pc][lg] = s_fk[k][pc][lg]; tot += s_fk[k][pc][lg]; }
+      place(l_th[pc][lg]);
+      for (int k = 0; k < 3; ++k) place(l_fk[k][pc][lg]);
     }
-    c->ot_td[lg] = tot; c->nt_td[lg] = s_td[lg]; tot += s_td[lg];
+    place(l_td[lg]);
   }
   if (tot > 0) {
     ensure_cap(c->d_tset, c->tset_cap, (size_t)tot, c->mem);
-    int f_th[2][4] = {}, f_td[4] = {}, f_fk[3][2][4] = {};  // fill positions
+    // fill positions
+    int f_th[2][4][PD_TILE_CLASSES] = {}, f_td[4][PD_TILE_CLASSES] = {}, f_fk[3][2][4][PD_TILE_CLASSES] = {};
     auto put = [&](int dst, const Env& e, const std::array<int, 3>& tb) {
       copies.push_back(PdCopy{reinterpret_cast<const uint32_t*>(e.d_tiles + tb[1]),
                               reinterpret_cast<uint32_t*>(c->d_tset + dst), (uint32_t)(sizeof(Tile) * tb[2] / 4),
@@ -1326,18 +1376,61 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
     for (int64_t id : ids) {
       Env& e = get_env(c, id);
       if (!e.tiled) continue;
-      const int lg = e.lgG;
+      const int lg = e.lgG, hm = (int)e.hmax, pr = e.precision;
+      const int pc = (pr == PD_PREC_FP32C) ? 1 : 0;
       for (auto& tb : e.tile_blocks) {
         const int kind = e.blocks[tb[0]].d.kind;
-        const int pc = (e.precision == PD_PREC_FP32C) ? 1 : 0;
-        put(c->ot_force[kind][pc][lg] + f_fk[kind][pc][lg], e, tb);
-        f_fk[kind][pc][lg] += tb[2];
-        if (kind != PD_BB) { put(c->ot_theta[pc][lg] + f_th[pc][lg], e, tb); f_th[pc][lg] += tb[2]; }
-        if (kind == PD_EP) { put(c->ot_td[lg] + f_td[lg], e, tb); f_td[lg] += tb[2]; }
+        const int hf = pd_tile_class(2, kind, pr, xf, hm);
+        put(l_fk[kind][pc][lg][hf].off + f_fk[kind][pc][lg][hf], e, tb);
+        f_fk[kind][pc][lg][hf] += tb[2];
+        if (kind != PD_BB) {
+          const int ht = pd_tile_class(0, kind, pr, xf, hm);
+          put(l_th[pc][lg][ht].off + f_th[pc][lg][ht], e, tb);
+          f_th[pc][lg][ht] += tb[2];
+        }
+        if (kind == PD_EP) {
+          const int hd = pd_tile_class(1, kind, pr, xf, hm);
+          put(l_td[lg][hd].off + f_td[lg][hd], e, tb);
+          f_td[lg][hd] += tb[2];
+        }
       }
     }
   }
-  c->set_hcap = hcap;
+  // Launch plan: one launch per class only where the class fills the chip
+  // (>= kMergeTiles tiles); smaller lists join the next non-empty class's
+  // launch, which runs them with its larger stride.  A launch lasts at least
+  // one tile's time, so a small set split over several classes would pay that
+  // once per class and pass.
+  auto merge = [](TileList* l) {
+    int start = -1, n = 0, hc = 1;
+    for (int h = 0; h < PD_TILE_CLASSES; ++h) {
+      if (!l[h].n) continue;
+      if (start < 0) start = l[h].off;
+      n += l[h].n;
+      hc = std::max(hc, l[h].hcap);
+      l[h].n = 0;
+      bool later = false;
+      for (int g = h + 1; g < PD_TILE_CLASSES; ++g) later = later || l[g].n > 0;
+      if (n >= kMergeTiles || !later) {
+        l[h].off = start;
+        l[h].n = n;
+        l[h].hcap = hc;
+        start = -1;
+        n = 0;
+        hc = 1;
+      }
+    }
+  };
+  for (int lg = 1; lg <= 3; ++lg) {
+    for (int pc = 0; pc < 2; ++pc) {
+      merge(l_th[pc][lg]);
+      for (int k = 0; k < 3; ++k) merge(l_fk[k][pc][lg]);
+    }
+    merge(l_td[lg]);
+  }
+  std::copy(&l_th[0][0][0], &l_th[0][0][0] + 2 * 4 * PD_TILE_CLASSES, &c->tl_theta[0][0][0]);
+  std::copy(&l_td[0][0], &l_td[0][0] + 4 * PD_TILE_CLASSES, &c->tl_td[0][0]);
+  std::copy(&l_fk[0][0][0][0], &l_fk[0][0][0][0] + 3 * 2 * 4 * PD_TILE_CLASSES, &c->tl_force[0][0][0][0]);
   c->set_tiled = tiled;
   c->set_xf = xf ? 1 : 0;
   // pair-contact items: (slot, contact index) and per-class workgroup offsets
@@ -1659,21 +1752,26 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
   auto step_kernels = [&](bool last, bool logged) {
     if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part); });
     Tile* dt = c->d_tset;
-    const int hc = c->set_hcap, xf = c->set_xf;
+    const int xf = c->set_xf;
     if (c->set_has_theta)
       timed(c, PD_PROF_THETA, [&] {
         if (c->n_theta_u) pd_launch_theta(s, prec, d + c->off_theta_u, c->n_theta_u, T);
-        for (int lg = 1; lg <= 3; ++lg) {
-          if (c->nt_theta[0][lg]) pd_launch_tile_theta(s, lg, xf, dt + c->ot_theta[0][lg], c->nt_theta[0][lg], T, hc);
-          if (c->nt_theta[1][lg])
-            pd_launch_tile_theta32(s, lg, dt + c->ot_theta[1][lg], c->nt_theta[1][lg], T, hc);
-        }
+        for (int lg = 1; lg <= 3; ++lg)
+          for (int h = 0; h < PD_TILE_CLASSES; ++h) {
+            const Ctx::TileList& l0 = c->tl_theta[0][lg][h];
+            const Ctx::TileList& l1 = c->tl_theta[1][lg][h];
+            if (l0.n) pd_launch_tile_theta(s, lg, xf, dt + l0.off, l0.n, T, l0.hcap);
+            if (l1.n) pd_launch_tile_theta32(s, lg, dt + l1.off, l1.n, T, l1.hcap);
+          }
       });
     if (c->set_has_ep)
       timed(c, PD_PROF_TD, [&] {
         if (c->n_td_u) pd_launch_td(s, prec, d + c->off_td_u, c->n_td_u, T);
         for (int lg = 1; lg <= 3; ++lg)
-          if (c->nt_td[lg]) pd_launch_tile_td(s, lg, xf, dt + c->ot_td[lg], c->nt_td[lg], T, hc);
+          for (int h = 0; h < PD_TILE_CLASSES; ++h) {
+            const Ctx::TileList& l = c->tl_td[lg][h];
+            if (l.n) pd_launch_tile_td(s, lg, xf, dt + l.off, l.n, T, l.hcap);
+          }
       });
     double* rows = logged ? c->d_obs_rows + (size_t)row * nenv * 8 : nullptr;
     if (logged) pd_launch_observables(s, 0, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
@@ -1683,12 +1781,13 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
         timed(c, PD_PROF_FORCE_BB + k, [&] {
           if (c->n_force_u[k])
             pd_launch_force(s, k, prec, d + c->off_force_u[k], c->n_force_u[k], T, fl, c->set_gpow);
-          for (int lg = 1; lg <= 3; ++lg) {
-            if (c->nt_force[k][0][lg])
-              pd_launch_tile_force(s, k, lg, xf, dt + c->ot_force[k][0][lg], c->nt_force[k][0][lg], T, fl, hc);
-            if (c->nt_force[k][1][lg])
-              pd_launch_tile_force32(s, k, lg, dt + c->ot_force[k][1][lg], c->nt_force[k][1][lg], T, fl, hc);
-          }
+          for (int lg = 1; lg <= 3; ++lg)
+            for (int h = 0; h < PD_TILE_CLASSES; ++h) {
+              const Ctx::TileList& l0 = c->tl_force[k][0][lg][h];
+              const Ctx::TileList& l1 = c->tl_force[k][1][lg][h];
+              if (l0.n) pd_launch_tile_force(s, k, lg, xf, dt + l0.off, l0.n, T, fl, l0.hcap);
+              if (l1.n) pd_launch_tile_force32(s, k, lg, dt + l1.off, l1.n, T, fl, l1.hcap);
+            }
         });
     if (logged) pd_launch_observables(s, 1, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
   };

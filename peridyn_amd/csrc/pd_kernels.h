@@ -88,6 +88,12 @@ void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvD
 // envs is exactly an FP32 value (compact records); hcap = largest halo of the
 // launched tiles
 int pd_tile_record_bytes(int pass, int kind, int prec, int xf);
+// LDS class of a pass's tiles whose largest halo is hmax: 0..3 for the
+// compiled strides that let 6, 5, 4, 3 workgroups share a CU (for the pass's
+// record size), 4 for the run-time capacity.  Envs are launched per class, so
+// that one env with a large halo does not lower the occupancy of the others.
+constexpr int PD_TILE_CLASSES = 5;
+int pd_tile_class(int pass, int kind, int prec, int xf, int hmax);
 void pd_tile_allow_lds(size_t bytes);
 void pd_launch_tile_theta(hipStream_t s, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs, int hcap);
 void pd_launch_tile_td(hipStream_t s, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs, int hcap);

@@ -13,6 +13,8 @@
 // so the results are bit-identical to theirs for the same lanes per particle.
 // Tiled envs hold single-type blocks only (the host checks), so type pairs
 // reduce to index 0 and no neighbour type is read.
+#include <type_traits>
+
 #include "pd_fp32c.h"
 
 namespace {
@@ -901,14 +903,37 @@ __global__ __launch_bounds__(256) void k_pk_sync(const Chunk* __restrict__ chunk
   }
 }
 
-// LDS stride of the FP64 record fields: the smaller of the compile-time
-// strides kStrideA / kStrideB that the set's largest halo fits (the field
-// offsets then fold into the LDS instructions' immediate offsets), else the
-// halo capacity at run time.  608 covers 4x4x4-brick halos at delta = 3 dx, 896
-// at 4 dx; neither costs tiles per CU against the run-time capacity.
-constexpr int kStrideA = 608, kStrideB = 896;
+// LDS stride of the record fields (records per field): compile-time strides
+// (the field offsets then fold into the LDS instructions' immediate offsets)
+// sized so that 6, 5, 4 or 3 workgroups fit a CU's 160 KiB for the kernel's own
+// record size, else the halo capacity at run time.  A launch takes the smallest
+// stride its largest halo fits.  The byte budgets are multiples of 1280 (the
+// coarsest LDS allocation granule, 160 KiB / 128) whose multiples by 6, 5, 4, 3
+// stay within 160 KiB also when rounded up to 512-byte granules.
+constexpr int kLdsClassBytes[4] = {26880, 32000, 40960, 53760};
 
-inline int tile_stride(int hcap) { return hcap <= kStrideA ? kStrideA : hcap <= kStrideB ? kStrideB : 0; }
+template <int REC, class F>
+void with_stride(int hcap, F&& f) {
+  constexpr int s0 = kLdsClassBytes[0] / REC, s1 = kLdsClassBytes[1] / REC;
+  constexpr int s2 = kLdsClassBytes[2] / REC, s3 = kLdsClassBytes[3] / REC;
+  if (hcap <= s0) f(std::integral_constant<int, s0>{});
+  else if (hcap <= s1) f(std::integral_constant<int, s1>{});
+  else if (hcap <= s2) f(std::integral_constant<int, s2>{});
+  else if (hcap <= s3) f(std::integral_constant<int, s3>{});
+  else f(std::integral_constant<int, 0>{});
+}
+template <int REC, class F>
+void each_stride(F&& f) {
+  f(std::integral_constant<int, kLdsClassBytes[0] / REC>{});
+  f(std::integral_constant<int, kLdsClassBytes[1] / REC>{});
+  f(std::integral_constant<int, kLdsClassBytes[2] / REC>{});
+  f(std::integral_constant<int, kLdsClassBytes[3] / REC>{});
+  f(std::integral_constant<int, 0>{});
+}
+
+constexpr int rec32_bytes(int pass, int kind) {
+  return (int)(3 * sizeof(double) + ((pass == TP_FORCE && kind == PD_OSB) ? 6 : 4) * sizeof(float));
+}
 
 template <int LG, bool XF, int S>
 void launch_theta_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
@@ -928,49 +953,39 @@ void launch_force_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int f
 
 template <int LG, bool XF>
 void launch_theta_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const int st = tile_stride(hcap);
-  if (st == kStrideA) launch_theta_s<LG, XF, kStrideA>(s, tiles, n, envs, hcap);
-  else if (st == kStrideB) launch_theta_s<LG, XF, kStrideB>(s, tiles, n, envs, hcap);
-  else launch_theta_s<LG, XF, 0>(s, tiles, n, envs, hcap);
+  with_stride<tile_record_bytes<TP_THETA, 0, XF>()>(
+      hcap, [&](auto S) { launch_theta_s<LG, XF, decltype(S)::value>(s, tiles, n, envs, hcap); });
 }
 template <int LG, bool XF>
 void launch_td_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const int st = tile_stride(hcap);
-  if (st == kStrideA) launch_td_s<LG, XF, kStrideA>(s, tiles, n, envs, hcap);
-  else if (st == kStrideB) launch_td_s<LG, XF, kStrideB>(s, tiles, n, envs, hcap);
-  else launch_td_s<LG, XF, 0>(s, tiles, n, envs, hcap);
+  with_stride<tile_record_bytes<TP_TD, 0, XF>()>(
+      hcap, [&](auto S) { launch_td_s<LG, XF, decltype(S)::value>(s, tiles, n, envs, hcap); });
 }
 template <int KIND, int LG, bool XF>
 void launch_force_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  const int st = tile_stride(hcap);
-  if (st == kStrideA) launch_force_s<KIND, LG, XF, kStrideA>(s, tiles, n, envs, flags, hcap);
-  else if (st == kStrideB) launch_force_s<KIND, LG, XF, kStrideB>(s, tiles, n, envs, flags, hcap);
-  else launch_force_s<KIND, LG, XF, 0>(s, tiles, n, envs, flags, hcap);
+  with_stride<tile_record_bytes<TP_FORCE, KIND, XF>()>(
+      hcap, [&](auto S) { launch_force_s<KIND, LG, XF, decltype(S)::value>(s, tiles, n, envs, flags, hcap); });
 }
 
 template <int LG, int S>
 void launch_theta32_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const size_t lds = (3 * sizeof(double) + 4 * sizeof(float)) * (size_t)(S > 0 ? S : hcap);
+  const size_t lds = (size_t)rec32_bytes(TP_THETA, PD_OSB) * (size_t)(S > 0 ? S : hcap);
   hipLaunchKernelGGL((k_tile_theta32<LG, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
 }
 template <int LG>
 void launch_theta32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  const int st = tile_stride(hcap);
-  if (st == kStrideA) launch_theta32_s<LG, kStrideA>(s, tiles, n, envs, hcap);
-  else if (st == kStrideB) launch_theta32_s<LG, kStrideB>(s, tiles, n, envs, hcap);
-  else launch_theta32_s<LG, 0>(s, tiles, n, envs, hcap);
+  with_stride<rec32_bytes(TP_THETA, PD_OSB)>(
+      hcap, [&](auto S) { launch_theta32_s<LG, decltype(S)::value>(s, tiles, n, envs, hcap); });
 }
 template <int KIND, int LG, int S>
 void launch_force32_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  const size_t lds = (3 * sizeof(double) + (KIND == PD_OSB ? 6 : 4) * sizeof(float)) * (size_t)(S > 0 ? S : hcap);
+  const size_t lds = (size_t)rec32_bytes(TP_FORCE, KIND) * (size_t)(S > 0 ? S : hcap);
   hipLaunchKernelGGL((k_tile_force32<KIND, LG, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
 }
 template <int KIND, int LG>
 void launch_force32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  const int st = tile_stride(hcap);
-  if (st == kStrideA) launch_force32_s<KIND, LG, kStrideA>(s, tiles, n, envs, flags, hcap);
-  else if (st == kStrideB) launch_force32_s<KIND, LG, kStrideB>(s, tiles, n, envs, flags, hcap);
-  else launch_force32_s<KIND, LG, 0>(s, tiles, n, envs, flags, hcap);
+  with_stride<rec32_bytes(TP_FORCE, KIND)>(
+      hcap, [&](auto S) { launch_force32_s<KIND, LG, decltype(S)::value>(s, tiles, n, envs, flags, hcap); });
 }
 
 // dynamic LDS beyond the default 64 KiB per workgroup must be enabled per kernel
@@ -978,31 +993,30 @@ template <class K>
 void allow_lds(K* k, size_t bytes) {
   (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
-template <int LG, int S>
+template <int LG>
 void allow_lds_fp32c(size_t bytes) {
-  allow_lds(k_tile_theta32<LG, S>, bytes);
-  allow_lds(k_tile_force32<PD_BB, LG, S>, bytes);
-  allow_lds(k_tile_force32<PD_OSB, LG, S>, bytes);
-}
-template <int LG, bool XF, int S>
-void allow_lds_fp64_s(size_t bytes) {
-  allow_lds(k_tile_theta<LG, XF, S>, bytes);
-  allow_lds(k_tile_td<LG, XF, S>, bytes);
-  allow_lds(k_tile_force<PD_BB, LG, XF, S>, bytes);
-  allow_lds(k_tile_force<PD_OSB, LG, XF, S>, bytes);
-  allow_lds(k_tile_force<PD_EP, LG, XF, S>, bytes);
+  each_stride<rec32_bytes(TP_THETA, PD_OSB)>([&](auto S) { allow_lds(k_tile_theta32<LG, decltype(S)::value>, bytes); });
+  each_stride<rec32_bytes(TP_FORCE, PD_BB)>(
+      [&](auto S) { allow_lds(k_tile_force32<PD_BB, LG, decltype(S)::value>, bytes); });
+  each_stride<rec32_bytes(TP_FORCE, PD_OSB)>(
+      [&](auto S) { allow_lds(k_tile_force32<PD_OSB, LG, decltype(S)::value>, bytes); });
 }
 template <int LG, bool XF>
 void allow_lds_fp64(size_t bytes) {
-  allow_lds_fp64_s<LG, XF, kStrideA>(bytes);
-  allow_lds_fp64_s<LG, XF, kStrideB>(bytes);
-  allow_lds_fp64_s<LG, XF, 0>(bytes);
+  each_stride<tile_record_bytes<TP_THETA, 0, XF>()>(
+      [&](auto S) { allow_lds(k_tile_theta<LG, XF, decltype(S)::value>, bytes); });
+  each_stride<tile_record_bytes<TP_TD, 0, XF>()>(
+      [&](auto S) { allow_lds(k_tile_td<LG, XF, decltype(S)::value>, bytes); });
+  each_stride<tile_record_bytes<TP_FORCE, PD_BB, XF>()>(
+      [&](auto S) { allow_lds(k_tile_force<PD_BB, LG, XF, decltype(S)::value>, bytes); });
+  each_stride<tile_record_bytes<TP_FORCE, PD_OSB, XF>()>(
+      [&](auto S) { allow_lds(k_tile_force<PD_OSB, LG, XF, decltype(S)::value>, bytes); });
+  each_stride<tile_record_bytes<TP_FORCE, PD_EP, XF>()>(
+      [&](auto S) { allow_lds(k_tile_force<PD_EP, LG, XF, decltype(S)::value>, bytes); });
 }
 template <int LG>
 void allow_lds_lg(size_t bytes) {
-  allow_lds_fp32c<LG, kStrideA>(bytes);
-  allow_lds_fp32c<LG, kStrideB>(bytes);
-  allow_lds_fp32c<LG, 0>(bytes);
+  allow_lds_fp32c<LG>(bytes);
   allow_lds_fp64<LG, false>(bytes);
   allow_lds_fp64<LG, true>(bytes);
 }
@@ -1016,11 +1030,17 @@ int fp64_record_bytes(int kind) {
 }  // namespace
 
 int pd_tile_record_bytes(int pass, int kind, int prec, int xf) {
-  if (prec == PD_PREC_FP32C && kind != PD_EP && pass != 1)
-    return (int)(3 * sizeof(double) + ((pass == 2 && kind == PD_OSB) ? 6 : 4) * sizeof(float));
+  if (prec == PD_PREC_FP32C && kind != PD_EP && pass != TP_TD) return rec32_bytes(pass, kind);
   if (pass == 0) return xf ? fp64_record_bytes<TP_THETA, true>(kind) : fp64_record_bytes<TP_THETA, false>(kind);
   if (pass == 1) return xf ? fp64_record_bytes<TP_TD, true>(kind) : fp64_record_bytes<TP_TD, false>(kind);
   return xf ? fp64_record_bytes<TP_FORCE, true>(kind) : fp64_record_bytes<TP_FORCE, false>(kind);
+}
+
+int pd_tile_class(int pass, int kind, int prec, int xf, int hmax) {
+  const int rec = pd_tile_record_bytes(pass, kind, prec, xf);
+  for (int c = 0; c < 4; ++c)
+    if (hmax <= kLdsClassBytes[c] / rec) return c;
+  return 4;
 }
 
 void pd_tile_allow_lds(size_t bytes) {
