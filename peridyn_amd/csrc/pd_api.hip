@@ -206,6 +206,9 @@ struct Env {
   std::vector<Chunk> chunks;
   Chunk* d_chunks = nullptr;
   int32_t* d_self = nullptr;  // this env's slot, for single-env bookkeeping launches
+  // tiled envs with contacts: {slot, 0}, the one-env list of pd_launch_cpart_build
+  int32_t* d_cself = nullptr;
+  int32_t cself_host[2] = {0, 0};
   EnvDev dev{};
   std::vector<ContactDev> cdev;
   DevMem* mem = nullptr;  // the context's device memory
@@ -250,6 +253,12 @@ struct Ctx {
   size_t cstarts_cap = 0;
   int n_citems = 0;
   int cgrid[PD_CB_NCLS] = {};
+  // tiled envs of the set with contacts: slots and first cpart-build workgroups
+  int32_t* d_cfin = nullptr;
+  size_t cfin_cap = 0;
+  int32_t* d_cfin_starts = nullptr;
+  size_t cfin_starts_cap = 0;
+  int n_cfin = 0, cfin_blocks = 0;
   PdCopy* d_copies = nullptr;  // build_set's copy descriptors
   size_t copies_cap = 0;
   Chunk* d_set = nullptr;
@@ -1215,6 +1224,18 @@ void commit_env(Ctx* c, Env& e) {
   }
   e.d_self = dalloc<int32_t>(e, 1);
   PD_HIP(hipMemcpyAsync(e.d_self, &e.slot, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (e.tiled && !e.contacts.empty()) {
+    // particles with contact terms, numbered at every list rebuild
+    D.cflag = dalloc<uint8_t>(e, N);
+    D.cpart = dalloc<int32_t>(e, N);
+    D.ccount = dalloc<int32_t>(e, 1);
+    PD_HIP(hipMemsetAsync(D.cflag, 0, (size_t)N, s));
+    PD_HIP(hipMemsetAsync(D.ccount, 0, sizeof(int32_t), s));
+    e.cself_host[0] = e.slot;
+    e.cself_host[1] = 0;
+    e.d_cself = dalloc<int32_t>(e, 2);
+    PD_HIP(hipMemcpyAsync(e.d_cself, e.cself_host, 2 * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
   e.d_chunks = dalloc<Chunk>(e, e.chunks.size());
   PD_HIP(hipMemcpyAsync(e.d_chunks, e.chunks.data(), sizeof(Chunk) * e.chunks.size(), hipMemcpyHostToDevice, s));
   e.tile_blocks.clear();
@@ -1259,6 +1280,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   static_assert(sizeof(Chunk) % 4 == 0 && sizeof(Tile) % 4 == 0, "copied as 4-byte words");
   std::vector<PdCopy> copies;  // per-env chunk and tile lists into the set's arrays
   std::vector<int2> citems;    // pair-contact items
+  std::vector<int32_t> cfin, cfin_starts;  // tiled envs with contacts
   std::vector<int32_t> cst;    // their per-class workgroup offsets
   std::vector<Chunk> th, td, fk[3], thu, tdu, fku[3];
   std::vector<int32_t> slots;
@@ -1467,6 +1489,27 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
                             c->stream));
     }
   }
+  {
+    std::vector<int32_t>& cf = cfin;
+    std::vector<int32_t>& cfs = cfin_starts;
+    int blocks = 0;
+    for (int64_t id : ids) {
+      const Env& e = get_env(c, id);
+      if (!e.dev.cflag) continue;
+      cf.push_back(e.slot);
+      cfs.push_back(blocks);
+      blocks += (int)((e.N + 255) / 256);
+    }
+    c->n_cfin = (int)cf.size();
+    c->cfin_blocks = blocks;
+    if (!cf.empty()) {
+      ensure_cap(c->d_cfin, c->cfin_cap, cf.size(), c->mem);
+      ensure_cap(c->d_cfin_starts, c->cfin_starts_cap, cfs.size(), c->mem);
+      PD_HIP(hipMemcpyAsync(c->d_cfin, cf.data(), sizeof(int32_t) * cf.size(), hipMemcpyHostToDevice, c->stream));
+      PD_HIP(hipMemcpyAsync(c->d_cfin_starts, cfs.data(), sizeof(int32_t) * cfs.size(), hipMemcpyHostToDevice,
+                            c->stream));
+    }
+  }
   ensure_cap(c->d_slots, c->slots_cap, slots.size(), c->mem);
   PD_HIP(hipMemcpyAsync(c->d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, c->stream));
   ensure_cap(c->d_obs_ranges, c->obs_ranges_cap, ranges.size(), c->mem);
@@ -1545,10 +1588,17 @@ void refresh_latch(Ctx* c, const std::vector<int64_t>& ids, int nl) {
   }
 }
 
+// numbers the particles with contact terms of a tiled env after its lists changed
+void cpart_update(Ctx* c, Env& e) {
+  if (e.dev.cflag)
+    pd_launch_cpart_build(c->stream, c->d_envs, e.d_cself, e.d_cself + 1, 1, (int)((e.N + 255) / 256));
+}
+
 void contact_update(Ctx* c, Env& e) {
   for (size_t k = 0; k < e.cdev.size(); ++k)
     if (e.cdev[k].self_contact) pd_launch_self_contact_update(c->stream, c->d_envs, e.slot, e.cdev[k], (int)k);
     else pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, e.cdev[k], (int)k);
+  cpart_update(c, e);
   e.lists_current = true;
 }
 
@@ -1562,6 +1612,7 @@ void contact_update_set(Ctx* c, const std::vector<int64_t>& ids) {
       if (e.cdev[k].self_contact) pd_launch_self_contact_update(c->stream, c->d_envs, e.slot, e.cdev[k], (int)k);
     if (!e.contacts.empty()) e.lists_current = true;
   }
+  pd_launch_cpart_build(c->stream, c->d_envs, c->d_cfin, c->d_cfin_starts, c->n_cfin, c->cfin_blocks);
 }
 
 // one force evaluation (update_acc!) for a chunk subset of one env
@@ -1789,6 +1840,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
               if (l1.n) pd_launch_tile_force32(s, k, lg, dt + l1.off, l1.n, T, fl, l1.hcap);
             }
         });
+    if (c->n_cfin) timed(c, PD_PROF_CONTACT, [&] { pd_launch_contact_finish(s, T, c->d_cfin, c->n_cfin, fl); });
     if (logged) pd_launch_observables(s, 1, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
   };
   auto book = [&](bool last) {
@@ -2376,6 +2428,7 @@ int pd_contact_update(void* ctx, int64_t env, int64_t cidx, const double* y) {
     if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
     if (cd.self_contact) pd_launch_self_contact_update(c->stream, c->d_envs, e.slot, cd, (int)cidx);
     else pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, cd, (int)cidx);
+    cpart_update(c, e);
     PD_HIP(hipGetLastError());
     PD_HIP(hipStreamSynchronize(c->stream));
     // the lists of the other models were built from earlier positions

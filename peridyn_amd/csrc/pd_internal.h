@@ -128,6 +128,13 @@ struct EnvDev {
   BlockDev* blocks;
   BcDev* bcs;
   ContactDev* contacts;
+  // tiled envs with contacts: the particles with contact terms under the
+  // current lists (cflag), compacted into cpart[0 .. *ccount) in no particular
+  // order; their tiled force kernels store the bond sum in f and leave the
+  // contact terms and the Verlet update to k_contact_finish
+  uint8_t* cflag;
+  int32_t* cpart;
+  int32_t* ccount;
 };
 
 struct Chunk {

@@ -55,6 +55,14 @@ void pd_launch_qs_update(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs
 // (one double4 per chunk): the kernels of pd_env_observables_noalloc
 // one contact model's terms added into acc (n = env particles, internal order)
 void pd_launch_contact_model(hipStream_t s, const EnvDev* envs, int slot, int cidx, bool gpow, double4* acc, int n);
+// tiled envs with contacts (slots[0 .. n)): number the particles with contact
+// terms (cflag, cpart, ccount) after a list rebuild; starts[k] = first of the
+// nblocks workgroups of env k (prefix over ceil(N / 256))
+void pd_launch_cpart_build(hipStream_t s, EnvDev* envs, const int32_t* slots, const int32_t* starts, int n,
+                           int nblocks);
+// contact terms + Verlet update of the particles the tiled force kernels left
+// unfinished (after all force passes of the step)
+void pd_launch_contact_finish(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int flags);
 void pd_launch_observables_env(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs, int slot, double4* part,
                                double* row);
 void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, EnvDev* envs, double4* part,

@@ -952,6 +952,167 @@ void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int
   else launch_force_kind<PD_EP>(s, prec, chunks, n, envs, flags, gpow);
 }
 
+// ---- contact finish of tiled envs ------------------------------------------
+// The listed envs (slots[0 .. n)) number their particles with contact terms:
+// counts zeroed, then every particle flagged and the flagged ones appended to
+// cpart (any order: each particle is finished on its own).  starts[k] is the
+// first workgroup of env k (a prefix over ceil(N / 256)).
+__global__ void k_cpart_zero(EnvDev* __restrict__ envs, const int32_t* __restrict__ slots, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) *envs[slots[i]].ccount = 0;
+}
+
+__global__ __launch_bounds__(256) void k_cpart_mark(EnvDev* __restrict__ envs, const int32_t* __restrict__ slots,
+                                                    const int32_t* __restrict__ starts, int n) {
+  int lo = 0, hi = n;  // starts[lo] <= blockIdx.x < starts[lo + 1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (starts[mid] <= (int)blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  EnvDev& E = envs[slots[lo]];
+  const int p = ((int)blockIdx.x - starts[lo]) * 256 + (int)threadIdx.x;
+  if (p >= E.N) return;
+  const bool has = pd_has_contact_terms(E, p);
+  E.cflag[p] = has ? 1 : 0;
+  if (has) E.cpart[atomicAdd(E.ccount, 1)] = p;
+}
+
+// The contact terms (model order, list order) and the Verlet update of the
+// particles the tiled force kernels left unfinished: f holds their bond sums.
+// One wavefront per particle (kCfinBlocks workgroups per env stride over its
+// cpart list): the lanes form the terms of 64 list entries at once, then every
+// lane adds them in list order (readlane), so the sums are add_contacts' own.
+constexpr int kCfinBlocks = 64;
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Adds (sign +1) or subtracts (-1) the terms t(k) = term(y_q(k), vol_q(k)) for
+// k = 0 .. n-1 in order; before entry `at` (0 <= at <= n) it runs ins() once.
+template <class Q, class T, class I>
+__device__ __forceinline__ void wave_list_terms(const double4* __restrict__ x, const double4* __restrict__ y, int n,
+                                                int at, Q&& q, T&& term, I&& ins, bool sub, double& a1,
+                                                double& a2, double& a3) {
+  const int lane = pd_lane();
+  for (int k0 = 0; k0 < n; k0 += 64) {
+    const int k = k0 + lane;
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (k < n) {
+      const int qq = q(k);
+      const double4 yq = y[qq];
+      term(yq, x[qq].w, t1, t2, t3);
+    }
+    const int m = n - k0 < 64 ? n - k0 : 64;
+    for (int u = 0; u < m; ++u) {
+      if (k0 + u == at) ins();
+      const double u1 = readlane_d(t1, u), u2 = readlane_d(t2, u), u3 = readlane_d(t3, u);
+      if (sub) {
+        a1 -= u1; a2 -= u2; a3 -= u3;
+      } else {
+        a1 += u1; a2 += u2; a3 += u3;
+      }
+    }
+  }
+  if (at >= n) ins();
+}
+
+__global__ __launch_bounds__(256) void k_contact_finish(EnvDev* __restrict__ envs, const int32_t* __restrict__ slots,
+                                                        int flags) {
+  EnvDev& E = envs[slots[blockIdx.x / kCfinBlocks]];
+  const int cnt = *E.ccount;
+  const double4* __restrict__ x = E.x;
+  const double4* __restrict__ y = E.y[E.cur];
+  constexpr int kWaves = kCfinBlocks * PD_WAVES_PER_BLOCK;
+  auto none = [] {};
+  for (int i = (int)(blockIdx.x % kCfinBlocks) * PD_WAVES_PER_BLOCK + pd_wave_id(); i < cnt; i += kWaves) {
+    const int p = E.cpart[i];  // the same for every lane of the wavefront
+    const double4 yp = y[p];
+    const double4 f = E.f[p];
+    double a1 = f.x, a2 = f.y, a3 = f.z;
+    for (int cm = 0; cm < E.ncontacts; ++cm) {
+      const ContactDev& C = E.contacts[cm];
+      const int i1 = C.inv1[p];
+      if (C.self_contact) {
+        // add_self_contact: reverse entries of members i' < i1, the own term, the rest
+        if (i1 < 0) continue;
+        const double vol_p = x[p].w;
+        const double den_p = (vol_p * E.blocks[E.pblock[p]].density[E.ptype[p]]) / vol_p;
+        const int off = C.rev_off[i1], n = C.rev_cnt[i1];
+        const double b = vol_p / den_p;
+        int split = 0;  // entries of members < i1 (the list is in member order)
+        for (int k0 = 0; k0 < n; k0 += 64) {
+          const int k = k0 + pd_lane();
+          const bool lt = k < n && C.rev[off + k] < i1;
+          split += __builtin_popcountll(__builtin_amdgcn_ballot_w64(lt));
+        }
+        auto own = [&] {
+          if (!C.own_cnt[i1]) return;
+          const int q = C.idx1[C.own[(size_t)i1 * C.max_neighs]];
+          const double4 yq = y[q];
+          double F1, F2, F3;
+          contact_fn<false>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+          const double a = x[q].w / den_p;
+          a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
+        };
+        wave_list_terms(
+            x, y, n, split, [&](int k) { return C.idx1[C.rev[off + k]]; },
+            [&](const double4& yq, double, double& t1, double& t2, double& t3) {
+              double F1, F2, F3;
+              contact_fn<false>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+              t1 = F1 * b; t2 = F2 * b; t3 = F3 * b;
+            },
+            own, true, a1, a2, a3);
+        continue;
+      }
+      // add_contact_model: own list, then the transposed list
+      const int i2 = C.inv2[p];
+      if (i1 >= 0) {
+        const double inv_den_p = E.inv_den[p];
+        wave_list_terms(
+            x, y, C.own_cnt[i1], 1 << 30, [&](int k) { return C.idx2[C.own[(size_t)i1 * C.max_neighs + k]]; },
+            [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
+              double F1, F2, F3;
+              contact_fn<false>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+              const double a = vq * inv_den_p;
+              t1 = F1 * a; t2 = F2 * a; t3 = F3 * a;
+            },
+            none, false, a1, a2, a3);
+      }
+      if (i2 >= 0) {
+        const double inv_den_p = E.inv_den[p];
+        const int off = C.rev_off[i2];
+        wave_list_terms(
+            x, y, C.rev_cnt[i2], 1 << 30, [&](int k) { return C.idx1[C.rev[off + k]]; },
+            [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
+              double F1, F2, F3;
+              contact_fn<false>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+              const double bq = vq * inv_den_p;
+              t1 = F1 * bq; t2 = F2 * bq; t3 = F3 * bq;
+            },
+            none, true, a1, a2, a3);
+      }
+    }
+    if (pd_lane() == 0) finish_particle(E, p, x[p], yp, a1, a2, a3, flags);  // tiled envs: integer exponents
+  }
+}
+
+void pd_launch_cpart_build(hipStream_t s, EnvDev* envs, const int32_t* slots, const int32_t* starts, int n,
+                           int nblocks) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cpart_zero, dim3((n + 255) / 256), dim3(256), 0, s, envs, slots, n);
+  if (nblocks > 0) hipLaunchKernelGGL(k_cpart_mark, dim3(nblocks), dim3(256), 0, s, envs, slots, starts, n);
+}
+
+void pd_launch_contact_finish(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int flags) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_contact_finish, dim3(n * kCfinBlocks), dim3(256), 0, s, envs, slots, flags);
+}
+
 void pd_launch_contact_model(hipStream_t s, const EnvDev* envs, int slot, int cidx, bool gpow, double4* acc, int n) {
   if (n <= 0) return;
   const dim3 g((n + 255) / 256), b(256);

@@ -193,6 +193,19 @@ __device__ __forceinline__ void pk_record_breaks(const uint64_t* __restrict__ pk
   }
 }
 
+// Epilogue of the tiled force kernels for particle p with bond sum a: a
+// particle with contact terms stores a in f and is finished by
+// k_contact_finish (contact terms in list order, then the Verlet update, as
+// the untiled kernels do in their epilogue); the others are finished here.
+__device__ __forceinline__ void finish_tiled(EnvDev& E, int p, const double4& xp, const double4& yp, double a1,
+                                             double a2, double a3, int flags) {
+  if (E.ncontacts > 0 && E.cflag[p]) {
+    E.f[p] = make_double4(a1, a2, a3, 0.0);
+    return;
+  }
+  finish_particle_v(E, p, xp, yp, a1, a2, a3, flags, gld4(E.v, p), E.bcid[p]);
+}
+
 // bond geometry against the staged neighbour j (expressions of the untiled kernels)
 template <class R>
 __device__ __forceinline__ void tile_geom(const R& rc, int j, const double4& xp, const double4& yp, double& X1,
@@ -452,9 +465,6 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
   TileWave w;
   const bool has = tile_wave<LG>(T, w);
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
-  // the epilogue's inputs are loaded after the bond loop: the loop needs their
-  // registers more (waves per SIMD as above)
-  EpiPre ep;
   if (deformed) stage_halo<XF, NX>(E, T, sh, hs);
   __syncthreads();
   if (!has) return;
@@ -548,10 +558,7 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
     a3 = group_sum(a3, C);
   }
   if (!w.active || w.g != 0) return;
-  epi_load_a(E, p, ep);
-  epi_load_b(E, ep);
-  add_contacts_pre<false>(E, p, yp, y, ep, a1, a2, a3);
-  finish_particle_v(E, p, xp, yp, a1, a2, a3, flags, ep.v, ep.bcid);
+  finish_tiled(E, p, xp, yp, a1, a2, a3, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -684,7 +691,6 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
   TileWave w;
   const bool has = tile_wave<LG>(T, w);
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
-  EpiPre ep;  // the epilogue's inputs, loaded after the bond loop (its registers)
   if (deformed) stage_halo32<KIND == PD_OSB ? ST_OSB : ST_BB>(E, B, T, sh, hcap);
   __syncthreads();
   if (!has) return;
@@ -742,10 +748,7 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
     a3 = comp_total(s3, c3, C);
   }
   if (!w.active || w.g != 0) return;
-  epi_load_a(E, p, ep);
-  epi_load_b(E, ep);
-  add_contacts_pre<false>(E, p, yp, y, ep, a1, a2, a3);
-  finish_particle_v(E, p, gld4(E.x, p), yp, a1, a2, a3, flags, ep.v, ep.bcid);
+  finish_tiled(E, p, gld4(E.x, p), yp, a1, a2, a3, flags);
 }
 
 // ---------------------------------------------------------------------------

@@ -143,78 +143,21 @@ __device__ __forceinline__ void add_contacts(const EnvDev& E, int p, const doubl
   for (int cm = 0; cm < E.ncontacts; ++cm) add_contact_model<GPOW>(E, E.contacts[cm], p, yp, y, a1, a2, a3);
 }
 
-// Per-particle inputs of the tiled force kernels' epilogue (the contact terms
-// of model 0 and the Verlet update), loaded in two batches whose loads are in
-// flight together: epi_load_a, then epi_load_b (the loads that depend on the
-// first ones).  The kernels load them after the bond loop: the loop's
-// registers are worth more than hiding this latency behind it.
-struct EpiPre {
-  int i1, i2, cnt1, off2, cnt2, bcid;
-  double inv_den;
-  double4 v;
-};
-
-__device__ __forceinline__ void epi_load_a(const EnvDev& E, int p, EpiPre& e) {
-  e.i1 = -1;
-  e.i2 = -1;
-  if (E.ncontacts > 0 && !E.contacts[0].self_contact) {
-    e.i1 = E.contacts[0].inv1[p];
-    e.i2 = E.contacts[0].inv2[p];
-  }
-  e.inv_den = E.inv_den[p];
-  e.v = E.v[p];
-  e.bcid = E.bcid[p];
-}
-
-__device__ __forceinline__ void epi_load_b(const EnvDev& E, EpiPre& e) {
-  e.cnt1 = 0;
-  e.cnt2 = 0;
-  e.off2 = 0;
-  if (e.i1 >= 0) e.cnt1 = E.contacts[0].own_cnt[e.i1];
-  if (e.i2 >= 0) {
-    e.off2 = E.contacts[0].rev_off[e.i2];
-    e.cnt2 = E.contacts[0].rev_cnt[e.i2];
-  }
-}
-
-// add_contacts with model 0's lookups taken from e (same terms, same order)
-template <bool GPOW>
-__device__ __forceinline__ void add_contacts_pre(const EnvDev& E, int p, const double4& yp, const double4* y,
-                                                 const EpiPre& e, double& a1, double& a2, double& a3) {
-  const double4* __restrict__ x = E.x;
+// True when particle p receives contact terms under the current lists (any
+// model): the terms add_contacts would add are then not all absent.
+__device__ __forceinline__ bool pd_has_contact_terms(const EnvDev& E, int p) {
   for (int cm = 0; cm < E.ncontacts; ++cm) {
     const ContactDev& C = E.contacts[cm];
+    const int i1 = C.inv1[p];
     if (C.self_contact) {
-      add_self_contact<GPOW>(E, C, p, yp, y, a1, a2, a3);
+      if (i1 >= 0 && (C.own_cnt[i1] > 0 || C.rev_cnt[i1] > 0)) return true;
       continue;
     }
-    const int i1 = cm == 0 ? e.i1 : C.inv1[p];
-    if (i1 >= 0) {
-      const double inv_den_p = e.inv_den;
-      const int cnt = cm == 0 ? e.cnt1 : C.own_cnt[i1];
-      for (int k = 0; k < cnt; ++k) {
-        const int q = C.idx2[C.own[(size_t)i1 * C.max_neighs + k]];
-        const double4 yq = y[q];
-        double F1, F2, F3;
-        contact_fn<GPOW>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
-        const double a = x[q].w * inv_den_p;
-        a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
-      }
-    }
-    const int i2 = cm == 0 ? e.i2 : C.inv2[p];
-    if (i2 >= 0) {
-      const double inv_den_p = e.inv_den;
-      const int off = cm == 0 ? e.off2 : C.rev_off[i2], cnt = cm == 0 ? e.cnt2 : C.rev_cnt[i2];
-      for (int k = 0; k < cnt; ++k) {
-        const int q = C.idx1[C.rev[off + k]];
-        const double4 yq = y[q];
-        double F1, F2, F3;
-        contact_fn<GPOW>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
-        const double b = x[q].w * inv_den_p;
-        a1 -= F1 * b; a2 -= F2 * b; a3 -= F3 * b;
-      }
-    }
+    if (i1 >= 0 && C.own_cnt[i1] > 0) return true;
+    const int i2 = C.inv2[p];
+    if (i2 >= 0 && C.rev_cnt[i2] > 0) return true;
   }
+  return false;
 }
 
 // Stores f and runs the FP64 velocity-Verlet part of the fused step:
