@@ -9,7 +9,10 @@ call, one launch per kernel class per step, steady steps replayed from a
 captured HIP graph.  Every specimen's grips move from step 1, so every block's
 deformed latch is set on step 1 and every timed step runs the bond passes.  A
 "step" is one velocity_verlet_step! over all 256 specimens; all state is
-resident in HBM before timing starts.
+resident in HBM before timing starts.  By default the timed run is each
+workload's own run length (500 steps here, after 5 untimed ones); every bond of
+the families counts as one bond-update per step, broken or not, as the
+reference's bond loop visits every bond and tests its mask.
 
 Reported on one JSON line:
   value          bond-updates/s of the timed (graph-replayed) steps
@@ -19,8 +22,8 @@ Reported on one JSON line:
   cpu_baseline   the oracle (C++ restatement, threaded loops) on a sample of
                  the same specimens on the host cores
   fp64_one_at_a_time   the same specimens run one at a time in plain FP64
-  served         the served stream (continuous batching) against the same
-                 requests served one at a time in plain FP64
+  served         the served stream (FP32C continuous batching by default)
+                 against the same requests served one at a time in plain FP64
 
 Other workloads (--workload):
   plates_ep    two EP plates 50x50x20 (100k particles), nonlinear contact
@@ -45,6 +48,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 METRIC = "bond-updates/s & % HBM roofline; served sims/s + p99 latency vs plain FP64"
+# timed steps per workload by default: the run length of its configuration
+RUN_STEPS = {"batched": 500, "plates_ep": 500, "disk_impact": 2000, "tensile_bb": 200}
 PROF_CLASSES = ("prologue", "latch", "theta", "td", "force_bb", "force_osb", "force_ep", "contact", "book")
 
 # Algorithmic bytes per launch, (per bond, per particle), of each kernel class
@@ -509,10 +514,12 @@ def bench_served(args):
     print(json.dumps(out))
 
 
-def main():
+def make_parser() -> argparse.ArgumentParser:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the workload's own run length: batched 500, plates_ep 500, "
+                         "disk_impact 2000, tensile_bb 200)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "plates_ep", "disk_impact", "tensile_bb", "served"])
@@ -538,17 +545,23 @@ def main():
                          "requests queue and the GPU stays saturated)")
     ap.add_argument("--served-lanes", type=int, default=4,
                     help="served: lanes per particle of the served envs (0: each env's own choice)")
-    ap.add_argument("--served-precision", default="fp64", choices=["fp64", "fp32c"],
+    ap.add_argument("--served-precision", default="fp32c", choices=["fp64", "fp32c"],
                     help="served: form of the batched simulations")
     ap.add_argument("--no-seq", action="store_true", help="served: skip the one-at-a-time FP64 baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (x3) whose HBM counter readings "
                          "scale FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = make_parser().parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = RUN_STEPS.get(args.workload, 100)
     if args.served_requests is None:
         args.served_requests = 64 if args.workload in ("batched", "served") else 0
     if args.workload == "batched":

@@ -50,3 +50,14 @@ def test_batched_and_served_configs():
     reqs = W.served_requests(count=c4["count"], seed=c4["seed"], rate=c4["arrival_rate_per_s"])
     assert all(r["steps"] % c4["neigh_update_freq"] == 0 and 200 <= r["steps"] <= 2000 for r in reqs)
     assert all(2.5 <= r["horizon"] <= 4.0 and 1000 <= r["n"] <= 200000 for r in reqs)
+
+
+def test_served_config_matches_bench_defaults():
+    """fixtures/configs/4_served.json describes the stream bench.py serves by default."""
+    import bench
+    c4 = load("4_served")
+    d = bench.make_parser().parse_args([])
+    assert d.served_rate == c4["arrival_rate_per_s"]
+    assert d.served_precision == c4["precision"]
+    assert d.served_lanes == c4["lanes_per_particle"]
+    assert c4["count"] == 64 and c4["chunk_steps"] == 50
