@@ -187,6 +187,7 @@ struct Env {
   std::vector<HostBc> bcs;
   bool committed = false;
   bool maybe_undeformed = true;
+  std::vector<uint8_t> block_latched;  // blocks read back as latched (latches never reset)
   // the contact lists were built from the current positions (nothing has moved
   // them since), so an update_neighs! now would rebuild the same sorted lists
   bool lists_current = false;
@@ -259,6 +260,7 @@ struct Ctx {
   int32_t* d_cfin_starts = nullptr;
   size_t cfin_starts_cap = 0;
   int n_cfin = 0, cfin_blocks = 0;
+  int cfin_grid = 0;  // workgroups of the contact-finish pass
   PdCopy* d_copies = nullptr;  // build_set's copy descriptors
   size_t copies_cap = 0;
   Chunk* d_set = nullptr;
@@ -278,11 +280,16 @@ struct Ctx {
   size_t obs_part_cap = 0;
   double* d_obs_rows = nullptr;
   size_t obs_rows_cap = 0;
-  int2* d_latch = nullptr;
+  int2* d_latch = nullptr;  // (env slot, block) items of the latch test
   size_t latch_cap = 0;
-  double4* d_latch_part = nullptr;  // per-slice partial sums of the latch test
+  int2* d_latch_units = nullptr;  // (item, slice): one workgroup each
+  size_t latch_units_cap = 0;
+  int32_t* d_latch_ustart = nullptr;  // first unit of each item
+  size_t latch_ustart_cap = 0;
+  int latch_nunits = 0;
+  std::vector<std::pair<int64_t, int>> latch_items;  // (env id, block) of the items
+  double4* d_latch_part = nullptr;  // per-unit partial sums of the latch test
   size_t latch_part_cap = 0;
-  int latch_nparts = 1;
   int32_t* d_latch_flags = nullptr;  // per latch item, read back after a run
   size_t latch_flags_cap = 0;
   // per-kernel-class HIP-event timing on the context stream (pd_ctx_profile)
@@ -1502,6 +1509,10 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
     }
     c->n_cfin = (int)cf.size();
     c->cfin_blocks = blocks;
+    // one wavefront per 16 particles of these envs (a bound on those with
+    // contact terms), at most the 4 per SIMD (of the chip's 1024) that the
+    // kernel's registers let reside at once
+    c->cfin_grid = std::min(1024, std::max(1, blocks * 4));
     if (!cf.empty()) {
       ensure_cap(c->d_cfin, c->cfin_cap, cf.size(), c->mem);
       ensure_cap(c->d_cfin_starts, c->cfin_starts_cap, cfs.size(), c->mem);
@@ -1543,47 +1554,62 @@ int precision_of(Ctx* c, const std::vector<int64_t>& ids) {
   return prec < 0 ? PD_PREC_FP64 : prec;
 }
 
-// deformed-latch work list for envs that may still hold undeformed blocks
+// deformed-latch work list: the blocks not yet read back as latched of the
+// envs that may still hold undeformed blocks, and their slices
 int prepare_latch(Ctx* c, const std::vector<int64_t>& ids) {
-  std::vector<int2> items;
-  int64_t max_n = 1;
+  std::vector<int2> items, units;
+  std::vector<int32_t> ustart;
+  c->latch_items.clear();
   for (int64_t id : ids) {
     Env& e = get_env(c, id);
     if (!e.maybe_undeformed) continue;
+    e.block_latched.resize(e.blocks.size(), 0);
     for (size_t b = 0; b < e.blocks.size(); ++b) {
+      if (e.block_latched[b]) continue;
+      const int item = (int)items.size();
       items.push_back(make_int2(e.slot, (int)b));
-      max_n = std::max<int64_t>(max_n, e.blocks[b].d.n);
+      c->latch_items.push_back({id, (int)b});
+      ustart.push_back((int32_t)units.size());
+      const int ns = pd_latch_slices(e.blocks[b].d.n);
+      for (int q = 0; q < ns; ++q) units.push_back(make_int2(item, q));
     }
   }
+  c->latch_nunits = (int)units.size();
   if (items.empty()) return 0;
-  c->latch_nparts = pd_latch_parts(max_n);
   ensure_cap(c->d_latch, c->latch_cap, items.size(), c->mem);
   ensure_cap(c->d_latch_flags, c->latch_flags_cap, items.size(), c->mem);
-  ensure_cap(c->d_latch_part, c->latch_part_cap, items.size() * (size_t)c->latch_nparts, c->mem);
+  ensure_cap(c->d_latch_units, c->latch_units_cap, units.size(), c->mem);
+  ensure_cap(c->d_latch_ustart, c->latch_ustart_cap, ustart.size(), c->mem);
+  ensure_cap(c->d_latch_part, c->latch_part_cap, units.size(), c->mem);
   PD_HIP(hipMemcpyAsync(c->d_latch, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, c->stream));
+  PD_HIP(hipMemcpyAsync(c->d_latch_units, units.data(), sizeof(int2) * units.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  PD_HIP(hipMemcpyAsync(c->d_latch_ustart, ustart.data(), sizeof(int32_t) * ustart.size(), hipMemcpyHostToDevice,
+                        c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
   return (int)items.size();
 }
 
-// After a run with nl latch items (prepare_latch, same ids): the latches of
+void launch_latch(Ctx* c, hipStream_t s, int nl) {
+  pd_launch_latch(s, c->d_envs, c->d_latch, nl, c->d_latch_units, c->latch_nunits, c->d_latch_ustart,
+                  c->d_latch_part);
+}
+
+// After a run with nl latch items (the last prepare_latch): the latches of
 // every listed block in one readback.
 void refresh_latch(Ctx* c, const std::vector<int64_t>& ids, int nl) {
+  (void)ids;
   if (nl <= 0) return;
   std::vector<int32_t> fl(nl);
   pd_launch_latch_flags(c->stream, c->d_envs, c->d_latch, nl, c->d_latch_flags);
   PD_HIP(hipMemcpyAsync(fl.data(), c->d_latch_flags, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
-  int k = 0;
-  for (int64_t id : ids) {
-    Env& e = get_env(c, id);
-    if (!e.maybe_undeformed) continue;
+  for (int k = 0; k < nl; ++k)
+    if (fl[k]) get_env(c, c->latch_items[k].first).block_latched[c->latch_items[k].second] = 1;
+  for (int k = 0; k < nl; ++k) {
+    Env& e = get_env(c, c->latch_items[k].first);
     bool any = false;
-    // every block's flag is consumed, also after an unlatched one was found:
-    // the flags of the envs that follow start right after this env's blocks
-    for (size_t b = 0; b < e.blocks.size(); ++b) {
-      const bool latched = fl[k++] != 0;
-      any = any || !latched;
-    }
+    for (uint8_t l : e.block_latched) any = any || !l;
     e.maybe_undeformed = any;
   }
 }
@@ -1637,7 +1663,7 @@ void force_pass(Ctx* c, Env& e, const std::vector<Chunk>& sub, int flags_extra) 
     PD_HIP(hipMemcpyAsync(d, cat.data(), sizeof(Chunk) * cat.size(), hipMemcpyHostToDevice, c->stream));
     std::vector<int64_t> one{e.id};
     nl = prepare_latch(c, one);
-    if (nl) pd_launch_latch(c->stream, c->d_envs, c->d_latch, nl, c->latch_nparts, c->d_latch_part);
+    if (nl) launch_latch(c, c->stream, nl);
     pd_launch_theta(c->stream, e.precision, d, (int)th.size(), c->d_envs);
     pd_launch_td(c->stream, e.precision, d + o_td, (int)td.size(), c->d_envs);
     for (int k = 0; k < 3; ++k)
@@ -1763,6 +1789,17 @@ struct StepGraph {
 constexpr int64_t kLatchPoll = 16;
 constexpr int64_t kGraphMinSteps = 32;  // shorter pd_run calls launch every step eagerly
 
+// Contact-finish shape: one lane per particle once the set's particles with
+// contact terms number at least this (PERIDYN_CFIN_LANE_MIN overrides), one
+// wavefront per particle below it.
+int cfin_lane_min() {
+  static const int v = [] {
+    const char* e = std::getenv("PERIDYN_CFIN_LANE_MIN");
+    return e ? std::atoi(e) : 32768;
+  }();
+  return v;
+}
+
 // PERIDYN_GRAPH=0 launches every step eagerly (for comparison)
 bool graphs_enabled() {
   const char* v = std::getenv("PERIDYN_GRAPH");
@@ -1801,7 +1838,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
   // force pass (update_misc! runs inside update_acc!) and the force and damage
   // sums after it
   auto step_kernels = [&](bool last, bool logged) {
-    if (nl) timed(c, PD_PROF_LATCH, [&] { pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part); });
+    if (nl) timed(c, PD_PROF_LATCH, [&] { launch_latch(c, s, nl); });
     Tile* dt = c->d_tset;
     const int xf = c->set_xf;
     if (c->set_has_theta)
@@ -1840,7 +1877,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
               if (l1.n) pd_launch_tile_force32(s, k, lg, dt + l1.off, l1.n, T, fl, l1.hcap);
             }
         });
-    if (c->n_cfin) timed(c, PD_PROF_CONTACT, [&] { pd_launch_contact_finish(s, T, c->d_cfin, c->n_cfin, fl); });
+    if (c->n_cfin) timed(c, PD_PROF_CONTACT, [&] { pd_launch_contact_finish(s, T, c->d_cfin, c->n_cfin, c->cfin_grid, cfin_lane_min(), fl); });
     if (logged) pd_launch_observables(s, 1, d, c->n_all, T, c->d_obs_part, c->d_obs_ranges, nenv, rows);
   };
   auto book = [&](bool last) {
@@ -2013,6 +2050,10 @@ int pd_ctx_destroy(void* ctx) {
   if (c->d_latch) (void)hipFree(c->d_latch);
   if (c->d_latch_part) (void)hipFree(c->d_latch_part);
   if (c->d_latch_flags) (void)hipFree(c->d_latch_flags);
+  if (c->d_latch_units) (void)hipFree(c->d_latch_units);
+  if (c->d_latch_ustart) (void)hipFree(c->d_latch_ustart);
+  if (c->d_cfin) (void)hipFree(c->d_cfin);
+  if (c->d_cfin_starts) (void)hipFree(c->d_cfin_starts);
   if (c->d_obs_ranges) (void)hipFree(c->d_obs_ranges);
   if (c->d_obs_part) (void)hipFree(c->d_obs_part);
   if (c->d_obs_rows) (void)hipFree(c->d_obs_rows);
@@ -2637,6 +2678,7 @@ int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes) {
     }
     e.time_step = hdr[8];
     e.maybe_undeformed = any_undeformed;
+    e.block_latched.assign(e.blocks.size(), 0);  // the loaded latches are read back when next tested
     e.lists_current = false;
     upload_env_table(c, e);
     if (e.tiled) pd_launch_pk_sync(s, e.d_chunks, (int)e.chunks.size(), c->d_envs);
@@ -2680,7 +2722,7 @@ int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64
       unsigned long long hm[2] = {0, 0};
       for (int64_t i = 1; i <= max_iter; ++i) {
         // update_acc! (solvers.jl:320-336): the env's own force passes, no Verlet
-        if (nl) pd_launch_latch(s, T, c->d_latch, nl, c->latch_nparts, c->d_latch_part);
+        if (nl) launch_latch(c, s, nl);
         if (c->set_has_theta) pd_launch_theta(s, prec, d + c->off_theta, c->n_theta, T);
         if (c->set_has_ep) pd_launch_td(s, prec, d + c->off_td, c->n_td, T);
         for (int k = 0; k < 3; ++k)

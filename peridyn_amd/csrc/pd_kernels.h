@@ -17,9 +17,12 @@ void pd_launch_force(hipStream_t s, int kind, int prec, const Chunk* chunks, int
                      bool gpow);
 void pd_launch_prologue(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_book(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int flags);
-// deformed latch over n (env slot, block) items; part holds n * nparts double4
-int pd_latch_parts(int64_t max_n);
-void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n, int nparts, double4* part);
+// deformed latch over n (env slot, block) items: the slices of item i (a block
+// of n_b particles has pd_latch_slices(n_b)) are the units (i, 0 .. ns - 1),
+// listed from ustart[i] on in the nunits units; part holds one double4 per unit
+int pd_latch_slices(int64_t n);
+void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n, const int2* units, int nunits,
+                     const int32_t* ustart, double4* part);
 
 // family build (pd_family.hip); all arrays device, env-local particle ids
 struct FamilyGrid {
@@ -61,8 +64,11 @@ void pd_launch_contact_model(hipStream_t s, const EnvDev* envs, int slot, int ci
 void pd_launch_cpart_build(hipStream_t s, EnvDev* envs, const int32_t* slots, const int32_t* starts, int n,
                            int nblocks);
 // contact terms + Verlet update of the particles the tiled force kernels left
-// unfinished (after all force passes of the step)
-void pd_launch_contact_finish(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int flags);
+// unfinished (after all force passes of the step), envs slots[0 .. n), nblocks
+// workgroups of 4 wavefronts striding over all their particles: one lane per
+// particle when they number lane_min or more, else one wavefront per particle
+void pd_launch_contact_finish(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int nblocks, int lane_min,
+                              int flags);
 void pd_launch_observables_env(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs, int slot, double4* part,
                                double* row);
 void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, EnvDev* envs, double4* part,

@@ -630,9 +630,10 @@ __global__ void k_book(EnvDev* __restrict__ envs, const int32_t* __restrict__ sl
 
 // Deformed latch (general_material.jl:243-255): isapprox(y .- y[:,1] .+ x[:,1], x)
 // with rtol = sqrt(eps(Float64)).  Two kernels: k_latch_part sums |a - x|^2,
-// |a|^2 and |x|^2 over `nparts` slices of each (env, block) entry, one workgroup
-// per slice; k_latch_fin adds the slices of an entry in a fixed order and sets
-// the latch.  Latched blocks skip both.
+// |a|^2 and |x|^2 over the slices of each (env, block) entry, one workgroup per
+// slice (the launch's units list every entry's slices); k_latch_fin adds the
+// slices of an entry in a fixed order and sets the latch.  Latched blocks skip
+// both.
 __device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
   for (int o = 32; o > 0; o >>= 1) {
     a += __shfl_down(a, o);
@@ -651,31 +652,31 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
 }
 
 // A block's slices follow from its own size only (kLatchSlice particles each,
-// at most kLatchMaxSlices), so its sums, and the step it latches on, do not
-// depend on the other entries of the launch: a specimen latches on the same
-// step alone and batched.  The grid's y extent (nparts) is the largest entry's
-// slice count; other entries skip the slices beyond their own.
-constexpr int kLatchSlice = 256 * 16;
-constexpr int kLatchMaxSlices = 1024;
+// two per thread, at most kLatchMaxSlices), so its sums, and the step it
+// latches on, do not depend on the other entries of the launch: a specimen
+// latches on the same step alone and batched.  Small slices keep every
+// workgroup's dependent loads short: a block that never deforms (a rigid
+// indenter moved by its BCs) is tested at every step.
+constexpr int kLatchSlice = 256 * 2;
+constexpr int kLatchMaxSlices = 16384;
 __host__ __device__ __forceinline__ int latch_slices(int64_t n) {
   const int64_t s = (n + kLatchSlice - 1) / kLatchSlice;
   return (int)(s < 1 ? 1 : (s > kLatchMaxSlices ? kLatchMaxSlices : s));
 }
 
 __global__ __launch_bounds__(256) void k_latch_part(EnvDev* __restrict__ envs, const int2* __restrict__ items,
-                                                    int nparts, double4* __restrict__ part) {
-  const int item = blockIdx.x;  // entries on x (no 65535 limit), slices on y
-  const int2 it = items[item];
+                                                    const int2* __restrict__ units, double4* __restrict__ part) {
+  const int2 un = units[blockIdx.x];  // (entry, slice)
+  const int2 it = items[un.x];
   const EnvDev& E = envs[it.x];
   const BlockDev& B = E.blocks[it.y];
   if (B.deformed) return;  // uniform per workgroup
   const int ns = latch_slices(B.n);
-  if ((int)blockIdx.y >= ns) return;  // uniform per workgroup
   const double4* __restrict__ y = E.y[E.cur];
   const double4* __restrict__ x = E.x;
   const double4 y1 = y[B.first], x1 = x[B.first];
   const int per = (B.n + ns - 1) / ns;
-  const int lo = blockIdx.y * per;
+  const int lo = un.y * per;
   const int hi = min(B.n, lo + per);
   double sd = 0.0, sa = 0.0, sx = 0.0;
   for (int i = lo + (int)threadIdx.x; i < hi; i += 256) {
@@ -691,19 +692,20 @@ __global__ __launch_bounds__(256) void k_latch_part(EnvDev* __restrict__ envs, c
     }
   }
   block_sum3(sd, sa, sx);
-  if (threadIdx.x == 0) part[(size_t)item * nparts + blockIdx.y] = make_double4(sd, sa, sx, 0.0);
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double4(sd, sa, sx, 0.0);
 }
 
 __global__ __launch_bounds__(64) void k_latch_fin(EnvDev* __restrict__ envs, const int2* __restrict__ items,
-                                                  int nparts, const double4* __restrict__ part) {
+                                                  const int32_t* __restrict__ ustart, const double4* __restrict__ part) {
   const int2 it = items[blockIdx.x];
   EnvDev& E = envs[it.x];
   BlockDev& B = E.blocks[it.y];
   if (B.deformed) return;
   const int ns = latch_slices(B.n);
+  const double4* __restrict__ mine = part + ustart[blockIdx.x];
   double D = 0.0, A = 0.0, X = 0.0;
   for (int q = threadIdx.x; q < ns; q += 64) {
-    const double4 v = part[(size_t)blockIdx.x * nparts + q];
+    const double4 v = mine[q];
     D += v.x; A += v.y; X += v.z;
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -972,18 +974,32 @@ __global__ __launch_bounds__(256) void k_cpart_mark(EnvDev* __restrict__ envs, c
   }
   EnvDev& E = envs[slots[lo]];
   const int p = ((int)blockIdx.x - starts[lo]) * 256 + (int)threadIdx.x;
-  if (p >= E.N) return;
-  const bool has = pd_has_contact_terms(E, p);
-  E.cflag[p] = has ? 1 : 0;
-  if (has) E.cpart[atomicAdd(E.ccount, 1)] = p;
+  const bool in = p < E.N;
+  const bool has = in && pd_has_contact_terms(E, p);
+  if (in) E.cflag[p] = has ? 1 : 0;
+  // one counter atomic per wavefront; its flagged lanes take consecutive entries
+  const uint64_t m = __builtin_amdgcn_ballot_w64(has);
+  if (!m) return;
+  const int leader = __builtin_ctzll(m);
+  int base = 0;
+  if (pd_lane() == leader) base = atomicAdd(E.ccount, __builtin_popcountll(m));
+  base = __shfl(base, leader);
+  if (has) E.cpart[base + lanes_below(m)] = p;
 }
 
 // The contact terms (model order, list order) and the Verlet update of the
 // particles the tiled force kernels left unfinished: f holds their bond sums.
-// One wavefront per particle (kCfinBlocks workgroups per env stride over its
-// cpart list): the lanes form the terms of 64 list entries at once, then every
-// lane adds them in list order (readlane), so the sums are add_contacts' own.
-constexpr int kCfinBlocks = 64;
+// The grid's wavefronts stride over the concatenation of the listed envs'
+// cpart lists (a wavefront-wide prefix of their counts maps an entry to its
+// env), in one of two shapes, both summing every particle's terms in list
+// order, so both give add_contacts' own sums:
+//  * few particles (fewer than lane_min in all): one wavefront per particle;
+//    the lanes form the terms of 64 list entries at once, then every lane adds
+//    them in list order (readlane).  Every particle's gathers are in flight
+//    together, which is what a short list needs;
+//  * many (deep contact: tens of thousands of particles, tens of entries
+//    each): one lane per particle summing its own lists, 64 particles per
+//    wavefront, about a ninth of the instructions per list entry.
 
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __builtin_bit_cast(long long, v);
@@ -1021,83 +1037,231 @@ __device__ __forceinline__ void wave_list_terms(const double4* __restrict__ x, c
   if (at >= n) ins();
 }
 
-__global__ __launch_bounds__(256) void k_contact_finish(EnvDev* __restrict__ envs, const int32_t* __restrict__ slots,
-                                                        int flags) {
-  EnvDev& E = envs[slots[blockIdx.x / kCfinBlocks]];
-  const int cnt = *E.ccount;
+// Contact terms and Verlet update of particle p of E (every lane of the
+// wavefront calls it with the same p).
+__device__ __forceinline__ void contact_finish_particle(EnvDev& E, int p, int flags) {
   const double4* __restrict__ x = E.x;
   const double4* __restrict__ y = E.y[E.cur];
-  constexpr int kWaves = kCfinBlocks * PD_WAVES_PER_BLOCK;
   auto none = [] {};
-  for (int i = (int)(blockIdx.x % kCfinBlocks) * PD_WAVES_PER_BLOCK + pd_wave_id(); i < cnt; i += kWaves) {
-    const int p = E.cpart[i];  // the same for every lane of the wavefront
-    const double4 yp = y[p];
-    const double4 f = E.f[p];
-    double a1 = f.x, a2 = f.y, a3 = f.z;
-    for (int cm = 0; cm < E.ncontacts; ++cm) {
-      const ContactDev& C = E.contacts[cm];
-      const int i1 = C.inv1[p];
-      if (C.self_contact) {
-        // add_self_contact: reverse entries of members i' < i1, the own term, the rest
-        if (i1 < 0) continue;
-        const double vol_p = x[p].w;
-        const double den_p = (vol_p * E.blocks[E.pblock[p]].density[E.ptype[p]]) / vol_p;
-        const int off = C.rev_off[i1], n = C.rev_cnt[i1];
-        const double b = vol_p / den_p;
-        int split = 0;  // entries of members < i1 (the list is in member order)
-        for (int k0 = 0; k0 < n; k0 += 64) {
-          const int k = k0 + pd_lane();
-          const bool lt = k < n && C.rev[off + k] < i1;
-          split += __builtin_popcountll(__builtin_amdgcn_ballot_w64(lt));
-        }
-        auto own = [&] {
-          if (!C.own_cnt[i1]) return;
-          const int q = C.idx1[C.own[(size_t)i1 * C.max_neighs]];
-          const double4 yq = y[q];
-          double F1, F2, F3;
-          contact_fn<false>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
-          const double a = x[q].w / den_p;
-          a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
-        };
-        wave_list_terms(
-            x, y, n, split, [&](int k) { return C.idx1[C.rev[off + k]]; },
-            [&](const double4& yq, double, double& t1, double& t2, double& t3) {
-              double F1, F2, F3;
-              contact_fn<false>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
-              t1 = F1 * b; t2 = F2 * b; t3 = F3 * b;
-            },
-            own, true, a1, a2, a3);
-        continue;
+  const double4 yp = y[p];
+  const double4 f = E.f[p];
+  double a1 = f.x, a2 = f.y, a3 = f.z;
+  for (int cm = 0; cm < E.ncontacts; ++cm) {
+    const ContactDev& C = E.contacts[cm];
+    const int i1 = C.inv1[p];
+    if (C.self_contact) {
+      // add_self_contact: reverse entries of members i' < i1, the own term, the rest
+      if (i1 < 0) continue;
+      const double vol_p = x[p].w;
+      const double den_p = (vol_p * E.blocks[E.pblock[p]].density[E.ptype[p]]) / vol_p;
+      const int off = C.rev_off[i1], n = C.rev_cnt[i1];
+      const double b = vol_p / den_p;
+      int split = 0;  // entries of members < i1 (the list is in member order)
+      for (int k0 = 0; k0 < n; k0 += 64) {
+        const int k = k0 + pd_lane();
+        const bool lt = k < n && C.rev[off + k] < i1;
+        split += __builtin_popcountll(__builtin_amdgcn_ballot_w64(lt));
       }
-      // add_contact_model: own list, then the transposed list
-      const int i2 = C.inv2[p];
-      if (i1 >= 0) {
-        const double inv_den_p = E.inv_den[p];
-        wave_list_terms(
-            x, y, C.own_cnt[i1], 1 << 30, [&](int k) { return C.idx2[C.own[(size_t)i1 * C.max_neighs + k]]; },
-            [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
-              double F1, F2, F3;
-              contact_fn<false>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
-              const double a = vq * inv_den_p;
-              t1 = F1 * a; t2 = F2 * a; t3 = F3 * a;
-            },
-            none, false, a1, a2, a3);
-      }
-      if (i2 >= 0) {
-        const double inv_den_p = E.inv_den[p];
-        const int off = C.rev_off[i2];
-        wave_list_terms(
-            x, y, C.rev_cnt[i2], 1 << 30, [&](int k) { return C.idx1[C.rev[off + k]]; },
-            [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
-              double F1, F2, F3;
-              contact_fn<false>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
-              const double bq = vq * inv_den_p;
-              t1 = F1 * bq; t2 = F2 * bq; t3 = F3 * bq;
-            },
-            none, true, a1, a2, a3);
+      auto own = [&] {
+        if (!C.own_cnt[i1]) return;
+        const int q = C.idx1[C.own[(size_t)i1 * C.max_neighs]];
+        const double4 yq = y[q];
+        double F1, F2, F3;
+        contact_fn<false>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+        const double a = x[q].w / den_p;
+        a1 += F1 * a; a2 += F2 * a; a3 += F3 * a;
+      };
+      wave_list_terms(
+          x, y, n, split, [&](int k) { return C.idx1[C.rev[off + k]]; },
+          [&](const double4& yq, double, double& t1, double& t2, double& t3) {
+            double F1, F2, F3;
+            contact_fn<false>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+            t1 = F1 * b; t2 = F2 * b; t3 = F3 * b;
+          },
+          own, true, a1, a2, a3);
+      continue;
+    }
+    // add_contact_model: own list, then the transposed list
+    const int i2 = C.inv2[p];
+    if (i1 >= 0) {
+      const double inv_den_p = E.inv_den[p];
+      wave_list_terms(
+          x, y, C.own_cnt[i1], 1 << 30, [&](int k) { return C.idx2[C.own[(size_t)i1 * C.max_neighs + k]]; },
+          [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
+            double F1, F2, F3;
+            contact_fn<false>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+            const double a = vq * inv_den_p;
+            t1 = F1 * a; t2 = F2 * a; t3 = F3 * a;
+          },
+          none, false, a1, a2, a3);
+    }
+    if (i2 >= 0) {
+      const double inv_den_p = E.inv_den[p];
+      const int off = C.rev_off[i2];
+      wave_list_terms(
+          x, y, C.rev_cnt[i2], 1 << 30, [&](int k) { return C.idx1[C.rev[off + k]]; },
+          [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
+            double F1, F2, F3;
+            contact_fn<false>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+            const double bq = vq * inv_den_p;
+            t1 = F1 * bq; t2 = F2 * bq; t3 = F3 * bq;
+          },
+          none, true, a1, a2, a3);
+    }
+  }
+  if (pd_lane() == 0) finish_particle(E, p, x[p], yp, a1, a2, a3, flags);  // tiled envs: integer exponents
+}
+
+// One pair-model list of particle p on its own lane: the terms of entries
+// k = 0 .. n-1 (partner q(k)), added (sub: subtracted) in list order, with the
+// gathers of 4 entries in flight at once.  term(yq, vol_q) -> (t1, t2, t3).
+template <class Q, class T>
+__device__ __forceinline__ void lane_list_terms(const double4* __restrict__ x, const double4* __restrict__ y, int n,
+                                                int pself, Q&& q, T&& term, bool sub, double& a1, double& a2,
+                                                double& a3) {
+  for (int k = 0; k < n; k += 4) {
+    int qs[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) qs[u] = (k + u < n) ? q(k + u) : pself;
+    double4 yq[4];
+    double vq[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      yq[u] = y[qs[u]];
+      vq[u] = x[qs[u]].w;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k + u >= n) break;
+      double t1, t2, t3;
+      term(yq[u], vq[u], t1, t2, t3);
+      if (sub) {
+        a1 -= t1; a2 -= t2; a3 -= t3;
+      } else {
+        a1 += t1; a2 += t2; a3 += t3;
       }
     }
-    if (pd_lane() == 0) finish_particle(E, p, x[p], yp, a1, a2, a3, flags);  // tiled envs: integer exponents
+  }
+}
+
+// add_contacts (integer exponents) for particle p on its own lane, the pair
+// models' lists through lane_list_terms: the same terms in the same order.
+__device__ __forceinline__ void lane_contact_terms(const EnvDev& E, int p, const double4& yp,
+                                                   const double4* __restrict__ y, double& a1, double& a2,
+                                                   double& a3) {
+  const double4* __restrict__ x = E.x;
+  for (int cm = 0; cm < E.ncontacts; ++cm) {
+    const ContactDev& C = E.contacts[cm];
+    if (C.self_contact) {
+      add_self_contact<false>(E, C, p, yp, y, a1, a2, a3);
+      continue;
+    }
+    const int i1 = C.inv1[p], i2 = C.inv2[p];
+    if (i1 >= 0) {
+      const double inv_den_p = E.inv_den[p];
+      const int32_t* __restrict__ own = C.own + (size_t)i1 * C.max_neighs;
+      lane_list_terms(
+          x, y, C.own_cnt[i1], p, [&](int k) { return C.idx2[own[k]]; },
+          [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
+            double F1, F2, F3;
+            contact_fn<false>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+            const double a = vq * inv_den_p;
+            t1 = F1 * a; t2 = F2 * a; t3 = F3 * a;
+          },
+          false, a1, a2, a3);
+    }
+    if (i2 >= 0) {
+      const double inv_den_p = E.inv_den[p];
+      const int32_t* __restrict__ rev = C.rev + C.rev_off[i2];
+      lane_list_terms(
+          x, y, C.rev_cnt[i2], p, [&](int k) { return C.idx1[rev[k]]; },
+          [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
+            double F1, F2, F3;
+            contact_fn<false>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+            const double bq = vq * inv_den_p;
+            t1 = F1 * bq; t2 = F2 * bq; t3 = F3 * bq;
+          },
+          true, a1, a2, a3);
+    }
+  }
+}
+
+// Total entries of the cpart lists of envs slots[0 .. n) (wave-uniform).
+__device__ __forceinline__ int contact_total(const EnvDev* __restrict__ envs, const int32_t* __restrict__ slots,
+                                             int n) {
+  int total = 0;
+  for (int k0 = 0; k0 < n; k0 += 64) {
+    const int k = k0 + pd_lane();
+    int cnt = k < n ? *envs[slots[k]].ccount : 0;
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    total += cnt;
+  }
+  return total;
+}
+
+// Entry t of the concatenated cpart lists of envs slots[0 .. n), one per lane
+// (t may differ between lanes; every lane must call it): its env index (into
+// slots) and its position in that env's list; -1 past the last entry.
+__device__ __forceinline__ int contact_entry_env(const EnvDev* __restrict__ envs, const int32_t* __restrict__ slots,
+                                                 int n, int t, int& pos) {
+  const int lane = pd_lane();
+  int found = -1;
+  pos = 0;
+  for (int k0 = 0; k0 < n; k0 += 64) {
+    const int k = k0 + lane;
+    const int cnt = k < n ? *envs[slots[k]].ccount : 0;
+    int incl = cnt;  // inclusive prefix over this group of envs
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off);
+      if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    // first lane of the group whose prefix exceeds t (64 if none)
+    int lo = 0, hi = 64;
+    for (int step = 0; step < 7; ++step) {
+      const int mid = (lo + hi) >> 1;
+      const int v = __shfl(incl, mid < 64 ? mid : 63);
+      if (lo < hi) {
+        if (v > t) hi = mid;
+        else lo = mid + 1;
+      }
+    }
+    const int excl = __shfl(incl - cnt, lo < 64 ? lo : 63);
+    if (found < 0 && t >= 0 && t < total) {
+      found = k0 + lo;
+      pos = t - excl;
+    }
+    t -= total;
+  }
+  return found;
+}
+
+__global__ __launch_bounds__(256) void k_contact_finish(EnvDev* __restrict__ envs, const int32_t* __restrict__ slots,
+                                                        int n, int lane_min, int flags) {
+  const int nwaves = (int)gridDim.x * PD_WAVES_PER_BLOCK;
+  const int w0 = (int)blockIdx.x * PD_WAVES_PER_BLOCK + pd_wave_id();
+  const int total = contact_total(envs, slots, n);
+  if (total >= lane_min) {
+    for (int t0 = w0 * 64; t0 < total; t0 += nwaves * 64) {
+      int pos;
+      const int k = contact_entry_env(envs, slots, n, t0 + pd_lane(), pos);
+      if (k < 0) continue;
+      EnvDev& E = envs[slots[k]];
+      const int p = E.cpart[pos];
+      const double4* __restrict__ y = E.y[E.cur];
+      const double4 yp = y[p];
+      const double4 f = E.f[p];
+      double a1 = f.x, a2 = f.y, a3 = f.z;
+      lane_contact_terms(E, p, yp, y, a1, a2, a3);
+      finish_particle(E, p, E.x[p], yp, a1, a2, a3, flags);  // tiled envs: integer exponents
+    }
+    return;
+  }
+  for (int t = w0; t < total; t += nwaves) {
+    int pos;
+    const int k = contact_entry_env(envs, slots, n, t, pos);
+    EnvDev& E = envs[slots[k]];
+    contact_finish_particle(E, E.cpart[pos], flags);
   }
 }
 
@@ -1108,9 +1272,10 @@ void pd_launch_cpart_build(hipStream_t s, EnvDev* envs, const int32_t* slots, co
   if (nblocks > 0) hipLaunchKernelGGL(k_cpart_mark, dim3(nblocks), dim3(256), 0, s, envs, slots, starts, n);
 }
 
-void pd_launch_contact_finish(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int flags) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_contact_finish, dim3(n * kCfinBlocks), dim3(256), 0, s, envs, slots, flags);
+void pd_launch_contact_finish(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, int nblocks, int lane_min,
+                              int flags) {
+  if (n <= 0 || nblocks <= 0) return;
+  hipLaunchKernelGGL(k_contact_finish, dim3(nblocks), dim3(256), 0, s, envs, slots, n, lane_min, flags);
 }
 
 void pd_launch_contact_model(hipStream_t s, const EnvDev* envs, int slot, int cidx, bool gpow, double4* acc, int n) {
@@ -1130,7 +1295,7 @@ void pd_launch_book(hipStream_t s, EnvDev* envs, const int32_t* slots, int n, in
   hipLaunchKernelGGL(k_book, dim3((n + 255) / 256), dim3(256), 0, s, envs, slots, n, flags);
 }
 
-int pd_latch_parts(int64_t max_n) { return latch_slices(max_n); }
+int pd_latch_slices(int64_t n) { return latch_slices(n); }
 
 // deformed latch of each (env slot, block) item, for one readback
 __global__ void k_latch_flags(const EnvDev* __restrict__ envs, const int2* __restrict__ items, int n,
@@ -1149,10 +1314,11 @@ void pd_launch_latch_flags(hipStream_t s, EnvDev* envs, const int2* items, int n
   hipLaunchKernelGGL(k_latch_flags, dim3((n + 255) / 256), dim3(256), 0, s, envs, items, n, out);
 }
 
-void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n, int nparts, double4* part) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_latch_part, dim3(n, nparts), dim3(256), 0, s, envs, items, nparts, part);
-  hipLaunchKernelGGL(k_latch_fin, dim3(n), dim3(64), 0, s, envs, items, nparts, (const double4*)part);
+void pd_launch_latch(hipStream_t s, EnvDev* envs, const int2* items, int n, const int2* units, int nunits,
+                     const int32_t* ustart, double4* part) {
+  if (n <= 0 || nunits <= 0) return;
+  hipLaunchKernelGGL(k_latch_part, dim3(nunits), dim3(256), 0, s, envs, items, units, part);
+  hipLaunchKernelGGL(k_latch_fin, dim3(n), dim3(64), 0, s, envs, items, ustart, (const double4*)part);
 }
 
 void pd_launch_observables(hipStream_t s, int mode, const Chunk* chunks, int n, EnvDev* envs, double4* part,
