@@ -478,7 +478,7 @@ def served_summary(ctx, count: int, rate: float, seq: bool = True, seed: int = 2
            "budget_bytes": budget, **st.as_dict()}
     if seq:
         one = serve.Server(serve.GpuBackend(ctx, "fp64", keep_state=False), budget, chunk=None, max_active=1,
-                           on_retire=_progress("fp64 one at a time", count))
+                           on_retire=_progress("fp64 one at a time", count), overlap=False)
         _, s1 = one.serve(reqs)
         out["fp64_one_at_a_time"] = s1.as_dict()
         out["sims_per_s_vs_fp64_one_at_a_time"] = st.sims_per_s / s1.sims_per_s

@@ -157,6 +157,21 @@ int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t row
  * after every step i with i % nuf == 0, i counted from 1 in each call (run!'s
  * default start_at = 0).  Returns when the steps have completed. */
 int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf);
+/* pd_run in two halves, for a serving loop that admits new simulations while a
+ * chunk of steps runs.  No counterpart in the reference (its run! returns when
+ * done); the stepping is pd_run's, bit for bit.  pd_run_begin enqueues the
+ * nsteps >= 1 steps of the listed envs and returns; pd_run_query sets *done to
+ * 1 once they have completed (or when nothing is in flight), else 0;
+ * pd_run_end waits for them and completes the call.  Between begin and end
+ * the only calls accepted on the context are pd_env_create, pd_env_add_block
+ * / _contact / _bc, pd_env_set_schedule, pd_env_schedule, pd_env_layout,
+ * pd_env_commit, pd_env_info, pd_env_time_step (the value before the run),
+ * pd_env_state_size, pd_ctx_mem_info and pd_env_destroy of an env that the
+ * run does not step; their device work goes to a second stream of the context
+ * and overlaps the run.  Every other call returns PD_E_STATE. */
+int pd_run_begin(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf);
+int pd_run_query(void* ctx, int32_t* done);
+int pd_run_end(void* ctx);
 /* pd_run plus the printdata columns (environment.jl:308-320) that run! logs
  * after step 1 and after every `every`-th step (solvers.jl:158-167, every =
  * average_prop_freq): per logged step and env 8 doubles, time (time_step * dt),

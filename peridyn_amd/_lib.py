@@ -26,7 +26,7 @@ EXPORTS = [
     "pd_env_set_schedule", "pd_env_schedule", "pd_env_layout", "pd_ctx_mem_info", "pd_run_logged", "pd_log_rows",
     "pd_env_observables", "pd_env_state_size", "pd_env_state_save", "pd_env_state_load",
     "pd_env_minimize", "pd_env_observables_noalloc", "pd_contact_update", "pd_contact_neighs",
-    "pd_contact_force",
+    "pd_contact_force", "pd_run_begin", "pd_run_query", "pd_run_end",
 ]
 PROF_CLASSES = {"prologue": 0, "latch": 1, "theta": 2, "td": 3, "force_bb": 4, "force_osb": 5,
                 "force_ep": 6, "contact": 7, "book": 8}
@@ -106,6 +106,9 @@ def load() -> ctypes.CDLL:
     L.pd_block_theta.argtypes = [P, I64, I64, P]
     L.pd_block_edp.argtypes = [P, I64, I64, P, I64]
     L.pd_run.argtypes = [P, P, I64, I64, I64]
+    L.pd_run_begin.argtypes = [P, P, I64, I64, I64]
+    L.pd_run_query.argtypes = [P, ctypes.POINTER(I32)]
+    L.pd_run_end.argtypes = [P]
     L.pd_env_update_acc.argtypes = [P, I64, P, P]
     L.pd_force_density.argtypes = [P, I64, I64, P, P, I64, I64]
     L.pd_env_update_neighs.argtypes = [P, I64]

@@ -382,6 +382,28 @@ def run_batch(envs: Iterable[Env], nsteps: int, nuf: int = 1):
     ctx.check(ctx.lib.pd_run(ctx.h, _ptr(ids), len(envs), int(nsteps), int(nuf)))
 
 
+def run_batch_begin(envs: Iterable[Env], nsteps: int, nuf: int = 1):
+    """Start run_batch and return once its steps are enqueued (pd_run_begin).
+    Until run_batch_end, the context accepts only env construction, commit,
+    info and the destruction of envs that the run does not step."""
+    envs = list(envs)
+    ctx = envs[0].ctx
+    ids = np.array([e.id for e in envs], dtype=np.int64)
+    ctx.check(ctx.lib.pd_run_begin(ctx.h, _ptr(ids), len(envs), int(nsteps), int(nuf)))
+
+
+def run_batch_done(ctx: Context) -> bool:
+    """Whether the steps of the run started by run_batch_begin have completed."""
+    done = ctypes.c_int32(0)
+    ctx.check(ctx.lib.pd_run_query(ctx.h, ctypes.byref(done)))
+    return bool(done.value)
+
+
+def run_batch_end(ctx: Context):
+    """Wait for the run started by run_batch_begin and complete it."""
+    ctx.check(ctx.lib.pd_run_end(ctx.h))
+
+
 def run_batch_logged(envs: Iterable[Env], nsteps: int, nuf: int = 1, every: int = 1) -> np.ndarray:
     """run_batch plus the printdata rows of every env: array (rows, nenv, 8)."""
     envs = list(envs)

@@ -219,7 +219,19 @@ struct Env {
 
 struct Ctx {
   int device = 0;
+  // the stream every call enqueues on (`stream`): the run stream, except
+  // while a run started by pd_run_begin is in flight, when the calls allowed
+  // then (env construction, commit, destroy) use the admission stream, so that
+  // their waits do not wait for the run
   hipStream_t stream = nullptr;
+  hipStream_t run_stream = nullptr;
+  hipStream_t adm_stream = nullptr;  // created with the first pd_run_begin
+  struct Inflight {
+    bool on = false;
+    std::vector<int64_t> ids;
+    int64_t nsteps = 0, nuf = 1;
+    int nl = 0;
+  } inflight;
   double4* scratch = nullptr;  // this GPU's copy of the file-scope scratch array
   DevMem mem;  // env arrays and build temporaries
   HostStage stage;  // commit staging
@@ -455,6 +467,8 @@ void upload_env_table(Ctx* c, Env& e) {
 void ensure_slot(Ctx* c, Env& e) {
   if (e.slot >= 0) return;
   if (c->free_slots.empty()) {
+    // an in-flight run reads the table: let it finish before the table moves
+    if (c->inflight.on) PD_HIP(hipStreamSynchronize(c->run_stream));
     const int32_t ncap = std::max(64, c->slot_cap * 2);
     EnvDev* nt = nullptr;
     PD_HIP(hipMalloc(&nt, sizeof(EnvDev) * ncap));
@@ -1806,11 +1820,26 @@ bool graphs_enabled() {
   return !(v && std::string(v) == "0");
 }
 
+// Host bookkeeping after the steps of a run have completed: every env's step
+// counter, buffer indices and list state, and the deformed latches read back.
+void end_run(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t nuf, int nl) {
+  for (int64_t id : ids) {
+    Env& e = get_env(c, id);
+    e.time_step += nsteps;
+    e.cur ^= (int32_t)(nsteps & 1);
+    if (e.has_ep) e.edp_cur ^= (int32_t)(nsteps & 1);
+    e.lists_current = (nsteps % nuf) == 0;  // rebuilt after the last step
+  }
+  refresh_latch(c, ids, nl);
+}
+
 // run! for the listed envs (pd_run); with every > 0 the printdata columns of
 // step 1 and of every every-th step (solvers.jl:158) go to log_host (rows x
-// nenv x 8, rows in step order).
+// nenv x 8, rows in step order).  async (pd_run_begin): every step is launched
+// eagerly, the deformed latches are read back only at the end, and the call
+// returns once the steps are enqueued; pd_run_end completes it.
 void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t nuf, int64_t every,
-              double* log_host) {
+              double* log_host, bool async = false) {
   PD_HIP(hipSetDevice(c->device));
   const int prec = precision_of(c, ids);
   build_set(c, ids);
@@ -1895,7 +1924,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
   // capturing and instantiating the two step graphs costs ~2-3 ms; measured on
   // the 256-specimen batch (4 launches per step) a 20-step call is faster
   // eager (1.48 vs 1.52 ms/step), a 100-step call faster replayed (1.38 vs 1.39)
-  const bool use_graph = graphs_enabled() && !c->prof && nsteps >= kGraphMinSteps;
+  const bool use_graph = graphs_enabled() && !c->prof && nsteps >= kGraphMinSteps && !async;
   for (int64_t st = 1; st <= nsteps; ++st) {
     const bool last = (st == nsteps);
     const bool logged = every > 0 && (st == 1 || st % every == 0);
@@ -1922,7 +1951,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
       if (logged) ++row;
       if (rebuild) timed(c, PD_PROF_CONTACT, [&] { contact_update_set(c, ids); });
       book(last);
-      if (st == 1 && nl && !last) {
+      if (st == 1 && nl && !last && !async) {
         refresh_latch(c, ids, nl);
         nl = prepare_latch(c, ids);
       }
@@ -1941,6 +1970,14 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
     }
   }
   PD_HIP(hipGetLastError());
+  if (async) {
+    c->inflight.on = true;
+    c->inflight.ids = ids;
+    c->inflight.nsteps = nsteps;
+    c->inflight.nuf = nuf;
+    c->inflight.nl = nl;
+    return;
+  }
   PD_HIP(hipStreamSynchronize(s));  // also before the step graph is destroyed
   if (nrows > 0) {
     PD_HIP(hipMemcpyAsync(log_host, c->d_obs_rows, sizeof(double) * (size_t)(nrows * nenv * 8),
@@ -1959,14 +1996,7 @@ void run_impl(Ctx* c, const std::vector<int64_t>& ids, int64_t nsteps, int64_t n
       ++r;
     }
   }
-  for (int64_t id : ids) {
-    Env& e = get_env(c, id);
-    e.time_step += nsteps;
-    e.cur ^= (int32_t)(nsteps & 1);
-    if (e.has_ep) e.edp_cur ^= (int32_t)(nsteps & 1);
-    e.lists_current = (nsteps % nuf) == 0;  // rebuilt after the last step
-  }
-  refresh_latch(c, ids, nl);
+  end_run(c, ids, nsteps, nuf, nl);
 }
 
 // ---- resident-state snapshot ----------------------------------------------
@@ -2005,6 +2035,35 @@ static int run_guard(void* ctx, const std::function<void()>& f) {
   }
 }
 
+// Entry points that read or change state an in-flight run uses take their
+// context through idle(): they fail while a pd_run_begin run is in flight.
+static Ctx* idle(void* ctx) {
+  Ctx* c = C(ctx);
+  if (c->inflight.on)
+    throw PdError(PD_E_STATE, "a run started by pd_run_begin is in flight: call pd_run_end first");
+  return c;
+}
+
+// The calls allowed during an in-flight run (env construction, commit,
+// destroy) enqueue on the admission stream meanwhile.
+struct AdmissionScope {
+  Ctx* c;
+  bool swapped = false;
+  explicit AdmissionScope(Ctx* cc) : c(cc) {
+    if (c->inflight.on) {
+      c->stream = c->adm_stream;
+      c->mem.stream = c->adm_stream;
+      swapped = true;
+    }
+  }
+  ~AdmissionScope() {
+    if (swapped) {
+      c->stream = c->run_stream;
+      c->mem.stream = c->run_stream;
+    }
+  }
+};
+
 int pd_ctx_create(int device, void** out) {
   return run_guard(nullptr, [&] {
     if (!out) throw PdError(PD_E_ARG, "null output pointer");
@@ -2016,6 +2075,7 @@ int pd_ctx_create(int device, void** out) {
     c->device = device;
     hipError_t r = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (r != hipSuccess) { delete c; PD_HIP(r); }
+    c->run_stream = c->stream;
     // the address of this GPU's copy of `scratch` (device is current)
     void* sp = nullptr;
     r = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
@@ -2039,7 +2099,10 @@ int pd_ctx_destroy(void* ctx) {
   if (!ctx) return PD_E_ARG;
   Ctx* c = C(ctx);
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->run_stream);
+  if (c->adm_stream) (void)hipStreamSynchronize(c->adm_stream);
+  c->inflight.on = false;
+  c->stream = c->run_stream;
   for (auto& kv : c->envs) { free_env_device(*kv.second); delete kv.second; }
   c->mem.trim(0);
   if (c->stage.pinned) (void)hipHostFree(c->stage.pinned);
@@ -2062,7 +2125,8 @@ int pd_ctx_destroy(void* ctx) {
   if (c->d_cstarts) (void)hipFree(c->d_cstarts);
   for (auto& r : c->pending) { c->pool.push_back(r.a); c->pool.push_back(r.b); }
   for (hipEvent_t ev : c->pool) (void)hipEventDestroy(ev);
-  (void)hipStreamDestroy(c->stream);
+  (void)hipStreamDestroy(c->run_stream);
+  if (c->adm_stream) (void)hipStreamDestroy(c->adm_stream);
   delete c;
   return PD_OK;
 }
@@ -2070,7 +2134,7 @@ int pd_ctx_destroy(void* ctx) {
 const char* pd_last_error(void* ctx) { return ctx ? C(ctx)->err.c_str() : g_err.c_str(); }
 
 int pd_sync(void* ctx) {
-  return run_guard(ctx, [&] { PD_HIP(hipStreamSynchronize(C(ctx)->stream)); });
+  return run_guard(ctx, [&] { PD_HIP(hipStreamSynchronize(idle(ctx)->stream)); });
 }
 
 int pd_ctx_mem_info(void* ctx, int64_t* free_bytes, int64_t* total_bytes) {
@@ -2211,6 +2275,7 @@ int pd_env_commit(void* ctx, int64_t env) {
     Ctx* c = C(ctx);
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
+    AdmissionScope adm(c);
     try {
       commit_env(c, e);
     } catch (...) {
@@ -2224,6 +2289,9 @@ int pd_env_destroy(void* ctx, int64_t env) {
   return run_guard(ctx, [&] {
     Ctx* c = C(ctx);
     Env& e = get_env(c, env);
+    if (c->inflight.on && std::find(c->inflight.ids.begin(), c->inflight.ids.end(), env) != c->inflight.ids.end())
+      throw PdError(PD_E_STATE, "env " + std::to_string(env) + " is stepped by the in-flight run: call pd_run_end first");
+    AdmissionScope adm(c);
     PD_HIP(hipStreamSynchronize(c->stream));
     free_env_device(e);
     if (e.slot >= 0) c->free_slots.push_back(e.slot);
@@ -2244,7 +2312,7 @@ int pd_env_info(void* ctx, int64_t env, int64_t* np, int64_t* nb, int64_t* bytes
 
 int pd_block_family_rows(void* ctx, int64_t env, int64_t block, int64_t* rows) {
   return run_guard(ctx, [&] {
-    Env& e = get_env(C(ctx), env);
+    Env& e = get_env(idle(ctx), env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     *rows = get_block(e, block).maxdeg + 1;
   });
@@ -2252,7 +2320,7 @@ int pd_block_family_rows(void* ctx, int64_t env, int64_t block, int64_t* rows) {
 
 int pd_block_family_export(void* ctx, int64_t env, int64_t block, int64_t* family, uint8_t* intact, int64_t rows) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     HostBlock& b = get_block(e, block);
@@ -2286,7 +2354,7 @@ static void download_block_vec(Ctx* c, Env& e, HostBlock& b, const double* src, 
 
 int pd_block_weighted_volume(void* ctx, int64_t env, int64_t block, double* m) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     download_block_vec(c, e, get_block(e, block), e.dev.m, m);
@@ -2295,7 +2363,7 @@ int pd_block_weighted_volume(void* ctx, int64_t env, int64_t block, double* m) {
 
 int pd_block_theta(void* ctx, int64_t env, int64_t block, double* theta) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     download_block_vec(c, e, get_block(e, block), e.dev.theta, theta);
@@ -2304,7 +2372,7 @@ int pd_block_theta(void* ctx, int64_t env, int64_t block, double* theta) {
 
 int pd_block_edp(void* ctx, int64_t env, int64_t block, double* edp, int64_t rows) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     HostBlock& b = get_block(e, block);
@@ -2335,7 +2403,52 @@ int pd_run(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t
     if (nenv <= 0 || !envs) throw PdError(PD_E_ARG, "empty env list");
     if (nsteps < 0) throw PdError(PD_E_ARG, "nsteps must be >= 0");
     if (nuf < 1) throw PdError(PD_E_ARG, "neigh_update_freq must be >= 1");
-    run_impl(C(ctx), std::vector<int64_t>(envs, envs + nenv), nsteps, nuf, 0, nullptr);
+    run_impl(idle(ctx), std::vector<int64_t>(envs, envs + nenv), nsteps, nuf, 0, nullptr);
+  });
+}
+
+int pd_run_begin(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, int64_t nuf) {
+  return run_guard(ctx, [&] {
+    Ctx* c = idle(ctx);
+    if (nenv <= 0 || !envs) throw PdError(PD_E_ARG, "empty env list");
+    if (nsteps < 1) throw PdError(PD_E_ARG, "pd_run_begin needs nsteps >= 1");
+    if (nuf < 1) throw PdError(PD_E_ARG, "neigh_update_freq must be >= 1");
+    if (c->prof) throw PdError(PD_E_STATE, "pd_run_begin does not run under pd_ctx_profile");
+    PD_HIP(hipSetDevice(c->device));
+    if (!c->adm_stream) PD_HIP(hipStreamCreateWithFlags(&c->adm_stream, hipStreamNonBlocking));
+    run_impl(c, std::vector<int64_t>(envs, envs + nenv), nsteps, nuf, 0, nullptr, true);
+  });
+}
+
+int pd_run_query(void* ctx, int32_t* done) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    if (!done) throw PdError(PD_E_ARG, "null output pointer");
+    if (!c->inflight.on) {
+      *done = 1;
+      return;
+    }
+    const hipError_t r = hipStreamQuery(c->run_stream);
+    if (r == hipErrorNotReady) {
+      *done = 0;
+      return;
+    }
+    PD_HIP(r);
+    *done = 1;
+  });
+}
+
+int pd_run_end(void* ctx) {
+  return run_guard(ctx, [&] {
+    Ctx* c = C(ctx);
+    if (!c->inflight.on) throw PdError(PD_E_STATE, "no run started by pd_run_begin is in flight");
+    PD_HIP(hipSetDevice(c->device));
+    Ctx::Inflight r = c->inflight;
+    c->inflight = Ctx::Inflight{};
+    c->stream = c->run_stream;
+    c->mem.stream = c->run_stream;
+    PD_HIP(hipStreamSynchronize(c->run_stream));
+    end_run(c, r.ids, r.nsteps, r.nuf, r.nl);
   });
 }
 
@@ -2347,13 +2460,13 @@ int pd_run_logged(void* ctx, const int64_t* envs, int64_t nenv, int64_t nsteps, 
     if (nuf < 1) throw PdError(PD_E_ARG, "neigh_update_freq must be >= 1");
     if (every < 1) throw PdError(PD_E_ARG, "average_prop_freq must be >= 1");
     if (!log && pd_log_rows(nsteps, every) > 0) throw PdError(PD_E_ARG, "null log buffer");
-    run_impl(C(ctx), std::vector<int64_t>(envs, envs + nenv), nsteps, nuf, every, log);
+    run_impl(idle(ctx), std::vector<int64_t>(envs, envs + nenv), nsteps, nuf, every, log);
   });
 }
 
 int pd_env_observables(void* ctx, int64_t env, double* out) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     if (!out) throw PdError(PD_E_ARG, "null output");
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
@@ -2373,7 +2486,7 @@ int pd_env_observables(void* ctx, int64_t env, double* out) {
 
 int pd_env_observables_noalloc(void* ctx, int64_t env, double* out_dev) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     if (!out_dev) throw PdError(PD_E_ARG, "null output");
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
@@ -2397,7 +2510,7 @@ int pd_env_observables_noalloc(void* ctx, int64_t env, double* out_dev) {
 
 int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
@@ -2412,7 +2525,7 @@ int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f) {
 
 int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, double* f, int64_t lo, int64_t hi) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
@@ -2446,7 +2559,7 @@ int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, dou
 
 int pd_env_update_neighs(void* ctx, int64_t env) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     contact_update(c, e);
@@ -2461,7 +2574,7 @@ static ContactDev& get_contact(Env& e, int64_t cidx) {
 
 int pd_contact_update(void* ctx, int64_t env, int64_t cidx, const double* y) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
@@ -2479,7 +2592,7 @@ int pd_contact_update(void* ctx, int64_t env, int64_t cidx, const double* y) {
 
 int pd_contact_neighs(void* ctx, int64_t env, int64_t cidx, int64_t* neighs, int64_t max_neighs) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
@@ -2500,7 +2613,7 @@ int pd_contact_neighs(void* ctx, int64_t env, int64_t cidx, int64_t* neighs, int
 
 int pd_contact_force(void* ctx, int64_t env, int64_t cidx, const double* y, double* f) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
@@ -2528,7 +2641,7 @@ int pd_contact_force(void* ctx, int64_t env, int64_t cidx, const double* y, doub
 
 int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     PD_HIP(hipStreamSynchronize(c->stream));
@@ -2540,7 +2653,7 @@ int pd_env_download(void* ctx, int64_t env, double* y, double* v, double* f) {
 
 int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, const double* f) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     PD_HIP(hipStreamSynchronize(c->stream));
@@ -2555,7 +2668,7 @@ int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, cons
 
 int pd_env_damage(void* ctx, int64_t env, double* damage) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     BondsHost h;
@@ -2571,7 +2684,7 @@ int pd_env_damage(void* ctx, int64_t env, double* damage) {
 
 int pd_ctx_profile(void* ctx, int32_t enable) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     drain_profile(c);
     for (int k = 0; k < PD_PROF_NCLASS; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
     c->prof = enable != 0;
@@ -2580,7 +2693,7 @@ int pd_ctx_profile(void* ctx, int32_t enable) {
 
 int pd_ctx_profile_read(void* ctx, int32_t cls, double* total_ms, int64_t* launches) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     if (cls < 0 || cls >= PD_PROF_NCLASS) throw PdError(PD_E_ARG, "unknown kernel class");
     drain_profile(c);
     if (total_ms) *total_ms = c->prof_ms[cls];
@@ -2603,7 +2716,7 @@ int pd_env_state_size(void* ctx, int64_t env, int64_t* bytes) {
 
 int pd_env_state_save(void* ctx, int64_t env, void* buf, int64_t bytes) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
@@ -2638,7 +2751,7 @@ int pd_env_state_save(void* ctx, int64_t env, void* buf, int64_t bytes) {
 
 int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     PD_HIP(hipSetDevice(c->device));
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
@@ -2689,7 +2802,7 @@ int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes) {
 int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64_t max_iter, double x_tol,
                     double f_tol, int64_t* iters, double* f_max, double* x_max) {
   return run_guard(ctx, [&] {
-    Ctx* c = C(ctx);
+    Ctx* c = idle(ctx);
     if (max_iter < 1) throw PdError(PD_E_ARG, "max_iter must be >= 1");
     if (!(step_size > 0.0) || !std::isfinite(step_size)) throw PdError(PD_E_ARG, "step_size must be > 0");
     PD_HIP(hipSetDevice(c->device));
@@ -2762,8 +2875,8 @@ int pd_env_minimize(void* ctx, int64_t env, double step_size, double drag, int64
 }
 
 int pd_env_check_finite(void* ctx, int64_t env) {
-  Ctx* c = C(ctx);
   int rc = run_guard(ctx, [&] {
+    Ctx* c = idle(ctx);
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     std::vector<double> a(3 * e.N);

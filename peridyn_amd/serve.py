@@ -10,7 +10,11 @@ peridyn_amd.workloads.make_spec), a step count and a contact-rebuild period
   * steps every resident simulation together, one pd_run over all of them per
     chunk (one launch per kernel class covers every specimen);
   * retires each simulation the moment its steps are done (results copied to
-    the host, device memory freed) and admits waiting ones into the room.
+    the host, device memory freed) and admits waiting ones into the room;
+  * with a backend that can start a chunk and return (step_begin / step_done /
+    step_end), admits the requests that arrive while a chunk runs during that
+    chunk: their upload and family build overlap the stepping on the device,
+    and they join the next chunk.
 
 Chunks are multiples of `nuf` and never pass a resident request's last step, so
 each request rebuilds contacts at the same steps as when it runs alone in one
@@ -106,6 +110,7 @@ class Stats:
     peak_bytes: int = 0
     chunks: int = 0
     admit_s: float = 0.0    # host + device time spent admitting (upload, family build)
+    admit_overlap_s: float = 0.0  # the part of admit_s spent while a chunk was running
     step_s: float = 0.0     # time inside pd_run
     retire_s: float = 0.0   # result download and free
     idle_s: float = 0.0     # waiting for arrivals with nothing resident
@@ -115,8 +120,8 @@ class Stats:
         return {"completed": self.completed, "makespan_s": self.makespan, "sims_per_s": self.sims_per_s,
                 "bond_updates_per_s": self.bond_updates_per_s, "p50_s": self.p50, "p99_s": self.p99,
                 "mean_active": self.mean_active, "max_active": self.max_active, "peak_bytes": self.peak_bytes,
-                "chunks": self.chunks, "admit_s": self.admit_s, "step_s": self.step_s, "retire_s": self.retire_s,
-                "idle_s": self.idle_s}
+                "chunks": self.chunks, "admit_s": self.admit_s, "admit_overlap_s": self.admit_overlap_s,
+                "step_s": self.step_s, "retire_s": self.retire_s, "idle_s": self.idle_s}
 
 
 def percentile(a, q: float) -> float:
@@ -136,13 +141,22 @@ class Server:
         retire(handle, result)                         fill result, free memory
         free_bytes() -> int                            device memory now free
 
+    and, to overlap admission with stepping, optionally
+
+        step_begin(handles, nsteps, nuf)   start the chunk and return
+        step_done() -> bool                whether it has completed
+        step_end()                         wait for it and complete it
+
     `chunk` is the largest number of steps per pd_run (None: run each batch to
-    the first completion); `max_active` caps the resident simulations."""
+    the first completion); `max_active` caps the resident simulations;
+    `poll_s` is how often arrivals are checked while a chunk runs; `overlap`
+    (default: whenever the backend can) admits during chunks."""
 
     def __init__(self, backend, budget_bytes: int, chunk: Optional[int] = 50, max_active: int = 1 << 30,
                  clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
                  reserve_bytes: int = 1 << 30, keep_log: bool = False,
-                 on_retire: Optional[Callable[[Result], None]] = None):
+                 on_retire: Optional[Callable[[Result], None]] = None, poll_s: float = 2e-4,
+                 overlap: Optional[bool] = None):
         self.backend = backend
         self.budget = int(budget_bytes)
         self.chunk = chunk
@@ -152,6 +166,11 @@ class Server:
         self.reserve = reserve_bytes
         self.keep_log = keep_log
         self.on_retire = on_retire
+        self.poll_s = poll_s
+        can = all(hasattr(backend, m) for m in ("step_begin", "step_done", "step_end"))
+        if overlap and not can:
+            raise ValueError("the backend cannot start a chunk and return")
+        self.overlap = can if overlap is None else bool(overlap)
 
     def serve(self, requests: List[dict], make_spec: Optional[Callable[[dict], dict]] = None):
         """Serve `requests` (dicts with id, arrival [s after start], steps, nuf,
@@ -182,9 +201,10 @@ class Server:
         t0 = self.clock()
         busy_t = 0.0
         active_t = 0.0
-        while pending or active:
-            now = self.clock() - t0
-            # admission: first fit in arrival order among the arrived requests
+        running = None  # overlapped chunk in flight: (stepped simulations, k, start time)
+
+        def admit_arrived(now: float):
+            nonlocal used
             i = 0
             while i < len(pending) and len(active) < self.max_active:
                 r = pending[i]
@@ -200,15 +220,59 @@ class Server:
                 handle, nbytes, bonds = self.backend.admit(specs[r["id"]])
                 res.admitted = self.clock() - t0
                 st.admit_s += res.admitted - (ta - t0)
+                if running is not None:
+                    st.admit_overlap_s += res.admitted - (ta - t0)
                 res.bonds = bonds
                 used += max(nbytes, 0)
                 active.append(_Active(r, handle, nbytes, int(r["steps"]), res))
                 st.peak_bytes = max(st.peak_bytes, used)
                 if self.keep_log:
                     st.log.append(("admit", r["id"], res.admitted, used))
+
+        def chunk_done(stepped: List[_Active], k: int, dt: float):
+            nonlocal used, busy_t, active_t, active
+            busy_t += dt
+            st.step_s += dt
+            active_t += dt * len(stepped)
+            st.chunks += 1
+            st.max_active = max(st.max_active, len(stepped))
+            if self.keep_log:
+                st.log.append(("step", tuple(a.req["id"] for a in stepped), k))
+            finished = set()
+            for a in stepped:
+                a.left -= k
+                if a.left == 0:
+                    tr = self.clock()
+                    self.backend.retire(a.handle, a.result)
+                    a.result.done = self.clock() - t0
+                    st.retire_s += a.result.done - (tr - t0)
+                    used -= max(a.bytes, 0)
+                    results[a.req["id"]] = a.result
+                    finished.add(id(a))
+                    if self.on_retire is not None:
+                        self.on_retire(a.result)
+                    if self.keep_log:
+                        st.log.append(("retire", a.req["id"], a.result.done, used))
+            active = [a for a in active if id(a) not in finished]
+
+        while pending or active or running is not None:
+            admit_arrived(self.clock() - t0)
+            if running is not None:
+                stepped, k, ts = running
+                if not self.backend.step_done():
+                    # every arrived request that fits was just admitted
+                    now = self.clock() - t0
+                    future = [r["arrival"] - now for r in pending if r["arrival"] > now]
+                    self.sleep(min([self.poll_s] + future))
+                    continue
+                self.backend.step_end()
+                running = None
+                chunk_done(stepped, k, self.clock() - ts)
+                continue
             if not active:
                 if not pending:
                     break
+                now = self.clock() - t0
                 future = [r["arrival"] for r in pending if r["arrival"] > now]
                 if not future:
                     raise RuntimeError("no waiting request fits the free device memory")
@@ -220,33 +284,14 @@ class Server:
             k = min(a.left for a in active)
             if self.chunk is not None:
                 k = min(k, self.chunk)
+            stepped = list(active)
             ts = self.clock()
-            self.backend.step([a.handle for a in active], k, nuf)
-            dt = self.clock() - ts
-            busy_t += dt
-            st.step_s += dt
-            active_t += dt * len(active)
-            st.chunks += 1
-            st.max_active = max(st.max_active, len(active))
-            if self.keep_log:
-                st.log.append(("step", tuple(a.req["id"] for a in active), k))
-            still = []
-            for a in active:
-                a.left -= k
-                if a.left == 0:
-                    tr = self.clock()
-                    self.backend.retire(a.handle, a.result)
-                    a.result.done = self.clock() - t0
-                    st.retire_s += a.result.done - (tr - t0)
-                    used -= max(a.bytes, 0)
-                    results[a.req["id"]] = a.result
-                    if self.on_retire is not None:
-                        self.on_retire(a.result)
-                    if self.keep_log:
-                        st.log.append(("retire", a.req["id"], a.result.done, used))
-                else:
-                    still.append(a)
-            active = still
+            if self.overlap:
+                self.backend.step_begin([a.handle for a in stepped], k, nuf)
+                running = (stepped, k, ts)
+                continue
+            self.backend.step([a.handle for a in stepped], k, nuf)
+            chunk_done(stepped, k, self.clock() - ts)
         lat = [r.latency for r in results.values()]
         st.completed = len(results)
         first = min((r.arrival for r in results.values()), default=0.0)
@@ -280,6 +325,18 @@ class GpuBackend:
     def step(self, handles, nsteps: int, nuf: int):
         from .core import run_batch
         run_batch(handles, nsteps, nuf)
+
+    def step_begin(self, handles, nsteps: int, nuf: int):
+        from .core import run_batch_begin
+        run_batch_begin(handles, nsteps, nuf)
+
+    def step_done(self) -> bool:
+        from .core import run_batch_done
+        return run_batch_done(self.ctx)
+
+    def step_end(self):
+        from .core import run_batch_end
+        run_batch_end(self.ctx)
 
     def retire(self, env, result: Result):
         if self.keep_state:

@@ -181,3 +181,61 @@ def test_latch_readback_after_a_two_block_env_with_a_pending_block(core):
     assert np.ptp(vz2) > 0.0  # the second plate deformed: its particles no longer move together
     for e in envs + [alone]:
         e.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_run_begin_end_equals_run_and_admits_meanwhile(core, prec):
+    """pd_run_begin / pd_run_end step exactly as pd_run does (bit for bit, an
+    indentation request with contacts and FixBC and a tensile one, 60 steps
+    with rebuilds every 10), and an env committed while the run is in flight
+    (on the admission stream) steps like one committed before it."""
+    reqs = requests(count=6, seed=23)
+    ind = next(r for r in reqs if r["shape"] == "indentation")
+    ten = next(r for r in reqs if r["shape"] == "tensile")
+    ctx = core.Context(0)
+    ref = [core.Env.from_spec(r["spec"], precision=prec, ctx=ctx) for r in (ind, ten)]
+    core.run_batch(ref, 60, 10)
+    want = [e.download() for e in ref]
+    late_ref = core.Env.from_spec(ten["spec"], precision=prec, ctx=ctx)
+    late_ref.run(30, 10)
+    want_late = late_ref.download()
+    for e in ref + [late_ref]:
+        e.close()
+    say(f"async run ({prec})")
+    envs = [core.Env.from_spec(r["spec"], precision=prec, ctx=ctx) for r in (ind, ten)]
+    core.run_batch_begin(envs, 60, 10)
+    late = core.Env.from_spec(ten["spec"], precision=prec, ctx=ctx)  # committed during the run
+    core.run_batch_end(ctx)
+    assert core.run_batch_done(ctx)
+    for e, w in zip(envs, want):
+        got = e.download()
+        for a, b in zip(got, w):
+            assert np.array_equal(a, b)
+        assert e.time_step == 60
+    late.run(30, 10)
+    for a, b in zip(late.download(), want_late):
+        assert np.array_equal(a, b)
+
+
+def test_calls_during_an_inflight_run_are_refused(core):
+    from peridyn_amd import _lib
+    reqs = requests(count=4, seed=29)
+    ctx = core.Context(0)
+    a = core.Env.from_spec(reqs[0]["spec"], precision="fp64", ctx=ctx)
+    b = core.Env.from_spec(reqs[1]["spec"], precision="fp64", ctx=ctx)
+    core.run_batch_begin([a], 40, 10)
+    L = ctx.lib
+    assert L.pd_sync(ctx.h) == _lib.PD_E_STATE
+    y = np.zeros((a.N, 3))
+    assert L.pd_env_download(ctx.h, a.id, y.ctypes.data_as(_lib.P), None, None) == _lib.PD_E_STATE
+    ids = np.array([b.id], dtype=np.int64)
+    assert L.pd_run(ctx.h, ids.ctypes.data_as(_lib.P), 1, 10, 10) == _lib.PD_E_STATE
+    assert L.pd_run_begin(ctx.h, ids.ctypes.data_as(_lib.P), 1, 10, 10) == _lib.PD_E_STATE
+    assert L.pd_env_destroy(ctx.h, a.id) == _lib.PD_E_STATE  # stepped by the run
+    info = b.info()  # allowed meanwhile
+    assert info["particles"] == b.N
+    b.close()  # not stepped by the run: allowed
+    core.run_batch_end(ctx)
+    assert L.pd_run_end(ctx.h) == _lib.PD_E_STATE  # nothing in flight
+    a.run(10, 10)
+    assert a.time_step == 50

@@ -150,3 +150,74 @@ def test_idle_server_waits_for_arrivals():
         assert res[r["id"]].admitted >= r["arrival"]
         assert res[r["id"]].latency < 0.1
     assert math.isclose(st.makespan, max(x.done for x in res.values()) - min(r["arrival"] for r in reqs))
+
+
+class FakeAsyncBackend(FakeBackend):
+    """FakeBackend whose chunks run in the background: step_begin returns at
+    once, the chunk completes per_step * n seconds later on the clock."""
+
+    def __init__(self, clock, **kw):
+        super().__init__(clock, **kw)
+        self.end_at = None
+        self.chunk_spans = []   # (start, end, handles)
+        self.admits = []        # (clock at admission, handle)
+
+    def admit(self, spec):
+        h, nb, bonds = super().admit(spec)
+        self.admits.append((self.clock.t, h))
+        return h, nb, bonds
+
+    def step_begin(self, handles, n, nuf):
+        assert self.end_at is None, "a chunk is already running"
+        self.calls.append((tuple(handles), n, nuf))
+        for h in handles:
+            self.steps_done[h] += n
+        self.end_at = self.clock.t + self.per_step * n
+        self.chunk_spans.append((self.clock.t, self.end_at, tuple(handles)))
+
+    def step_done(self):
+        return self.clock.t >= self.end_at
+
+    def step_end(self):
+        self.clock.t = max(self.clock.t, self.end_at)
+        self.end_at = None
+
+    def retire(self, h, result):
+        assert self.end_at is None, "retired while a chunk runs"
+        super().retire(h, result)
+
+
+def test_overlapped_admission_joins_the_next_chunk():
+    clock = FakeClock()
+    be = FakeAsyncBackend(clock, per_step=2e-4, per_admit=1e-3)
+    reqs = small_requests(count=24, rate=400.0, seed=11)
+    srv = serve.Server(be, budget_bytes=1 << 40, chunk=50, clock=clock, sleep=clock.sleep, reserve_bytes=0,
+                       keep_log=True)
+    res, st = srv.serve(reqs)
+    assert st.completed == len(reqs)
+    for r in reqs:
+        assert int(res[r["id"]].y[0]) == r["steps"]
+        assert res[r["id"]].done >= res[r["id"]].admitted >= r["arrival"]
+    # some admissions happened while a chunk ran, and a simulation admitted
+    # during a chunk is not stepped by it
+    inside = 0
+    for t, h in be.admits:
+        for (a, b, hs) in be.chunk_spans:
+            if a < t < b:
+                inside += 1
+                assert h not in hs
+    assert inside > 0 and st.admit_overlap_s > 0
+    for handles, n, nuf in be.calls:
+        assert n % nuf == 0 and 0 < n <= 50
+
+
+def test_overlap_can_be_turned_off():
+    clock = FakeClock()
+    be = FakeAsyncBackend(clock)
+    reqs = small_requests(count=6)
+    srv = serve.Server(be, budget_bytes=1 << 40, chunk=None, max_active=1, clock=clock, sleep=clock.sleep,
+                       reserve_bytes=0, overlap=False)
+    res, st = srv.serve(reqs)
+    assert st.completed == len(reqs) and not be.chunk_spans and st.admit_overlap_s == 0
+    with pytest.raises(ValueError):
+        serve.Server(FakeBackend(clock), budget_bytes=1 << 40, overlap=True)
