@@ -127,23 +127,26 @@ def test_batched_specimens_equal_alone(core):
         e.close()
 
 
-def test_full_served_stream_equals_alone(core):
+@pytest.mark.parametrize("prec", ["fp32c", "fp64"])
+def test_full_served_stream_equals_alone(core, prec):
     """The served benchmark's stream at full size (64 Poisson requests,
     1k-200k particles, 200-2000 steps, tensile and indentation, BB and OSB),
-    served as the bench serves it (FP64, 4 lanes per particle, 50-step chunks):
-    every request equals the same request run alone in one call, bit for bit."""
+    served as the bench serves it (4 lanes per particle, 50-step chunks,
+    admission overlapping the chunks; FP32C as benchmarked, and FP64): every
+    request equals the same request run alone in one call, bit for bit."""
     reqs = W.served_requests(count=64, seed=20261204, rate=200.0)
     for r in reqs:
         r["spec"] = W.make_spec(r)
     ctx = core.Context(0)
-    srv = serve.Server(serve.GpuBackend(ctx, "fp64", keep_state=True, lanes_per_particle=4),
+    srv = serve.Server(serve.GpuBackend(ctx, prec, keep_state=True, lanes_per_particle=4),
                        int(0.8 * ctx.mem_info()[0]), chunk=50)
-    say(f"serve the full stream of {len(reqs)} requests")
+    say(f"serve the full stream of {len(reqs)} requests ({prec})")
     res, st = srv.serve(reqs)
     say(f"  done: {st.completed} sims, max resident {st.max_active}, {st.chunks} chunks, {st.makespan:.2f} s")
     assert st.completed == len(reqs) and st.max_active > 32
+    assert st.admit_overlap_s > 0
     for r in reqs:
-        e = core.Env.from_spec(r["spec"], precision="fp64", ctx=ctx, lanes_per_particle=4)
+        e = core.Env.from_spec(r["spec"], precision=prec, ctx=ctx, lanes_per_particle=4)
         e.run(r["steps"], r["nuf"])
         y, v, f = e.download()
         d = e.damage()
