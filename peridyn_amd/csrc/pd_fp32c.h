@@ -10,6 +10,24 @@ __device__ __forceinline__ void kahan(float& s, float& c, float v) {
   s = t;
 }
 
+// Compensated sum of FP32 terms taken in groups: the terms of one group (a
+// lane's 4 consecutive bond steps, the bonds of one packed word in the tiled
+// kernels) are accumulated in plain FP32, and each group's total enters a
+// Kahan sum.  The error stays a constant multiple of eps * sum |term| (at most
+// 4 roundings per group, 2 for the compensated sum), independent of the
+// number of groups, at a quarter of the per-term compensation cost.  Tiled and
+// untiled kernels group the same bond steps, so their sums stay bit-identical.
+// A group's running total starts at -0 (the additive identity, so its first
+// term enters exactly) and takes each term a * b as a fused multiply-add.
+struct GroupKahan {
+  float g = -0.0f, s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float a, float b) { g = __builtin_fmaf(a, b, g); }
+  __device__ __forceinline__ void flush() {
+    kahan(s, c, g);
+    g = -0.0f;
+  }
+};
+
 struct G32 {
   float Y1, Y2, Y3, inv_x, inv_y, xij, ext, s;
 };
@@ -18,9 +36,15 @@ struct G32 {
 // w = y - (corner + x32), with x32 the FP32 offset from the block's corner (exact
 // for lattice positions).  Formed once per particle (per halo record in the
 // tiled kernels); a bond's displacement difference is then U = float(w_j - w_i).
+__device__ __forceinline__ double3 disp32(const double3& corner, const float4& x32, const double4& y) {
+  return make_double3(y.x - (corner.x + (double)x32.x), y.y - (corner.y + (double)x32.y),
+                      y.z - (corner.z + (double)x32.z));
+}
+__device__ __forceinline__ double3 corner_of(const BlockDev& B) {
+  return make_double3(B.corner[0], B.corner[1], B.corner[2]);
+}
 __device__ __forceinline__ double3 disp32(const BlockDev& B, const float4& x32, const double4& y) {
-  return make_double3(y.x - (B.corner[0] + (double)x32.x), y.y - (B.corner[1] + (double)x32.y),
-                      y.z - (B.corner[2] + (double)x32.z));
+  return disp32(corner_of(B), x32, y);
 }
 
 // Bond geometry from the FP32 reference offsets and the FP64 displacements:

@@ -11,8 +11,9 @@
 //                           motion of the body
 //   |Y|^2 = |X|^2 + (2 X.U + U.U),  e = (2 X.U + U.U) / (|Y| + |X|),  s = e / |X|
 //                           (no |Y| - |X| cancellation), all FP32
-// Force, theta and td_norm2 sums are Kahan-compensated FP32 per lane; the G
-// lanes of a particle are combined in FP64.  Positions, velocities, the Verlet
+// Force and theta sums are Kahan-compensated FP32 per lane over groups of 4
+// bond steps (GroupKahan), td_norm2 sums per bond; the G lanes of a particle
+// are combined in FP64.  Positions, velocities, the Verlet
 // update and the weighted volume stay FP64.  Elastoplastic blocks
 // (elasto_plastic.jl:228-255,282-295) keep their plastic state in FP64: the
 // plastic extensions edp (accumulated over the run from small increments that
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
   const double3 wp = disp32(B, xp, y[p]);
   const int d = w.active ? E.deg[p] : 0;
   const uint32_t base = w.ch.slot0 + w.lane;
-  float s = 0.0f, c = 0.0f;
+  GroupKahan acc;  // groups of 4 bond steps, as the tiled kernel's packed words
   constexpr int U = 2;  // bonds per batch; column/state loaded one batch ahead
   const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
   int jn[U];
@@ -71,13 +72,17 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
     if (i0 + U < ni) load_stream(i0 + U);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (!sv[u]) continue;
-      G32 g;
-      geom32(xp, wp, xj[u], disp32(B, xj[u], yj[u]), g);
-      kahan(s, c, g.ext * xj[u].w);
+      if (sv[u]) {
+        G32 g;
+        geom32(xp, wp, xj[u], disp32(B, xj[u], yj[u]), g);
+        acc.add(g.ext, xj[u].w);
+      }
+      if (i0 + u < ni && (((i0 + u) & 3) == 3 || i0 + u == ni - 1)) acc.flush();
     }
   }
-  const double tot = comp_total(s, c, w.C);
+  // the tiled kernel runs every packed word of the chunk, empty ones too
+  for (int q = (ni + 3) >> 2; q < ((w.ch.width + 3) >> 2); ++q) acc.flush();
+  const double tot = comp_total(acc.s, acc.c, w.C);
   if (w.active && w.g == 0) {
     const float th = (float)((3.0 * (double)E.aux32[p].y) * tot);
     E.aux32[p].x = th;
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
       edp_out = E.edp[E.edp_cur ^ 1];
     }
     const float c11 = (float)B.c11;
-    float s1 = 0.f, s2 = 0.f, s3 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+    GroupKahan k1, k2, k3;  // groups of 4 bond steps, as the tiled kernel's packed words
     constexpr int U = 2;  // bonds per batch; column/state loaded one batch ahead
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
     int jn[U];
@@ -256,7 +261,15 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
       if (i0 + U < ni) load_stream(i0 + U);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (!sv[u]) continue;
+        const bool grp_end = i0 + u < ni && (((i0 + u) & 3) == 3 || i0 + u == ni - 1);
+        if (!sv[u]) {
+          if (grp_end) {
+            k1.flush();
+            k2.flush();
+            k3.flush();
+          }
+          continue;
+        }
         const uint32_t slot = base + 64u * (uint32_t)(i0 + u);
         G32 g;
         geom32(xp, wp, xj[u], disp32(B, xj[u], yj[u]), g);
@@ -301,14 +314,25 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
           gst(st, slot, (uint8_t)0);
           if (KIND == PD_EP && yielded_p) gst(edp_out, slot, eo[u]);
         }
-        kahan(s1, c1, xx * g.Y1);
-        kahan(s2, c2, xx * g.Y2);
-        kahan(s3, c3, xx * g.Y3);
+        k1.add(xx, g.Y1);
+        k2.add(xx, g.Y2);
+        k3.add(xx, g.Y3);
+        if (grp_end) {
+          k1.flush();
+          k2.flush();
+          k3.flush();
+        }
       }
     }
-    a1 = comp_total(s1, c1, w.C);
-    a2 = comp_total(s2, c2, w.C);
-    a3 = comp_total(s3, c3, w.C);
+    // the tiled kernel runs every packed word of the chunk, empty ones too
+    for (int q = (ni + 3) >> 2; q < ((w.ch.width + 3) >> 2); ++q) {
+      k1.flush();
+      k2.flush();
+      k3.flush();
+    }
+    a1 = comp_total(k1.s, k1.c, w.C);
+    a2 = comp_total(k2.s, k2.c, w.C);
+    a3 = comp_total(k3.s, k3.c, w.C);
   }
   if (!w.active || w.g != 0) return;
   add_contacts<GPOW>(E, p, yp, y, a1, a2, a3);

@@ -580,6 +580,7 @@ __device__ __forceinline__ void stage_halo32(const EnvDev& E, const BlockDev& B,
   float2* __restrict__ f2 = (float2*)(sh + 3 * hcap);
   const float inv_den = (float)(1.0 / B.density[0]);
   const float c11 = (float)B.c11;
+  const double3 corner = corner_of(B);  // read once: the loop's LDS stores could alias B for the compiler
   for (int i0 = threadIdx.x; i0 < T.hn; i0 += 256 * kStageR) {
     uint32_t id[kStageR];
 #pragma unroll
@@ -601,7 +602,7 @@ __device__ __forceinline__ void stage_halo32(const EnvDev& E, const BlockDev& B,
     for (int r = 0; r < kStageR; ++r) {
       const int i = i0 + 256 * r;
       if (i < T.hn) {
-        const double3 wd = disp32(B, a[r], b[r]);
+        const double3 wd = disp32(corner, a[r], b[r]);
         sh[i] = wd.x;
         sh[hcap + i] = wd.y;
         sh[2 * hcap + i] = wd.z;
@@ -653,7 +654,7 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
   const double3 wp = disp32(B, xp, gld4(E.y[E.cur], p));
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
-  float s = 0.0f, c = 0.0f;
+  GroupKahan acc;
   pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -663,10 +664,11 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
       const Rec32 r = rec32(sh, f2, hcap, j);
       G32 g;
       geom32(xp, wp, r.x, r.w, g);
-      kahan(s, c, g.ext * r.x.w);
+      acc.add(g.ext, r.x.w);
     }
+    acc.flush();
   });
-  const double tot = comp_total(s, c, C);
+  const double tot = comp_total(acc.s, acc.c, C);
   if (w.active && w.g == 0) {
     const float th = (float)((3.0 * (double)E.aux32[p].y) * tot);
     E.aux32[p].x = th;
@@ -710,7 +712,7 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
     const float s0f = B.s0f_up;  // (double)s < s0 exactly when s < s0f
     const float Kf = (float)B.K[0], Gf = (float)B.G[0];
     const float A = 3.0f * Kf - 5.0f * Gf, B15 = 15.0f * Gf;
-    float s1 = 0.f, s2 = 0.f, s3 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+    GroupKahan k1, k2, k3;
     const int nt4 = (w.ch.width + 3) >> 2;
     const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
     const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
@@ -731,10 +733,13 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
         const bool intact = g.s < s0f;
         brk = brk || !intact;
         const float xx = intact ? tt * r.x.w : 0.0f;
-        kahan(s1, c1, xx * g.Y1);
-        kahan(s2, c2, xx * g.Y2);
-        kahan(s3, c3, xx * g.Y3);
+        k1.add(xx, g.Y1);
+        k2.add(xx, g.Y2);
+        k3.add(xx, g.Y3);
       }
+      k1.flush();
+      k2.flush();
+      k3.flush();
     });
     if (brk)
       pk_record_breaks(pk, pk16, st, wb, base, nt4, [&](int j) {
@@ -743,9 +748,9 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
         geom32(xp, wp, r.x, r.w, g);
         return !(g.s < s0f);
       });
-    a1 = comp_total(s1, c1, C);
-    a2 = comp_total(s2, c2, C);
-    a3 = comp_total(s3, c3, C);
+    a1 = comp_total(k1.s, k1.c, C);
+    a2 = comp_total(k2.s, k2.c, C);
+    a3 = comp_total(k3.s, k3.c, C);
   }
   if (!w.active || w.g != 0) return;
   finish_tiled(E, p, gld4(E.x, p), yp, a1, a2, a3, flags);
