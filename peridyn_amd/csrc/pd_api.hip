@@ -1127,6 +1127,13 @@ void commit_env(Ctx* c, Env& e) {
     for (int d = 0; d < 3; ++d) B.corner[d] = INFINITY;
     for (int64_t i = 0; i < b.d.n; ++i)
       for (int d = 0; d < 3; ++d) B.corner[d] = std::min(B.corner[d], b.x[3 * i + d]);
+    // integer lattice: bond lengths from the tiled kernels' table
+    {
+      bool xi = b.d.horizon * b.d.horizon <= 32.0;
+      for (int64_t i = 0; xi && i < 3 * b.d.n; ++i)
+        xi = std::floor(b.x[i]) == b.x[i] && std::fabs(b.x[i]) < 4194304.0;
+      B.xint = xi ? 1 : 0;
+    }
     {
       float f = (float)b.s0[0];
       if ((double)f < b.s0[0]) f = std::nextafter(f, INFINITY);

@@ -42,7 +42,10 @@ struct BlockDev {
   int32_t first, n;          // env-local particle range
   int32_t chunk0, nchunks;   // chunk range (env-local chunk ids)
   int32_t deformed;          // latch (general_material.jl:243-255)
-  int32_t pad;
+  // every reference position of the block is an integer (|x| < 2^22) and
+  // horizon^2 <= 32: each bond's |X|^2 is an integer 1..31, and the tiled
+  // kernels take |X| and 1/|X| from a table (pd_xtab_size entries)
+  int32_t xint;
   double c11;                // BB bond stiffness c[1,1]
   double horizon;
   double density[PD_MAX_TYPES];

@@ -206,13 +206,51 @@ __device__ __forceinline__ void finish_tiled(EnvDev& E, int p, const double4& xp
   finish_particle_v(E, p, xp, yp, a1, a2, a3, flags, gld4(E.v, p), E.bcid[p]);
 }
 
-// bond geometry against the staged neighbour j (expressions of the untiled kernels)
+// deformed bond vector to the staged neighbour j (expressions of the untiled kernels)
 template <class R>
-__device__ __forceinline__ void tile_geom(const R& rc, int j, const double4& xp, const double4& yp, double& X1,
-                                          double& X2, double& X3, double& Y1, double& Y2, double& Y3) {
-  X1 = rc.x(0, j) - xp.x; X2 = rc.x(1, j) - xp.y; X3 = rc.x(2, j) - xp.z;
+__device__ __forceinline__ void tile_y(const R& rc, int j, const double4& yp, double& Y1, double& Y2, double& Y3) {
   Y1 = rc.y(0, j) - yp.x; Y2 = rc.y(1, j) - yp.y; Y3 = rc.y(2, j) - yp.z;
 }
+
+// Reference bond lengths of integer lattices.  In a block whose reference
+// positions are all integers (BlockDev::xint) every bond vector X is integral,
+// so |X|^2 is an integer below kXTab and |X|, 1/|X| are read from a table the
+// workgroup fills first, with the very expressions the general path evaluates
+// (xij = xx2 * pd_rsqrt(xx2)): both paths give the same bits.
+constexpr int kXTab = 32;
+__device__ __forceinline__ void fill_xtab64(double2* __restrict__ t) {
+  if (threadIdx.x < kXTab) {
+    const double a = (double)threadIdx.x;
+    const double r = pd_rsqrt(a);
+    t[threadIdx.x] = make_double2(a * r, r);
+  }
+}
+
+// The owner's reference position and where |X| comes from (FP64 kernels)
+struct XGeo {
+  double4 xp;
+  float f0, f1, f2;  // xp in FP32 (exact when the table is used)
+  bool lat;          // the integer-lattice table applies
+  const double2* tab;
+  __device__ __forceinline__ XGeo(const double4& x, bool l, const double2* t)
+      : xp(x), f0((float)x.x), f1((float)x.y), f2((float)x.z), lat(l), tab(t) {}
+  // |X| (xij) and 1/|X| of the bond to staged record j
+  template <class R>
+  __device__ __forceinline__ void xlen(const R& rc, int j, double& xij, double& inv_x) const {
+    if (lat) {
+      const float* __restrict__ fs = rc.fs();
+      const float X1 = fs[j] - f0, X2 = fs[rc.hcap + j] - f1, X3 = fs[2 * rc.hcap + j] - f2;
+      const double2 v = tab[(int)__builtin_fmaf(X3, X3, __builtin_fmaf(X2, X2, X1 * X1))];
+      xij = v.x;
+      inv_x = v.y;
+    } else {
+      const double X1 = rc.x(0, j) - xp.x, X2 = rc.x(1, j) - xp.y, X3 = rc.x(2, j) - xp.z;
+      const double xx2 = pd_norm2(X1, X2, X3);
+      inv_x = pd_rsqrt(xx2);
+      xij = xx2 * inv_x;
+    }
+  }
+};
 
 // ---------------------------------------------------------------------------
 // dilatation theta (src/peri.jl:155-172), OSB and EP blocks
@@ -228,6 +266,8 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   if (B.kind == PD_BB || !B.deformed) return;
+  __shared__ double2 xtab[XF ? kXTab : 1];
+  if (XF) fill_xtab64(xtab);
   stage_halo<XF, 0>(E, T, sh, hs);
   __syncthreads();
   TileWave w;
@@ -236,6 +276,7 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
   const uint64_t* __restrict__ pk = E.pk;
   const int p = w.active ? w.p : w.ch.first;
   const double4 xp = gld4(E.x, p), yp = gld4(E.y[E.cur], p);
+  const XGeo xg(xp, XF && B.xint, xtab);
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   double acc = 0.0;
@@ -245,11 +286,11 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
       const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
       if (!(e & 0x8000u)) continue;
       const int j = (int)(e & 0x7fffu);
-      double X1, X2, X3, Y1, Y2, Y3;
-      tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-      const double xx2 = pd_norm2(X1, X2, X3);
+      double Y1, Y2, Y3, xij, inv_x;
+      tile_y(rc, j, yp, Y1, Y2, Y3);
+      xg.xlen(rc, j, xij, inv_x);
       const double yy2 = pd_norm2(Y1, Y2, Y3);
-      const double ext = yy2 * pd_rsqrt(yy2) - xx2 * pd_rsqrt(xx2);
+      const double ext = yy2 * pd_rsqrt(yy2) - xij;
       acc = __builtin_fma(ext, rc.vol(j), acc);
     }
   });
@@ -276,6 +317,8 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
   EnvDev& E = envs[T.env];
   const BlockDev& B = E.blocks[T.block];
   if (B.kind != PD_EP || !B.deformed) return;
+  __shared__ double2 xtab[XF ? kXTab : 1];
+  if (XF) fill_xtab64(xtab);
   stage_halo<XF, 2>(E, T, sh, hs);
   __syncthreads();
   TileWave w;
@@ -288,6 +331,7 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
   double* __restrict__ edp_o = E.edp[E.edp_cur ^ 1];
   const int p = w.active ? w.p : w.ch.first;
   const double4 xp = gld4(E.x, p), yp = gld4(E.y[E.cur], p);
+  const XGeo xg(xp, XF && B.xint, xtab);
   const double4 ap = E.aux[p];  // (theta, 1/m, +-td_norm2 of the last step, C)
   const bool yielded = pd_has_yielded(ap);
   const double s0 = B.s0[0];
@@ -321,12 +365,10 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
       if (!(e & 0x8000u)) continue;
       const int j = (int)(e & 0x7fffu);
       const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
-      double X1, X2, X3, Y1, Y2, Y3;
-      tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-      const double xx2 = pd_norm2(X1, X2, X3);
+      double Y1, Y2, Y3, xij, inv_x;
+      tile_y(rc, j, yp, Y1, Y2, Y3);
+      xg.xlen(rc, j, xij, inv_x);
       const double yy2 = pd_norm2(Y1, Y2, Y3);
-      const double inv_x = pd_rsqrt(xx2);
-      const double xij = xx2 * inv_x;
       const double ext = yy2 * pd_rsqrt(yy2) - xij;
       if (ext * inv_x < s0) {
         const double th_j = rc.th(j), rm_j = rc.rm(j);
@@ -390,7 +432,7 @@ __device__ __forceinline__ void ep_stage_b(EpGrp& G, const double* __restrict__ 
 }
 
 template <class R>
-__device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, uint32_t wb, const double4& xp,
+__device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, uint32_t wb, const XGeo& xg,
                                          const double4& yp, const double4& ap, bool yielded_p, double s0_pp,
                                          double K0, double G0, double inv_den, const R& rc,
                                          double* __restrict__ edp_out, uint8_t* __restrict__ st,
@@ -401,13 +443,11 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
     if (!(e & 0x8000u)) continue;
     const int j = (int)(e & 0x7fffu);
     const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
-    double X1, X2, X3, Y1, Y2, Y3;
-    tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-    const double xx2 = pd_norm2(X1, X2, X3);
+    double Y1, Y2, Y3, xij, inv_x;
+    tile_y(rc, j, yp, Y1, Y2, Y3);
+    xg.xlen(rc, j, xij, inv_x);
     const double yy2 = pd_norm2(Y1, Y2, Y3);
-    const double inv_x = pd_rsqrt(xx2);
     const double inv_y = pd_rsqrt(yy2);
-    const double xij = xx2 * inv_x;
     const double yij = yy2 * inv_y;
     const double ext = yij - xij;
     if (ext * inv_x < s0_pp) {
@@ -465,12 +505,15 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
   TileWave w;
   const bool has = tile_wave<LG>(T, w);
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
+  __shared__ double2 xtab[XF ? kXTab : 1];
+  if (XF) fill_xtab64(xtab);
   if (deformed) stage_halo<XF, NX>(E, T, sh, hs);
   __syncthreads();
   if (!has) return;
   const Rec<XF, NX> rc(sh, hs);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 xp = gld4(E.x, p), yp = gld4(y, p);
+  const XGeo xg(xp, XF && B.xint, xtab);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
   if (deformed) {
     const uint64_t* __restrict__ pk = E.pk;
@@ -494,7 +537,7 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
       for (int t4 = 0; t4 < nt4; ++t4) {
         ep_stage_b(g1, edp_in);
         ep_stage_a(g2, ps.next(t4 + 2), base, t4 + 2, yielded_p, edp_in, E.rev, rc);
-        ep_group(g0, t4, base, wb, xp, yp, ap, yielded_p, s0_pp, K0, G0, inv_den, rc, edp_out, st, pk16, a1, a2,
+        ep_group(g0, t4, base, wb, xg, yp, ap, yielded_p, s0_pp, K0, G0, inv_den, rc, edp_out, st, pk16, a1, a2,
                  a3);
         g0 = g1;
         g1 = g2;
@@ -511,13 +554,11 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
           const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
           if (!(e & 0x8000u)) continue;
           const int j = (int)(e & 0x7fffu);
-          double X1, X2, X3, Y1, Y2, Y3;
-          tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-          const double xx2 = pd_norm2(X1, X2, X3);
+          double Y1, Y2, Y3, xij, inv_x;
+          tile_y(rc, j, yp, Y1, Y2, Y3);
+          xg.xlen(rc, j, xij, inv_x);
           const double yy2 = pd_norm2(Y1, Y2, Y3);
-          const double inv_x = pd_rsqrt(xx2);
           const double inv_y = pd_rsqrt(yy2);
-          const double xij = xx2 * inv_x;
           const double yij = yy2 * inv_y;
           const double ext = yij - xij;
           if (ext * inv_x < s0_pp) {
@@ -541,13 +582,11 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
       });
       if (brk)
         pk_record_breaks(pk, pk16, st, wb, base, nt4, [&](int j) {
-          double X1, X2, X3, Y1, Y2, Y3;
-          tile_geom(rc, j, xp, yp, X1, X2, X3, Y1, Y2, Y3);
-          const double xx2 = pd_norm2(X1, X2, X3);
+          double Y1, Y2, Y3, xij, inv_x;
+          tile_y(rc, j, yp, Y1, Y2, Y3);
+          xg.xlen(rc, j, xij, inv_x);
           const double yy2 = pd_norm2(Y1, Y2, Y3);
-          const double inv_x = pd_rsqrt(xx2);
           const double inv_y = pd_rsqrt(yy2);
-          const double xij = xx2 * inv_x;
           const double yij = yy2 * inv_y;
           const double ext = yij - xij;
           return !(ext * inv_x < s0_pp);
@@ -918,24 +957,28 @@ __global__ __launch_bounds__(256) void k_pk_sync(const Chunk* __restrict__ chunk
 // stride its largest halo fits.  The byte budgets are multiples of 1280 (the
 // coarsest LDS allocation granule, 160 KiB / 128) whose multiples by 6, 5, 4, 3
 // stay within 160 KiB also when rounded up to 512-byte granules.
+// The FP64 kernels' records leave kXTabLds bytes for their static lattice
+// table (RES); the FP32C kernels have none (RES = 0).
 constexpr int kLdsClassBytes[4] = {26880, 32000, 40960, 53760};
+constexpr int kXTabLds = 512;
+constexpr int halo_stride(int c, int rec, int reserve) { return (kLdsClassBytes[c] - reserve) / rec; }
 
-template <int REC, class F>
+template <int REC, int RES = kXTabLds, class F>
 void with_stride(int hcap, F&& f) {
-  constexpr int s0 = kLdsClassBytes[0] / REC, s1 = kLdsClassBytes[1] / REC;
-  constexpr int s2 = kLdsClassBytes[2] / REC, s3 = kLdsClassBytes[3] / REC;
+  constexpr int s0 = halo_stride(0, REC, RES), s1 = halo_stride(1, REC, RES);
+  constexpr int s2 = halo_stride(2, REC, RES), s3 = halo_stride(3, REC, RES);
   if (hcap <= s0) f(std::integral_constant<int, s0>{});
   else if (hcap <= s1) f(std::integral_constant<int, s1>{});
   else if (hcap <= s2) f(std::integral_constant<int, s2>{});
   else if (hcap <= s3) f(std::integral_constant<int, s3>{});
   else f(std::integral_constant<int, 0>{});
 }
-template <int REC, class F>
+template <int REC, int RES = kXTabLds, class F>
 void each_stride(F&& f) {
-  f(std::integral_constant<int, kLdsClassBytes[0] / REC>{});
-  f(std::integral_constant<int, kLdsClassBytes[1] / REC>{});
-  f(std::integral_constant<int, kLdsClassBytes[2] / REC>{});
-  f(std::integral_constant<int, kLdsClassBytes[3] / REC>{});
+  f(std::integral_constant<int, halo_stride(0, REC, RES)>{});
+  f(std::integral_constant<int, halo_stride(1, REC, RES)>{});
+  f(std::integral_constant<int, halo_stride(2, REC, RES)>{});
+  f(std::integral_constant<int, halo_stride(3, REC, RES)>{});
   f(std::integral_constant<int, 0>{});
 }
 
@@ -982,7 +1025,7 @@ void launch_theta32_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int
 }
 template <int LG>
 void launch_theta32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap) {
-  with_stride<rec32_bytes(TP_THETA, PD_OSB)>(
+  with_stride<rec32_bytes(TP_THETA, PD_OSB), 0>(
       hcap, [&](auto S) { launch_theta32_s<LG, decltype(S)::value>(s, tiles, n, envs, hcap); });
 }
 template <int KIND, int LG, int S>
@@ -992,7 +1035,7 @@ void launch_force32_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int
 }
 template <int KIND, int LG>
 void launch_force32_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  with_stride<rec32_bytes(TP_FORCE, KIND)>(
+  with_stride<rec32_bytes(TP_FORCE, KIND), 0>(
       hcap, [&](auto S) { launch_force32_s<KIND, LG, decltype(S)::value>(s, tiles, n, envs, flags, hcap); });
 }
 
@@ -1003,10 +1046,10 @@ void allow_lds(K* k, size_t bytes) {
 }
 template <int LG>
 void allow_lds_fp32c(size_t bytes) {
-  each_stride<rec32_bytes(TP_THETA, PD_OSB)>([&](auto S) { allow_lds(k_tile_theta32<LG, decltype(S)::value>, bytes); });
-  each_stride<rec32_bytes(TP_FORCE, PD_BB)>(
+  each_stride<rec32_bytes(TP_THETA, PD_OSB), 0>([&](auto S) { allow_lds(k_tile_theta32<LG, decltype(S)::value>, bytes); });
+  each_stride<rec32_bytes(TP_FORCE, PD_BB), 0>(
       [&](auto S) { allow_lds(k_tile_force32<PD_BB, LG, decltype(S)::value>, bytes); });
-  each_stride<rec32_bytes(TP_FORCE, PD_OSB)>(
+  each_stride<rec32_bytes(TP_FORCE, PD_OSB), 0>(
       [&](auto S) { allow_lds(k_tile_force32<PD_OSB, LG, decltype(S)::value>, bytes); });
 }
 template <int LG, bool XF>
@@ -1046,8 +1089,9 @@ int pd_tile_record_bytes(int pass, int kind, int prec, int xf) {
 
 int pd_tile_class(int pass, int kind, int prec, int xf, int hmax) {
   const int rec = pd_tile_record_bytes(pass, kind, prec, xf);
+  const int res = (prec == PD_PREC_FP32C && kind != PD_EP && pass != TP_TD) ? 0 : kXTabLds;
   for (int c = 0; c < 4; ++c)
-    if (hmax <= kLdsClassBytes[c] / rec) return c;
+    if (hmax <= halo_stride(c, rec, res)) return c;
   return 4;
 }
 
