@@ -92,11 +92,21 @@ typedef struct {
   int64_t max_neighs;           /* min(max_neighs, n2) */
 } pd_contact_desc;
 
-/* FixBC / ToFroBC / MoveBC (= ToFroBC with freq = Inf), all dims
- * (FixBC.jl:46-52, ToFroBC.jl:51-62,79-81,107-111). */
+/* FixBC / ToFroBC / MoveBC (= ToFroBC with freq = Inf) (FixBC.jl:46-52,
+ * ToFroBC.jl:51-62,79-81,107-111).  options:
+ *   PD_BC_ONLYATSTART  the constructors' onlyatstart keyword: minimize!
+ *       (minimize.jl:138-143) leaves the BC's particles free after applying it
+ *       once; run! applies the BC every step regardless, as the reference does;
+ *   PD_BC_DIMS(d)      the dims keyword as a 3-bit mask (bit k = dimension k+1;
+ *       0 = all three).  The reference assigns its 3 x n update into
+ *       y[dims, bool] (boundary_conditions.jl:108-110,123-125), which raises
+ *       DimensionMismatch unless every dimension is selected, so a subset is
+ *       refused here with PD_E_ARG. */
+enum { PD_BC_ONLYATSTART = 1 };
+#define PD_BC_DIMS(d) (((int32_t)(d) & 7) << 8)
 typedef struct {
   int32_t kind;                 /* PD_BC_FIX / PD_BC_TOFRO */
-  int32_t reserved;
+  int32_t options;              /* PD_BC_ONLYATSTART | PD_BC_DIMS(mask); 0: defaults */
   const uint8_t* mask;          /* N (env particles) */
   double rate[3];               /* ToFroBC rate (a velocity) */
   double freq;                  /* ToFroBC freq (integer-valued or +Inf) */

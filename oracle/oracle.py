@@ -90,6 +90,8 @@ def lib():
         P, I64, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double
         L.orc_julia_pow_int.argtypes = [D, I64]
         L.orc_julia_pow_int.restype = D
+        L.orc_contact_power.argtypes = [D, D, ctypes.c_int]
+        L.orc_contact_power.restype = D
         L.orc_cal_family.argtypes = [I64, P, D, I64, P, P]
         L.orc_cal_family.restype = I64
         L.orc_weighted_volume.argtypes = [I64, I64, P, P, P, P]
@@ -116,6 +118,11 @@ def _p(a: np.ndarray):
 def julia_pow_int(x: float, n: int) -> float:
     """Julia `x::Float64 ^ n::Integer` (compensated power by squaring)."""
     return lib().orc_julia_pow_int(float(x), int(n))
+
+
+def contact_power(strain: float, expo: float) -> float:
+    """strain^expo as the contact force takes it (orc_contact_power)."""
+    return lib().orc_contact_power(float(strain), float(expo), int(_julia_int_pow(expo)))
 
 
 def make_matrix(S) -> np.ndarray:
@@ -401,6 +408,11 @@ class OracleEnv:
             d["mask"] = np.asarray(bc["mask"], dtype=bool)
             d["last"] = np.zeros((int(d["mask"].sum()), 3))
             d["dir"] = 1
+            d["onlyatstart"] = bool(bc.get("onlyatstart", False))
+            dims = bc.get("dims")
+            if dims is not None and not all(bool(v) for v in dims):
+                # y[dims, bool] .= (3 x n update) (boundary_conditions.jl:108-110)
+                raise ValueError("DimensionMismatch: a BC on a subset of dims cannot take its 3 x n update")
             if d["kind"] == "tofro":
                 d["rate"] = np.asarray(bc["rate"], dtype=np.float64)
                 d.setdefault("freq", math.inf)
@@ -585,7 +597,8 @@ class OracleEnv:
         a = self.blocks[0].particle_size / 10.0
         held = np.zeros(self.N, dtype=bool)
         for bc in self.bcs:
-            held |= bc["mask"]
+            if not bc["onlyatstart"]:  # minimize.jl:138-143
+                held |= bc["mask"]
         free = ~held
         opt = np.zeros((int(free.sum()), 3))
         k = drag / 10.0

@@ -541,6 +541,14 @@ void orc_contact_neighs_omp(int64_t n1, const double* y1, int64_t n2, const doub
   contact_neighs_loop(n1, y1, n2, y2, distance, max_neighs, neighs_out, true);
 }
 
+// strain^expo of the NonLinearSpring force (nonlinearspring.jl:140, expo a
+// Float64): Julia's integer power for integer-valued exponents (see
+// orc_julia_pow_int), else the C library's pow, which like Julia's general
+// Float64 power is within one ulp of the exact value.
+double orc_contact_power(double strain, double expo, int expo_is_int) {
+  return expo_is_int ? orc_julia_pow_int(strain, (int64_t)expo) : std::pow(strain, expo);
+}
+
 // short_range_contact! RM12 cpu (src/contacts/contacts.jl:210-247) with the
 // NonLinearSpring force function (src/contacts/nonlinearspring.jl:124-145).
 // f1/f2 are the masked force views (3 x n1, 3 x n2) updated in place.
@@ -563,7 +571,7 @@ void orc_contact_force(int64_t n1, const double* y1, const double* vol1, const d
         const double strain = del_x / equi_size;
         double F1 = 0.0, F2 = 0.0, F3 = 0.0;
         if (!(del_x < 0.0)) {
-          const double p = expo_is_int ? orc_julia_pow_int(strain, (int64_t)expo) : std::pow(strain, expo);
+          const double p = orc_contact_power(strain, expo, expo_is_int);
           const double s = (K * p) / (mag_dr + 1e-10);
           F1 = s * d1; F2 = s * d2; F3 = s * d3;
         }

@@ -15,6 +15,11 @@ PD_OK, PD_E_HIP, PD_E_ARG, PD_E_OVERFLOW, PD_E_STATE, PD_E_NOMEM, PD_E_NONFINITE
 PD_BB, PD_OSB, PD_EP = 0, 1, 2
 PD_PREC_EXACT, PD_PREC_FP64, PD_PREC_FP32C = 0, 1, 2
 PD_BC_FIX, PD_BC_TOFRO = 0, 1
+PD_BC_ONLYATSTART = 1
+
+
+def PD_BC_DIMS(d: int) -> int:
+    return (int(d) & 7) << 8
 
 EXPORTS = [
     "pd_ctx_create", "pd_ctx_destroy", "pd_last_error", "pd_sync",
@@ -53,7 +58,7 @@ class ContactDesc(ctypes.Structure):
 
 
 class BcDesc(ctypes.Structure):
-    _fields_ = [("kind", I32), ("reserved", I32), ("mask", P), ("rate", D * 3),
+    _fields_ = [("kind", I32), ("options", I32), ("mask", P), ("rate", D * 3),
                 ("freq", D), ("applyafter", I64)]
 
 

@@ -209,6 +209,12 @@ class Env:
                         bd.rate[i] = float(r[i])
                     bd.freq = float(bc.get("freq", math.inf))
                     bd.applyafter = int(bc.get("applyafter", 0))
+                # onlyatstart / dims keywords of the BC constructors (FixBC.jl:46, ToFroBC.jl:51)
+                opts = _lib.PD_BC_ONLYATSTART if bc.get("onlyatstart", False) else 0
+                dims = bc.get("dims")
+                if dims is not None:
+                    opts |= _lib.PD_BC_DIMS(sum(1 << k for k in range(3) if dims[k]))
+                bd.options = opts
                 keep.append([mask])
                 ctx.check(L.pd_env_add_bc(ctx.h, env_id, ctypes.byref(bd)))
             ctx.check(L.pd_env_commit(ctx.h, env_id))

@@ -154,8 +154,9 @@ struct HostStage {
   }
 };
 // staged bytes per particle in a commit (x, y, v, block, BC, type, 1/density,
-// degree, row start, reference index, FP32C x32, aux32 and m), and the padding
-constexpr size_t kStagePerParticle = 3 * 32 + 4 + 4 + 1 + 8 + 4 + 4 + 4 + 16 + 16 + 8;
+// degree, row start, reference index, FP32C x32, aux32 and m, minimize!'s held
+// flag), and the padding
+constexpr size_t kStagePerParticle = 3 * 32 + 4 + 4 + 1 + 8 + 4 + 4 + 4 + 16 + 16 + 8 + 1;
 constexpr size_t kStagePad = 16 * 256;
 
 struct HostBlock {
@@ -922,10 +923,12 @@ void commit_env(Ctx* c, Env& e) {
       inv_den[p] = 1.0 / den;
     }
   }
+  bool any_onlyatstart = false;
   for (size_t k = 0; k < e.bcs.size(); ++k) {
     if ((int64_t)e.bcs[k].mask.size() != N) throw PdError(PD_E_ARG, "BC mask length must equal env particles");
     for (int64_t r = 0; r < N; ++r)
       if (e.bcs[k].mask[r]) bcid[internal_of(e, r)] = (int32_t)k;
+    any_onlyatstart = any_onlyatstart || (e.bcs[k].d.options & PD_BC_ONLYATSTART);
   }
   EnvDev& D = e.dev;
   std::memset(&D, 0, sizeof(D));
@@ -959,6 +962,17 @@ void commit_env(Ctx* c, Env& e) {
   PD_HIP(hipMemcpyAsync(D.ptype, ptype.data(), N, hipMemcpyHostToDevice, s));
   PD_HIP(hipMemcpyAsync(D.inv_den, inv_den.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
   PD_HIP(hipMemcpyAsync(D.bcid, bcid.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+  if (any_onlyatstart) {
+    // minimize!'s held particles: those of any BC without onlyatstart (minimize.jl:138-143)
+    auto held = hs.take<uint8_t>(N);
+    std::fill(held.p, held.p + N, (uint8_t)0);
+    for (const HostBc& hb : e.bcs)
+      if (!(hb.d.options & PD_BC_ONLYATSTART))
+        for (int64_t r = 0; r < N; ++r)
+          if (hb.mask[r]) held[internal_of(e, r)] = 1;
+    D.qheld = dalloc<uint8_t>(e, N);
+    PD_HIP(hipMemcpyAsync(D.qheld, held.data(), N, hipMemcpyHostToDevice, s));
+  }
   PD_HIP(hipMemsetAsync(D.theta, 0, sizeof(double) * N, s));
   PD_HIP(hipMemsetAsync(D.td, 0, sizeof(double) * N, s));
   PD_HIP(hipMemsetAsync(D.f, 0, sizeof(double4) * N, s));
@@ -2235,6 +2249,12 @@ int pd_env_add_bc(void* ctx, int64_t env, const pd_bc_desc* d) {
     if (d->kind != PD_BC_FIX && d->kind != PD_BC_TOFRO) throw PdError(PD_E_ARG, "unknown BC kind");
     if (d->kind == PD_BC_TOFRO && !std::isinf(d->freq) && (d->freq != std::floor(d->freq) || d->freq <= 0))
       throw PdError(PD_E_ARG, "ToFroBC freq must be a positive integer or Inf");
+    if (d->options & ~(PD_BC_ONLYATSTART | PD_BC_DIMS(7))) throw PdError(PD_E_ARG, "unknown BC options");
+    const int dims = (d->options >> 8) & 7;
+    if (dims != 0 && dims != 7)
+      throw PdError(PD_E_ARG,
+                    "BC dims must select all three dimensions: the reference assigns its 3 x n update into "
+                    "y[dims, bool] (boundary_conditions.jl:108-110), a DimensionMismatch for a subset");
     int64_t N = 0;
     for (auto& b : e.blocks) N += b.d.n;
     HostBc hb;

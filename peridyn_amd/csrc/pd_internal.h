@@ -116,6 +116,7 @@ struct EnvDev {
   uint8_t* ptype;            // 0-based type-local index
   double* inv_den;           // 1 / env density (mass/volume), contacts.jl:231,235
   int32_t* bcid;             // effective BC per particle (last in list order) or -1
+  uint8_t* qheld;            // minimize!'s held particles when some BC has onlyatstart, else null (bcid >= 0)
   int32_t* deg;              // degree per particle
   uint32_t* pslot0;          // slot of (p, k = 0); slot(p, k) = pslot0[p] + 64 k
   double* m;                 // weighted volume

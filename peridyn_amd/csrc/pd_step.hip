@@ -863,7 +863,7 @@ __global__ __launch_bounds__(256) void k_qs_update(const Chunk* __restrict__ chu
   if (!wave_chunk(chunks, nchunks, envs, w)) return;
   EnvDev& E = envs[w.ch.env];
   unsigned long long mf = 0, mx = 0;
-  if (w.active && w.g == 0 && E.bcid[w.p] < 0) {
+  if (w.active && w.g == 0 && !(E.qheld ? E.qheld[w.p] != 0 : E.bcid[w.p] >= 0)) {
     const int p = w.p;
     const double4 f = E.f[p];
     double4 o = opt[p];

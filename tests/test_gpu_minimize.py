@@ -87,3 +87,38 @@ def test_minimize_converges_at_the_oracle_iteration(core, orc, prec):
         assert gres == ores and np.array_equal(y, o.y)
     else:
         assert np.abs(y - o.y).max() <= 1e-12
+
+
+def test_minimize_onlyatstart_bc_against_oracle(core, orc):
+    """An onlyatstart grip (minimize.jl:138-143): applied once, then relaxed
+    with the free particles; reference-order form bit-identical to the oracle
+    (positions, forces, iterations), and different from the held grip."""
+    spec = pulled_block("bb")
+    spec["bcs"][1]["onlyatstart"] = True
+    o = orc.OracleEnv(spec)
+    o.run(20, 1)
+    ores = o.minimize(1e-5, 100.0, max_iter=40, x_tol=1e-12, f_tol=1e-6)
+    g = core.Env.from_spec(spec, precision="exact")
+    g.run(20, 1)
+    gres = g.minimize(1e-5, 100.0, max_iter=40, x_tol=1e-12, f_tol=1e-6)
+    assert gres == ores
+    y, v, f = g.download()
+    assert np.array_equal(y, o.y) and np.array_equal(f, o.f)
+    held = core.Env.from_spec(pulled_block("bb"), precision="exact")
+    held.run(20, 1)
+    held.minimize(1e-5, 100.0, max_iter=40, x_tol=1e-12, f_tol=1e-6)
+    top = spec["bcs"][1]["mask"]
+    assert not np.array_equal(held.download()[0][top], y[top])
+
+
+def test_bc_dims_subset_is_refused(core):
+    """dims selecting all three dimensions is accepted; a subset is refused with
+    PD_E_ARG, where the reference raises DimensionMismatch."""
+    from peridyn_amd import _lib
+    spec = pulled_block("bb")
+    spec["bcs"][0]["dims"] = [True, True, True]
+    core.Env.from_spec(spec, precision="fp64").close()
+    spec["bcs"][0]["dims"] = [False, False, True]
+    with pytest.raises(_lib.PeriDynError) as ei:
+        core.Env.from_spec(spec, precision="fp64")
+    assert ei.value.code == _lib.PD_E_ARG

@@ -327,6 +327,32 @@ def test_contact_exponent_value_selects_the_power(orc):
     assert np.isclose(forces[2.5][0, 2], (s25 * -0.7) * (1.0 / 2.0), rtol=1e-14, atol=0)
 
 
+def test_contact_general_power_within_one_ulp_of_exact(orc):
+    """The general Float64 power of non-integer exponents (nonlinearspring.jl:140):
+    Julia's `^(::Float64, ::Float64)` returns a value within one ulp of the exact
+    x^y; the oracle's power is checked against x^y evaluated in 60-digit decimal
+    arithmetic (exp(y ln x), rounded once to Float64) on strains across (0, 1]
+    and exponents 0.5 .. 4.75: every value is within one ulp (and most are the
+    correctly rounded one), so a wrong base, exponent or formula fails."""
+    import decimal
+    ctx = decimal.Context(prec=60)
+    rng = np.random.default_rng(17)
+    strains = np.concatenate([rng.uniform(1e-6, 1.0, 400), [1.0, 0.5, 0.1, 1e-3, 0.999999]])
+    exact_hits = total = 0
+    for e in (0.5, 0.75, 1.5, 2.25, 2.5, 3.3, 4.75):
+        assert not orc._julia_int_pow(e)
+        for x in strains:
+            got = orc.contact_power(float(x), e)
+            ex = ctx.exp(ctx.multiply(decimal.Decimal(e), ctx.ln(decimal.Decimal(float(x)))))
+            want = float(ex)  # correctly rounded to Float64
+            assert abs(got - want) <= np.spacing(want), (x, e, got, want)
+            exact_hits += got == want
+            total += 1
+    assert exact_hits >= 0.99 * total
+    # the integer-valued exponents take Julia's power by squaring instead
+    assert orc.contact_power(0.3, 2.0) == orc.julia_pow_int(0.3, 2)
+
+
 def test_contact_search_matches_brute_force(orc):
     rng = np.random.default_rng(5)
     x1 = rng.uniform(0, 6, size=(300, 3))
@@ -452,6 +478,47 @@ def test_qsdrag_bc_particles_move_once(orc):
     o.minimize(1e-3, 10.0, max_iter=5)
     top = spec["bcs"][1]["mask"]
     assert np.allclose(o.y[top] - y0[top], [0.0, 0.0, 1e-3 * 10.0], rtol=0, atol=1e-15)
+
+
+def test_qsdrag_onlyatstart_bc_particles_are_free(orc):
+    """minimize.jl:138-143 masks only the particles of BCs without onlyatstart:
+    an onlyatstart grip is applied once (moved by dt * rate) and then relaxes
+    with the free particles.  One iteration from rest (opt = 0) moves a free
+    particle by clip(0.5 f dt^2, ps/10), f the force at the BC-moved positions."""
+    dt = 1e-3
+    base = W.tensile_bb(n=8, dt=dt, pull_per_step=0.01)
+    top = base["bcs"][1]["mask"]
+    for only in (False, True):
+        spec = W.tensile_bb(n=8, dt=dt, pull_per_step=0.01)
+        spec["bcs"][1]["onlyatstart"] = only
+        o = orc.OracleEnv(spec)
+        o.apply_bc_at0()
+        y0 = o.y.copy()
+        moved = orc.OracleEnv(spec)
+        moved.apply_bc_at0()
+        for k, sgn in ((0, -1.0), (1, 1.0)):  # both grips applied once
+            m = spec["bcs"][k]["mask"]
+            moved.y[m] = y0[m] + np.array([0.0, 0.0, sgn * dt * 10.0])
+        moved.update_acc()
+        o.minimize(dt, 10.0, max_iter=1, x_tol=0.0, f_tol=0.0)
+        want = y0[top] + np.array([0.0, 0.0, dt * 10.0])
+        if only:
+            a = o.blocks[0].particle_size / 10.0
+            want = want + np.clip(0.5 * moved.f[top] * (dt * dt), -a, a)
+            assert np.abs(moved.f[top]).max() > 0
+        assert np.array_equal(o.y[top], want), only
+
+
+def test_bc_dims_subset_is_a_dimension_mismatch(orc):
+    """A BC on a subset of dimensions cannot take the reference's 3 x n update
+    into y[dims, bool] (boundary_conditions.jl:108-110): the oracle refuses it
+    as the reference raises DimensionMismatch; all three dims are accepted."""
+    spec = W.tensile_bb(n=6)
+    spec["bcs"][0]["dims"] = [True, True, True]
+    orc.OracleEnv(spec)
+    spec["bcs"][0]["dims"] = [False, False, True]
+    with pytest.raises(ValueError):
+        orc.OracleEnv(spec)
 
 
 def _derived_rm11(y, fam, intact, equi):
