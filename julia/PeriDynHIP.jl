@@ -36,6 +36,7 @@ const LIB = joinpath(@__DIR__, "..", "peridyn_amd", "libperidyn_hip.so")
 const PD_BB, PD_OSB, PD_EP = Int32(0), Int32(1), Int32(2)
 const PD_PREC_EXACT, PD_PREC_FP64, PD_PREC_FP32C = Int32(0), Int32(1), Int32(2)
 const PD_BC_FIX, PD_BC_TOFRO = Int32(0), Int32(1)
+const PD_BC_ONLYATSTART = 1
 
 struct BlockDesc
     kind::Int32; ntype::Int32; n::Int64; type_start::Int64
@@ -53,7 +54,7 @@ struct ContactDesc
 end
 
 struct BcDesc
-    kind::Int32; reserved::Int32; mask::Ptr{UInt8}; rate::NTuple{3,Float64}
+    kind::Int32; options::Int32; mask::Ptr{UInt8}; rate::NTuple{3,Float64}
     freq::Float64; applyafter::Int64
 end
 
@@ -303,11 +304,16 @@ function upload_env(env; precision=PD_PREC_FP64)
     for bc in env.boundary_conditions
         mask = UInt8.(Array(bc.bool))
         push!(keep, mask)
+        # onlyatstart and dims of the BC constructors (FixBC.jl:46, ToFroBC.jl:51); the
+        # library refuses a dims subset, which the reference's update cannot take either
+        dims = Array(bc.dims)
+        opts = Int32((bc.onlyatstart ? PD_BC_ONLYATSTART : 0) |
+                     ((Int(dims[1] != 0) | (Int(dims[2] != 0) << 1) | (Int(dims[3] != 0) << 2)) << 8))
         if bc isa PeriDyn.FixBC
-            bd = BcDesc(PD_BC_FIX, 0, pointer(mask), (0.0, 0.0, 0.0), Inf, 0)
+            bd = BcDesc(PD_BC_FIX, opts, pointer(mask), (0.0, 0.0, 0.0), Inf, 0)
         else
             r = Array(bc.rate)
-            bd = BcDesc(PD_BC_TOFRO, 0, pointer(mask), (r[1], r[2], r[3]), Float64(bc.freq), bc.applyafter)
+            bd = BcDesc(PD_BC_TOFRO, opts, pointer(mask), (r[1], r[2], r[3]), Float64(bc.freq), bc.applyafter)
         end
         GC.@preserve mask check(ccall((:pd_env_add_bc, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{BcDesc}),
                                       ctx.h, id[], bd), ctx)
