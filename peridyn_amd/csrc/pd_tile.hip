@@ -24,9 +24,12 @@ enum { TP_THETA = 0, TP_TD = 1, TP_FORCE = 2 };
 // LDS halo records of the FP64 passes: one array per field.  Double fields in
 // order: x0 x1 x2 vol y0 y1 y2 | theta 1/m | Cs, where Cs = C with the
 // neighbour's "has yielded" flag as its sign (C >= 0).  With XF (every x and
-// volume of the set's envs is exactly an FP32 value) x0 x1 x2 vol go to a float
-// region after the doubles instead -- the values read back are the same
-// doubles -- which shrinks the records and lets more tiles reside per CU.
+// volume of the set's envs is exactly an FP32 value) x0 x1 x2 vol go to two
+// 8-byte fields after the doubles instead, (x0, x1) and (x2, vol) as float
+// pairs -- the values read back are the same doubles -- which shrinks the
+// records and lets more tiles reside per CU; every field of record j then sits
+// at j * 8 plus a constant.  In an integer-lattice block the first pair holds
+// the lattice coordinates instead (lat_pack).
 template <int PASS, int KIND>
 __host__ __device__ constexpr int tile_extra() {  // double fields beyond x, vol, y
   return PASS == TP_THETA ? 0 : PASS == TP_TD ? 2 : (KIND == PD_BB ? 0 : KIND == PD_OSB ? 2 : 3);
@@ -41,16 +44,40 @@ struct Rec {
   const double* sh;
   int hcap;
   __device__ __forceinline__ Rec(const double* s, int h) : sh(s), hcap(h) {}
-  __device__ __forceinline__ const float* fs() const { return (const float*)(sh + (3 + NX) * hcap); }
-  __device__ __forceinline__ double x(int k, int j) const {
-    return XF ? (double)fs()[k * hcap + j] : sh[k * hcap + j];
-  }
-  __device__ __forceinline__ double vol(int j) const { return XF ? (double)fs()[3 * hcap + j] : sh[3 * hcap + j]; }
+  __device__ __forceinline__ const float2* xa() const { return (const float2*)(sh + (3 + NX) * hcap); }
+  __device__ __forceinline__ const float2* xb() const { return (const float2*)(sh + (4 + NX) * hcap); }
+  __device__ __forceinline__ uint2 lat(int j) const { return ((const uint2*)xa())[j]; }
+  __device__ __forceinline__ double vol(int j) const { return XF ? (double)xb()[j].y : sh[3 * hcap + j]; }
   __device__ __forceinline__ double y(int k, int j) const { return sh[((XF ? 0 : 4) + k) * hcap + j]; }
   __device__ __forceinline__ double th(int j) const { return sh[(XF ? 3 : 7) * hcap + j]; }
   __device__ __forceinline__ double rm(int j) const { return sh[(XF ? 4 : 8) * hcap + j]; }
   __device__ __forceinline__ double cs(int j) const { return sh[(XF ? 5 : 9) * hcap + j]; }
+  // reference position of record j
+  __device__ __forceinline__ void x(int j, double& a, double& b, double& c) const {
+    if (XF) {
+      const float2 p = xa()[j];
+      a = p.x; b = p.y; c = xb()[j].x;
+    } else {
+      a = sh[j]; b = sh[hcap + j]; c = sh[2 * hcap + j];
+    }
+  }
 };
+
+// Lattice coordinates of an integer-lattice position, scaled by 4 and packed as
+// 16-bit two's-complement pairs (x0, x1) and (x2, 0), each modulo 2^16: the
+// packed difference of two such words is 4 (X0, X1) and 4 (X2, 0) exactly when
+// |4 X| < 2^15, and the dot products give 16 |X|^2, the byte offset of |X|^2's
+// entry in a table of 16-byte records.
+__device__ __forceinline__ uint2 lat_pack(double x0, double x1, double x2) {
+  const uint32_t a = (uint32_t)(4 * (int32_t)x0) & 0xffffu, b = (uint32_t)(4 * (int32_t)x1);
+  return make_uint2(a | (b << 16), (uint32_t)(4 * (int32_t)x2) & 0xffffu);
+}
+typedef short pd_short2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int lat_offset(uint2 a, uint2 b) {
+  const pd_short2 d01 = __builtin_bit_cast(pd_short2, a.x) - __builtin_bit_cast(pd_short2, b.x);
+  const pd_short2 d2 = __builtin_bit_cast(pd_short2, a.y) - __builtin_bit_cast(pd_short2, b.y);
+  return __builtin_amdgcn_sdot2(d01, d01, __builtin_amdgcn_sdot2(d2, d2, 0, false), false);
+}
 
 struct TileWave {
   Chunk ch;
@@ -64,12 +91,14 @@ struct TileWave {
 constexpr int kStageR = 4;
 
 template <bool XF, int NX>
-__device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap) {
+__device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap,
+                                           bool lat) {
   const double4* __restrict__ x = E.x;
   const double4* __restrict__ y = E.y[E.cur];
   const int32_t* __restrict__ halo = E.halo;
   constexpr int Y0 = XF ? 0 : 4, T0 = XF ? 3 : 7;
-  float* __restrict__ fs = (float*)(sh + (3 + NX) * hcap);
+  float2* __restrict__ xa = (float2*)(sh + (3 + NX) * hcap);
+  float2* __restrict__ xb = (float2*)(sh + (4 + NX) * hcap);
   for (int i0 = threadIdx.x; i0 < T.hn; i0 += 256 * kStageR) {
     uint32_t id[kStageR];
 #pragma unroll
@@ -91,10 +120,9 @@ __device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, doubl
       const int i = i0 + 256 * r;
       if (i < T.hn) {
         if (XF) {
-          fs[i] = (float)a[r].x;
-          fs[hcap + i] = (float)a[r].y;
-          fs[2 * hcap + i] = (float)a[r].z;
-          fs[3 * hcap + i] = (float)a[r].w;
+          if (lat) ((uint2*)xa)[i] = lat_pack(a[r].x, a[r].y, a[r].z);
+          else xa[i] = make_float2((float)a[r].x, (float)a[r].y);
+          xb[i] = make_float2((float)a[r].z, (float)a[r].w);
         } else {
           sh[i] = a[r].x;
           sh[hcap + i] = a[r].y;
@@ -229,28 +257,42 @@ __device__ __forceinline__ void fill_xtab64(double2* __restrict__ t) {
 // The owner's reference position and where |X| comes from (FP64 kernels)
 struct XGeo {
   double4 xp;
-  float f0, f1, f2;  // xp in FP32 (exact when the table is used)
-  bool lat;          // the integer-lattice table applies
+  uint2 lp;  // the owner's lat_pack (when the table is used)
+  bool lat;  // the integer-lattice table applies (uniform over the workgroup)
   const double2* tab;
   __device__ __forceinline__ XGeo(const double4& x, bool l, const double2* t)
-      : xp(x), f0((float)x.x), f1((float)x.y), f2((float)x.z), lat(l), tab(t) {}
-  // |X| (xij) and 1/|X| of the bond to staged record j
-  template <class R>
+      : xp(x), lp(l ? lat_pack(x.x, x.y, x.z) : make_uint2(0u, 0u)), lat(l), tab(t) {}
+  // |X| (xij) and 1/|X| of the bond to staged record j; LAT = lat, a
+  // compile-time copy so that the bond loops carry no branch on it
+  template <bool LAT, class R>
   __device__ __forceinline__ void xlen(const R& rc, int j, double& xij, double& inv_x) const {
-    if (lat) {
-      const float* __restrict__ fs = rc.fs();
-      const float X1 = fs[j] - f0, X2 = fs[rc.hcap + j] - f1, X3 = fs[2 * rc.hcap + j] - f2;
-      const double2 v = tab[(int)__builtin_fmaf(X3, X3, __builtin_fmaf(X2, X2, X1 * X1))];
+    if constexpr (LAT) {
+      const double2 v = *(const double2*)((const char*)tab + lat_offset(rc.lat(j), lp));
       xij = v.x;
       inv_x = v.y;
     } else {
-      const double X1 = rc.x(0, j) - xp.x, X2 = rc.x(1, j) - xp.y, X3 = rc.x(2, j) - xp.z;
+      double a, b, c;
+      rc.x(j, a, b, c);
+      const double X1 = a - xp.x, X2 = b - xp.y, X3 = c - xp.z;
       const double xx2 = pd_norm2(X1, X2, X3);
       inv_x = pd_rsqrt(xx2);
       xij = xx2 * inv_x;
     }
   }
 };
+
+// Runs f(std::integral_constant<bool, lat>) -- a bond loop specialised for the
+// lattice table or the general |X| -- where lat is uniform over the workgroup.
+template <bool XF, class F>
+__device__ __forceinline__ void with_lat(bool lat, F&& f) {
+  if constexpr (XF) {
+    if (lat) {
+      f(std::true_type{});
+      return;
+    }
+  }
+  f(std::false_type{});
+}
 
 // ---------------------------------------------------------------------------
 // dilatation theta (src/peri.jl:155-172), OSB and EP blocks
@@ -268,7 +310,8 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
   if (B.kind == PD_BB || !B.deformed) return;
   __shared__ double2 xtab[XF ? kXTab : 1];
   if (XF) fill_xtab64(xtab);
-  stage_halo<XF, 0>(E, T, sh, hs);
+  const bool lat = XF && B.xint;
+  stage_halo<XF, 0>(E, T, sh, hs, lat);
   __syncthreads();
   TileWave w;
   if (!tile_wave<LG>(T, w)) return;
@@ -276,23 +319,25 @@ __global__ __launch_bounds__(256) void k_tile_theta(const Tile* __restrict__ til
   const uint64_t* __restrict__ pk = E.pk;
   const int p = w.active ? w.p : w.ch.first;
   const double4 xp = gld4(E.x, p), yp = gld4(E.y[E.cur], p);
-  const XGeo xg(xp, XF && B.xint, xtab);
+  const XGeo xg(xp, lat, xtab);
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   double acc = 0.0;
-  pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
+  with_lat<XF>(lat, [&](auto LT) {
+    pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
-      if (!(e & 0x8000u)) continue;
-      const int j = (int)(e & 0x7fffu);
-      double Y1, Y2, Y3, xij, inv_x;
-      tile_y(rc, j, yp, Y1, Y2, Y3);
-      xg.xlen(rc, j, xij, inv_x);
-      const double yy2 = pd_norm2(Y1, Y2, Y3);
-      const double ext = yy2 * pd_rsqrt(yy2) - xij;
-      acc = __builtin_fma(ext, rc.vol(j), acc);
-    }
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+        if (!(e & 0x8000u)) continue;
+        const int j = (int)(e & 0x7fffu);
+        double Y1, Y2, Y3, xij, inv_x;
+        tile_y(rc, j, yp, Y1, Y2, Y3);
+        xg.template xlen<decltype(LT)::value>(rc, j, xij, inv_x);
+        const double yy2 = pd_norm2(Y1, Y2, Y3);
+        const double ext = yy2 * pd_rsqrt(yy2) - xij;
+        acc = __builtin_fma(ext, rc.vol(j), acc);
+      }
+    });
   });
   acc = group_sum(acc, C);
   if (w.active && w.g == 0) {
@@ -319,7 +364,8 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
   if (B.kind != PD_EP || !B.deformed) return;
   __shared__ double2 xtab[XF ? kXTab : 1];
   if (XF) fill_xtab64(xtab);
-  stage_halo<XF, 2>(E, T, sh, hs);
+  const bool lat = XF && B.xint;
+  stage_halo<XF, 2>(E, T, sh, hs, lat);
   __syncthreads();
   TileWave w;
   if (!tile_wave<LG>(T, w)) return;
@@ -331,7 +377,7 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
   double* __restrict__ edp_o = E.edp[E.edp_cur ^ 1];
   const int p = w.active ? w.p : w.ch.first;
   const double4 xp = gld4(E.x, p), yp = gld4(E.y[E.cur], p);
-  const XGeo xg(xp, XF && B.xint, xtab);
+  const XGeo xg(xp, lat, xtab);
   const double4 ap = E.aux[p];  // (theta, 1/m, +-td_norm2 of the last step, C)
   const bool yielded = pd_has_yielded(ap);
   const double s0 = B.s0[0];
@@ -348,42 +394,44 @@ __global__ __launch_bounds__(256) void k_tile_td(const Tile* __restrict__ tiles,
       eo[u] = (on && yielded) ? gld(edp, base + 64u * (uint32_t)(4 * g + u)) : 0.0;
     }
   };
-  PkStream<3> ps(pk, wb, nt4);
-  uint64_t cn = ps.next(0);
-  double en[4];
-  load_eo(cn, 0, en);
-  for (int t4 = 0; t4 < nt4; ++t4) {
-    const uint64_t cw = cn;
-    double eo[4];
+  with_lat<XF>(lat, [&](auto LT) {
+    PkStream<3> ps(pk, wb, nt4);
+    uint64_t cn = ps.next(0);
+    double en[4];
+    load_eo(cn, 0, en);
+    for (int t4 = 0; t4 < nt4; ++t4) {
+      const uint64_t cw = cn;
+      double eo[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) eo[u] = en[u];
-    cn = ps.next(t4 + 1);
-    load_eo(cn, t4 + 1, en);
+      for (int u = 0; u < 4; ++u) eo[u] = en[u];
+      cn = ps.next(t4 + 1);
+      load_eo(cn, t4 + 1, en);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
-      if (!(e & 0x8000u)) continue;
-      const int j = (int)(e & 0x7fffu);
-      const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
-      double Y1, Y2, Y3, xij, inv_x;
-      tile_y(rc, j, yp, Y1, Y2, Y3);
-      xg.xlen(rc, j, xij, inv_x);
-      const double yy2 = pd_norm2(Y1, Y2, Y3);
-      const double ext = yy2 * pd_rsqrt(yy2) - xij;
-      if (ext * inv_x < s0) {
-        const double th_j = rc.th(j), rm_j = rc.rm(j);
-        const double sm = ap.y + rm_j;
-        const double twm = ap.x * ap.y + th_j * rm_j;
-        const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
-        const double td = g15 * (ed - eo[u]) * (inv_x * sm);
-        acc = __builtin_fma(td * td, rc.vol(j), acc);
-      } else {
-        gst(st, slot, (uint8_t)0);
-        gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
-        if (yielded) gst(edp_o, slot, eo[u]);  // else 0 in both buffers already
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+        if (!(e & 0x8000u)) continue;
+        const int j = (int)(e & 0x7fffu);
+        const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
+        double Y1, Y2, Y3, xij, inv_x;
+        tile_y(rc, j, yp, Y1, Y2, Y3);
+        xg.template xlen<decltype(LT)::value>(rc, j, xij, inv_x);
+        const double yy2 = pd_norm2(Y1, Y2, Y3);
+        const double ext = yy2 * pd_rsqrt(yy2) - xij;
+        if (ext * inv_x < s0) {
+          const double th_j = rc.th(j), rm_j = rc.rm(j);
+          const double sm = ap.y + rm_j;
+          const double twm = ap.x * ap.y + th_j * rm_j;
+          const double ed = ext - (twm * pd_rcp(sm)) * xij * (1.0 / 3.0);
+          const double td = g15 * (ed - eo[u]) * (inv_x * sm);
+          acc = __builtin_fma(td * td, rc.vol(j), acc);
+        } else {
+          gst(st, slot, (uint8_t)0);
+          gst(pk16, 4u * (wb + 64u * (uint32_t)t4) + (uint32_t)u, (uint16_t)(e & 0x7fffu));
+          if (yielded) gst(edp_o, slot, eo[u]);  // else 0 in both buffers already
+        }
       }
     }
-  }
+  });
   acc = group_sum(acc, C);
   if (w.active && w.g == 0) {
     const double Cp = __builtin_sqrt((2.0 * B.psi[0]) / acc);
@@ -431,7 +479,7 @@ __device__ __forceinline__ void ep_stage_b(EpGrp& G, const double* __restrict__ 
   for (int u = 0; u < 4; ++u) G.er[u] = ((G.need >> u) & 1u) ? gld(edp_in, G.rv[u]) : 0.0;
 }
 
-template <class R>
+template <bool LAT, class R>
 __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, uint32_t wb, const XGeo& xg,
                                          const double4& yp, const double4& ap, bool yielded_p, double s0_pp,
                                          double K0, double G0, double inv_den, const R& rc,
@@ -445,7 +493,7 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
     const uint32_t slot = base + 64u * (uint32_t)(4 * t4 + u);
     double Y1, Y2, Y3, xij, inv_x;
     tile_y(rc, j, yp, Y1, Y2, Y3);
-    xg.xlen(rc, j, xij, inv_x);
+    xg.template xlen<LAT>(rc, j, xij, inv_x);
     const double yy2 = pd_norm2(Y1, Y2, Y3);
     const double inv_y = pd_rsqrt(yy2);
     const double yij = yy2 * inv_y;
@@ -507,13 +555,14 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
   __shared__ double2 xtab[XF ? kXTab : 1];
   if (XF) fill_xtab64(xtab);
-  if (deformed) stage_halo<XF, NX>(E, T, sh, hs);
+  const bool lat = XF && B.xint;
+  if (deformed) stage_halo<XF, NX>(E, T, sh, hs, lat);
   __syncthreads();
   if (!has) return;
   const Rec<XF, NX> rc(sh, hs);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 xp = gld4(E.x, p), yp = gld4(y, p);
-  const XGeo xg(xp, XF && B.xint, xtab);
+  const XGeo xg(xp, lat, xtab);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
   if (deformed) {
     const uint64_t* __restrict__ pk = E.pk;
@@ -529,68 +578,72 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
       const double* __restrict__ edp_in = E.edp[E.edp_cur];
       double* __restrict__ edp_out = E.edp[E.edp_cur ^ 1];
       const bool yielded_p = pd_has_yielded(ap);
-      PkStream<2> ps(pk, wb, nt4);
-      EpGrp g0, g1, g2;
-      ep_stage_a(g0, ps.next(0), base, 0, yielded_p, edp_in, E.rev, rc);
-      ep_stage_b(g0, edp_in);
-      ep_stage_a(g1, ps.next(1), base, 1, yielded_p, edp_in, E.rev, rc);
-      for (int t4 = 0; t4 < nt4; ++t4) {
-        ep_stage_b(g1, edp_in);
-        ep_stage_a(g2, ps.next(t4 + 2), base, t4 + 2, yielded_p, edp_in, E.rev, rc);
-        ep_group(g0, t4, base, wb, xg, yp, ap, yielded_p, s0_pp, K0, G0, inv_den, rc, edp_out, st, pk16, a1, a2,
-                 a3);
-        g0 = g1;
-        g1 = g2;
-      }
+      with_lat<XF>(lat, [&](auto LT) {
+        PkStream<2> ps(pk, wb, nt4);
+        EpGrp g0, g1, g2;
+        ep_stage_a(g0, ps.next(0), base, 0, yielded_p, edp_in, E.rev, rc);
+        ep_stage_b(g0, edp_in);
+        ep_stage_a(g1, ps.next(1), base, 1, yielded_p, edp_in, E.rev, rc);
+        for (int t4 = 0; t4 < nt4; ++t4) {
+          ep_stage_b(g1, edp_in);
+          ep_stage_a(g2, ps.next(t4 + 2), base, t4 + 2, yielded_p, edp_in, E.rev, rc);
+          ep_group<decltype(LT)::value>(g0, t4, base, wb, xg, yp, ap, yielded_p, s0_pp, K0, G0, inv_den, rc,
+                                        edp_out, st, pk16, a1, a2, a3);
+          g0 = g1;
+          g1 = g2;
+        }
+      });
     } else {
       // bond-based and ordinary state-based blocks
       double4 ap = make_double4(0.0, 0.0, 0.0, 0.0);
       if (KIND != PD_BB) ap = E.aux[p];
       const double c11 = B.c11;
       bool brk = false;  // this lane broke a bond
-      pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
+      with_lat<XF>(lat, [&](auto LT) {
+        pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
-          if (!(e & 0x8000u)) continue;
-          const int j = (int)(e & 0x7fffu);
-          double Y1, Y2, Y3, xij, inv_x;
-          tile_y(rc, j, yp, Y1, Y2, Y3);
-          xg.xlen(rc, j, xij, inv_x);
-          const double yy2 = pd_norm2(Y1, Y2, Y3);
-          const double inv_y = pd_rsqrt(yy2);
-          const double yij = yy2 * inv_y;
-          const double ext = yij - xij;
-          if (ext * inv_x < s0_pp) {
-            double tt;
-            if (KIND == PD_BB) {
-              tt = ext * inv_x * c11;
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
+            if (!(e & 0x8000u)) continue;
+            const int j = (int)(e & 0x7fffu);
+            double Y1, Y2, Y3, xij, inv_x;
+            tile_y(rc, j, yp, Y1, Y2, Y3);
+            xg.template xlen<decltype(LT)::value>(rc, j, xij, inv_x);
+            const double yy2 = pd_norm2(Y1, Y2, Y3);
+            const double inv_y = pd_rsqrt(yy2);
+            const double yij = yy2 * inv_y;
+            const double ext = yij - xij;
+            if (ext * inv_x < s0_pp) {
+              double tt;
+              if (KIND == PD_BB) {
+                tt = ext * inv_x * c11;
+              } else {
+                const double th_j = rc.th(j), rm_j = rc.rm(j);
+                const double sm = ap.y + rm_j;
+                const double twm = ap.x * ap.y + th_j * rm_j;
+                tt = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
+              }
+              const double xx = tt * inv_y * (rc.vol(j) * inv_den);
+              a1 = __builtin_fma(xx, Y1, a1);
+              a2 = __builtin_fma(xx, Y2, a2);
+              a3 = __builtin_fma(xx, Y3, a3);
             } else {
-              const double th_j = rc.th(j), rm_j = rc.rm(j);
-              const double sm = ap.y + rm_j;
-              const double twm = ap.x * ap.y + th_j * rm_j;
-              tt = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
+              brk = true;
             }
-            const double xx = tt * inv_y * (rc.vol(j) * inv_den);
-            a1 = __builtin_fma(xx, Y1, a1);
-            a2 = __builtin_fma(xx, Y2, a2);
-            a3 = __builtin_fma(xx, Y3, a3);
-          } else {
-            brk = true;
           }
-        }
-      });
-      if (brk)
-        pk_record_breaks(pk, pk16, st, wb, base, nt4, [&](int j) {
-          double Y1, Y2, Y3, xij, inv_x;
-          tile_y(rc, j, yp, Y1, Y2, Y3);
-          xg.xlen(rc, j, xij, inv_x);
-          const double yy2 = pd_norm2(Y1, Y2, Y3);
-          const double inv_y = pd_rsqrt(yy2);
-          const double yij = yy2 * inv_y;
-          const double ext = yij - xij;
-          return !(ext * inv_x < s0_pp);
         });
+        if (brk)
+          pk_record_breaks(pk, pk16, st, wb, base, nt4, [&](int j) {
+            double Y1, Y2, Y3, xij, inv_x;
+            tile_y(rc, j, yp, Y1, Y2, Y3);
+            xg.template xlen<decltype(LT)::value>(rc, j, xij, inv_x);
+            const double yy2 = pd_norm2(Y1, Y2, Y3);
+            const double inv_y = pd_rsqrt(yy2);
+            const double yij = yy2 * inv_y;
+            const double ext = yij - xij;
+            return !(ext * inv_x < s0_pp);
+          });
+      });
     }
     a1 = group_sum(a1, C);
     a2 = group_sum(a2, C);

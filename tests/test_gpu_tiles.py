@@ -324,3 +324,38 @@ def test_halo_stride_classes_equal_untiled(pd, kind, horizon, stride):
     same_state(a, b)
     for e in (a, b):
         e.close()
+
+
+def moved(spec, shift=(0.0, 0.0, 0.0), scale=1.0):
+    for b in spec["blocks"]:
+        x = np.asarray(b["x"], dtype=np.float64) * scale + np.asarray(shift)
+        b["x"] = np.ascontiguousarray(x)
+        if scale != 1.0:
+            b["vol"] = np.asarray(b["vol"], dtype=np.float64) * scale ** 3
+            b["horizon"] = float(b["horizon"]) * scale
+    return spec
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+@pytest.mark.parametrize("case", ["far", "negative", "half"])
+def test_tiled_lattice_positions_equal_untiled(pd, kind, case):
+    """Compact records with and without the integer-lattice table of |X|: an
+    integer lattice far from the origin (its coordinates wrap the table's 16-bit
+    lattice words), one at negative coordinates, and a half-integer lattice (FP32
+    values, general |X|).  Each equals the untiled kernels bit for bit."""
+    say("lattice positions", kind, case)
+    spec = ep_tensile() if kind == "ep" else W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.002)
+    if case == "far":
+        spec = moved(spec, shift=(40000.0, -20011.0, 123457.0))
+    elif case == "negative":
+        spec = moved(spec, shift=(-37.0, -8.0, -1000.0))
+    else:
+        spec = moved(spec, scale=0.5)
+    a = make(pd, spec, True, 4)
+    assert a.layout()["compact_records"]
+    b = make(pd, spec, False, 4)
+    a.run(40, 1)
+    b.run(40, 1)
+    same_state(a, b)
+    a.close()
+    b.close()
