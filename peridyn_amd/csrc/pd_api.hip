@@ -201,6 +201,9 @@ struct Env {
   // built (tiled: the halos fit the LDS budget); brick of each internal particle
   bool want_tiles = false, tiled = false, xf32 = false;
   std::vector<int32_t> brick;
+  // per brick: row (the brick's y-z position within its block) and column (x)
+  std::vector<int64_t> brick_row;
+  std::vector<int32_t> brick_col;
   std::vector<Tile> tiles;
   Tile* d_tiles = nullptr;                       // device copy (run sets copy from it)
   std::vector<std::array<int, 3>> tile_blocks;   // (block, first tile, count), tiles in block order
@@ -527,6 +530,8 @@ void build_brick_order(Env& e) {
   e.perm.resize(e.N);
   e.iperm.resize(e.N);
   e.brick.assign(e.N, 0);
+  e.brick_row.clear();
+  e.brick_col.clear();
   int bx, by, bz;
   brick_dims(e.lgG, bx, by, bz);
   int32_t next_brick = 0;
@@ -589,6 +594,10 @@ void build_brick_order(Env& e) {
       const int32_t i = order[k];
       const int64_t br = brk[i];
       if (k > 0 && br != prev) ++next_brick;
+      if (k == 0 || br != prev) {
+        e.brick_row.push_back(br / nb[0]);
+        e.brick_col.push_back((int32_t)(br % nb[0]));
+      }
       prev = br;
       e.perm[b.first + k] = b.first + i;
       e.iperm[b.first + i] = (int32_t)(b.first + k);
@@ -602,6 +611,8 @@ void build_order(Env& e) {
   e.perm.clear();
   e.iperm.clear();
   e.brick.clear();
+  e.brick_row.clear();
+  e.brick_col.clear();
   if (e.precision == PD_PREC_EXACT) return;
   if (e.want_tiles) {
     build_brick_order(e);
@@ -652,7 +663,8 @@ constexpr size_t kTileLdsBudget = 80 * 1024;
 constexpr int kMergeTiles = 2 * 5 * 256;
 
 // Tiles of a brick-ordered env: consecutive chunks of one brick, up to
-// PD_TILE_CHUNKS per tile.  Each tile's halo is the sorted set of neighbour ids
+// PD_TILE_CHUNKS per tile, or of two partial bricks next to each other
+// (merge_partial_tiles).  Each tile's halo is the sorted set of neighbour ids
 // of its particles; pk receives every bond's halo-local index with the intact
 // bit (every bond is intact after the family build).  Leaves e.tiled false
 // (untiled passes) when a halo exceeds 15-bit indices or the LDS budget.
@@ -730,9 +742,111 @@ bool build_tiles_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& 
   return true;
 }
 
+// Halo sizes (distinct neighbour ids) of the tiles with chunk ranges tr, by
+// k_halo_count on the device.
+std::vector<int32_t> halo_counts_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& tr, int ncap) {
+  hipStream_t s = c->stream;
+  const int nt = (int)tr.size();
+  std::vector<Tile> tt(nt);
+  for (int t = 0; t < nt; ++t) {
+    Tile& T = tt[t];
+    std::memset(&T, 0, sizeof(T));
+    T.block = e.chunks[tr[t].first].block;
+    T.nch = tr[t].second;
+    for (int k = 0; k < T.nch; ++k) T.ch[k] = e.chunks[tr[t].first + k];
+  }
+  std::vector<int32_t> cnt(nt);
+  const size_t cls_t = DevMem::size_class(sizeof(Tile) * std::max(nt, 1));
+  const size_t cls_c = DevMem::size_class(sizeof(int32_t) * std::max(nt, 1));
+  Tile* dt = nullptr;
+  int32_t* dcnt = nullptr;
+  try {
+    dt = static_cast<Tile*>(c->mem.get(cls_t));
+    dcnt = static_cast<int32_t*>(c->mem.get(cls_c));
+    PD_HIP(hipMemcpyAsync(dt, tt.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
+    pd_launch_halo_count(s, dt, nt, e.dev.col, dcnt, std::max(2 * ncap, 512));
+    PD_HIP(hipGetLastError());
+    PD_HIP(hipMemcpyAsync(cnt.data(), dcnt, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    c->mem.put(dt, cls_t);
+    c->mem.put(dcnt, cls_c);
+    throw;
+  }
+  c->mem.put(dt, cls_t);
+  c->mem.put(dcnt, cls_c);
+  return cnt;
+}
+
+// The same sizes on the host, from the downloaded neighbour columns.
+std::vector<int32_t> halo_counts_host(const Env& e, const std::vector<int32_t>& col,
+                                      const std::vector<std::pair<int, int>>& tr) {
+  std::vector<int32_t> cnt(tr.size(), 0);
+  std::vector<int32_t> stamp((size_t)e.N, -1);
+  for (size_t t = 0; t < tr.size(); ++t)
+    for (int k = tr[t].first; k < tr[t].first + tr[t].second; ++k) {
+      const Chunk& ch = e.chunks[k];
+      const size_t nsl = (size_t)PD_WAVE * ch.width;
+      for (size_t i = 0; i < nsl; ++i) {
+        const int32_t j = col[ch.slot0 + i];
+        if (j >= 0 && stamp[j] != (int32_t)t) {
+          stamp[j] = (int32_t)t;
+          ++cnt[t];
+        }
+      }
+    }
+  return cnt;
+}
+
+// Partial bricks (fewer than PD_TILE_CHUNKS chunks: the last brick layer of a
+// block whose extent is not a multiple of the brick's) leave waves of their
+// workgroup idle while the workgroup holds a full LDS class.  Two such bricks
+// next to each other in x (consecutive in the brick order) become one tile
+// when their chunks fit one and the merged halo is no larger than the largest
+// halo of the env's unmerged tiles, so that the env's LDS class is unchanged.
+// counts(ranges) returns the halo sizes of chunk ranges.
+template <class F>
+void merge_partial_tiles(const Env& e, std::vector<std::pair<int, int>>& tr, int64_t maxent, F&& counts) {
+  auto ent = [&](int first, int n) {
+    int64_t v = 0;
+    for (int k = first; k < first + n; ++k) v += (int64_t)PD_WAVE * e.chunks[k].width;
+    return v;
+  };
+  std::vector<int> cand;  // t: merge tiles t and t+1
+  for (int t = 0; t + 1 < (int)tr.size(); ++t) {
+    const Chunk& a = e.chunks[tr[t].first];
+    const Chunk& b = e.chunks[tr[t + 1].first];
+    const int ba = e.brick[a.first], bb = e.brick[b.first];
+    if (a.block != b.block || tr[t].second + tr[t + 1].second > PD_TILE_CHUNKS) continue;
+    if (e.brick_row[ba] != e.brick_row[bb] || e.brick_col[bb] != e.brick_col[ba] + 1) continue;
+    if (ent(tr[t].first, tr[t].second + tr[t + 1].second) > maxent) continue;
+    cand.push_back(t);
+  }
+  if (cand.empty()) return;
+  std::vector<std::pair<int, int>> q(tr);
+  for (int t : cand) q.emplace_back(tr[t].first, tr[t].second + tr[t + 1].second);
+  const std::vector<int32_t> cnt = counts(q);
+  int32_t h0 = 0;
+  for (size_t t = 0; t < tr.size(); ++t) h0 = std::max(h0, cnt[t]);
+  std::vector<std::pair<int, int>> out;
+  out.reserve(tr.size());
+  size_t ci = 0;
+  for (int t = 0; t < (int)tr.size(); ++t) {
+    while (ci < cand.size() && cand[ci] < t) ++ci;
+    if (ci < cand.size() && cand[ci] == t && cnt[tr.size() + ci] <= h0) {
+      out.emplace_back(tr[t].first, tr[t].second + tr[t + 1].second);
+      ++t;
+    } else {
+      out.push_back(tr[t]);
+    }
+  }
+  tr.swap(out);
+}
+
 void build_tiles(Ctx* c, Env& e) {
   hipStream_t s = c->stream;
-  // tiles: chunk ranges
+  // tiles: chunk ranges, one brick each
   std::vector<std::pair<int, int>> tr;
   for (int k = 0; k < (int)e.chunks.size();) {
     const int br = e.brick[e.chunks[k].first];
@@ -743,7 +857,6 @@ void build_tiles(Ctx* c, Env& e) {
     tr.emplace_back(k, n);
     k += n;
   }
-  const int nt = (int)tr.size();
   int64_t maxent = 1;
   for (auto& r : tr) {
     int64_t ent = 0;
@@ -755,16 +868,24 @@ void build_tiles(Ctx* c, Env& e) {
   // PERIDYN_TILE_BUILD=host selects the host build below (it is the path of
   // tiles beyond the device sort buffer; tests compare the two)
   const char* hb = std::getenv("PERIDYN_TILE_BUILD");
-  const bool host_build = hb && std::string(hb) == "host";
-  if (ncap <= 16384 && !host_build) {
+  const bool host_build = (hb && std::string(hb) == "host") || ncap > 16384;
+  std::vector<int32_t> col;
+  if (host_build) {
+    col.resize((size_t)std::max<int64_t>(e.nslots, 1));
+    PD_HIP(hipMemcpyAsync(col.data(), e.dev.col, sizeof(int32_t) * (size_t)e.nslots, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+  }
+  // merged tiles hold at most maxent bond slots, so ncap stands
+  merge_partial_tiles(e, tr, maxent, [&](const std::vector<std::pair<int, int>>& q) {
+    return host_build ? halo_counts_host(e, col, q) : halo_counts_device(c, e, q, ncap);
+  });
+  const int nt = (int)tr.size();
+  if (!host_build) {
     e.tiled = build_tiles_device(c, e, tr, ncap);
     if (!e.tiled) e.tiles.clear();
     return;
   }
   // host build for tiles beyond the device sort buffer
-  std::vector<int32_t> col((size_t)std::max<int64_t>(e.nslots, 1));
-  PD_HIP(hipMemcpyAsync(col.data(), e.dev.col, sizeof(int32_t) * (size_t)e.nslots, hipMemcpyDeviceToHost, s));
-  PD_HIP(hipStreamSynchronize(s));
   std::vector<std::vector<int32_t>> halos(nt);
   std::vector<uint64_t> pk((size_t)std::max<int64_t>(e.npk, 1), 0ull);
   const int64_t N = e.N;

@@ -359,3 +359,37 @@ def test_tiled_lattice_positions_equal_untiled(pd, kind, case):
     same_state(a, b)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
+@pytest.mark.parametrize("build", ["device", "host"])
+def test_partial_brick_tiles_equal_untiled(pd, kind, build):
+    """A 12 x 12 x 10 slab: its top brick layer holds 2 of a brick's 4 chunk
+    layers, and x-neighbouring partial bricks share one tile.  Tiled (device or
+    host tile build) equals untiled bit for bit."""
+    say("partial bricks", kind, build)
+    x = W.lattice_box(12, 12, 10)
+    K = W.bulk_from_E(W.E0, W.NU0)
+    params = dict(bulk_modulus=K, critical_stretch=W.S00, density=W.RHO0)
+    if kind != "bb":
+        params["shear_modulus"] = 3.0 * K * (1 - 2 * W.NU0) / (2 * (1 + W.NU0))
+    if kind == "ep":
+        params["yield_stress"] = 20.0
+    spec = {"dt": 1e-4, "blocks": [W.block(kind, x, W.DELTA, **params)], "contacts": [],
+            "bcs": W.grips_z(x, 2, 0.002 / 1e-4)}
+    old = os.environ.get("PERIDYN_TILE_BUILD")
+    if build == "host":
+        os.environ["PERIDYN_TILE_BUILD"] = "host"
+    try:
+        a = make(pd, spec, True, 4)
+    finally:
+        if old is None:
+            os.environ.pop("PERIDYN_TILE_BUILD", None)
+        else:
+            os.environ["PERIDYN_TILE_BUILD"] = old
+    b = make(pd, spec, False, 4)
+    a.run(40, 1)
+    b.run(40, 1)
+    same_state(a, b)
+    a.close()
+    b.close()
