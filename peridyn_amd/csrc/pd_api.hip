@@ -201,9 +201,8 @@ struct Env {
   // built (tiled: the halos fit the LDS budget); brick of each internal particle
   bool want_tiles = false, tiled = false, xf32 = false;
   std::vector<int32_t> brick;
-  // per brick: row (the brick's y-z position within its block) and column (x)
-  std::vector<int64_t> brick_row;
-  std::vector<int32_t> brick_col;
+  // per brick: its (x, y, z) position in its block's brick grid
+  std::vector<std::array<int32_t, 3>> brick_pos;
   std::vector<Tile> tiles;
   Tile* d_tiles = nullptr;                       // device copy (run sets copy from it)
   std::vector<std::array<int, 3>> tile_blocks;   // (block, first tile, count), tiles in block order
@@ -530,8 +529,7 @@ void build_brick_order(Env& e) {
   e.perm.resize(e.N);
   e.iperm.resize(e.N);
   e.brick.assign(e.N, 0);
-  e.brick_row.clear();
-  e.brick_col.clear();
+  e.brick_pos.clear();
   int bx, by, bz;
   brick_dims(e.lgG, bx, by, bz);
   int32_t next_brick = 0;
@@ -594,10 +592,8 @@ void build_brick_order(Env& e) {
       const int32_t i = order[k];
       const int64_t br = brk[i];
       if (k > 0 && br != prev) ++next_brick;
-      if (k == 0 || br != prev) {
-        e.brick_row.push_back(br / nb[0]);
-        e.brick_col.push_back((int32_t)(br % nb[0]));
-      }
+      if (k == 0 || br != prev)
+        e.brick_pos.push_back({(int32_t)(br % nb[0]), (int32_t)((br / nb[0]) % nb[1]), (int32_t)(br / (nb[0] * nb[1]))});
       prev = br;
       e.perm[b.first + k] = b.first + i;
       e.iperm[b.first + i] = (int32_t)(b.first + k);
@@ -611,8 +607,7 @@ void build_order(Env& e) {
   e.perm.clear();
   e.iperm.clear();
   e.brick.clear();
-  e.brick_row.clear();
-  e.brick_col.clear();
+  e.brick_pos.clear();
   if (e.precision == PD_PREC_EXACT) return;
   if (e.want_tiles) {
     build_brick_order(e);
@@ -679,20 +674,29 @@ size_t tile_record_bytes(const Env& e) {
   return rb;
 }
 
+// A tile before its build: the indices (into Env::chunks) of its chunks.
+struct TileChunks {
+  int n = 0;
+  int ck[PD_TILE_CHUNKS] = {};
+};
+
+Tile tile_of(const Env& e, const TileChunks& r) {
+  Tile T;
+  std::memset(&T, 0, sizeof(T));
+  T.block = e.chunks[r.ck[0]].block;
+  T.nch = r.n;
+  for (int k = 0; k < r.n; ++k) T.ch[k] = e.chunks[r.ck[k]];
+  return T;
+}
+
 // Device build (k_halo_count / k_halo_fill, one workgroup per tile): tiles of
 // at most 16384 bond slots (an LDS hash set of twice that).  Returns false
 // (nothing allocated) when a halo exceeds the limits.
-bool build_tiles_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& tr, int ncap) {
+bool build_tiles_device(Ctx* c, Env& e, const std::vector<TileChunks>& tr, int ncap) {
   hipStream_t s = c->stream;
   const int nt = (int)tr.size();
   std::vector<Tile> tt(nt);
-  for (int t = 0; t < nt; ++t) {
-    Tile& T = tt[t];
-    std::memset(&T, 0, sizeof(T));
-    T.block = e.chunks[tr[t].first].block;
-    T.nch = tr[t].second;
-    for (int k = 0; k < T.nch; ++k) T.ch[k] = e.chunks[tr[t].first + k];
-  }
+  for (int t = 0; t < nt; ++t) tt[t] = tile_of(e, tr[t]);
   Tile* dt = nullptr;
   int32_t* dcnt = nullptr;
   std::vector<int32_t> cnt(nt);
@@ -744,17 +748,11 @@ bool build_tiles_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& 
 
 // Halo sizes (distinct neighbour ids) of the tiles with chunk ranges tr, by
 // k_halo_count on the device.
-std::vector<int32_t> halo_counts_device(Ctx* c, Env& e, const std::vector<std::pair<int, int>>& tr, int ncap) {
+std::vector<int32_t> halo_counts_device(Ctx* c, Env& e, const std::vector<TileChunks>& tr, int ncap) {
   hipStream_t s = c->stream;
   const int nt = (int)tr.size();
   std::vector<Tile> tt(nt);
-  for (int t = 0; t < nt; ++t) {
-    Tile& T = tt[t];
-    std::memset(&T, 0, sizeof(T));
-    T.block = e.chunks[tr[t].first].block;
-    T.nch = tr[t].second;
-    for (int k = 0; k < T.nch; ++k) T.ch[k] = e.chunks[tr[t].first + k];
-  }
+  for (int t = 0; t < nt; ++t) tt[t] = tile_of(e, tr[t]);
   std::vector<int32_t> cnt(nt);
   const size_t cls_t = DevMem::size_class(sizeof(Tile) * std::max(nt, 1));
   const size_t cls_c = DevMem::size_class(sizeof(int32_t) * std::max(nt, 1));
@@ -781,12 +779,12 @@ std::vector<int32_t> halo_counts_device(Ctx* c, Env& e, const std::vector<std::p
 
 // The same sizes on the host, from the downloaded neighbour columns.
 std::vector<int32_t> halo_counts_host(const Env& e, const std::vector<int32_t>& col,
-                                      const std::vector<std::pair<int, int>>& tr) {
+                                      const std::vector<TileChunks>& tr) {
   std::vector<int32_t> cnt(tr.size(), 0);
   std::vector<int32_t> stamp((size_t)e.N, -1);
   for (size_t t = 0; t < tr.size(); ++t)
-    for (int k = tr[t].first; k < tr[t].first + tr[t].second; ++k) {
-      const Chunk& ch = e.chunks[k];
+    for (int q = 0; q < tr[t].n; ++q) {
+      const Chunk& ch = e.chunks[tr[t].ck[q]];
       const size_t nsl = (size_t)PD_WAVE * ch.width;
       for (size_t i = 0; i < nsl; ++i) {
         const int32_t j = col[ch.slot0 + i];
@@ -801,44 +799,75 @@ std::vector<int32_t> halo_counts_host(const Env& e, const std::vector<int32_t>& 
 
 // Partial bricks (fewer than PD_TILE_CHUNKS chunks: the last brick layer of a
 // block whose extent is not a multiple of the brick's) leave waves of their
-// workgroup idle while the workgroup holds a full LDS class.  Two such bricks
-// next to each other in x (consecutive in the brick order) become one tile
-// when their chunks fit one and the merged halo is no larger than the largest
-// halo of the env's unmerged tiles, so that the env's LDS class is unchanged.
-// counts(ranges) returns the halo sizes of chunk ranges.
+// workgroup idle while the workgroup holds a full LDS class.  Two partial
+// bricks next to each other (in x, y or z) become one tile when their chunks
+// fit one and the merged halo is no larger than the largest halo of the env's
+// unmerged tiles, so that the env's LDS classes are unchanged.  Candidate
+// pairs are taken greedily, those that fill a whole tile first, then in brick
+// order.  counts(tiles) returns the halo sizes of chunk lists.
 template <class F>
-void merge_partial_tiles(const Env& e, std::vector<std::pair<int, int>>& tr, int64_t maxent, F&& counts) {
-  auto ent = [&](int first, int n) {
+void merge_partial_tiles(const Env& e, std::vector<TileChunks>& tr, int64_t maxent, F&& counts) {
+  auto ent = [&](const TileChunks& a, const TileChunks& b) {
     int64_t v = 0;
-    for (int k = first; k < first + n; ++k) v += (int64_t)PD_WAVE * e.chunks[k].width;
+    for (int k = 0; k < a.n; ++k) v += (int64_t)PD_WAVE * e.chunks[a.ck[k]].width;
+    for (int k = 0; k < b.n; ++k) v += (int64_t)PD_WAVE * e.chunks[b.ck[k]].width;
     return v;
   };
-  std::vector<int> cand;  // t: merge tiles t and t+1
-  for (int t = 0; t + 1 < (int)tr.size(); ++t) {
-    const Chunk& a = e.chunks[tr[t].first];
-    const Chunk& b = e.chunks[tr[t + 1].first];
-    const int ba = e.brick[a.first], bb = e.brick[b.first];
-    if (a.block != b.block || tr[t].second + tr[t + 1].second > PD_TILE_CHUNKS) continue;
-    if (e.brick_row[ba] != e.brick_row[bb] || e.brick_col[bb] != e.brick_col[ba] + 1) continue;
-    if (ent(tr[t].first, tr[t].second + tr[t + 1].second) > maxent) continue;
-    cand.push_back(t);
+  // partial tiles by (block, brick position)
+  std::map<std::array<int32_t, 4>, int> at;
+  for (int t = 0; t < (int)tr.size(); ++t) {
+    if (tr[t].n >= PD_TILE_CHUNKS) continue;
+    const Chunk& c0 = e.chunks[tr[t].ck[0]];
+    const auto& bp = e.brick_pos[e.brick[c0.first]];
+    at[{c0.block, bp[0], bp[1], bp[2]}] = t;
+  }
+  std::vector<std::pair<int, int>> cand;  // (t, u): merge tiles t < u
+  for (const auto& kv : at) {
+    const int t = kv.second;
+    for (int d = 0; d < 3; ++d) {
+      std::array<int32_t, 4> k = kv.first;
+      ++k[1 + d];
+      const auto it = at.find(k);
+      if (it == at.end()) continue;
+      const int u = it->second;
+      if (tr[t].n + tr[u].n > PD_TILE_CHUNKS || ent(tr[t], tr[u]) > maxent) continue;
+      cand.emplace_back(std::min(t, u), std::max(t, u));
+    }
   }
   if (cand.empty()) return;
-  std::vector<std::pair<int, int>> q(tr);
-  for (int t : cand) q.emplace_back(tr[t].first, tr[t].second + tr[t + 1].second);
+  std::sort(cand.begin(), cand.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+    const bool fa = tr[a.first].n + tr[a.second].n == PD_TILE_CHUNKS;
+    const bool fb = tr[b.first].n + tr[b.second].n == PD_TILE_CHUNKS;
+    if (fa != fb) return fa;
+    return a < b;
+  });
+  std::vector<TileChunks> q(tr);
+  for (const auto& pr : cand) {
+    TileChunks m = tr[pr.first];
+    for (int k = 0; k < tr[pr.second].n; ++k) m.ck[m.n++] = tr[pr.second].ck[k];
+    q.push_back(m);
+  }
   const std::vector<int32_t> cnt = counts(q);
   int32_t h0 = 0;
   for (size_t t = 0; t < tr.size(); ++t) h0 = std::max(h0, cnt[t]);
-  std::vector<std::pair<int, int>> out;
+  std::vector<int> partner(tr.size(), -1);
+  for (size_t i = 0; i < cand.size(); ++i) {
+    const int t = cand[i].first, u = cand[i].second;
+    if (partner[t] >= 0 || partner[u] >= 0 || cnt[tr.size() + i] > h0) continue;
+    partner[t] = u;
+    partner[u] = t;
+  }
+  // a merged tile takes the place of its first brick
+  std::vector<TileChunks> out;
   out.reserve(tr.size());
-  size_t ci = 0;
   for (int t = 0; t < (int)tr.size(); ++t) {
-    while (ci < cand.size() && cand[ci] < t) ++ci;
-    if (ci < cand.size() && cand[ci] == t && cnt[tr.size() + ci] <= h0) {
-      out.emplace_back(tr[t].first, tr[t].second + tr[t + 1].second);
-      ++t;
-    } else {
+    if (partner[t] < 0) {
       out.push_back(tr[t]);
+    } else if (partner[t] > t) {
+      TileChunks m = tr[t];
+      const TileChunks& o = tr[partner[t]];
+      for (int k = 0; k < o.n; ++k) m.ck[m.n++] = o.ck[k];
+      out.push_back(m);
     }
   }
   tr.swap(out);
@@ -846,21 +875,21 @@ void merge_partial_tiles(const Env& e, std::vector<std::pair<int, int>>& tr, int
 
 void build_tiles(Ctx* c, Env& e) {
   hipStream_t s = c->stream;
-  // tiles: chunk ranges, one brick each
-  std::vector<std::pair<int, int>> tr;
+  // tiles: the chunks of one brick each
+  std::vector<TileChunks> tr;
   for (int k = 0; k < (int)e.chunks.size();) {
-    const int br = e.brick[e.chunks[k].first];
-    int n = 1;
-    while (n < PD_TILE_CHUNKS && k + n < (int)e.chunks.size() && e.chunks[k + n].block == e.chunks[k].block &&
-           e.brick[e.chunks[k + n].first] == br)
-      ++n;
-    tr.emplace_back(k, n);
-    k += n;
+    const int br = e.brick[e.chunks[k].first], blk = e.chunks[k].block;
+    TileChunks r;
+    r.ck[r.n++] = k++;
+    while (r.n < PD_TILE_CHUNKS && k < (int)e.chunks.size() && e.chunks[k].block == blk &&
+           e.brick[e.chunks[k].first] == br)
+      r.ck[r.n++] = k++;
+    tr.push_back(r);
   }
   int64_t maxent = 1;
   for (auto& r : tr) {
     int64_t ent = 0;
-    for (int k = r.first; k < r.first + r.second; ++k) ent += (int64_t)PD_WAVE * e.chunks[k].width;
+    for (int k = 0; k < r.n; ++k) ent += (int64_t)PD_WAVE * e.chunks[r.ck[k]].width;
     maxent = std::max(maxent, ent);
   }
   int ncap = 1;
@@ -876,7 +905,7 @@ void build_tiles(Ctx* c, Env& e) {
     PD_HIP(hipStreamSynchronize(s));
   }
   // merged tiles hold at most maxent bond slots, so ncap stands
-  merge_partial_tiles(e, tr, maxent, [&](const std::vector<std::pair<int, int>>& q) {
+  merge_partial_tiles(e, tr, maxent, [&](const std::vector<TileChunks>& q) {
     return host_build ? halo_counts_host(e, col, q) : halo_counts_device(c, e, q, ncap);
   });
   const int nt = (int)tr.size();
@@ -893,8 +922,8 @@ void build_tiles(Ctx* c, Env& e) {
     std::vector<int32_t> stamp(N, -1), pos(N, 0);
     for (int t = t0; t < t1; ++t) {
       std::vector<int32_t>& h = halos[t];
-      for (int k = tr[t].first; k < tr[t].first + tr[t].second; ++k) {
-        const Chunk& ch = e.chunks[k];
+      for (int q = 0; q < tr[t].n; ++q) {
+        const Chunk& ch = e.chunks[tr[t].ck[q]];
         const size_t nsl = (size_t)PD_WAVE * ch.width;
         for (size_t i = 0; i < nsl; ++i) {
           const int32_t j = col[ch.slot0 + i];
@@ -907,8 +936,8 @@ void build_tiles(Ctx* c, Env& e) {
       std::sort(h.begin(), h.end());
       if (h.size() > 0x7fff) continue;  // rejected below
       for (size_t i = 0; i < h.size(); ++i) pos[h[i]] = (int32_t)i;
-      for (int k = tr[t].first; k < tr[t].first + tr[t].second; ++k) {
-        const Chunk& ch = e.chunks[k];
+      for (int q = 0; q < tr[t].n; ++q) {
+        const Chunk& ch = e.chunks[tr[t].ck[q]];
         for (int st = 0; st < ch.width; ++st)
           for (int l = 0; l < PD_WAVE; ++l) {
             const int32_t j = col[ch.slot0 + (size_t)PD_WAVE * st + l];
@@ -939,12 +968,10 @@ void build_tiles(Ctx* c, Env& e) {
   e.tiles.assign(nt, Tile{});
   for (int t = 0; t < nt; ++t) {
     Tile& T = e.tiles[t];
+    T = tile_of(e, tr[t]);
     T.env = 0;
-    T.block = e.chunks[tr[t].first].block;
-    T.nch = tr[t].second;
     T.hn = (int32_t)halos[t].size();
     T.hoff = (uint32_t)hall.size();
-    for (int k = 0; k < T.nch; ++k) T.ch[k] = e.chunks[tr[t].first + k];
     hall.insert(hall.end(), halos[t].begin(), halos[t].end());
   }
   e.hmax = (int32_t)hmax;
