@@ -613,23 +613,25 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
             const double inv_y = pd_rsqrt(yy2);
             const double yij = yy2 * inv_y;
             const double ext = yij - xij;
-            if (ext * inv_x < s0_pp) {
-              double tt;
-              if (KIND == PD_BB) {
-                tt = ext * inv_x * c11;
-              } else {
-                const double th_j = rc.th(j), rm_j = rc.rm(j);
-                const double sm = ap.y + rm_j;
-                const double twm = ap.x * ap.y + th_j * rm_j;
-                tt = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
-              }
-              const double xx = tt * inv_y * (rc.vol(j) * inv_den);
-              a1 = __builtin_fma(xx, Y1, a1);
-              a2 = __builtin_fma(xx, Y2, a2);
-              a3 = __builtin_fma(xx, Y3, a3);
+            // the bond's term, dropped by the select when it breaks (adding
+            // fma(0, Y, a) leaves a unchanged: Y is finite and a never -0),
+            // so that the loop carries no branch but the intact test
+            const bool ok = ext * inv_x < s0_pp;
+            brk = brk || !ok;
+            double tt;
+            if (KIND == PD_BB) {
+              tt = ext * inv_x * c11;
             } else {
-              brk = true;
+              const double th_j = rc.th(j), rm_j = rc.rm(j);
+              const double sm = ap.y + rm_j;
+              const double twm = ap.x * ap.y + th_j * rm_j;
+              tt = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
             }
+            const double xt = tt * inv_y * (rc.vol(j) * inv_den);
+            const double xx = ok ? xt : 0.0;
+            a1 = __builtin_fma(xx, Y1, a1);
+            a2 = __builtin_fma(xx, Y2, a2);
+            a3 = __builtin_fma(xx, Y3, a3);
           }
         });
         if (brk)
