@@ -11,14 +11,20 @@ __device__ __forceinline__ void kahan(float& s, float& c, float v) {
 }
 
 // Compensated sum of FP32 terms taken in groups: the terms of one group (a
-// lane's 4 consecutive bond steps, the bonds of one packed word in the tiled
-// kernels) are accumulated in plain FP32, and each group's total enters a
-// Kahan sum.  The error stays a constant multiple of eps * sum |term| (at most
-// 4 roundings per group, 2 for the compensated sum), independent of the
-// number of groups, at a quarter of the per-term compensation cost.  Tiled and
-// untiled kernels group the same bond steps, so their sums stay bit-identical.
+// lane's kGroupSteps = 8 consecutive bond steps, the bonds of two packed words
+// in the tiled kernels) are accumulated in plain FP32, and each group's total
+// enters a Kahan sum.  The error stays a constant multiple of eps * sum |term|
+// (at most 8 roundings per group, 2 for the compensated sum), independent of
+// the number of groups, at an eighth of the per-term compensation cost.  Tiled
+// and untiled kernels group the same bond steps, so their sums stay
+// bit-identical: a group ends after an odd packed word or the chunk's last.
 // A group's running total starts at -0 (the additive identity, so its first
 // term enters exactly) and takes each term a * b as a fused multiply-add.
+// Bond steps per group: two packed words of the tiled kernels.
+constexpr int kGroupSteps = 8;
+__device__ __forceinline__ bool group_end_word(int t4, int nt4) { return (t4 & 1) || t4 == nt4 - 1; }
+__device__ __forceinline__ int group_count(int width) { return (width + kGroupSteps - 1) / kGroupSteps; }
+
 struct GroupKahan {
   float g = -0.0f, s = 0.0f, c = 0.0f;
   __device__ __forceinline__ void add(float a, float b) { g = __builtin_fmaf(a, b, g); }

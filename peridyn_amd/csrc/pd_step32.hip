@@ -11,7 +11,7 @@
 //                           motion of the body
 //   |Y|^2 = |X|^2 + (2 X.U + U.U),  e = (2 X.U + U.U) / (|Y| + |X|),  s = e / |X|
 //                           (no |Y| - |X| cancellation), all FP32
-// Force and theta sums are Kahan-compensated FP32 per lane over groups of 4
+// Force and theta sums are Kahan-compensated FP32 per lane over groups of 8
 // bond steps (GroupKahan), td_norm2 sums per bond; the G lanes of a particle
 // are combined in FP64.  Positions, velocities, the Verlet
 // update and the weighted volume stay FP64.  Elastoplastic blocks
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
   const double3 wp = disp32(B, xp, y[p]);
   const int d = w.active ? E.deg[p] : 0;
   const uint32_t base = w.ch.slot0 + w.lane;
-  GroupKahan acc;  // groups of 4 bond steps, as the tiled kernel's packed words
+  GroupKahan acc;  // groups of kGroupSteps bond steps, as the tiled kernel's
   constexpr int U = 2;  // bonds per batch; column/state loaded one batch ahead
   const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
   int jn[U];
@@ -77,11 +77,11 @@ __global__ __launch_bounds__(256) void k_theta32(const Chunk* __restrict__ chunk
         geom32(xp, wp, xj[u], disp32(B, xj[u], yj[u]), g);
         acc.add(g.ext, xj[u].w);
       }
-      if (i0 + u < ni && (((i0 + u) & 3) == 3 || i0 + u == ni - 1)) acc.flush();
+      if (i0 + u < ni && (((i0 + u) % kGroupSteps) == kGroupSteps - 1 || i0 + u == ni - 1)) acc.flush();
     }
   }
   // the tiled kernel runs every packed word of the chunk, empty ones too
-  for (int q = (ni + 3) >> 2; q < ((w.ch.width + 3) >> 2); ++q) acc.flush();
+  for (int q = group_count(ni); q < group_count(w.ch.width); ++q) acc.flush();
   const double tot = comp_total(acc.s, acc.c, w.C);
   if (w.active && w.g == 0) {
     const float th = (float)((3.0 * (double)E.aux32[p].y) * tot);
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
       edp_out = E.edp[E.edp_cur ^ 1];
     }
     const float c11 = (float)B.c11;
-    GroupKahan k1, k2, k3;  // groups of 4 bond steps, as the tiled kernel's packed words
+    GroupKahan k1, k2, k3;  // groups of kGroupSteps bond steps, as the tiled kernel's
     constexpr int U = 2;  // bonds per batch; column/state loaded one batch ahead
     const int ni = (d > w.g) ? ((d - w.g + w.G - 1) >> w.lgG) : 0;
     int jn[U];
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
       if (i0 + U < ni) load_stream(i0 + U);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const bool grp_end = i0 + u < ni && (((i0 + u) & 3) == 3 || i0 + u == ni - 1);
+        const bool grp_end = i0 + u < ni && (((i0 + u) % kGroupSteps) == kGroupSteps - 1 || i0 + u == ni - 1);
         if (!sv[u]) {
           if (grp_end) {
             k1.flush();
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_force32(const Chunk* __restrict__ chunk
       }
     }
     // the tiled kernel runs every packed word of the chunk, empty ones too
-    for (int q = (ni + 3) >> 2; q < ((w.ch.width + 3) >> 2); ++q) {
+    for (int q = group_count(ni); q < group_count(w.ch.width); ++q) {
       k1.flush();
       k2.flush();
       k3.flush();

@@ -749,7 +749,7 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
   const int nt4 = (w.ch.width + 3) >> 2;
   const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
   GroupKahan acc;
-  pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
+  pk_groups(pk, wb, nt4, [&](int t4, uint64_t cw) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(256) void k_tile_theta32(const Tile* __restrict__ t
       geom32(xp, wp, r.x, r.w, g);
       acc.add(g.ext, r.x.w);
     }
-    acc.flush();
+    if (group_end_word(t4, nt4)) acc.flush();
   });
   const double tot = comp_total(acc.s, acc.c, C);
   if (w.active && w.g == 0) {
@@ -811,7 +811,7 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
     const uint32_t wb = w.ch.pk0 + (uint32_t)w.lane;
     const uint32_t base = w.ch.slot0 + (uint32_t)w.lane;
     bool brk = false;  // this lane broke a bond
-    pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
+    pk_groups(pk, wb, nt4, [&](int t4, uint64_t cw) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t e = (uint32_t)(cw >> (16 * u)) & 0xffffu;
@@ -831,9 +831,11 @@ void k_tile_force32(const Tile* __restrict__ tiles, int ntiles,
         k2.add(xx, g.Y2);
         k3.add(xx, g.Y3);
       }
-      k1.flush();
-      k2.flush();
-      k3.flush();
+      if (group_end_word(t4, nt4)) {
+        k1.flush();
+        k2.flush();
+        k3.flush();
+      }
     });
     if (brk)
       pk_record_breaks(pk, pk16, st, wb, base, nt4, [&](int j) {
