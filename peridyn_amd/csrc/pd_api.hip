@@ -189,6 +189,12 @@ struct Env {
   bool committed = false;
   bool maybe_undeformed = true;
   std::vector<uint8_t> block_latched;  // blocks read back as latched (latches never reset)
+  // Blocks whose every particle one BC holds (not onlyatstart): the BC moves
+  // them by one displacement per step, so y - y[1] + x[1] - x stays at
+  // rounding level (~1e-16 |y|), far inside the latch's rtol = sqrt(eps); once
+  // such a block has been read back unlatched after y was last set from the
+  // host (block_rigid_ok), later latch tests would all fail too and are skipped.
+  std::vector<uint8_t> block_rigid, block_rigid_ok;
   // the contact lists were built from the current positions (nothing has moved
   // them since), so an update_neighs! now would rebuild the same sorted lists
   bool lists_current = false;
@@ -1078,6 +1084,16 @@ void commit_env(Ctx* c, Env& e) {
       if (e.bcs[k].mask[r]) bcid[internal_of(e, r)] = (int32_t)k;
     any_onlyatstart = any_onlyatstart || (e.bcs[k].d.options & PD_BC_ONLYATSTART);
   }
+  e.block_rigid.assign(e.blocks.size(), 0);
+  e.block_rigid_ok.assign(e.blocks.size(), 0);
+  for (size_t bi = 0; bi < e.blocks.size(); ++bi) {
+    const HostBlock& b = e.blocks[bi];
+    if (b.d.n <= 0) continue;
+    const int32_t k0 = bcid[internal_of(e, b.first)];
+    bool one = k0 >= 0 && !(e.bcs[k0].d.options & PD_BC_ONLYATSTART);
+    for (int64_t i = 1; one && i < b.d.n; ++i) one = bcid[internal_of(e, b.first + i)] == k0;
+    e.block_rigid[bi] = one ? 1 : 0;
+  }
   EnvDev& D = e.dev;
   std::memset(&D, 0, sizeof(D));
   D.N = (int32_t)N;
@@ -1747,8 +1763,9 @@ int prepare_latch(Ctx* c, const std::vector<int64_t>& ids) {
     Env& e = get_env(c, id);
     if (!e.maybe_undeformed) continue;
     e.block_latched.resize(e.blocks.size(), 0);
+    e.block_rigid_ok.resize(e.blocks.size(), 0);
     for (size_t b = 0; b < e.blocks.size(); ++b) {
-      if (e.block_latched[b]) continue;
+      if (e.block_latched[b] || e.block_rigid_ok[b]) continue;
       const int item = (int)items.size();
       items.push_back(make_int2(e.slot, (int)b));
       c->latch_items.push_back({id, (int)b});
@@ -1787,8 +1804,12 @@ void refresh_latch(Ctx* c, const std::vector<int64_t>& ids, int nl) {
   pd_launch_latch_flags(c->stream, c->d_envs, c->d_latch, nl, c->d_latch_flags);
   PD_HIP(hipMemcpyAsync(fl.data(), c->d_latch_flags, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < nl; ++k)
-    if (fl[k]) get_env(c, c->latch_items[k].first).block_latched[c->latch_items[k].second] = 1;
+  for (int k = 0; k < nl; ++k) {
+    Env& e = get_env(c, c->latch_items[k].first);
+    const int b = c->latch_items[k].second;
+    if (fl[k]) e.block_latched[b] = 1;
+    else if (!e.block_rigid.empty() && e.block_rigid[b]) e.block_rigid_ok[b] = 1;
+  }
   for (int k = 0; k < nl; ++k) {
     Env& e = get_env(c, c->latch_items[k].first);
     bool any = false;
@@ -1896,6 +1917,13 @@ void upload_xyz(Ctx* c, const Env& e, double4* dst, const double* in, bool keep_
   }
   PD_HIP(hipMemcpyAsync(dst, h.data(), sizeof(double4) * N, hipMemcpyHostToDevice, c->stream));
   PD_HIP(hipStreamSynchronize(c->stream));
+}
+
+// Positions from the host into the current y buffer; rigidly held blocks are
+// tested again (block_rigid_ok).
+void upload_y(Ctx* c, Env& e, const double* y) {
+  upload_xyz(c, e, e.dev.y[e.cur], y, false);
+  std::fill(e.block_rigid_ok.begin(), e.block_rigid_ok.end(), (uint8_t)0);
 }
 
 struct BondsHost {
@@ -2690,7 +2718,7 @@ int pd_env_update_acc(void* ctx, int64_t env, const double* y, double* f) {
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     if (y) {
-      upload_xyz(c, e, e.dev.y[e.cur], y, false);
+      upload_y(c, e, y);
       e.lists_current = false;
     }
     force_pass(c, e, e.chunks, 0);
@@ -2707,7 +2735,7 @@ int pd_force_density(void* ctx, int64_t env, int64_t block, const double* y, dou
     HostBlock& b = get_block(e, block);
     if (lo != b.first + 1 || hi != b.first + b.d.n) throw PdError(PD_E_ARG, "limits do not match the block");
     if (y) {
-      upload_xyz(c, e, e.dev.y[e.cur], y, false);
+      upload_y(c, e, y);
       e.lists_current = false;
     }
     std::vector<Chunk> sub(e.chunks.begin() + b.chunk0, e.chunks.begin() + b.chunk0 + b.nchunks);
@@ -2754,7 +2782,7 @@ int pd_contact_update(void* ctx, int64_t env, int64_t cidx, const double* y) {
     Env& e = get_env(c, env);
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     ContactDev& cd = get_contact(e, cidx);
-    if (y) upload_xyz(c, e, e.dev.y[e.cur], y, false);
+    if (y) upload_y(c, e, y);
     if (cd.self_contact) pd_launch_self_contact_update(c->stream, c->d_envs, e.slot, cd, (int)cidx);
     else pd_launch_contact_update(c->stream, c->d_envs, e.slot, e.dev, cd, (int)cidx);
     cpart_update(c, e);
@@ -2795,7 +2823,7 @@ int pd_contact_force(void* ctx, int64_t env, int64_t cidx, const double* y, doub
     const ContactDev& cd = get_contact(e, cidx);
     if (!f) throw PdError(PD_E_ARG, "null f");
     if (y) {
-      upload_xyz(c, e, e.dev.y[e.cur], y, false);
+      upload_y(c, e, y);
       e.lists_current = false;
     }
     const size_t cls = DevMem::size_class(sizeof(double4) * (size_t)std::max<int64_t>(e.N, 1));
@@ -2833,7 +2861,7 @@ int pd_env_upload(void* ctx, int64_t env, const double* y, const double* v, cons
     if (!e.committed) throw PdError(PD_E_STATE, "env not committed");
     PD_HIP(hipStreamSynchronize(c->stream));
     if (y) {
-      upload_xyz(c, e, e.dev.y[e.cur], y, false);
+      upload_y(c, e, y);
       e.lists_current = false;
     }
     if (v) upload_xyz(c, e, e.dev.v, v, false);
@@ -2967,6 +2995,7 @@ int pd_env_state_load(void* ctx, int64_t env, const void* buf, int64_t bytes) {
     e.time_step = hdr[8];
     e.maybe_undeformed = any_undeformed;
     e.block_latched.assign(e.blocks.size(), 0);  // the loaded latches are read back when next tested
+    e.block_rigid_ok.assign(e.blocks.size(), 0);
     e.lists_current = false;
     upload_env_table(c, e);
     if (e.tiled) pd_launch_pk_sync(s, e.d_chunks, (int)e.chunks.size(), c->d_envs);
