@@ -456,3 +456,51 @@ def test_contact_seam_per_model_against_oracle(pd, orc):
         assert not np.array_equal(out, f0)
     assert (o.contacts[1]["neighs"] > 0).sum(axis=1).max() == 5  # truncated
     g.close()
+
+
+@pytest.mark.parametrize("prec", ["exact", "fp64"])
+def test_rigidly_held_block_latch_and_upload(pd, orc, prec):
+    """A block every particle of which one ToFroBC holds moves rigidly and never
+    latches (its latch tests stop once it has read back unlatched); positions
+    set from the host are tested again: a stretched upload of that block latches
+    it at the next step, as the oracle's latch (general_material.jl:240-256)
+    does.  A second, free block is pulled by grips meanwhile.  Bars: exact form
+    bit-identical to the oracle, FP64 within 1e-10, over three calls, the
+    upload and two more calls."""
+    K = W.bulk_from_E(W.E0, W.NU0)
+    p = dict(bulk_modulus=K, critical_stretch=W.S00, density=W.RHO0)
+    xa = W.lattice_box(7, 7, 7)
+    xb = W.lattice_box(5, 5, 5, origin=(20.0, 0.0, 0.0))
+    spec = {"dt": 1e-4, "blocks": [W.block("bb", xa, W.DELTA, type_id=1, **p),
+                                   W.block("bb", xb, W.DELTA, type_id=2, **p)], "contacts": []}
+    n = len(xa) + len(xb)
+    grips = W.grips_z(xa, 2, 20.0)
+    for g_ in grips:
+        g_["mask"] = np.concatenate([g_["mask"], np.zeros(len(xb), dtype=bool)])
+    held = np.zeros(n, dtype=bool)
+    held[len(xa):] = True
+    spec["bcs"] = grips + [{"kind": "tofro", "mask": held, "rate": np.array([3.0, -1.0, 2.0]), "freq": 25.0,
+                            "applyafter": 0}]
+    o = orc.OracleEnv(spec)
+    g = pd.Env.from_spec(spec, precision=prec)
+    for k in (7, 20, 13):
+        o.run(k, 1, start=True)
+        g.run(k, 1)
+    assert not o.blocks[1].deformed
+    y = o.y.copy()
+    c = y[len(xa):].mean(axis=0)
+    y[len(xa):] = c + (y[len(xa):] - c) * np.array([1.0, 1.0, 1.002])
+    o.y = y.copy()
+    g.upload(y=y)
+    for k in (5, 9):
+        o.run(k, 1, start=True)
+        g.run(k, 1)
+    assert o.blocks[1].deformed
+    yy, vv, ff = g.download()
+    if prec == "exact":
+        assert np.array_equal(yy, o.y) and np.array_equal(vv, o.v) and np.array_equal(ff, o.f)
+    else:
+        x = np.concatenate([xa, xb])
+        assert rel_l2(yy - x, o.y - x) <= 1e-10 and rel_l2(ff, o.f) <= 1e-10
+    assert np.any(ff[len(xa):] != 0.0)
+    g.close()
