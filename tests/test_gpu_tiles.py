@@ -393,3 +393,34 @@ def test_partial_brick_tiles_equal_untiled(pd, kind, build):
     same_state(a, b)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_forked_bond_based_passes_equal_one_stream(pd, precision):
+    """A batch of bond-based, ordinary state-based and elastoplastic specimens:
+    the bond-based force passes run on a second stream beside the dilatation
+    passes (PERIDYN_FORK=1 forks whatever the sizes; unset, sets whose
+    launches are short fork); the results equal the one-stream run bit for
+    bit, graph-replayed and eager."""
+    say("fork", precision)
+    specs = [W.tensile_bb(n=9, dt=1e-4, kind="bb", pull_per_step=0.002),
+             W.tensile_bb(n=10, dt=1e-4, kind="osb", pull_per_step=0.002), ep_tensile(8),
+             W.tensile_bb(n=11, dt=1e-4, kind="bb", pull_per_step=0.003)]
+    out = {}
+    for mode in ("0", "1"):
+        old = os.environ.get("PERIDYN_FORK")
+        os.environ["PERIDYN_FORK"] = mode
+        try:
+            envs = [pd.Env.from_spec(s, precision=precision, lanes_per_particle=4) for s in specs]
+            pd.run_batch(envs, 40, 1)   # graph-replayed steps
+            pd.run_batch(envs, 7, 1)    # eager (short call)
+            out[mode] = envs
+        finally:
+            if old is None:
+                os.environ.pop("PERIDYN_FORK", None)
+            else:
+                os.environ["PERIDYN_FORK"] = old
+    for a, b in zip(out["0"], out["1"]):
+        same_state(a, b)
+    for e in out["0"] + out["1"]:
+        e.close()
