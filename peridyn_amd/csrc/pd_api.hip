@@ -1440,6 +1440,7 @@ void commit_env(Ctx* c, Env& e) {
     D.cflag = dalloc<uint8_t>(e, N);
     D.cpart = dalloc<int32_t>(e, N);
     D.ccount = dalloc<int32_t>(e, 1);
+    if (e.cdev.size() == 1 && !e.cdev[0].self_contact) D.crec = dalloc<int32_t>(e, 8 * (size_t)N);
     PD_HIP(hipMemsetAsync(D.cflag, 0, (size_t)N, s));
     PD_HIP(hipMemsetAsync(D.ccount, 0, sizeof(int32_t), s));
     e.cself_host[0] = e.slot;

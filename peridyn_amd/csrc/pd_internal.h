@@ -139,6 +139,10 @@ struct EnvDev {
   uint8_t* cflag;
   int32_t* cpart;
   int32_t* ccount;
+  // envs with one pair contact model: per cpart entry (p, i1, n1, i2, n2,
+  // rev_off, 0, 0): the member indices, list lengths and transposed-list
+  // offset that the finish pass would otherwise look up one after another
+  int32_t* crec;
 };
 
 struct Chunk {

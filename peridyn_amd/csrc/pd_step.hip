@@ -984,7 +984,16 @@ __global__ __launch_bounds__(256) void k_cpart_mark(EnvDev* __restrict__ envs, c
   int base = 0;
   if (pd_lane() == leader) base = atomicAdd(E.ccount, __builtin_popcountll(m));
   base = __shfl(base, leader);
-  if (has) E.cpart[base + lanes_below(m)] = p;
+  if (!has) return;
+  const int at = base + lanes_below(m);
+  E.cpart[at] = p;
+  if (E.crec) {
+    const ContactDev& C = E.contacts[0];
+    const int i1 = C.inv1[p], i2 = C.inv2[p];
+    int4* r = (int4*)(E.crec + 8 * (size_t)at);
+    r[0] = make_int4(p, i1, i1 >= 0 ? C.own_cnt[i1] : 0, i2);
+    r[1] = make_int4(i2 >= 0 ? C.rev_cnt[i2] : 0, i2 >= 0 ? C.rev_off[i2] : 0, 0, 0);
+  }
 }
 
 // The contact terms (model order, list order) and the Verlet update of the
@@ -1035,6 +1044,52 @@ __device__ __forceinline__ void wave_list_terms(const double4* __restrict__ x, c
     }
   }
   if (at >= n) ins();
+}
+
+// The same for the entry at of an env with one pair model, from its record
+// (k_cpart_mark): the own list, then the transposed list, as one run of
+// n1 + n2 entries whose gathers are in flight together (entries below n1
+// added, the others subtracted, in that order).
+__device__ __forceinline__ void contact_finish_rec(EnvDev& E, int at, int flags) {
+  const int4* r = (const int4*)(E.crec + 8 * (size_t)at);
+  const int4 r0 = r[0], r1 = r[1];
+  const int p = r0.x, i1 = r0.y, n1 = r0.z, n2 = r1.x, off2 = r1.y;
+  const double4* __restrict__ x = E.x;
+  const double4* __restrict__ y = E.y[E.cur];
+  const double4 yp = y[p];
+  const double4 f = E.f[p];
+  const double4 xp = x[p];
+  const double4 vp = E.v[p];
+  const int bci = E.bcid[p];
+  const double inv_den_p = E.inv_den[p];
+  const ContactDev& C = E.contacts[0];
+  const int32_t* __restrict__ own = C.own + (size_t)(i1 >= 0 ? i1 : 0) * C.max_neighs;
+  double a1 = f.x, a2 = f.y, a3 = f.z;
+  const int nn = n1 + n2;
+  for (int k0 = 0; k0 < nn; k0 += 64) {
+    const int k = k0 + pd_lane();
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (k < nn) {
+      const bool mine = k < n1;
+      const int q = mine ? C.idx2[own[k]] : C.idx1[C.rev[off2 + (k - n1)]];
+      const double4 yq = y[q];
+      const double a = x[q].w * inv_den_p;
+      double F1, F2, F3;
+      if (mine) contact_fn<false>(C, yp.x - yq.x, yp.y - yq.y, yp.z - yq.z, F1, F2, F3);
+      else contact_fn<false>(C, yq.x - yp.x, yq.y - yp.y, yq.z - yp.z, F1, F2, F3);
+      t1 = F1 * a; t2 = F2 * a; t3 = F3 * a;
+    }
+    const int m = nn - k0 < 64 ? nn - k0 : 64;
+    for (int u = 0; u < m; ++u) {
+      const double u1 = readlane_d(t1, u), u2 = readlane_d(t2, u), u3 = readlane_d(t3, u);
+      if (k0 + u < n1) {
+        a1 += u1; a2 += u2; a3 += u3;
+      } else {
+        a1 -= u1; a2 -= u2; a3 -= u3;
+      }
+    }
+  }
+  if (pd_lane() == 0) finish_particle_v(E, p, xp, yp, a1, a2, a3, flags, vp, bci);
 }
 
 // Contact terms and Verlet update of particle p of E (every lane of the
@@ -1186,6 +1241,43 @@ __device__ __forceinline__ void lane_contact_terms(const EnvDev& E, int p, const
   }
 }
 
+// lane_contact_terms for an env with one pair model, from the record of its
+// cpart entry at (k_cpart_mark); returns p.
+__device__ __forceinline__ int lane_contact_rec(const EnvDev& E, int at, const double4* __restrict__ y,
+                                                double4& yp, double& a1, double& a2, double& a3) {
+  const int4* r = (const int4*)(E.crec + 8 * (size_t)at);
+  const int4 r0 = r[0], r1 = r[1];
+  const int p = r0.x, i1 = r0.y, n1 = r0.z, n2 = r1.x, off2 = r1.y;
+  const double4* __restrict__ x = E.x;
+  yp = y[p];
+  const double4 f = E.f[p];
+  const double inv_den_p = E.inv_den[p];
+  const ContactDev& C = E.contacts[0];
+  a1 = f.x; a2 = f.y; a3 = f.z;
+  const int32_t* __restrict__ own = C.own + (size_t)(i1 >= 0 ? i1 : 0) * C.max_neighs;
+  const int32_t* __restrict__ rev = C.rev + off2;
+  const double4 ypp = yp;
+  lane_list_terms(
+      x, y, n1, p, [&](int k) { return C.idx2[own[k]]; },
+      [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
+        double F1, F2, F3;
+        contact_fn<false>(C, ypp.x - yq.x, ypp.y - yq.y, ypp.z - yq.z, F1, F2, F3);
+        const double a = vq * inv_den_p;
+        t1 = F1 * a; t2 = F2 * a; t3 = F3 * a;
+      },
+      false, a1, a2, a3);
+  lane_list_terms(
+      x, y, n2, p, [&](int k) { return C.idx1[rev[k]]; },
+      [&](const double4& yq, double vq, double& t1, double& t2, double& t3) {
+        double F1, F2, F3;
+        contact_fn<false>(C, yq.x - ypp.x, yq.y - ypp.y, yq.z - ypp.z, F1, F2, F3);
+        const double bq = vq * inv_den_p;
+        t1 = F1 * bq; t2 = F2 * bq; t3 = F3 * bq;
+      },
+      true, a1, a2, a3);
+  return p;
+}
+
 // Total entries of the cpart lists of envs slots[0 .. n) (wave-uniform).
 __device__ __forceinline__ int contact_total(const EnvDev* __restrict__ envs, const int32_t* __restrict__ slots,
                                              int n) {
@@ -1236,32 +1328,82 @@ __device__ __forceinline__ int contact_entry_env(const EnvDev* __restrict__ envs
   return found;
 }
 
+// Up to kCfinEnvs envs, every workgroup first puts the prefix of their counts
+// in LDS, so that an entry finds its env without global loads.
+constexpr int kCfinEnvs = 512;
+
 __global__ __launch_bounds__(256) void k_contact_finish(EnvDev* __restrict__ envs, const int32_t* __restrict__ slots,
                                                         int n, int lane_min, int flags) {
   const int nwaves = (int)gridDim.x * PD_WAVES_PER_BLOCK;
   const int w0 = (int)blockIdx.x * PD_WAVES_PER_BLOCK + pd_wave_id();
-  const int total = contact_total(envs, slots, n);
+  __shared__ int32_t pre[kCfinEnvs + 1];  // pre[k] = entries of envs 0 .. k-1
+  const bool lds = n <= kCfinEnvs;
+  if (lds) {
+    if (threadIdx.x < 64) {
+      const int lane = pd_lane();
+      int run = 0;
+      for (int k0 = 0; k0 < n; k0 += 64) {
+        const int k = k0 + lane;
+        const int cnt = k < n ? *envs[slots[k]].ccount : 0;
+        int incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+          const int v = __shfl_up(incl, off);
+          if (lane >= off) incl += v;
+        }
+        if (k < n) pre[k + 1] = run + incl;
+        run += __shfl(incl, 63);
+      }
+      if (lane == 0) pre[0] = 0;
+    }
+    __syncthreads();
+  }
+  // env of entry t (0 <= t < total, lds only): the k with pre[k] <= t < pre[k + 1]
+  auto find = [&](int t, int& pos) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    pos = t - pre[lo];
+    return lo;
+  };
+  const int total = lds ? pre[n] : contact_total(envs, slots, n);
   if (total >= lane_min) {
     for (int t0 = w0 * 64; t0 < total; t0 += nwaves * 64) {
-      int pos;
-      const int k = contact_entry_env(envs, slots, n, t0 + pd_lane(), pos);
-      if (k < 0) continue;
+      const int t = t0 + pd_lane();
+      int pos, k;
+      if (lds) {
+        if (t >= total) continue;
+        k = find(t, pos);
+      } else {  // every lane searches (shuffles); past the end k = -1
+        k = contact_entry_env(envs, slots, n, t, pos);
+        if (k < 0) continue;
+      }
       EnvDev& E = envs[slots[k]];
-      const int p = E.cpart[pos];
       const double4* __restrict__ y = E.y[E.cur];
-      const double4 yp = y[p];
-      const double4 f = E.f[p];
-      double a1 = f.x, a2 = f.y, a3 = f.z;
-      lane_contact_terms(E, p, yp, y, a1, a2, a3);
+      double4 yp;
+      double a1, a2, a3;
+      int p;
+      if (E.crec) {
+        p = lane_contact_rec(E, pos, y, yp, a1, a2, a3);
+      } else {
+        p = E.cpart[pos];
+        yp = y[p];
+        const double4 f = E.f[p];
+        a1 = f.x; a2 = f.y; a3 = f.z;
+        lane_contact_terms(E, p, yp, y, a1, a2, a3);
+      }
       finish_particle(E, p, E.x[p], yp, a1, a2, a3, flags);  // tiled envs: integer exponents
     }
     return;
   }
   for (int t = w0; t < total; t += nwaves) {
     int pos;
-    const int k = contact_entry_env(envs, slots, n, t, pos);
+    const int k = lds ? find(t, pos) : contact_entry_env(envs, slots, n, t, pos);
     EnvDev& E = envs[slots[k]];
-    contact_finish_particle(E, E.cpart[pos], flags);
+    if (E.crec) contact_finish_rec(E, pos, flags);
+    else contact_finish_particle(E, E.cpart[pos], flags);
   }
 }
 
