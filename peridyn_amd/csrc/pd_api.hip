@@ -165,6 +165,7 @@ struct HostBlock {
   std::vector<int64_t> type;
   std::vector<double> density, s0, c, K, G, psi;
   int32_t first = 0, chunk0 = 0, nchunks = 0, maxdeg = 0;
+  bool xint = false;  // integer lattice with horizon^2 <= 32 (BlockDev::xint)
 };
 
 struct HostContact {
@@ -1316,6 +1317,7 @@ void commit_env(Ctx* c, Env& e) {
       for (int64_t i = 0; xi && i < 3 * b.d.n; ++i)
         xi = std::floor(b.x[i]) == b.x[i] && std::fabs(b.x[i]) < 4194304.0;
       B.xint = xi ? 1 : 0;
+      b.xint = xi;
     }
     {
       float f = (float)b.s0[0];
@@ -1502,10 +1504,15 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   // tile lists: theta [pc][lg][cls], td [lg][cls], force [kind][pc][lg][cls]
   using TileList = Ctx::TileList;
   TileList l_th[2][4][PD_TILE_CLASSES], l_td[4][PD_TILE_CLASSES], l_fk[3][2][4][PD_TILE_CLASSES];
+  bool lat = true;  // every tiled FP64 bond-based block of the set is an integer lattice
   for (int64_t id : ids) {
     const Env& e = get_env(c, id);
     if (e.tiled) xf = xf && e.xf32;  // the FP64 records' size, hence the LDS classes
+    if (e.tiled && e.precision == PD_PREC_FP64)
+      for (auto& b : e.blocks)
+        if (b.d.kind == PD_BB) lat = lat && b.xint;
   }
+  const int xfl = xf ? (lat ? 2 : 1) : 0;
   auto add = [](TileList& l, int n, int hmax) {
     l.n += n;
     l.hcap = std::max(l.hcap, hmax);
@@ -1543,9 +1550,9 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
       const int pc = (pr == PD_PREC_FP32C) ? 1 : 0;  // FP32C envs tile BB / OSB blocks only
       for (auto& tb : e.tile_blocks) {
         const int kind = e.blocks[tb[0]].d.kind;
-        add(l_fk[kind][pc][lg][pd_tile_class(2, kind, pr, xf, hm)], tb[2], hm);
-        if (kind != PD_BB) add(l_th[pc][lg][pd_tile_class(0, kind, pr, xf, hm)], tb[2], hm);
-        if (kind == PD_EP) add(l_td[lg][pd_tile_class(1, kind, pr, xf, hm)], tb[2], hm);
+        add(l_fk[kind][pc][lg][pd_tile_class(2, kind, pr, xfl, hm)], tb[2], hm);
+        if (kind != PD_BB) add(l_th[pc][lg][pd_tile_class(0, kind, pr, xfl, hm)], tb[2], hm);
+        if (kind == PD_EP) add(l_td[lg][pd_tile_class(1, kind, pr, xfl, hm)], tb[2], hm);
       }
     }
   }
@@ -1614,16 +1621,16 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
       const int pc = (pr == PD_PREC_FP32C) ? 1 : 0;
       for (auto& tb : e.tile_blocks) {
         const int kind = e.blocks[tb[0]].d.kind;
-        const int hf = pd_tile_class(2, kind, pr, xf, hm);
+        const int hf = pd_tile_class(2, kind, pr, xfl, hm);
         put(l_fk[kind][pc][lg][hf].off + f_fk[kind][pc][lg][hf], e, tb);
         f_fk[kind][pc][lg][hf] += tb[2];
         if (kind != PD_BB) {
-          const int ht = pd_tile_class(0, kind, pr, xf, hm);
+          const int ht = pd_tile_class(0, kind, pr, xfl, hm);
           put(l_th[pc][lg][ht].off + f_th[pc][lg][ht], e, tb);
           f_th[pc][lg][ht] += tb[2];
         }
         if (kind == PD_EP) {
-          const int hd = pd_tile_class(1, kind, pr, xf, hm);
+          const int hd = pd_tile_class(1, kind, pr, xfl, hm);
           put(l_td[lg][hd].off + f_td[lg][hd], e, tb);
           f_td[lg][hd] += tb[2];
         }
@@ -1666,7 +1673,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   std::copy(&l_td[0][0], &l_td[0][0] + 4 * PD_TILE_CLASSES, &c->tl_td[0][0]);
   std::copy(&l_fk[0][0][0][0], &l_fk[0][0][0][0] + 3 * 2 * 4 * PD_TILE_CLASSES, &c->tl_force[0][0][0][0]);
   c->set_tiled = tiled;
-  c->set_xf = xf ? 1 : 0;
+  c->set_xf = xfl;
   // pair-contact items: (slot, contact index) and per-class workgroup offsets
   {
     std::vector<int2>& items = citems;

@@ -98,15 +98,18 @@ void pd_launch_td32(hipStream_t s, const Chunk* chunks, int n, EnvDev* envs);
 void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvDev* envs, int flags, bool gpow);
 
 // tiled passes (pd_tile.hip): LDS bytes per halo record for pass (0 theta,
-// 1 td, 2 force), kind and form; xf = every x and volume of the launched
-// envs is exactly an FP32 value (compact records); hcap = largest halo of the
-// launched tiles
+// 1 td, 2 force), kind and form; xf = 1: every x and volume of the launched
+// envs is exactly an FP32 value (compact records), 2: besides, every tiled
+// bond-based block is an integer lattice (compact lattice records of the
+// bond-based force pass); hcap = largest halo of the launched tiles
 int pd_tile_record_bytes(int pass, int kind, int prec, int xf);
-// LDS class of a pass's tiles whose largest halo is hmax: 0..3 for the
+// LDS class of a pass's tiles whose largest halo is hmax: 1..4 for the
 // compiled strides that let 6, 5, 4, 3 workgroups share a CU (for the pass's
-// record size), 4 for the run-time capacity.  Envs are launched per class, so
-// that one env with a large halo does not lower the occupancy of the others.
-constexpr int PD_TILE_CLASSES = 5;
+// record size), 5 for the run-time capacity, 0 for the 7-workgroup stride of
+// the compact lattice records (bond-based force, xf = 2).  Envs are launched
+// per class, so that one env with a large halo does not lower the occupancy
+// of the others.
+constexpr int PD_TILE_CLASSES = 6;
 int pd_tile_class(int pass, int kind, int prec, int xf, int hmax);
 void pd_tile_allow_lds(size_t bytes);
 void pd_launch_tile_theta(hipStream_t s, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs, int hcap);

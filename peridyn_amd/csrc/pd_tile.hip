@@ -34,12 +34,16 @@ template <int PASS, int KIND>
 __host__ __device__ constexpr int tile_extra() {  // double fields beyond x, vol, y
   return PASS == TP_THETA ? 0 : PASS == TP_TD ? 2 : (KIND == PD_BB ? 0 : KIND == PD_OSB ? 2 : 3);
 }
-template <int PASS, int KIND, bool XF>
+// LAT (compact lattice records, the force pass of sets whose bond-based
+// blocks are all integer lattices): the lattice pair and then the volume as
+// one float, without x2: 12 bytes after the doubles instead of 16.
+template <int PASS, int KIND, bool XF, bool LAT = false>
 __host__ __device__ constexpr int tile_record_bytes() {
-  return XF ? 8 * (3 + tile_extra<PASS, KIND>()) + 4 * 4 : 8 * (7 + tile_extra<PASS, KIND>());
+  return LAT ? 8 * (3 + tile_extra<PASS, KIND>()) + 12
+             : XF ? 8 * (3 + tile_extra<PASS, KIND>()) + 4 * 4 : 8 * (7 + tile_extra<PASS, KIND>());
 }
 
-template <bool XF, int NX>  // NX = extra double fields staged
+template <bool XF, int NX, bool LAT = false>  // NX = extra double fields staged
 struct Rec {
   const double* sh;
   int hcap;
@@ -47,7 +51,9 @@ struct Rec {
   __device__ __forceinline__ const float2* xa() const { return (const float2*)(sh + (3 + NX) * hcap); }
   __device__ __forceinline__ const float2* xb() const { return (const float2*)(sh + (4 + NX) * hcap); }
   __device__ __forceinline__ uint2 lat(int j) const { return ((const uint2*)xa())[j]; }
-  __device__ __forceinline__ double vol(int j) const { return XF ? (double)xb()[j].y : sh[3 * hcap + j]; }
+  __device__ __forceinline__ double vol(int j) const {
+    return LAT ? (double)((const float*)(sh + (4 + NX) * hcap))[j] : XF ? (double)xb()[j].y : sh[3 * hcap + j];
+  }
   __device__ __forceinline__ double y(int k, int j) const { return sh[((XF ? 0 : 4) + k) * hcap + j]; }
   __device__ __forceinline__ double th(int j) const { return sh[(XF ? 3 : 7) * hcap + j]; }
   __device__ __forceinline__ double rm(int j) const { return sh[(XF ? 4 : 8) * hcap + j]; }
@@ -90,7 +96,7 @@ struct TileWave {
 // then the records, then the LDS stores).
 constexpr int kStageR = 4;
 
-template <bool XF, int NX>
+template <bool XF, int NX, bool LAT = false>
 __device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap,
                                            bool lat) {
   const double4* __restrict__ x = E.x;
@@ -119,7 +125,10 @@ __device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, doubl
     for (int r = 0; r < kStageR; ++r) {
       const int i = i0 + 256 * r;
       if (i < T.hn) {
-        if (XF) {
+        if (LAT) {
+          ((uint2*)xa)[i] = lat_pack(a[r].x, a[r].y, a[r].z);
+          ((float*)xb)[i] = (float)a[r].w;
+        } else if (XF) {
           if (lat) ((uint2*)xa)[i] = lat_pack(a[r].x, a[r].y, a[r].z);
           else xa[i] = make_float2((float)a[r].x, (float)a[r].y);
           xb[i] = make_float2((float)a[r].z, (float)a[r].w);
@@ -283,8 +292,12 @@ struct XGeo {
 
 // Runs f(std::integral_constant<bool, lat>) -- a bond loop specialised for the
 // lattice table or the general |X| -- where lat is uniform over the workgroup.
-template <bool XF, class F>
-__device__ __forceinline__ void with_lat(bool lat, F&& f) {
+template <bool XF, class F, bool LAT = false>
+__device__ __forceinline__ void with_lat(bool lat, F&& f, std::integral_constant<bool, LAT> = {}) {
+  if constexpr (LAT) {  // compact lattice records: the table only
+    f(std::true_type{});
+    return;
+  }
   if constexpr (XF) {
     if (lat) {
       f(std::true_type{});
@@ -536,8 +549,11 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
 // compact LDS records, which leave LDS for as many tiles per CU): BB 6 (80
 // VGPRs), OSB 5 (96), EP 4 (126); the BB and OSB targets spill 20-28 B per lane
 // outside the bond loop.
-template <int KIND, int LG, bool XF, int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == PD_BB ? 6 : KIND == PD_OSB ? 5 : 4)))
+// LAT: every block of the launch is an integer lattice and its records are
+// the compact lattice ones (bond-based blocks only); 7 workgroups of them fit
+// a CU's LDS in the densest class, so the kernel aims at 7 waves per SIMD.
+template <int KIND, int LG, bool XF, int S, bool LAT = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == PD_BB ? (LAT ? 7 : 6) : KIND == PD_OSB ? 5 : 4)))
 void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
                                                     EnvDev* __restrict__ envs, int flags, int hcap) {
   const int hs = S > 0 ? S : hcap;
@@ -555,11 +571,11 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
   const int p = has ? (w.active ? w.p : w.ch.first) : T.ch[0].first;
   __shared__ double2 xtab[XF ? kXTab : 1];
   if (XF) fill_xtab64(xtab);
-  const bool lat = XF && B.xint;
-  if (deformed) stage_halo<XF, NX>(E, T, sh, hs, lat);
+  const bool lat = LAT || (XF && B.xint);
+  if (deformed) stage_halo<XF, NX, LAT>(E, T, sh, hs, lat);
   __syncthreads();
   if (!has) return;
-  const Rec<XF, NX> rc(sh, hs);
+  const Rec<XF, NX, LAT> rc(sh, hs);
   const double4* __restrict__ y = E.y[E.cur];
   const double4 xp = gld4(E.x, p), yp = gld4(y, p);
   const XGeo xg(xp, lat, xtab);
@@ -592,7 +608,7 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
           g0 = g1;
           g1 = g2;
         }
-      });
+      }, std::integral_constant<bool, LAT>{});
     } else {
       // bond-based and ordinary state-based blocks
       double4 ap = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -645,7 +661,7 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
             const double ext = yij - xij;
             return !(ext * inv_x < s0_pp);
           });
-      });
+      }, std::integral_constant<bool, LAT>{});
     }
     a1 = group_sum(a1, C);
     a2 = group_sum(a2, C);
@@ -1019,6 +1035,11 @@ __global__ __launch_bounds__(256) void k_pk_sync(const Chunk* __restrict__ chunk
 constexpr int kLdsClassBytes[4] = {26880, 32000, 40960, 53760};
 constexpr int kXTabLds = 512;
 constexpr int halo_stride(int c, int rec, int reserve) { return (kLdsClassBytes[c] - reserve) / rec; }
+// The class of 7 workgroups per CU (45 allocation granules of 512 bytes),
+// for the compact lattice records of the bond-based force pass only: its
+// kernel fits 7 waves per SIMD in registers, the other FP64 kernels do not.
+constexpr int kLdsClass7Bytes = 23040;
+constexpr int halo_stride7(int rec, int reserve) { return (kLdsClass7Bytes - reserve) / rec; }
 
 template <int REC, int RES = kXTabLds, class F>
 void with_stride(int hcap, F&& f) {
@@ -1029,6 +1050,12 @@ void with_stride(int hcap, F&& f) {
   else if (hcap <= s2) f(std::integral_constant<int, s2>{});
   else if (hcap <= s3) f(std::integral_constant<int, s3>{});
   else f(std::integral_constant<int, 0>{});
+}
+template <int REC, int RES = kXTabLds, class F>
+void with_stride7(int hcap, F&& f) {
+  constexpr int s7 = halo_stride7(REC, RES);
+  if (hcap <= s7) f(std::integral_constant<int, s7>{});
+  else with_stride<REC, RES>(hcap, f);
 }
 template <int REC, int RES = kXTabLds, class F>
 void each_stride(F&& f) {
@@ -1053,10 +1080,10 @@ void launch_td_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap
   const size_t lds = (size_t)tile_record_bytes<TP_TD, 0, XF>() * (size_t)(S > 0 ? S : hcap);
   hipLaunchKernelGGL((k_tile_td<LG, XF, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
 }
-template <int KIND, int LG, bool XF, int S>
+template <int KIND, int LG, bool XF, int S, bool LAT = false>
 void launch_force_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  const size_t lds = (size_t)tile_record_bytes<TP_FORCE, KIND, XF>() * (size_t)(S > 0 ? S : hcap);
-  hipLaunchKernelGGL((k_tile_force<KIND, LG, XF, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
+  const size_t lds = (size_t)tile_record_bytes<TP_FORCE, KIND, XF, LAT>() * (size_t)(S > 0 ? S : hcap);
+  hipLaunchKernelGGL((k_tile_force<KIND, LG, XF, S, LAT>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
 }
 
 template <int LG, bool XF>
@@ -1073,6 +1100,12 @@ template <int KIND, int LG, bool XF>
 void launch_force_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
   with_stride<tile_record_bytes<TP_FORCE, KIND, XF>()>(
       hcap, [&](auto S) { launch_force_s<KIND, LG, XF, decltype(S)::value>(s, tiles, n, envs, flags, hcap); });
+}
+template <int LG>
+void launch_force_lat_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
+  with_stride7<tile_record_bytes<TP_FORCE, PD_BB, true, true>()>(hcap, [&](auto S) {
+    launch_force_s<PD_BB, LG, true, decltype(S)::value, true>(s, tiles, n, envs, flags, hcap);
+  });
 }
 
 template <int LG, int S>
@@ -1121,6 +1154,11 @@ void allow_lds_fp64(size_t bytes) {
       [&](auto S) { allow_lds(k_tile_force<PD_OSB, LG, XF, decltype(S)::value>, bytes); });
   each_stride<tile_record_bytes<TP_FORCE, PD_EP, XF>()>(
       [&](auto S) { allow_lds(k_tile_force<PD_EP, LG, XF, decltype(S)::value>, bytes); });
+  if constexpr (XF) {
+    constexpr int rec = tile_record_bytes<TP_FORCE, PD_BB, true, true>();
+    allow_lds(k_tile_force<PD_BB, LG, true, halo_stride7(rec, kXTabLds), true>, bytes);
+    each_stride<rec>([&](auto S) { allow_lds(k_tile_force<PD_BB, LG, true, decltype(S)::value, true>, bytes); });
+  }
 }
 template <int LG>
 void allow_lds_lg(size_t bytes) {
@@ -1139,6 +1177,7 @@ int fp64_record_bytes(int kind) {
 
 int pd_tile_record_bytes(int pass, int kind, int prec, int xf) {
   if (prec == PD_PREC_FP32C && kind != PD_EP && pass != TP_TD) return rec32_bytes(pass, kind);
+  if (xf == 2 && pass == TP_FORCE && kind == PD_BB) return tile_record_bytes<TP_FORCE, PD_BB, true, true>();
   if (pass == 0) return xf ? fp64_record_bytes<TP_THETA, true>(kind) : fp64_record_bytes<TP_THETA, false>(kind);
   if (pass == 1) return xf ? fp64_record_bytes<TP_TD, true>(kind) : fp64_record_bytes<TP_TD, false>(kind);
   return xf ? fp64_record_bytes<TP_FORCE, true>(kind) : fp64_record_bytes<TP_FORCE, false>(kind);
@@ -1147,9 +1186,11 @@ int pd_tile_record_bytes(int pass, int kind, int prec, int xf) {
 int pd_tile_class(int pass, int kind, int prec, int xf, int hmax) {
   const int rec = pd_tile_record_bytes(pass, kind, prec, xf);
   const int res = (prec == PD_PREC_FP32C && kind != PD_EP && pass != TP_TD) ? 0 : kXTabLds;
+  const bool lat = xf == 2 && pass == TP_FORCE && kind == PD_BB && prec != PD_PREC_FP32C;
+  if (lat && hmax <= halo_stride7(rec, res)) return 0;
   for (int c = 0; c < 4; ++c)
-    if (hmax <= halo_stride(c, rec, res)) return c;
-  return 4;
+    if (hmax <= halo_stride(c, rec, res)) return c + 1;
+  return 5;
 }
 
 void pd_tile_allow_lds(size_t bytes) {
@@ -1193,6 +1234,12 @@ void launch_force_any(int kind, hipStream_t s, const Tile* tiles, int n, EnvDev*
 void pd_launch_tile_force(hipStream_t s, int kind, int lgG, int xf, const Tile* tiles, int n, EnvDev* envs,
                           int flags, int hcap) {
   if (n <= 0) return;
+  if (xf == 2 && kind == PD_BB) {
+    if (lgG == 1) launch_force_lat_lg<1>(s, tiles, n, envs, flags, hcap);
+    else if (lgG == 2) launch_force_lat_lg<2>(s, tiles, n, envs, flags, hcap);
+    else launch_force_lat_lg<3>(s, tiles, n, envs, flags, hcap);
+    return;
+  }
   PD_LG_XF(launch_force_any, kind, s, tiles, n, envs, flags, hcap);
 }
 #undef PD_LG_XF
