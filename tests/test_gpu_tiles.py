@@ -424,3 +424,35 @@ def test_forked_bond_based_passes_equal_one_stream(pd, precision):
         same_state(a, b)
     for e in out["0"] + out["1"]:
         e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_cost_ordered_tiles_equal_brick_order(pd, precision):
+    """Specimens of different horizons and sizes stepped together (tiles of
+    unequal cost in one launch, partial bricks among them): with the tiles of
+    every launch ordered by cost and dealt over the XCDs (PERIDYN_TILE_ORDER=1)
+    the results equal the brick-ordered run bit for bit, graph-replayed and
+    eager; one indentation request adds its contact pass."""
+    say("tile order", precision)
+    specs = [W.tensile(3000, horizon=3.9, kind="bb"), W.tensile(5000, horizon=2.6, kind="osb"),
+             W.tensile(1200, horizon=3.1, kind="osb"), W.tensile(7000, horizon=2.8, kind="bb"),
+             W.indentation(2000, horizon=3.0, kind="bb", steps=60)]
+    out = {}
+    for mode in ("0", "1"):
+        old = os.environ.get("PERIDYN_TILE_ORDER")
+        os.environ["PERIDYN_TILE_ORDER"] = mode
+        try:
+            envs = [pd.Env.from_spec(s, precision=precision, lanes_per_particle=4) for s in specs]
+            pd.run_batch(envs, 40, 10)  # graph-replayed steps
+            pd.run_batch(envs, 7, 10)   # eager (short call)
+            out[mode] = envs
+        finally:
+            if old is None:
+                os.environ.pop("PERIDYN_TILE_ORDER", None)
+            else:
+                os.environ["PERIDYN_TILE_ORDER"] = old
+    for a, b in zip(out["0"], out["1"]):
+        assert a.layout()["tiled"]
+        same_state(a, b)
+    for e in out["0"] + out["1"]:
+        e.close()
