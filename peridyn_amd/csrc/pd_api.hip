@@ -14,6 +14,7 @@
 #include <map>
 #include <thread>
 #include <string>
+#include <queue>
 #include <vector>
 
 #include "pd_device.h"
@@ -1501,14 +1502,42 @@ int tile_cost(const Tile& T) {
 }
 
 // PERIDYN_TILE_ORDER=0 keeps every launch's tiles in env and brick order,
-// =1 orders every launch by cost; unset, launches of fewer than kOrderTiles
-// tiles are ordered by cost (below)
+// =1 orders every launch by cost; unset, a launch of fewer than kOrderTiles
+// tiles is ordered by cost where that shortens its predicted span by
+// kOrderGain percent or more (below)
 int tile_order_mode() {
   const char* v = std::getenv("PERIDYN_TILE_ORDER");
   if (!v) return -1;
   return std::string(v) == "0" ? 0 : 1;
 }
-constexpr int kOrderTiles = 16384;
+constexpr int kOrderTiles = 16384, kOrderGain = 5, kOrderRun = 16;
+
+// Predicted span of a launch whose tiles (costs in list order) are dispatched
+// as pd_xcd_block does: XCD x runs its contiguous range of the list in order,
+// each tile on the first of its 32 CUs' wg workgroup slots to come free.
+int64_t tile_span(const std::vector<int>& cost, int wg) {
+  const int n = (int)cost.size(), q = n / 8, r = n % 8;
+  const size_t slots = (size_t)32 * (size_t)std::max(wg, 1);
+  int64_t span = 0;
+  int at = 0;
+  std::priority_queue<int64_t, std::vector<int64_t>, std::greater<int64_t>> free_at;
+  for (int x = 0; x < 8; ++x) {
+    const int m = q + (x < r ? 1 : 0);
+    free_at = decltype(free_at)();
+    for (int i = at; i < at + m; ++i) {
+      int64_t t0 = 0;
+      if (free_at.size() >= slots) {
+        t0 = free_at.top();
+        free_at.pop();
+      }
+      const int64_t t1 = t0 + cost[i];
+      span = std::max(span, t1);
+      free_at.push(t1);
+    }
+    at += m;
+  }
+  return span;
+}
 
 // Copies of the tile runs into the set's tile array, one launch range of the
 // lists at a time.  In brick order, a launch's workgroups go to the chip's 8
@@ -1517,16 +1546,24 @@ constexpr int kOrderTiles = 16384;
 // decreasing cost (stable) and dealt round-robin over the 8 ranges, so that
 // each XCD receives the same mix of costly and cheap tiles and runs its own
 // in decreasing cost: the last workgroups of a launch are the cheapest ones.
-// That pays in short launches of unequal tiles (specimens of different
-// horizons stepped together: 22 served requests, 2k-10k tiles per launch,
-// 3% faster per step); a launch of kOrderTiles tiles or more runs many rounds
-// of workgroups, its tail is short, and it loses more to the neighbouring
-// bricks' halos missing the XCD's L2 (the 256 tensile specimens, 33k tiles
-// per launch: 6% slower ordered by cost).
+// That pays where the tiles' costs are unequal and the launch runs few
+// rounds of workgroups (the disk's partial bricks: force pass 129 -> 117 us;
+// specimens of different horizons stepped together: 22 served requests, 3%
+// faster per step).  Dealt one tile at a time, neighbouring bricks' shared
+// halo records miss the XCD's L2 (the plates' near-equal tiles: force pass
+// 10% slower), so the deal hands out runs of kOrderRun consecutive ranks
+// (brick order within equal costs: the plates then 1% faster, runs of 2: 5%
+// slower).  A launch is ordered by cost where tile_span predicts that order
+// to be kOrderGain percent shorter, and only below kOrderTiles tiles: a
+// longer launch runs many rounds of workgroups and has little tail to gain
+// (the 256 tensile specimens, 33k tiles per launch: 6% slower dealt one tile
+// at a time).
 void place_tile_runs(Ctx* c, std::vector<TileRun>& runs, std::vector<PdCopy>& copies,
                      Ctx::TileList (&l_th)[2][4][PD_TILE_CLASSES], Ctx::TileList (&l_td)[4][PD_TILE_CLASSES],
-                     Ctx::TileList (&l_fk)[3][2][4][PD_TILE_CLASSES]) {
+                     Ctx::TileList (&l_fk)[3][2][4][PD_TILE_CLASSES], int xfl) {
   const int mode = tile_order_mode();
+  const char* lg_env = std::getenv("PERIDYN_TILE_ORDER_LOG");
+  const bool log = lg_env && std::string(lg_env) == "1";
   auto run_copy = [&](const Env* e, int t, int n, int dst) {
     copies.push_back(PdCopy{reinterpret_cast<const uint32_t*>(e->d_tiles + t),
                             reinterpret_cast<uint32_t*>(c->d_tset + dst), (uint32_t)(sizeof(Tile) * n / 4), 0u});
@@ -1537,15 +1574,17 @@ void place_tile_runs(Ctx* c, std::vector<TileRun>& runs, std::vector<PdCopy>& co
   }
   std::sort(runs.begin(), runs.end(), [](const TileRun& a, const TileRun& b) { return a.dst < b.dst; });
   std::vector<std::pair<const Env*, int>> tl;
-  std::vector<int> cost, ord;
+  std::vector<int> cost, ord, arranged;
   size_t ri = 0;
-  auto launch = [&](const Ctx::TileList& l) {
+  // rec: LDS bytes per halo record of the launch's kernel (its workgroups per CU)
+  auto launch = [&](const Ctx::TileList& l, int rec) {
     if (l.n <= 0) return;
-    if (mode < 0 && l.n >= kOrderTiles) {
-      for (; ri < runs.size() && runs[ri].dst < l.off + l.n; ++ri)
+    const size_t r0 = ri;
+    auto in_order = [&] {
+      for (ri = r0; ri < runs.size() && runs[ri].dst < l.off + l.n; ++ri)
         if (runs[ri].dst >= l.off) run_copy(runs[ri].e, runs[ri].first, runs[ri].n, runs[ri].dst);
-      return;
-    }
+    };
+    if (mode < 0 && l.n >= kOrderTiles) return in_order();
     tl.clear();
     while (ri < runs.size() && runs[ri].dst < l.off) ++ri;
     for (; ri < runs.size() && runs[ri].dst < l.off + l.n; ++ri)
@@ -1554,30 +1593,55 @@ void place_tile_runs(Ctx* c, std::vector<TileRun>& runs, std::vector<PdCopy>& co
     if (n != l.n) throw PdError(PD_E_STATE, "tile runs do not cover their launch");
     cost.resize(n);
     ord.resize(n);
+    arranged.resize(n);
     for (int i = 0; i < n; ++i) {
       cost[i] = tile_cost(tl[i].first->tiles[tl[i].second]);
       ord[i] = i;
     }
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    const int q = n / 8, r = n % 8;
-    for (int b = 0; b < n; ++b) {
-      const int x = b % 8;
-      const int pos = x * q + std::min(x, r) + b / 8;
-      run_copy(tl[ord[b]].first, tl[ord[b]].second, 1, l.off + pos);
-    }
+    // rank b -> list position: runs of kOrderRun consecutive ranks (often
+    // neighbouring bricks of equal cost, which share halo records) are dealt
+    // round-robin over the XCD ranges, the ranks past the last whole round of
+    // runs one at a time, so that XCD x still receives q (+1 if x < r) tiles
+    const int q = n / 8, r = n % 8, whole = n / (8 * kOrderRun) * (8 * kOrderRun);
+    auto pos_of = [&](int b) {
+      int x, i;
+      if (b < whole) {
+        const int run = b / kOrderRun;
+        x = run % 8;
+        i = run / 8 * kOrderRun + b % kOrderRun;
+      } else {
+        x = (b - whole) % 8;
+        i = whole / 8 + (b - whole) / 8;
+      }
+      return x * q + std::min(x, r) + i;
+    };
+    for (int b = 0; b < n; ++b) arranged[pos_of(b)] = cost[ord[b]];
+    const int wg = (int)std::min<int64_t>(8, std::max<int64_t>(1, 163840 / ((int64_t)rec * std::max(l.hcap, 1))));
+    const int64_t s_brick = tile_span(cost, wg), s_cost = tile_span(arranged, wg);
+    const bool by_cost = mode == 1 || (mode < 0 && s_cost * 100 <= s_brick * (100 - kOrderGain));
+    if (log)
+      std::fprintf(stderr, "tile launch at %d: %d tiles, %d per CU, span %lld in brick order, %lld by cost: %s\n",
+                   l.off, n, wg, (long long)s_brick, (long long)s_cost, by_cost ? "by cost" : "brick order");
+    if (!by_cost) return in_order();
+    for (int b = 0; b < n; ++b) run_copy(tl[ord[b]].first, tl[ord[b]].second, 1, l.off + pos_of(b));
   };
   // launch ranges in ascending set offset (the order place() gave them)
-  std::vector<const Ctx::TileList*> ls;
+  std::vector<std::pair<const Ctx::TileList*, int>> ls;
   for (int lg = 1; lg <= 3; ++lg) {
     for (int pc = 0; pc < 2; ++pc) {
-      for (int h = 0; h < PD_TILE_CLASSES; ++h) ls.push_back(&l_th[pc][lg][h]);
+      const int prec = pc ? PD_PREC_FP32C : PD_PREC_FP64;
+      for (int h = 0; h < PD_TILE_CLASSES; ++h)
+        ls.push_back({&l_th[pc][lg][h], pd_tile_record_bytes(0, PD_OSB, prec, xfl)});
       for (int k = 0; k < 3; ++k)
-        for (int h = 0; h < PD_TILE_CLASSES; ++h) ls.push_back(&l_fk[k][pc][lg][h]);
+        for (int h = 0; h < PD_TILE_CLASSES; ++h)
+          ls.push_back({&l_fk[k][pc][lg][h], pd_tile_record_bytes(2, k, prec, xfl)});
     }
-    for (int h = 0; h < PD_TILE_CLASSES; ++h) ls.push_back(&l_td[lg][h]);
+    for (int h = 0; h < PD_TILE_CLASSES; ++h)
+      ls.push_back({&l_td[lg][h], pd_tile_record_bytes(1, PD_EP, PD_PREC_FP64, xfl)});
   }
-  std::sort(ls.begin(), ls.end(), [](const Ctx::TileList* a, const Ctx::TileList* b) { return a->off < b->off; });
-  for (const Ctx::TileList* l : ls) launch(*l);
+  std::sort(ls.begin(), ls.end(), [](const auto& a, const auto& b) { return a.first->off < b.first->off; });
+  for (const auto& l : ls) launch(*l.first, l.second);
 }
 
 // Run set of the listed envs.  The all-chunk list and the tile lists are
@@ -1763,7 +1827,7 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
     }
     merge(l_td[lg]);
   }
-  place_tile_runs(c, truns, copies, l_th, l_td, l_fk);
+  place_tile_runs(c, truns, copies, l_th, l_td, l_fk, xfl);
   std::copy(&l_th[0][0][0], &l_th[0][0][0] + 2 * 4 * PD_TILE_CLASSES, &c->tl_theta[0][0][0]);
   std::copy(&l_td[0][0], &l_td[0][0] + 4 * PD_TILE_CLASSES, &c->tl_td[0][0]);
   std::copy(&l_fk[0][0][0][0], &l_fk[0][0][0][0] + 3 * 2 * 4 * PD_TILE_CLASSES, &c->tl_force[0][0][0][0]);
