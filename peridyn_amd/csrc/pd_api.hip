@@ -1320,6 +1320,9 @@ void commit_env(Ctx* c, Env& e) {
       B.xint = xi ? 1 : 0;
       b.xint = xi;
     }
+    B.vol0 = b.d.n > 0 ? b.vol[0] : 0.0;
+    B.vol1 = 1;
+    for (int64_t i = 0; i < b.d.n; ++i) B.vol1 = B.vol1 && b.vol[i] == B.vol0;
     {
       float f = (float)b.s0[0];
       if ((double)f < b.s0[0]) f = std::nextafter(f, INFINITY);

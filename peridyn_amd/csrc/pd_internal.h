@@ -58,7 +58,10 @@ struct BlockDev {
   // s < s0f_up exactly when (double)s < s0[0]
   double corner[3];
   float s0f_up;
-  int32_t pad2;
+  // every particle of the block has the volume vol0 (the tiled bond-based
+  // kernels of lattice records then take it from here, not per record)
+  int32_t vol1;
+  double vol0;
 };
 
 struct BcDev {

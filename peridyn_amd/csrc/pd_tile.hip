@@ -615,7 +615,11 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
       if (KIND != PD_BB) ap = E.aux[p];
       const double c11 = B.c11;
       bool brk = false;  // this lane broke a bond
+      // a lattice block of one volume: vol_j / density is the same for every
+      // bond (the value the records give, formed once)
+      const double vd0 = B.vol0 * inv_den;
       with_lat<XF>(lat, [&](auto LT) {
+        auto bonds = [&](auto U) {
         pk_groups(pk, wb, nt4, [&](int, uint64_t cw) {
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
@@ -643,13 +647,20 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
               const double twm = ap.x * ap.y + th_j * rm_j;
               tt = (3.0 * K0 - 5.0 * G0) * twm + 15.0 * G0 * ext * (inv_x * sm);
             }
-            const double xt = tt * inv_y * (rc.vol(j) * inv_den);
+            const double xt = tt * inv_y * (decltype(U)::value ? vd0 : rc.vol(j) * inv_den);
             const double xx = ok ? xt : 0.0;
             a1 = __builtin_fma(xx, Y1, a1);
             a2 = __builtin_fma(xx, Y2, a2);
             a3 = __builtin_fma(xx, Y3, a3);
           }
         });
+        };
+        if constexpr (LAT) {
+          if (B.vol1) bonds(std::true_type{});
+          else bonds(std::false_type{});
+        } else {
+          bonds(std::false_type{});
+        }
         if (brk)
           pk_record_breaks(pk, pk16, st, wb, base, nt4, [&](int j) {
             double Y1, Y2, Y3, xij, inv_x;

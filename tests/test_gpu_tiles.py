@@ -337,18 +337,24 @@ def moved(spec, shift=(0.0, 0.0, 0.0), scale=1.0):
 
 
 @pytest.mark.parametrize("kind", ["bb", "osb", "ep"])
-@pytest.mark.parametrize("case", ["far", "negative", "half"])
+@pytest.mark.parametrize("case", ["far", "negative", "half", "volumes"])
 def test_tiled_lattice_positions_equal_untiled(pd, kind, case):
     """Compact records with and without the integer-lattice table of |X|: an
     integer lattice far from the origin (its coordinates wrap the table's 16-bit
-    lattice words), one at negative coordinates, and a half-integer lattice (FP32
-    values, general |X|).  Each equals the untiled kernels bit for bit."""
+    lattice words), one at negative coordinates, a half-integer lattice (FP32
+    values, general |X|), and an integer lattice of unequal FP32 volumes (the
+    volumes from the records, not the block's one volume).  Each equals the
+    untiled kernels bit for bit."""
     say("lattice positions", kind, case)
     spec = ep_tensile() if kind == "ep" else W.tensile_bb(n=9, dt=1e-4, kind=kind, pull_per_step=0.002)
     if case == "far":
         spec = moved(spec, shift=(40000.0, -20011.0, 123457.0))
     elif case == "negative":
         spec = moved(spec, shift=(-37.0, -8.0, -1000.0))
+    elif case == "volumes":
+        for b in spec["blocks"]:
+            v = np.asarray(b["vol"], dtype=np.float64)
+            b["vol"] = np.ascontiguousarray(v * (1.0 + (np.arange(v.size) % 7) / 8.0))
     else:
         spec = moved(spec, scale=0.5)
     a = make(pd, spec, True, 4)
