@@ -167,6 +167,7 @@ struct HostBlock {
   std::vector<double> density, s0, c, K, G, psi;
   int32_t first = 0, chunk0 = 0, nchunks = 0, maxdeg = 0;
   bool xint = false;  // integer lattice with horizon^2 <= 32 (BlockDev::xint)
+  bool vol1 = false;  // one volume (BlockDev::vol1)
 };
 
 struct HostContact {
@@ -1323,6 +1324,7 @@ void commit_env(Ctx* c, Env& e) {
     B.vol0 = b.d.n > 0 ? b.vol[0] : 0.0;
     B.vol1 = 1;
     for (int64_t i = 0; i < b.d.n; ++i) B.vol1 = B.vol1 && b.vol[i] == B.vol0;
+    b.vol1 = B.vol1 != 0;
     {
       float f = (float)b.s0[0];
       if ((double)f < b.s0[0]) f = std::nextafter(f, INFINITY);
@@ -1667,15 +1669,19 @@ void build_set(Ctx* c, const std::vector<int64_t>& ids, bool full = false) {
   // tile lists: theta [pc][lg][cls], td [lg][cls], force [kind][pc][lg][cls]
   using TileList = Ctx::TileList;
   TileList l_th[2][4][PD_TILE_CLASSES], l_td[4][PD_TILE_CLASSES], l_fk[3][2][4][PD_TILE_CLASSES];
-  bool lat = true;  // every tiled FP64 bond-based block of the set is an integer lattice
+  bool lat = true;   // every tiled FP64 bond-based block of the set is an integer lattice
+  bool onev = true;  // ... and has one volume
   for (int64_t id : ids) {
     const Env& e = get_env(c, id);
     if (e.tiled) xf = xf && e.xf32;  // the FP64 records' size, hence the LDS classes
     if (e.tiled && e.precision == PD_PREC_FP64)
       for (auto& b : e.blocks)
-        if (b.d.kind == PD_BB) lat = lat && b.xint;
+        if (b.d.kind == PD_BB) {
+          lat = lat && b.xint;
+          onev = onev && b.vol1;
+        }
   }
-  const int xfl = xf ? (lat ? 2 : 1) : 0;
+  const int xfl = xf ? (lat ? (onev ? 3 : 2) : 1) : 0;
   auto add = [](TileList& l, int n, int hmax) {
     l.n += n;
     l.hcap = std::max(l.hcap, hmax);

@@ -101,12 +101,14 @@ void pd_launch_force32(hipStream_t s, int kind, const Chunk* chunks, int n, EnvD
 // 1 td, 2 force), kind and form; xf = 1: every x and volume of the launched
 // envs is exactly an FP32 value (compact records), 2: besides, every tiled
 // bond-based block is an integer lattice (compact lattice records of the
-// bond-based force pass); hcap = largest halo of the launched tiles
+// bond-based force pass), 3: besides, every such block has one volume (the
+// lattice records without it); hcap = largest halo of the launched tiles
 int pd_tile_record_bytes(int pass, int kind, int prec, int xf);
 // LDS class of a pass's tiles whose largest halo is hmax: 1..4 for the
 // compiled strides that let 6, 5, 4, 3 workgroups share a CU (for the pass's
 // record size), 5 for the run-time capacity, 0 for the 7-workgroup stride of
-// the compact lattice records (bond-based force, xf = 2).  Envs are launched
+// the compact lattice records (bond-based force, xf = 2) or the 8-workgroup
+// stride of the one-volume ones (xf = 3).  Envs are launched
 // per class, so that one env with a large halo does not lower the occupancy
 // of the others.
 constexpr int PD_TILE_CLASSES = 6;

@@ -36,10 +36,12 @@ __host__ __device__ constexpr int tile_extra() {  // double fields beyond x, vol
 }
 // LAT (compact lattice records, the force pass of sets whose bond-based
 // blocks are all integer lattices): the lattice pair and then the volume as
-// one float, without x2: 12 bytes after the doubles instead of 16.
-template <int PASS, int KIND, bool XF, bool LAT = false>
+// one float, without x2: 12 bytes after the doubles instead of 16.  ONEV (the
+// same where every such block has one volume, BlockDev::vol1): the lattice
+// pair only, 8 bytes.
+template <int PASS, int KIND, bool XF, bool LAT = false, bool ONEV = false>
 __host__ __device__ constexpr int tile_record_bytes() {
-  return LAT ? 8 * (3 + tile_extra<PASS, KIND>()) + 12
+  return LAT ? 8 * (3 + tile_extra<PASS, KIND>()) + (ONEV ? 8 : 12)
              : XF ? 8 * (3 + tile_extra<PASS, KIND>()) + 4 * 4 : 8 * (7 + tile_extra<PASS, KIND>());
 }
 
@@ -96,7 +98,7 @@ struct TileWave {
 // then the records, then the LDS stores).
 constexpr int kStageR = 4;
 
-template <bool XF, int NX, bool LAT = false>
+template <bool XF, int NX, bool LAT = false, bool ONEV = false>
 __device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, double* __restrict__ sh, int hcap,
                                            bool lat) {
   const double4* __restrict__ x = E.x;
@@ -127,7 +129,7 @@ __device__ __forceinline__ void stage_halo(const EnvDev& E, const Tile& T, doubl
       if (i < T.hn) {
         if (LAT) {
           ((uint2*)xa)[i] = lat_pack(a[r].x, a[r].y, a[r].z);
-          ((float*)xb)[i] = (float)a[r].w;
+          if (!ONEV) ((float*)xb)[i] = (float)a[r].w;
         } else if (XF) {
           if (lat) ((uint2*)xa)[i] = lat_pack(a[r].x, a[r].y, a[r].z);
           else xa[i] = make_float2((float)a[r].x, (float)a[r].y);
@@ -552,8 +554,12 @@ __device__ __forceinline__ void ep_group(const EpGrp& G, int t4, uint32_t base, 
 // LAT: every block of the launch is an integer lattice and its records are
 // the compact lattice ones (bond-based blocks only); 7 workgroups of them fit
 // a CU's LDS in the densest class, so the kernel aims at 7 waves per SIMD.
-template <int KIND, int LG, bool XF, int S, bool LAT = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == PD_BB ? (LAT ? 7 : 6) : KIND == PD_OSB ? 5 : 4)))
+// ONEV: every block of the launch also has one volume; its records are 32
+// bytes, 8 workgroups fit, and the kernel aims at 8 waves per SIMD (64 VGPRs:
+// the compiler spills a few of the halo staging's registers, none of the bond
+// loop's).
+template <int KIND, int LG, bool XF, int S, bool LAT = false, bool ONEV = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == PD_BB ? (LAT ? (ONEV ? 8 : 7) : 6) : KIND == PD_OSB ? 5 : 4)))
 void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
                                                     EnvDev* __restrict__ envs, int flags, int hcap) {
   const int hs = S > 0 ? S : hcap;
@@ -572,7 +578,7 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
   __shared__ double2 xtab[XF ? kXTab : 1];
   if (XF) fill_xtab64(xtab);
   const bool lat = LAT || (XF && B.xint);
-  if (deformed) stage_halo<XF, NX, LAT>(E, T, sh, hs, lat);
+  if (deformed) stage_halo<XF, NX, LAT, ONEV>(E, T, sh, hs, lat);
   __syncthreads();
   if (!has) return;
   const Rec<XF, NX, LAT> rc(sh, hs);
@@ -655,7 +661,9 @@ void k_tile_force(const Tile* __restrict__ tiles, int ntiles,
           }
         });
         };
-        if constexpr (LAT) {
+        if constexpr (ONEV) {
+          bonds(std::true_type{});
+        } else if constexpr (LAT) {
           if (B.vol1) bonds(std::true_type{});
           else bonds(std::false_type{});
         } else {
@@ -1051,6 +1059,9 @@ constexpr int halo_stride(int c, int rec, int reserve) { return (kLdsClassBytes[
 // kernel fits 7 waves per SIMD in registers, the other FP64 kernels do not.
 constexpr int kLdsClass7Bytes = 23040;
 constexpr int halo_stride7(int rec, int reserve) { return (kLdsClass7Bytes - reserve) / rec; }
+// and of 8 (40 granules) for the one-volume lattice records of the same pass
+constexpr int kLdsClass8Bytes = 20480;
+constexpr int halo_stride8(int rec, int reserve) { return (kLdsClass8Bytes - reserve) / rec; }
 
 template <int REC, int RES = kXTabLds, class F>
 void with_stride(int hcap, F&& f) {
@@ -1066,6 +1077,12 @@ template <int REC, int RES = kXTabLds, class F>
 void with_stride7(int hcap, F&& f) {
   constexpr int s7 = halo_stride7(REC, RES);
   if (hcap <= s7) f(std::integral_constant<int, s7>{});
+  else with_stride<REC, RES>(hcap, f);
+}
+template <int REC, int RES = kXTabLds, class F>
+void with_stride8(int hcap, F&& f) {
+  constexpr int s8 = halo_stride8(REC, RES);
+  if (hcap <= s8) f(std::integral_constant<int, s8>{});
   else with_stride<REC, RES>(hcap, f);
 }
 template <int REC, int RES = kXTabLds, class F>
@@ -1091,10 +1108,11 @@ void launch_td_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int hcap
   const size_t lds = (size_t)tile_record_bytes<TP_TD, 0, XF>() * (size_t)(S > 0 ? S : hcap);
   hipLaunchKernelGGL((k_tile_td<LG, XF, S>), dim3(n), dim3(256), lds, s, tiles, n, envs, hcap);
 }
-template <int KIND, int LG, bool XF, int S, bool LAT = false>
+template <int KIND, int LG, bool XF, int S, bool LAT = false, bool ONEV = false>
 void launch_force_s(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
-  const size_t lds = (size_t)tile_record_bytes<TP_FORCE, KIND, XF, LAT>() * (size_t)(S > 0 ? S : hcap);
-  hipLaunchKernelGGL((k_tile_force<KIND, LG, XF, S, LAT>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags, hcap);
+  const size_t lds = (size_t)tile_record_bytes<TP_FORCE, KIND, XF, LAT, ONEV>() * (size_t)(S > 0 ? S : hcap);
+  hipLaunchKernelGGL((k_tile_force<KIND, LG, XF, S, LAT, ONEV>), dim3(n), dim3(256), lds, s, tiles, n, envs, flags,
+                     hcap);
 }
 
 template <int LG, bool XF>
@@ -1116,6 +1134,12 @@ template <int LG>
 void launch_force_lat_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
   with_stride7<tile_record_bytes<TP_FORCE, PD_BB, true, true>()>(hcap, [&](auto S) {
     launch_force_s<PD_BB, LG, true, decltype(S)::value, true>(s, tiles, n, envs, flags, hcap);
+  });
+}
+template <int LG>
+void launch_force_onev_lg(hipStream_t s, const Tile* tiles, int n, EnvDev* envs, int flags, int hcap) {
+  with_stride8<tile_record_bytes<TP_FORCE, PD_BB, true, true, true>()>(hcap, [&](auto S) {
+    launch_force_s<PD_BB, LG, true, decltype(S)::value, true, true>(s, tiles, n, envs, flags, hcap);
   });
 }
 
@@ -1169,6 +1193,10 @@ void allow_lds_fp64(size_t bytes) {
     constexpr int rec = tile_record_bytes<TP_FORCE, PD_BB, true, true>();
     allow_lds(k_tile_force<PD_BB, LG, true, halo_stride7(rec, kXTabLds), true>, bytes);
     each_stride<rec>([&](auto S) { allow_lds(k_tile_force<PD_BB, LG, true, decltype(S)::value, true>, bytes); });
+    constexpr int rec1 = tile_record_bytes<TP_FORCE, PD_BB, true, true, true>();
+    allow_lds(k_tile_force<PD_BB, LG, true, halo_stride8(rec1, kXTabLds), true, true>, bytes);
+    each_stride<rec1>(
+        [&](auto S) { allow_lds(k_tile_force<PD_BB, LG, true, decltype(S)::value, true, true>, bytes); });
   }
 }
 template <int LG>
@@ -1189,6 +1217,7 @@ int fp64_record_bytes(int kind) {
 int pd_tile_record_bytes(int pass, int kind, int prec, int xf) {
   if (prec == PD_PREC_FP32C && kind != PD_EP && pass != TP_TD) return rec32_bytes(pass, kind);
   if (xf == 2 && pass == TP_FORCE && kind == PD_BB) return tile_record_bytes<TP_FORCE, PD_BB, true, true>();
+  if (xf == 3 && pass == TP_FORCE && kind == PD_BB) return tile_record_bytes<TP_FORCE, PD_BB, true, true, true>();
   if (pass == 0) return xf ? fp64_record_bytes<TP_THETA, true>(kind) : fp64_record_bytes<TP_THETA, false>(kind);
   if (pass == 1) return xf ? fp64_record_bytes<TP_TD, true>(kind) : fp64_record_bytes<TP_TD, false>(kind);
   return xf ? fp64_record_bytes<TP_FORCE, true>(kind) : fp64_record_bytes<TP_FORCE, false>(kind);
@@ -1197,8 +1226,8 @@ int pd_tile_record_bytes(int pass, int kind, int prec, int xf) {
 int pd_tile_class(int pass, int kind, int prec, int xf, int hmax) {
   const int rec = pd_tile_record_bytes(pass, kind, prec, xf);
   const int res = (prec == PD_PREC_FP32C && kind != PD_EP && pass != TP_TD) ? 0 : kXTabLds;
-  const bool lat = xf == 2 && pass == TP_FORCE && kind == PD_BB && prec != PD_PREC_FP32C;
-  if (lat && hmax <= halo_stride7(rec, res)) return 0;
+  const bool lat = (xf == 2 || xf == 3) && pass == TP_FORCE && kind == PD_BB && prec != PD_PREC_FP32C;
+  if (lat && hmax <= (xf == 3 ? halo_stride8(rec, res) : halo_stride7(rec, res))) return 0;
   for (int c = 0; c < 4; ++c)
     if (hmax <= halo_stride(c, rec, res)) return c + 1;
   return 5;
@@ -1249,6 +1278,12 @@ void pd_launch_tile_force(hipStream_t s, int kind, int lgG, int xf, const Tile* 
     if (lgG == 1) launch_force_lat_lg<1>(s, tiles, n, envs, flags, hcap);
     else if (lgG == 2) launch_force_lat_lg<2>(s, tiles, n, envs, flags, hcap);
     else launch_force_lat_lg<3>(s, tiles, n, envs, flags, hcap);
+    return;
+  }
+  if (xf == 3 && kind == PD_BB) {
+    if (lgG == 1) launch_force_onev_lg<1>(s, tiles, n, envs, flags, hcap);
+    else if (lgG == 2) launch_force_onev_lg<2>(s, tiles, n, envs, flags, hcap);
+    else launch_force_onev_lg<3>(s, tiles, n, envs, flags, hcap);
     return;
   }
   PD_LG_XF(launch_force_any, kind, s, tiles, n, envs, flags, hcap);
