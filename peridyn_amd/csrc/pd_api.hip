@@ -14,7 +14,6 @@
 #include <map>
 #include <thread>
 #include <string>
-#include <queue>
 #include <vector>
 
 #include "pd_device.h"
@@ -1515,30 +1514,27 @@ int tile_order_mode() {
   if (!v) return -1;
   return std::string(v) == "0" ? 0 : 1;
 }
-constexpr int kOrderTiles = 16384, kOrderGain = 5, kOrderRun = 16;
+constexpr int kOrderTiles = 16384, kOrderGain = 4, kOrderRun = 16;
 
 // Predicted span of a launch whose tiles (costs in list order) are dispatched
-// as pd_xcd_block does: XCD x runs its contiguous range of the list in order,
-// each tile on the first of its 32 CUs' wg workgroup slots to come free.
+// as pd_xcd_block does: XCD x runs its contiguous range of the list in order
+// on its 32 CUs' wg workgroup slots, taken here as rounds of that many tiles,
+// each round lasting as long as its costliest tile (linear in the tiles, as
+// the set build runs before every served chunk).
 int64_t tile_span(const std::vector<int>& cost, int wg) {
   const int n = (int)cost.size(), q = n / 8, r = n % 8;
-  const size_t slots = (size_t)32 * (size_t)std::max(wg, 1);
+  const int slots = 32 * std::max(wg, 1);
   int64_t span = 0;
   int at = 0;
-  std::priority_queue<int64_t, std::vector<int64_t>, std::greater<int64_t>> free_at;
   for (int x = 0; x < 8; ++x) {
     const int m = q + (x < r ? 1 : 0);
-    free_at = decltype(free_at)();
-    for (int i = at; i < at + m; ++i) {
-      int64_t t0 = 0;
-      if (free_at.size() >= slots) {
-        t0 = free_at.top();
-        free_at.pop();
-      }
-      const int64_t t1 = t0 + cost[i];
-      span = std::max(span, t1);
-      free_at.push(t1);
+    int64_t t = 0;
+    for (int i0 = at; i0 < at + m; i0 += slots) {
+      int c = 0;
+      for (int i = i0; i < std::min(at + m, i0 + slots); ++i) c = std::max(c, cost[i]);
+      t += c;
     }
+    span = std::max(span, t);
     at += m;
   }
   return span;
@@ -1603,7 +1599,14 @@ void place_tile_runs(Ctx* c, std::vector<TileRun>& runs, std::vector<PdCopy>& co
       cost[i] = tile_cost(tl[i].first->tiles[tl[i].second]);
       ord[i] = i;
     }
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    {  // stable counting sort by decreasing cost (costs are small integers)
+      int cmax = 0;
+      for (int i = 0; i < n; ++i) cmax = std::max(cmax, cost[i]);
+      std::vector<int> at(cmax + 2, 0);
+      for (int i = 0; i < n; ++i) ++at[cmax - cost[i] + 1];
+      for (int k = 1; k <= cmax + 1; ++k) at[k] += at[k - 1];
+      for (int i = 0; i < n; ++i) ord[at[cmax - cost[i]]++] = i;
+    }
     // rank b -> list position: runs of kOrderRun consecutive ranks (often
     // neighbouring bricks of equal cost, which share halo records) are dealt
     // round-robin over the XCD ranges, the ranks past the last whole round of
@@ -1629,7 +1632,17 @@ void place_tile_runs(Ctx* c, std::vector<TileRun>& runs, std::vector<PdCopy>& co
       std::fprintf(stderr, "tile launch at %d: %d tiles, %d per CU, span %lld in brick order, %lld by cost: %s\n",
                    l.off, n, wg, (long long)s_brick, (long long)s_cost, by_cost ? "by cost" : "brick order");
     if (!by_cost) return in_order();
-    for (int b = 0; b < n; ++b) run_copy(tl[ord[b]].first, tl[ord[b]].second, 1, l.off + pos_of(b));
+    // one copy per run of ranks whose tiles follow each other both in their
+    // env's array and in the list
+    for (int b = 0; b < n;) {
+      const Env* e = tl[ord[b]].first;
+      const int t0 = tl[ord[b]].second, p0 = pos_of(b);
+      int k = 1;
+      while (b + k < n && tl[ord[b + k]].first == e && tl[ord[b + k]].second == t0 + k && pos_of(b + k) == p0 + k)
+        ++k;
+      run_copy(e, t0, k, l.off + p0);
+      b += k;
+    }
   };
   // launch ranges in ascending set offset (the order place() gave them)
   std::vector<std::pair<const Ctx::TileList*, int>> ls;
